@@ -1,0 +1,1337 @@
+// libncorr_hip.so: MI355X (gfx950) three-point triangle counting.
+//
+// Hot path of the reference (ahnyu/cudaNcorr): ncorr::calculateD1D2D3
+// (source/cudaNcorr.cu:220-430) and its kernels countTrianglesThreeDBox /
+// countTrianglesThreeDSurvey (source/device.cu:259-315, 381-431).
+//
+// Design (not a port): instead of one thread per unsorted primary walking
+// per-cell device allocations, the library
+//   1. builds a cell list on the device: FP64 cell key per point
+//      (cudaNcorr.cu:50-58), a counting sort by key (atomic rank + exclusive
+//      scan of the per-cell counts) and an SoA scatter, so neighbour cells are
+//      contiguous, coalesced ranges;
+//   2. runs one persistent triplet kernel: each workgroup pulls
+//      (term, primary tile) items from a device work queue, each wave takes one
+//      primary at a time, compacts the neighbours within (rmin, rmax) of its
+//      27 neighbour entries with wave ballots into a scratch list (storing the
+//      exact bin and FP32 sqrt of the r12 / r13 sides once), then loops over
+//      pairs of list entries -- one entry broadcast from LDS, one per lane --
+//      binning (r12, r13, r23) into a workgroup-private LDS histogram over the
+//      C(nb+2,3) sorted bins, merged once per term into global memory.
+//   When p2 and p3 are the same catalogue the (p2,p3) exchange symmetry of the
+//   reference's ordered loop is used: every unordered pair is evaluated once
+//   and counted twice (its squared separation is bitwise symmetric).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ncorr_hip.h"
+#include "ncorr_kernels.hpp"
+
+using namespace nck;
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+// Squared separation in the canonical evaluation order of the oracle:
+// fma(dz,dz, fma(dy,dy, dx*dx)), the left-to-right contraction of
+// dx*dx + dy*dy + dz*dz (device.cu:30-36).  Built with -ffp-contract=off so
+// no other contraction is introduced.
+__device__ __forceinline__ double sqrsep(double dx, double dy, double dz) {
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+}
+
+// Cell index along one axis: int(v / cs) with truncation (cudaNcorr.cu:51-56,
+// device.cu:76), optional clamp of ==n to n-1.  Out of grid => bad.
+__device__ __forceinline__ int axis_cell(double v, double cs, int n, bool clamp, bool &bad) {
+    const double q = v / cs;
+    if (!(q > -1.0 && q < (double)n + 1.0)) {
+        bad = true;
+        return 0;
+    }
+    int c = (int)q;
+    if (clamp && c == n) c = n - 1;
+    if (c < 0 || c >= n) {
+        bad = true;
+        return 0;
+    }
+    return c;
+}
+
+__device__ __forceinline__ int lane_rank(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Order LDS / global accesses of the lanes of one wave (wave-private data).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Exact bin of a squared separation d with r2min < d < r2max: lut[j] is a
+// lower bound of the bin of every d with (int)(d*scale) == j, and the loop
+// steps up over the squared edges (half-open bins, device.cu:181-191).
+__device__ __forceinline__ int bin_of(double d, double scale, const uint16_t *lut, const double *sq) {
+    int k = lut[(int)(d * scale)];
+    while (d >= sq[k + 1]) ++k;
+    return k;
+}
+
+// One of the 27 neighbour entries (x-major shift order, cudaNcorr.cu:37-48).
+// Box: one periodic wrap with image shift +/-L (device.cu:89-122).
+// Survey: entries outside the grid are skipped (device.cu:123-153) => -1.
+__device__ __forceinline__ int neighbour(const KArgs &A, int cx, int cy, int cz, int s, double &shx, double &shy,
+                                         double &shz) {
+    int ex = cx + s / 9 - 1, ey = cy + (s / 3) % 3 - 1, ez = cz + s % 3 - 1;
+    shx = shy = shz = 0.0;
+    if (A.survey) {
+        if (ex < 0 || ex >= A.nc[0] || ey < 0 || ey >= A.nc[1] || ez < 0 || ez >= A.nc[2]) return -1;
+    } else {
+        if (ex == A.nc[0]) { ex = 0; shx = A.box[0]; }
+        if (ey == A.nc[1]) { ey = 0; shy = A.box[1]; }
+        if (ez == A.nc[2]) { ez = 0; shz = A.box[2]; }
+        if (ex == -1) { ex = A.nc[0] - 1; shx = -A.box[0]; }
+        if (ey == -1) { ey = A.nc[1] - 1; shy = -A.box[1]; }
+        if (ez == -1) { ez = A.nc[2] - 1; shz = -A.box[2]; }
+    }
+    return ez + A.nc[2] * (ey + A.nc[1] * ex);
+}
+
+// ---------------------------------------------------------------------------
+// cell-list build
+
+struct GridGeo {
+    double cs[3];
+    int nc[3];
+    int survey;
+};
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Cell key of every point (host clamp rule, cudaNcorr.cu:50-58) and its slot in
+// the cell.  Survey primaries are also checked without the clamp, as the
+// reference's device index does not clamp them (device.cu:84-87).
+__global__ void k_cell_keys(const double4 *__restrict__ raw, int n, GridGeo g, int check_unclamped,
+                            uint32_t *__restrict__ key, uint32_t *__restrict__ rank, uint32_t *__restrict__ cnt,
+                            unsigned *__restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 p = raw[i];
+    bool bad = false;
+    int cx = axis_cell(p.x, g.cs[0], g.nc[0], true, bad);
+    int cy = axis_cell(p.y, g.cs[1], g.nc[1], true, bad);
+    int cz = axis_cell(p.z, g.cs[2], g.nc[2], true, bad);
+    if (check_unclamped) {
+        axis_cell(p.x, g.cs[0], g.nc[0], false, bad);
+        axis_cell(p.y, g.cs[1], g.nc[1], false, bad);
+        axis_cell(p.z, g.cs[2], g.nc[2], false, bad);
+    }
+    if (bad) {
+        atomicOr(err, kErrDomain);
+        cx = cy = cz = 0;
+    }
+    const uint32_t k = (uint32_t)(cz + g.nc[2] * (cy + g.nc[1] * cx));
+    key[i] = k;
+    rank[i] = atomicAdd(&cnt[k], 1u);
+}
+
+// Exclusive scan of the per-cell counts in one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ start,
+                                               int ncells) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (ncells + 1023) / 1024;
+    const int b = min(t * per, ncells), e = min(b + per, ncells);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += cnt[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (int i = b; i < e; ++i) {
+        start[i] = run;
+        run += cnt[i];
+    }
+    if (t == 1023) start[ncells] = part[1023];
+}
+
+__global__ void k_scatter(const double4 *__restrict__ raw, int n, const uint32_t *__restrict__ key,
+                          const uint32_t *__restrict__ rank, const uint32_t *__restrict__ start, double *__restrict__ x,
+                          double *__restrict__ y, double *__restrict__ z, double *__restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t d = start[key[i]] + rank[i];
+    const double4 p = raw[i];
+    x[d] = p.x;
+    y[d] = p.y;
+    z[d] = p.z;
+    w[d] = p.w;
+}
+
+// Upper bound of the neighbour-list length: over cells holding primaries, the
+// number of secondaries in the 27 neighbour entries.
+__global__ void k_nbr_bound(const uint32_t *__restrict__ st1, const uint32_t *__restrict__ st2, GridGeo g,
+                            unsigned *__restrict__ out) {
+    const int ncells = g.nc[0] * g.nc[1] * g.nc[2];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells || st1[c + 1] == st1[c]) return;
+    const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1], cx = c / (g.nc[2] * g.nc[1]);
+    unsigned tot = 0;
+    for (int s = 0; s < 27; ++s) {
+        int ex = cx + s / 9 - 1, ey = cy + (s / 3) % 3 - 1, ez = cz + s % 3 - 1;
+        if (g.survey) {
+            if (ex < 0 || ex >= g.nc[0] || ey < 0 || ey >= g.nc[1] || ez < 0 || ez >= g.nc[2]) continue;
+        } else {
+            ex = ex == g.nc[0] ? 0 : (ex == -1 ? g.nc[0] - 1 : ex);
+            ey = ey == g.nc[1] ? 0 : (ey == -1 ? g.nc[1] - 1 : ey);
+            ez = ez == g.nc[2] ? 0 : (ez == -1 ? g.nc[2] - 1 : ez);
+        }
+        const int k = ez + g.nc[2] * (ey + g.nc[1] * ex);
+        tot += st2[k + 1] - st2[k];
+    }
+    atomicMax(out, tot);
+}
+
+__global__ void k_unit_weight(const double4 *__restrict__ raw, long long n, unsigned *__restrict__ flag) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        if (raw[i].w != 1.0) atomicOr(flag, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// triplet kernel
+//
+// Per wave and primary p1, the neighbour lists S2 (catalogue c2) and S3 (c3)
+// hold every image point q of the 27 neighbour entries with
+// rmin^2 < |p1-q|^2 < rmax^2 -- exactly the p2 candidates the reference
+// accepts at device.cu:287 and the p3 candidates passing the r13 half of
+// device.cu:301.  The (p2,p3) loop of device.cu:288-308 is then a loop over
+// S2 x S3 that tests r23 only.
+//
+// Plain list entry (FP64, 40 B):   Elem {x,y,z image position, k, f} + weight
+// Mixed list entry (52 B):         float4 {x,y,z relative to p1, f} + k
+//                                  + double4 {x,y,z image position, weight}
+// k = exact bin of |p1-q|^2, f = sqrtf(float(|p1-q|^2)) as in device.cu:173.
+
+constexpr int kQueueCap = 256;  // deferred exact rechecks per wave (mixed path)
+
+struct LdsTabs {
+    unsigned long long *hist;
+    const double *sq;
+    const uint16_t *lut;
+    const int *ctab;
+    const uint16_t *lut32;
+    const float *band;
+};
+
+// Bytes of one list entry in a wave's scratch slot.
+__host__ __device__ constexpr size_t list_entry_bytes(bool mixed) {
+    return mixed ? sizeof(float4) + sizeof(int) + sizeof(double4) : sizeof(Elem) + sizeof(double);
+}
+
+// Compact the neighbours of primary (px,py,pz) in catalogue (X,Y,Z,Wt,st)
+// whose squared separation lies in (r2min, r2max) into the list at `base`.
+// Returns the list length (entries beyond cap are not stored).
+template <bool MIXED>
+__device__ int build_list(const KArgs &A, const double *__restrict__ X, const double *__restrict__ Y,
+                          const double *__restrict__ Z, const double *__restrict__ Wt, const uint32_t *__restrict__ st,
+                          double px, double py, double pz, int cx, int cy, int cz, char *__restrict__ base, int lane,
+                          const double *sq, const uint16_t *lut) {
+    int n = 0;
+    for (int s = 0; s < 27; ++s) {
+        double shx, shy, shz;
+        const int cell = neighbour(A, cx, cy, cz, s, shx, shy, shz);
+        if (cell < 0) continue;
+        const int beg = (int)st[cell], end = (int)st[cell + 1];
+        for (int j0 = beg; j0 < end; j0 += kWave) {
+            const int j = j0 + lane;
+            const bool v = j < end;
+            double qx = 0.0, qy = 0.0, qz = 0.0, qw = 0.0;
+            if (v) {
+                // image position: point + shift (device.cu:282-284)
+                qx = X[j] + shx;
+                qy = Y[j] + shy;
+                qz = Z[j] + shz;
+                qw = Wt[j];
+            }
+            const double d = sqrsep(px - qx, py - qy, pz - qz);
+            const bool pass = v && d > A.r2min && d < A.r2max;
+            const unsigned long long m = __ballot(pass);
+            const int pos = n + lane_rank(m);
+            if (pass && pos < A.cap) {
+                const int k = bin_of(d, A.lut_scale, lut, sq);
+                const float f = __builtin_sqrtf((float)d);
+                if constexpr (MIXED) {
+                    float4 *R = reinterpret_cast<float4 *>(base);
+                    int *K = reinterpret_cast<int *>(base + (size_t)A.cap * sizeof(float4));
+                    double4 *D = reinterpret_cast<double4 *>(base + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
+                    R[pos] = make_float4((float)(qx - px), (float)(qy - py), (float)(qz - pz), f);
+                    K[pos] = k;
+                    D[pos] = make_double4(qx, qy, qz, qw);
+                } else {
+                    Elem e;
+                    e.x = qx;
+                    e.y = qy;
+                    e.z = qz;
+                    e.k = k;
+                    e.f = f;
+                    reinterpret_cast<Elem *>(base)[pos] = e;
+                    reinterpret_cast<double *>(base + (size_t)A.cap * sizeof(Elem))[pos] = qw;
+                }
+            }
+            n += __popcll(m);
+        }
+    }
+    return n;
+}
+
+// Add one accepted (p2=a, p3=b) pair with bins (ka, kb, k) to the LDS
+// histogram over sorted bins.  SYM: the pair also stands for (p2=b, p3=a).
+template <bool SYM, bool WEIGHTED>
+__device__ __forceinline__ void accumulate(const KArgs &A, const LdsTabs &t, int ka, int kb, int k, double w1a,
+                                           double wa, double w1b, double wb) {
+    const int klo = min(min(ka, kb), k);
+    const int khi = max(max(ka, kb), k);
+    const int kmid = ka + kb + k - klo - khi;
+    const int idx = t.ctab[klo * A.nb + kmid] + (khi - kmid);
+    if constexpr (WEIGHTED) {
+        double v = w1a * wb;  // (w1*w2)*w3, device.cu:305
+        if (SYM) v += w1b * wa;
+        atomicAdd(reinterpret_cast<double *>(&t.hist[idx]), v);
+    } else {
+        atomicAdd(&t.hist[idx], SYM ? 2ull : 1ull);
+    }
+}
+
+// Exact FP64 evaluation of one pair (device.cu:299-307 restated): r23 from
+// the image positions, strict range filter, exact bin, FP32 triangle test.
+template <bool SYM, bool WEIGHTED>
+__device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, double ax, double ay, double az, int ka,
+                                           float fa, double w1a, double wa, double bx, double by, double bz, int kb,
+                                           float fb, double w1b, double wb, bool ok) {
+    const double d = sqrsep(ax - bx, ay - by, az - bz);
+    if (ok && d > A.r2min && d < A.r2max) {
+        const int k = bin_of(d, A.lut_scale, t.lut, t.sq);
+        const float f = __builtin_sqrtf((float)d);
+        const float lo = fminf(fminf(fa, fb), f);
+        const float hi = fmaxf(fmaxf(fa, fb), f);
+        const float mid = __builtin_amdgcn_fmed3f(fa, fb, f);
+        if (lo + mid > hi)  // the reference rejects sqrtf(r1)+sqrtf(r2) <= sqrtf(r3), device.cu:173
+            accumulate<SYM, WEIGHTED>(A, t, ka, kb, k, w1a, wa, w1b, wb);
+    }
+}
+
+// Plain path: all pairs of S2 (broadcast from LDS, a) x S3 (one per lane, b).
+template <bool SYM, bool WEIGHTED>
+__device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
+                                int n3, char *wlds, int lane) {
+    const int C = A.chunk_elems;
+    const Elem *L2 = reinterpret_cast<const Elem *>(l2);
+    const double *W2 = reinterpret_cast<const double *>(l2 + (size_t)A.cap * sizeof(Elem));
+    const Elem *L3 = reinterpret_cast<const Elem *>(l3);
+    const double *W3 = reinterpret_cast<const double *>(l3 + (size_t)A.cap * sizeof(Elem));
+    Elem *la = reinterpret_cast<Elem *>(wlds);
+    double *lw = reinterpret_cast<double *>(wlds + (size_t)C * sizeof(Elem));
+    for (int a0 = 0; a0 < n2; a0 += C) {
+        const int a1 = min(a0 + C, n2);
+        wave_sync();
+        for (int i = lane; i < a1 - a0; i += kWave) {
+            la[i] = L2[a0 + i];
+            if (WEIGHTED) lw[i] = W2[a0 + i];
+        }
+        wave_sync();
+        for (int b0 = SYM ? a0 : 0; b0 < n3; b0 += kWave) {
+            const int b = b0 + lane;
+            const bool vb = b < n3;
+            Elem eb;
+            double wb = 0.0;
+            if (vb) {
+                eb = L3[b];
+                if (WEIGHTED) wb = W3[b];
+            } else {
+                eb.x = eb.y = eb.z = 0.0;
+                eb.k = 0;
+                eb.f = 0.0f;
+            }
+            const double w1b = w1 * wb;
+            const int alim = SYM ? min(a1, min(n3, b0 + kWave) - 1) : a1;
+            for (int a = a0; a < alim; ++a) {
+                const Elem ea = la[a - a0];
+                double wa = 0.0;
+                if (WEIGHTED) wa = lw[a - a0];
+                exact_pair<SYM, WEIGHTED>(A, t, ea.x, ea.y, ea.z, ea.k, ea.f, w1 * wa, wa, eb.x, eb.y, eb.z, eb.k, eb.f,
+                                          w1b, wb, vb && (!SYM || a < b));
+            }
+        }
+    }
+}
+
+// Mixed path, deferred exact rechecks: every queued (a, b) pair is evaluated
+// with the FP64 image positions, one pair per lane.
+template <bool SYM, bool WEIGHTED>
+__device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
+                            const uint2 *q, int qn, int lane) {
+    const float4 *R2 = reinterpret_cast<const float4 *>(l2);
+    const int *K2 = reinterpret_cast<const int *>(l2 + (size_t)A.cap * sizeof(float4));
+    const double4 *D2 = reinterpret_cast<const double4 *>(l2 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
+    const float4 *R3 = reinterpret_cast<const float4 *>(l3);
+    const int *K3 = reinterpret_cast<const int *>(l3 + (size_t)A.cap * sizeof(float4));
+    const double4 *D3 = reinterpret_cast<const double4 *>(l3 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
+    wave_sync();
+    for (int i = lane; i < qn; i += kWave) {
+        const uint2 e = q[i];
+        const double4 pa = D2[e.x], pb = D3[e.y];
+        const double wa = WEIGHTED ? pa.w : 0.0, wb = WEIGHTED ? pb.w : 0.0;
+        exact_pair<SYM, WEIGHTED>(A, t, pa.x, pa.y, pa.z, K2[e.x], R2[e.x].w, w1 * wa, wa, pb.x, pb.y, pb.z, K3[e.y],
+                                  R3[e.y].w, w1 * wb, wb, true);
+    }
+    wave_sync();
+}
+
+// Mixed path: r23^2 in FP32 from coordinates re-centred on p1.  A lookup over
+// d*scale32 gives the bin directly when the cell is clear of every guard band;
+// a cell holding one band compares against it; d inside a band, a triangle
+// slack within its error bound, or a near-zero separation sends the pair to
+// the exact FP64 queue.
+template <bool SYM, bool WEIGHTED>
+__device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
+                                int n3, char *wlds, int lane) {
+    const int C = A.chunk_elems;
+    const float4 *R2 = reinterpret_cast<const float4 *>(l2);
+    const int *K2 = reinterpret_cast<const int *>(l2 + (size_t)A.cap * sizeof(float4));
+    const double4 *D2 = reinterpret_cast<const double4 *>(l2 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
+    const float4 *R3 = reinterpret_cast<const float4 *>(l3);
+    const int *K3 = reinterpret_cast<const int *>(l3 + (size_t)A.cap * sizeof(float4));
+    const double4 *D3 = reinterpret_cast<const double4 *>(l3 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
+    float4 *la = reinterpret_cast<float4 *>(wlds);
+    int *lk = reinterpret_cast<int *>(wlds + (size_t)C * sizeof(float4));
+    double *lw = reinterpret_cast<double *>(wlds + (size_t)C * (sizeof(float4) + sizeof(int)));
+    uint2 *q = reinterpret_cast<uint2 *>(wlds + (size_t)C * (sizeof(float4) + sizeof(int) + sizeof(double)));
+    const float s32 = A.lut_scale32;
+    const float jmax = (float)(A.lut32_m + 1);
+    const float tol_abs = A.tri_tol_abs, tol_rel = A.tri_tol_rel, d_tiny = A.d_tiny32;
+    const int nb = A.nb;
+    int qn = 0;
+    for (int a0 = 0; a0 < n2; a0 += C) {
+        const int a1 = min(a0 + C, n2);
+        wave_sync();
+        for (int i = lane; i < a1 - a0; i += kWave) {
+            la[i] = R2[a0 + i];
+            lk[i] = K2[a0 + i];
+            if (WEIGHTED) lw[i] = D2[a0 + i].w;
+        }
+        wave_sync();
+        for (int b0 = SYM ? a0 : 0; b0 < n3; b0 += kWave) {
+            const int b = b0 + lane;
+            const bool vb = b < n3;
+            float4 rb = make_float4(0.f, 0.f, 0.f, 0.f);
+            int kb = 0;
+            double wb = 0.0;
+            if (vb) {
+                rb = R3[b];
+                kb = K3[b];
+                if (WEIGHTED) wb = D3[b].w;
+            }
+            const double w1b = w1 * wb;
+            const int alim = SYM ? min(a1, min(n3, b0 + kWave) - 1) : a1;
+            for (int a = a0; a < alim; ++a) {
+                const float4 ra = la[a - a0];
+                const bool ok = vb && (!SYM || a < b);
+                const float dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+                const float d = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                bool amb = false, acc = false;
+                int k = 0;
+                if (ok) {
+                    const unsigned j = (unsigned)fminf(d * s32, jmax);
+                    const unsigned e = t.lut32[j];
+                    int c = (int)(e & 0xffu);
+                    bool amb_b = false;
+                    if (e & 0x100u) {
+                        const float blo = t.band[2 * c], bhi = t.band[2 * c + 1];
+                        amb_b = d >= blo && d <= bhi;
+                        c += d > bhi ? 1 : 0;
+                    }
+                    const bool valid = c >= 1 && c <= nb;
+                    if (valid || amb_b) {
+                        const float f = __builtin_amdgcn_sqrtf(d);
+                        const float lo = fminf(fminf(ra.w, rb.w), f);
+                        const float hi = fmaxf(fmaxf(ra.w, rb.w), f);
+                        const float mid = __builtin_amdgcn_fmed3f(ra.w, rb.w, f);
+                        const float slack = (lo + mid) - hi;
+                        const bool amb_t = fabsf(slack) <= __builtin_fmaf(tol_rel, hi, tol_abs) || d < d_tiny;
+                        amb = amb_b || amb_t;
+                        acc = !amb && valid && slack > 0.0f;
+                        k = c - 1;
+                    }
+                }
+                if (acc) {
+                    double wa = 0.0;
+                    if (WEIGHTED) wa = lw[a - a0];
+                    accumulate<SYM, WEIGHTED>(A, t, lk[a - a0], kb, k, w1 * wa, wa, w1b, wb);
+                }
+                const unsigned long long m = __ballot(amb);
+                if (m) {
+                    const int pos = qn + lane_rank(m);
+                    if (amb) q[pos] = make_uint2((unsigned)a, (unsigned)b);
+                    qn += __popcll(m);
+                    if (qn > kQueueCap - kWave) {
+                        drain_queue<SYM, WEIGHTED>(A, t, w1, l2, l3, q, qn, lane);
+                        qn = 0;
+                    }
+                }
+            }
+        }
+    }
+    if (qn > 0) drain_queue<SYM, WEIGHTED>(A, t, w1, l2, l3, q, qn, lane);
+}
+
+__device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long long *hist) {
+    __syncthreads();
+    unsigned long long *out = A.out + (long long)term * A.nbc;
+    for (int i = threadIdx.x; i < A.nbc; i += blockDim.x) {
+        const unsigned long long v = hist[i];
+        if (v != 0ull) {
+            if (weighted)
+                unsafeAtomicAdd(reinterpret_cast<double *>(&out[i]), __longlong_as_double((long long)v));
+            else
+                atomicAdd(&out[i], v);
+            hist[i] = 0ull;
+        }
+    }
+    __syncthreads();
+}
+
+// Persistent triplet kernel.  Workgroups pull (term, primary range) chunks
+// from the queue A.chunks with one atomic per chunk; the waves of a workgroup
+// take the primaries of the chunk one at a time.  The LDS histogram of the
+// workgroup is merged into the term's global histogram whenever the workgroup
+// moves to another term and at exit.
+template <bool MIXED>
+__global__ __launch_bounds__(512) void k_triplets(KArgs A) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    __shared__ int s_chunk;
+    __shared__ int s_next;
+    LdsTabs t;
+    t.hist = reinterpret_cast<unsigned long long *>(lds);
+    double *sq = reinterpret_cast<double *>(lds + A.off_sq);
+    uint16_t *lut = reinterpret_cast<uint16_t *>(lds + A.off_lut);
+    int *ctab = reinterpret_cast<int *>(lds + A.off_ctab);
+    uint16_t *lut32 = reinterpret_cast<uint16_t *>(lds + A.off_lut32);
+    float *band = reinterpret_cast<float *>(lds + A.off_band);
+    t.sq = sq;
+    t.lut = lut;
+    t.ctab = ctab;
+    t.lut32 = lut32;
+    t.band = band;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int i = tid; i < A.nbc; i += nthr) t.hist[i] = 0ull;
+    for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
+    for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
+    for (int i = tid; i < A.nb * A.nb; i += nthr) ctab[i] = A.ctab[i];
+    if constexpr (MIXED) {
+        for (int i = tid; i <= A.lut32_m + 1; i += nthr) lut32[i] = A.lut32[i];
+        for (int i = tid; i < 2 * (A.nb + 1); i += nthr) band[i] = A.band[i];
+    }
+    const int lane = tid & (kWave - 1), wave = tid / kWave, nw = nthr / kWave;
+    char *wlds = lds + A.off_wave + wave * A.wave_bytes;
+    char *slot = A.scratch + (long long)(blockIdx.x * nw + wave) * A.slot_bytes;
+    char *l2 = slot;
+    char *l3 = slot + (size_t)A.cap * list_entry_bytes(MIXED);
+    int cur = -1, cur_weighted = 0;
+    __syncthreads();
+    for (;;) {
+        if (tid == 0) {
+            s_chunk = (int)atomicAdd(A.counter, 1u);
+            s_next = 0;
+        }
+        __syncthreads();
+        const int ci = s_chunk;
+        if (ci >= A.nchunks) break;
+        const Chunk ch = A.chunks[ci];
+        const TermDesc T = A.terms[ch.term];
+        if (ch.term != cur) {
+            if (cur >= 0) flush_hist(A, cur, cur_weighted, t.hist);
+            cur = ch.term;
+            cur_weighted = T.weighted;
+        }
+        for (;;) {
+            int p = 0;
+            if (lane == 0) p = atomicAdd(&s_next, 1);
+            p = __builtin_amdgcn_readfirstlane(p) + ch.begin;
+            if (p >= ch.end) break;
+            const double px = T.x1[p], py = T.y1[p], pz = T.z1[p], pw = T.w1[p];
+            bool bad = false;
+            const bool clamp = !A.survey;  // device.cu:78-80 vs 84-87
+            const int cx = axis_cell(px, A.cs[0], A.nc[0], clamp, bad);
+            const int cy = axis_cell(py, A.cs[1], A.nc[1], clamp, bad);
+            const int cz = axis_cell(pz, A.cs[2], A.nc[2], clamp, bad);
+            if (bad) {
+                if (lane == 0) atomicOr(A.err, kErrDomain);
+                continue;
+            }
+            const int n2 =
+                build_list<MIXED>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, sq, lut);
+            const int n3 = T.sym ? n2
+                                 : build_list<MIXED>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3,
+                                                     lane, sq, lut);
+            if (n2 > A.cap || n3 > A.cap) {
+                if (lane == 0) atomicOr(A.err, kErrOverflow);
+                continue;
+            }
+            if (n2 == 0 || n3 == 0) continue;
+            wave_sync();
+            const char *lb = T.sym ? l2 : l3;
+            if constexpr (MIXED) {
+                if (T.sym) {
+                    if (T.weighted)
+                        pair_loop_mixed<true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    else
+                        pair_loop_mixed<true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                } else {
+                    if (T.weighted)
+                        pair_loop_mixed<false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    else
+                        pair_loop_mixed<false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                }
+            } else {
+                if (T.sym) {
+                    if (T.weighted)
+                        pair_loop_plain<true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    else
+                        pair_loop_plain<true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                } else {
+                    if (T.weighted)
+                        pair_loop_plain<false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    else
+                        pair_loop_plain<false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                }
+            }
+            wave_sync();
+        }
+        __syncthreads();
+    }
+    if (cur >= 0) flush_hist(A, cur, cur_weighted, t.hist);
+}
+
+template __global__ void k_triplets<false>(KArgs);
+template __global__ void k_triplets<true>(KArgs);
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+hipError_t dgrow(DBuf &b, size_t need) {
+    if (need == 0) need = 16;
+    if (need <= b.bytes) return hipSuccess;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    hipError_t e = hipMalloc(&b.p, need);
+    if (e == hipSuccess) b.bytes = need;
+    return e;
+}
+
+void dfree(DBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+struct Geometry {
+    double box[3];
+    double cs[3];
+    int nc[3];
+    long long ncells;
+    int survey;
+    int nb, nbc;
+    std::vector<double> edges, sq;
+    std::vector<uint16_t> lut;
+    double lut_scale;
+    int lut_m;
+    std::vector<int> ctab;
+    // mixed path: guard bands and the FP32 lookup (valid when mixed_ok)
+    bool mixed_ok = false;
+    std::vector<uint16_t> lut32;
+    std::vector<float> band;
+    float lut_scale32 = 0.f;
+    int lut32_m = 0;
+    float tri_tol_abs = 0.f, tri_tol_rel = 0.f, d_tiny32 = 0.f;
+};
+
+}  // namespace
+
+struct nc_catalog {
+    nc_context *ctx = nullptr;
+    int64_t n = 0;
+    const double *d_raw = nullptr;
+    bool owns = false;
+    bool unit_w = true;
+    DBuf raw, key, rank, cnt, start, x, y, z, w;
+};
+
+struct nc_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int ncu = 0;
+    size_t lds_max = 0;
+    DBuf tables, terms, chunks, out, counter, errw, scratch, bound;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    nc_timing timing{};
+    int waves_per_group = 4;
+    int chunk = 16;
+    int wgs_per_cu = 0;
+    int chunk_elems = 128;
+};
+
+namespace {
+
+int fail(nc_context *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define NC_HIP(ctx, call)                                                                               \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? NC_ERR_NOMEM : NC_ERR_HIP,                     \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                             \
+    } while (0)
+
+int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
+
+// Guard bands and FP32 lookup of the mixed path.
+//
+// Error bound.  With u = 2^-24, coordinates re-centred on p1 and rounded to
+// FP32 (|component| < rmax, error <= u*rmax each), differences rounded once
+// and the FMA chain rounded three times, the FP32 squared separation d32 of a
+// pair with true squared separation d obeys
+//     |d32 - d| <= u*(4*sqrt(3)*rmax*sqrt(d) + 5*d) + 12*u^2*rmax^2,
+// and the reference's FP64 value is within a few 1e-16*d of d.  Every squared
+// edge E gets the band E -/+ 2x that bound (rounded outward to FP32): a d32
+// outside every band is on the same side of every edge as the FP64 value.
+// The triangle-test inputs differ from the reference's in the r23 side only;
+// sqrt(d32) is within 32*u*rmax + 16*u*hi of sqrtf(float(d64)) unless d32 is
+// tiny, so slacks within that margin, and d32 < (64*u*rmax)^2, are rechecked.
+//
+// Lookup: j = min(fl32(d32*s32), M+1) over [0, 1.25*rmax^2); any d32 mapped
+// to j lies in [(j-1)/s32, (j+2)/s32].  Entry j holds the number of bands
+// entirely below that interval (bin+1 code) and a flag when one band meets it;
+// tables where one interval meets two bands are rejected (M is doubled).
+static void make_mixed_tables(Geometry &G) {
+    G.mixed_ok = false;
+    const double u = std::ldexp(1.0, -24);
+    const double R = G.edges.back();
+    const int ne = G.nb + 1;
+    std::vector<float> lo(ne), hi(ne);
+    for (int k = 0; k < ne; ++k) {
+        const double E = G.sq[k];
+        const double g = 2.0 * (u * (6.93 * R * std::sqrt(E) + 5.0 * E) + 12.0 * u * u * R * R) + 1e-12 * E;
+        float l = (float)(E - g), h = (float)(E + g);
+        if ((double)l > E - g) l = std::nextafter(l, -INFINITY);
+        if ((double)h < E + g) h = std::nextafter(h, INFINITY);
+        lo[k] = l;
+        hi[k] = h;
+    }
+    for (int k = 0; k + 1 < ne; ++k)
+        if (!(hi[k] < lo[k + 1])) return;  // bins narrower than the guard bands
+    const double r2max = G.sq[G.nb];
+    for (int M = 2048; M <= 16384; M *= 2) {
+        const float s32 = (float)((double)M / (1.25 * r2max));
+        const double s = (double)s32;
+        std::vector<uint16_t> lut(M + 2);
+        bool ok = true;
+        for (int j = 0; j <= M + 1 && ok; ++j) {
+            const double a = j == 0 ? -INFINITY : (double)(j - 1) / s * (1.0 - 1e-6);
+            const double b = j == M + 1 ? INFINITY : (double)(j + 2) / s * (1.0 + 1e-6);
+            int below = 0, meet = 0, which = -1;
+            for (int k = 0; k < ne; ++k) {
+                if ((double)hi[k] < a)
+                    ++below;
+                else if ((double)lo[k] <= b) {
+                    ++meet;
+                    which = k;
+                }
+            }
+            if (meet > 1 || (meet == 1 && which != below)) ok = false;
+            lut[j] = (uint16_t)(below | (meet ? 0x100 : 0));
+        }
+        if (!ok) continue;
+        G.lut32 = std::move(lut);
+        G.lut32_m = M;
+        G.lut_scale32 = s32;
+        G.band.resize(2 * ne);
+        for (int k = 0; k < ne; ++k) {
+            G.band[2 * k] = lo[k];
+            G.band[2 * k + 1] = hi[k];
+        }
+        G.tri_tol_abs = (float)(32.0 * u * R);
+        G.tri_tol_rel = (float)(16.0 * u);
+        G.d_tiny32 = (float)((64.0 * u * R) * (64.0 * u * R));
+        G.mixed_ok = G.nb < 255;
+        return;
+    }
+}
+
+// Geometry of the reference's threeD constructor (cudaNcorr.cu:118-157) plus
+// the lookup tables of the kernels.
+int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
+    if (!P || !P->edges || P->nedges < 2) return fail(ctx, NC_ERR_BINS, "need at least two bin edges");
+    const int ne = P->nedges;
+    G.edges.assign(P->edges, P->edges + ne);
+    if (!(G.edges[0] >= 0.0)) return fail(ctx, NC_ERR_BINS, "first bin edge must be >= 0");
+    for (int i = 1; i < ne; ++i)
+        if (!(G.edges[i] > G.edges[i - 1])) return fail(ctx, NC_ERR_BINS, "bin edges must be strictly increasing");
+    G.nb = ne - 1;
+    if (G.nb > 40) return fail(ctx, NC_ERR_LIMIT, "at most 40 bins per side are supported");
+    G.nbc = (int)choose_compact(G.nb);
+    G.sq.resize(ne);
+    for (int i = 0; i < ne; ++i) G.sq[i] = G.edges[i] * G.edges[i];  // cudaNcorr.cu:124-127
+    G.survey = P->survey ? 1 : 0;
+    G.ncells = 1;
+    for (int a = 0; a < 3; ++a) {
+        G.box[a] = P->box[a];
+        G.cs[a] = G.edges[ne - 1];  // cellSize = rmax, cudaNcorr.cu:137
+        const double q = std::ceil(G.box[a] / G.cs[a]);
+        if (!(q >= 1.0 && q <= 1048576.0)) return fail(ctx, NC_ERR_ARG, "box must be positive and finite");
+        G.nc[a] = (int)q;
+        G.ncells *= G.nc[a];
+    }
+    if (G.ncells > (1LL << 27)) return fail(ctx, NC_ERR_LIMIT, "cell grid larger than 2^27 cells");
+    // lower-bound bin lookup over d*scale (see bin_of)
+    G.lut_m = 2048;
+    const double r2max = G.sq[G.nb];
+    G.lut_scale = (double)G.lut_m / r2max;
+    G.lut.assign(G.lut_m + 1, 0);
+    for (int j = 0; j <= G.lut_m; ++j) {
+        int k = 0;
+        for (int e = 1; e < G.nb; ++e)
+            if (G.sq[e] * G.lut_scale * (1.0 + 1e-12) <= (double)(j - 1)) k = e;
+        G.lut[j] = (uint16_t)k;
+    }
+    // compact index of sorted (i <= j <= k) bins in lexicographic order
+    G.ctab.assign((size_t)G.nb * G.nb, 0);
+    int c = 0;
+    for (int i = 0; i < G.nb; ++i)
+        for (int j = i; j < G.nb; ++j) {
+            G.ctab[i * G.nb + j] = c;
+            c += G.nb - j;
+        }
+    make_mixed_tables(G);
+    return NC_OK;
+}
+
+struct CatGrid {
+    nc_catalog *cat;
+    bool is_p1;
+};
+
+int build_grid(nc_context *ctx, const Geometry &G, nc_catalog *c, bool check_unclamped) {
+    const int n = (int)c->n;
+    const size_t nc = (size_t)G.ncells;
+    NC_HIP(ctx, dgrow(c->key, sizeof(uint32_t) * (size_t)n));
+    NC_HIP(ctx, dgrow(c->rank, sizeof(uint32_t) * (size_t)n));
+    NC_HIP(ctx, dgrow(c->cnt, sizeof(uint32_t) * nc));
+    NC_HIP(ctx, dgrow(c->start, sizeof(uint32_t) * (nc + 1)));
+    NC_HIP(ctx, dgrow(c->x, sizeof(double) * (size_t)n));
+    NC_HIP(ctx, dgrow(c->y, sizeof(double) * (size_t)n));
+    NC_HIP(ctx, dgrow(c->z, sizeof(double) * (size_t)n));
+    NC_HIP(ctx, dgrow(c->w, sizeof(double) * (size_t)n));
+    GridGeo g;
+    for (int a = 0; a < 3; ++a) {
+        g.cs[a] = G.cs[a];
+        g.nc[a] = G.nc[a];
+    }
+    g.survey = G.survey;
+    const int fb = (int)std::min<long long>((G.ncells + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, ctx->stream, (uint32_t *)c->cnt.p, 0u, (long long)nc);
+    if (n > 0) {
+        const int nb = (n + 255) / 256;
+        hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n, g,
+                           check_unclamped ? 1 : 0, (uint32_t *)c->key.p, (uint32_t *)c->rank.p, (uint32_t *)c->cnt.p,
+                           (unsigned *)ctx->errw.p);
+    }
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)c->cnt.p, (uint32_t *)c->start.p,
+                       (int)nc);
+    if (n > 0) {
+        const int nb = (n + 255) / 256;
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n,
+                           (const uint32_t *)c->key.p, (const uint32_t *)c->rank.p, (const uint32_t *)c->start.p,
+                           (double *)c->x.p, (double *)c->y.p, (double *)c->z.p, (double *)c->w.p);
+    }
+    NC_HIP(ctx, hipGetLastError());
+    return NC_OK;
+}
+
+struct Term {
+    nc_catalog *c1, *c2, *c3;
+};
+
+int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &terms, int path, double *hist,
+              uint64_t *counts) {
+    if (!ctx) return NC_ERR_ARG;
+    if (!P || !hist) return fail(ctx, NC_ERR_ARG, "null params or output");
+    if (path != NC_PATH_PLAIN && path != NC_PATH_MIXED) return fail(ctx, NC_ERR_ARG, "unknown path");
+    for (const Term &t : terms)
+        if (!t.c1 || !t.c2 || !t.c3 || t.c1->ctx != ctx || t.c2->ctx != ctx || t.c3->ctx != ctx)
+            return fail(ctx, NC_ERR_ARG, "null catalogue or catalogue of another context");
+    Geometry G;
+    int rc = make_geometry(ctx, P, G);
+    if (rc) return rc;
+    NC_HIP(ctx, hipSetDevice(ctx->device));
+    const int nterms = (int)terms.size();
+    const size_t nh = (size_t)G.nb * G.nb * G.nb;
+    const bool mixed = path == NC_PATH_MIXED && G.mixed_ok;
+    // --- tables (copied into LDS by the kernel)
+    auto al16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
+    const size_t off_sq = 0;
+    const size_t off_lut = al16(off_sq + sizeof(double) * G.sq.size());
+    const size_t off_ctab = al16(off_lut + sizeof(uint16_t) * G.lut.size());
+    const size_t off_lut32 = al16(off_ctab + sizeof(int) * G.ctab.size());
+    const size_t off_band = al16(off_lut32 + (mixed ? sizeof(uint16_t) * G.lut32.size() : 0));
+    const size_t tab_bytes = al16(off_band + (mixed ? sizeof(float) * G.band.size() : 0));
+    std::vector<char> tabs(tab_bytes);
+    std::memcpy(tabs.data() + off_sq, G.sq.data(), sizeof(double) * G.sq.size());
+    std::memcpy(tabs.data() + off_lut, G.lut.data(), sizeof(uint16_t) * G.lut.size());
+    std::memcpy(tabs.data() + off_ctab, G.ctab.data(), sizeof(int) * G.ctab.size());
+    if (mixed) {
+        std::memcpy(tabs.data() + off_lut32, G.lut32.data(), sizeof(uint16_t) * G.lut32.size());
+        std::memcpy(tabs.data() + off_band, G.band.data(), sizeof(float) * G.band.size());
+    }
+    NC_HIP(ctx, dgrow(ctx->tables, tab_bytes));
+    NC_HIP(ctx, hipMemcpyAsync(ctx->tables.p, tabs.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+    NC_HIP(ctx, dgrow(ctx->errw, 16));
+    NC_HIP(ctx, dgrow(ctx->counter, 16));
+    NC_HIP(ctx, dgrow(ctx->bound, 16));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->errw.p, 0u, 4LL);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->bound.p, 0u, 4LL);
+    for (const Term &t : terms)
+        for (nc_catalog *c : {t.c1, t.c2, t.c3})
+            if (c->n > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
+
+    // --- cell lists, one per distinct catalogue
+    NC_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    std::vector<CatGrid> grids;
+    for (const Term &t : terms) {
+        nc_catalog *cs[3] = {t.c1, t.c2, t.c3};
+        for (int r = 0; r < 3; ++r) {
+            bool found = false;
+            for (CatGrid &g : grids)
+                if (g.cat == cs[r]) {
+                    found = true;
+                    g.is_p1 = g.is_p1 || r == 0;
+                }
+            if (!found) grids.push_back({cs[r], r == 0});
+        }
+    }
+    for (CatGrid &g : grids) {
+        rc = build_grid(ctx, G, g.cat, G.survey && g.is_p1);
+        if (rc) return rc;
+    }
+    NC_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    GridGeo gg;
+    for (int a = 0; a < 3; ++a) {
+        gg.cs[a] = G.cs[a];
+        gg.nc[a] = G.nc[a];
+    }
+    gg.survey = G.survey;
+    const int cb = (int)((G.ncells + 255) / 256);
+    for (const Term &t : terms) {
+        hipLaunchKernelGGL(k_nbr_bound, dim3(cb), dim3(256), 0, ctx->stream, (const uint32_t *)t.c1->start.p,
+                           (const uint32_t *)t.c2->start.p, gg, (unsigned *)ctx->bound.p);
+        if (t.c3 != t.c2)
+            hipLaunchKernelGGL(k_nbr_bound, dim3(cb), dim3(256), 0, ctx->stream, (const uint32_t *)t.c1->start.p,
+                               (const uint32_t *)t.c3->start.p, gg, (unsigned *)ctx->bound.p);
+    }
+    NC_HIP(ctx, hipGetLastError());
+    unsigned host_flags[2] = {0, 0};
+    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(&host_flags[1], ctx->bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
+    const int cap = (int)((std::max(1u, host_flags[1]) + 15u) & ~15u);
+
+    // --- launch shape
+    const int W = ctx->waves_per_group;
+    const int block = W * kWave;
+    int C = ctx->chunk_elems;
+    const size_t hist_bytes = al16((size_t)G.nbc * 8);
+    const size_t lds_tab = al16(hist_bytes + tab_bytes);
+    auto wave_lds = [&](int c) {
+        return mixed ? al16((size_t)c * (sizeof(float4) + sizeof(int) + sizeof(double)) + kQueueCap * sizeof(uint2))
+                     : al16((size_t)c * (sizeof(Elem) + sizeof(double)));
+    };
+    while (C > 16 && lds_tab + W * wave_lds(C) > ctx->lds_max) C /= 2;
+    const size_t wave_bytes = wave_lds(C);
+    const size_t lds_bytes = lds_tab + W * wave_bytes;
+    if (lds_bytes > ctx->lds_max) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
+    const void *kfn = mixed ? (const void *)k_triplets<true> : (const void *)k_triplets<false>;
+    int per_cu = ctx->wgs_per_cu;
+    if (per_cu <= 0) {
+        int occ = 0;
+        NC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, block, lds_bytes));
+        per_cu = std::max(1, occ);
+    }
+    const int grid = ctx->ncu * per_cu;
+    const long long nwaves = (long long)grid * W;
+    const long long slot_bytes = (long long)((2 * (size_t)cap * list_entry_bytes(mixed) + 255) & ~(size_t)255);
+    if ((double)nwaves * (double)slot_bytes > 64.0 * (1ULL << 30))
+        return fail(ctx, NC_ERR_LIMIT, "neighbour lists need more than 64 GiB of scratch");
+    NC_HIP(ctx, dgrow(ctx->scratch, (size_t)(nwaves * slot_bytes)));
+
+    // --- terms and work queue
+    std::vector<TermDesc> td(nterms);
+    std::vector<Chunk> chunks;
+    std::vector<int> weighted(nterms);
+    for (int i = 0; i < nterms; ++i) {
+        const Term &t = terms[i];
+        TermDesc &d = td[i];
+        d.x1 = (const double *)t.c1->x.p;
+        d.y1 = (const double *)t.c1->y.p;
+        d.z1 = (const double *)t.c1->z.p;
+        d.w1 = (const double *)t.c1->w.p;
+        d.x2 = (const double *)t.c2->x.p;
+        d.y2 = (const double *)t.c2->y.p;
+        d.z2 = (const double *)t.c2->z.p;
+        d.w2 = (const double *)t.c2->w.p;
+        d.st2 = (const uint32_t *)t.c2->start.p;
+        d.x3 = (const double *)t.c3->x.p;
+        d.y3 = (const double *)t.c3->y.p;
+        d.z3 = (const double *)t.c3->z.p;
+        d.w3 = (const double *)t.c3->w.p;
+        d.st3 = (const uint32_t *)t.c3->start.p;
+        d.n1 = (int)t.c1->n;
+        d.sym = t.c2 == t.c3;
+        d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
+        d.pad = 0;
+        weighted[i] = d.weighted;
+        if (t.c2->n == 0 || t.c3->n == 0) continue;
+        for (int b = 0; b < d.n1; b += ctx->chunk) chunks.push_back({i, b, std::min(d.n1, b + ctx->chunk), 0});
+    }
+    NC_HIP(ctx, dgrow(ctx->terms, sizeof(TermDesc) * td.size()));
+    NC_HIP(ctx, dgrow(ctx->chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size())));
+    NC_HIP(ctx, dgrow(ctx->out, sizeof(unsigned long long) * (size_t)nterms * G.nbc));
+    if (!td.empty())
+        NC_HIP(ctx, hipMemcpyAsync(ctx->terms.p, td.data(), sizeof(TermDesc) * td.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    if (!chunks.empty())
+        NC_HIP(ctx, hipMemcpyAsync(ctx->chunks.p, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->out.p, 0ull,
+                       (long long)nterms * G.nbc);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
+
+    KArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.terms = (const TermDesc *)ctx->terms.p;
+    A.chunks = (const Chunk *)ctx->chunks.p;
+    A.nchunks = (int)chunks.size();
+    A.counter = (unsigned *)ctx->counter.p;
+    A.err = (unsigned *)ctx->errw.p;
+    A.out = (unsigned long long *)ctx->out.p;
+    A.scratch = (char *)ctx->scratch.p;
+    A.slot_bytes = slot_bytes;
+    A.cap = cap;
+    A.chunk_elems = C;
+    for (int a = 0; a < 3; ++a) {
+        A.box[a] = G.box[a];
+        A.cs[a] = G.cs[a];
+        A.nc[a] = G.nc[a];
+    }
+    A.survey = G.survey;
+    A.r2min = G.sq[0];
+    A.r2max = G.sq[G.nb];
+    A.nb = G.nb;
+    A.nbc = G.nbc;
+    A.lut_scale = G.lut_scale;
+    A.lut_m = G.lut_m;
+    A.sq = (const double *)((char *)ctx->tables.p + off_sq);
+    A.lut = (const uint16_t *)((char *)ctx->tables.p + off_lut);
+    A.ctab = (const int *)((char *)ctx->tables.p + off_ctab);
+    A.lut32 = (const uint16_t *)((char *)ctx->tables.p + off_lut32);
+    A.band = (const float *)((char *)ctx->tables.p + off_band);
+    A.lut_scale32 = G.lut_scale32;
+    A.lut32_m = G.lut32_m;
+    A.tri_tol_abs = G.tri_tol_abs;
+    A.tri_tol_rel = G.tri_tol_rel;
+    A.d_tiny32 = G.d_tiny32;
+    A.off_sq = (int)hist_bytes;
+    A.off_lut = (int)(hist_bytes + off_lut);
+    A.off_ctab = (int)(hist_bytes + off_ctab);
+    A.off_lut32 = (int)(hist_bytes + off_lut32);
+    A.off_band = (int)(hist_bytes + off_band);
+    A.off_wave = (int)lds_tab;
+    A.wave_bytes = (int)wave_bytes;
+    A.lds_bytes = (int)lds_bytes;
+
+    NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    if (A.nchunks > 0) {
+        if (mixed)
+            hipLaunchKernelGGL(k_triplets<true>, dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+        else
+            hipLaunchKernelGGL(k_triplets<false>, dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+        NC_HIP(ctx, hipGetLastError());
+    }
+    NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    std::vector<unsigned long long> res((size_t)nterms * G.nbc);
+    NC_HIP(ctx, hipMemcpyAsync(res.data(), ctx->out.p, sizeof(unsigned long long) * res.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms_b = 0.f, ms_c = 0.f;
+    NC_HIP(ctx, hipEventElapsedTime(&ms_b, ctx->ev[0], ctx->ev[1]));
+    NC_HIP(ctx, hipEventElapsedTime(&ms_c, ctx->ev[2], ctx->ev[3]));
+    ctx->timing.ms_build = ms_b;
+    ctx->timing.ms_count = ms_c;
+    ctx->timing.ms_total = ms_b + ms_c;
+    ctx->timing.n_kernel = A.nchunks > 0 ? 1 : 0;
+    if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
+    if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
+
+    // --- compact sorted bins -> dense nb^3 (device.cu:198 layout)
+    for (int i = 0; i < nterms; ++i) {
+        double *h = hist + (size_t)i * nh;
+        uint64_t *cn = counts ? counts + (size_t)i * nh : nullptr;
+        std::fill(h, h + nh, 0.0);
+        if (cn) std::fill(cn, cn + nh, (uint64_t)0);
+        const unsigned long long *r = res.data() + (size_t)i * G.nbc;
+        int c = 0;
+        for (int a = 0; a < G.nb; ++a)
+            for (int b = a; b < G.nb; ++b)
+                for (int k = b; k < G.nb; ++k, ++c) {
+                    const size_t di = (size_t)k + (size_t)b * G.nb + (size_t)a * G.nb * G.nb;
+                    if (weighted[i]) {
+                        double v;
+                        std::memcpy(&v, &r[c], 8);
+                        h[di] = v;
+                    } else {
+                        h[di] = (double)r[c];
+                        if (cn) cn[di] = r[c];
+                    }
+                }
+    }
+    return NC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+
+extern "C" {
+
+int nc_create(nc_context **out, int device) {
+    if (!out) return NC_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return NC_ERR_HIP;
+    nc_context *ctx = new (std::nothrow) nc_context();
+    if (!ctx) return NC_ERR_NOMEM;
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return NC_ERR_HIP;
+    }
+    ctx->ncu = prop.multiProcessorCount;
+    ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 65536) - 256;
+    for (auto &e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            nc_destroy(ctx);
+            return NC_ERR_HIP;
+        }
+    *out = ctx;
+    return NC_OK;
+}
+
+int nc_destroy(nc_context *ctx) {
+    if (!ctx) return NC_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (DBuf *b : {&ctx->tables, &ctx->terms, &ctx->chunks, &ctx->out, &ctx->counter, &ctx->errw, &ctx->scratch,
+                    &ctx->bound})
+        dfree(*b);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return NC_OK;
+}
+
+const char *nc_last_error(const nc_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int64_t nc_hist_size(int nedges) { return nedges < 2 ? 0 : (int64_t)(nedges - 1) * (nedges - 1) * (nedges - 1); }
+
+int64_t nc_compact_size(int nedges) { return nedges < 2 ? 0 : choose_compact(nedges - 1); }
+
+int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog **cat) {
+    if (!ctx || !cat || n < 0 || (n > 0 && !xyzw)) return ctx ? fail(ctx, NC_ERR_ARG, "bad catalogue") : NC_ERR_ARG;
+    if (n > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
+    *cat = nullptr;
+    NC_HIP(ctx, hipSetDevice(ctx->device));
+    nc_catalog *c = new (std::nothrow) nc_catalog();
+    if (!c) return fail(ctx, NC_ERR_NOMEM, "host allocation");
+    c->ctx = ctx;
+    c->n = n;
+    c->owns = true;
+    for (int64_t i = 0; i < n; ++i)
+        if (xyzw[4 * i + 3] != 1.0) {
+            c->unit_w = false;
+            break;
+        }
+    hipError_t e = dgrow(c->raw, sizeof(double) * 4 * (size_t)n);
+    if (e == hipSuccess && n > 0)
+        e = hipMemcpyAsync(c->raw.p, xyzw, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        dfree(c->raw);
+        delete c;
+        return fail(ctx, e == hipErrorOutOfMemory ? NC_ERR_NOMEM : NC_ERR_HIP, hipGetErrorString(e));
+    }
+    c->d_raw = (const double *)c->raw.p;
+    *cat = c;
+    return NC_OK;
+}
+
+int nc_catalog_wrap_device(nc_context *ctx, const double *d_xyzw, int64_t n, nc_catalog **cat) {
+    if (!ctx || !cat || n < 0 || (n > 0 && !d_xyzw)) return ctx ? fail(ctx, NC_ERR_ARG, "bad catalogue") : NC_ERR_ARG;
+    if (n > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
+    *cat = nullptr;
+    NC_HIP(ctx, hipSetDevice(ctx->device));
+    nc_catalog *c = new (std::nothrow) nc_catalog();
+    if (!c) return fail(ctx, NC_ERR_NOMEM, "host allocation");
+    c->ctx = ctx;
+    c->n = n;
+    c->owns = false;
+    c->d_raw = d_xyzw;
+    if (n > 0) {
+        hipError_t e = dgrow(ctx->errw, 16);
+        unsigned flag = 0;
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->errw.p, 0u, 4LL);
+            hipLaunchKernelGGL(k_unit_weight, dim3(1024), dim3(256), 0, ctx->stream, (const double4 *)d_xyzw,
+                               (long long)n, (unsigned *)ctx->errw.p);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&flag, ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(ctx, NC_ERR_HIP, hipGetErrorString(e));
+        }
+        c->unit_w = flag == 0;
+    }
+    *cat = c;
+    return NC_OK;
+}
+
+int nc_catalog_destroy(nc_catalog *c) {
+    if (!c) return NC_ERR_ARG;
+    if (c->ctx) {
+        (void)hipSetDevice(c->ctx->device);
+        (void)hipStreamSynchronize(c->ctx->stream);
+    }
+    for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->cnt, &c->start, &c->x, &c->y, &c->z, &c->w}) dfree(*b);
+    delete c;
+    return NC_OK;
+}
+
+int64_t nc_catalog_size(const nc_catalog *c) { return c ? c->n : -1; }
+
+int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalog *c2, nc_catalog *c3, int path,
+             double *hist, uint64_t *counts) {
+    std::vector<Term> t{{c1, c2, c3}};
+    return run_count(ctx, params, t, path, hist, counts);
+}
+
+int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, int64_t n1, const double *p2, int64_t n2,
+                  const double *p3, int64_t n3, int path, double *hist, uint64_t *counts) {
+    if (!ctx) return NC_ERR_ARG;
+    nc_catalog *c[3] = {nullptr, nullptr, nullptr};
+    // identical host buffers share one device catalogue (and with it the
+    // (p2,p3) exchange symmetry)
+    int rc = nc_catalog_create(ctx, p1, n1, &c[0]);
+    if (!rc) {
+        if (p2 == p1 && n2 == n1)
+            c[1] = c[0];
+        else
+            rc = nc_catalog_create(ctx, p2, n2, &c[1]);
+    }
+    if (!rc) {
+        if (p3 == p2 && n3 == n2)
+            c[2] = c[1];
+        else if (p3 == p1 && n3 == n1)
+            c[2] = c[0];
+        else
+            rc = nc_catalog_create(ctx, p3, n3, &c[2]);
+    }
+    if (!rc) rc = nc_count(ctx, params, c[0], c[1], c[2], path, hist, counts);
+    if (c[2] && c[2] != c[1] && c[2] != c[0]) nc_catalog_destroy(c[2]);
+    if (c[1] && c[1] != c[0]) nc_catalog_destroy(c[1]);
+    if (c[0]) nc_catalog_destroy(c[0]);
+    return rc;
+}
+
+int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1,
+                     nc_catalog *const *c2, nc_catalog *const *c3, int path, double *hist, uint64_t *counts) {
+    if (!ctx) return NC_ERR_ARG;
+    if (nterms < 1 || !c1 || !c2 || !c3) return fail(ctx, NC_ERR_ARG, "bad term list");
+    std::vector<Term> t(nterms);
+    for (int i = 0; i < nterms; ++i) t[i] = {c1[i], c2[i], c3[i]};
+    return run_count(ctx, params, t, path, hist, counts);
+}
+
+int nc_last_timing(const nc_context *ctx, nc_timing *out) {
+    if (!ctx || !out) return NC_ERR_ARG;
+    *out = ctx->timing;
+    return NC_OK;
+}
+
+int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu) {
+    if (!ctx) return NC_ERR_ARG;
+    if (waves_per_group > 0) {
+        if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4 && waves_per_group != 8)
+            return fail(ctx, NC_ERR_ARG, "waves_per_group must be 1, 2, 4 or 8");
+        ctx->waves_per_group = waves_per_group;
+    }
+    if (chunk > 0) ctx->chunk = chunk;
+    if (wgs_per_cu >= 0) ctx->wgs_per_cu = wgs_per_cu;
+    return NC_OK;
+}
+
+int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu) {
+    if (!ctx) return NC_ERR_ARG;
+    if (waves_per_group) *waves_per_group = ctx->waves_per_group;
+    if (chunk) *chunk = ctx->chunk;
+    if (wgs_per_cu) *wgs_per_cu = ctx->wgs_per_cu;
+    return NC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Device-side definitions shared by the HIP kernels of libncorr_hip.so.
+//
+// Data layout in HBM (per catalogue, per count geometry):
+//   raw    : n x double4 {x,y,z,w}, as handed over by the caller
+//   key    : n x u32 cell key  z + nz*(y + ny*x)      (cudaNcorr.cu:50-58)
+//   rank   : n x u32 slot of the point inside its cell
+//   start  : ncells+1 x u32 exclusive scan of the per-cell counts
+//   x,y,z,w: n x double each, points sorted by cell key (SoA, coalesced)
+// Per persistent wave, a scratch slot holds the compacted neighbour lists of
+// the primary it is working on (Elem + weight per entry).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nck {
+
+constexpr int kWave = 64;
+constexpr unsigned kErrDomain = 1u;
+constexpr unsigned kErrOverflow = 2u;
+constexpr int kMaxWavesPerGroup = 8;
+
+// One compacted neighbour of a primary: the image position used by the
+// reference (point + periodic shift, FP64), the exact bin of its squared
+// separation from the primary and sqrtf(float(r^2)) of the FP32 triangle test.
+struct __attribute__((aligned(16))) Elem {
+    double x, y, z;
+    int k;
+    float f;
+};
+
+struct TermDesc {
+    const double *x1, *y1, *z1, *w1;
+    const double *x2, *y2, *z2, *w2;
+    const uint32_t *st2;
+    const double *x3, *y3, *z3, *w3;
+    const uint32_t *st3;
+    int n1;
+    int sym;       // c2 == c3: count unordered (p2,p3) pairs once, doubled
+    int weighted;  // accumulate (w1*w2)*w3 in FP64, else integer counts
+    int pad;
+};
+
+struct Chunk {
+    int term;
+    int begin;
+    int end;
+    int pad;
+};
+
+struct KArgs {
+    const TermDesc *terms;
+    const Chunk *chunks;
+    int nchunks;
+    unsigned *counter;
+    unsigned *err;
+    unsigned long long *out;  // nterms x nbc, u64 counts or f64 sums (bit pattern)
+    char *scratch;
+    long long slot_bytes;
+    int cap;
+    int chunk_elems;  // a-entries staged in LDS per wave
+    double box[3];
+    double cs[3];
+    int nc[3];
+    int survey;
+    double r2min, r2max;
+    int nb, nbc;
+    double lut_scale;
+    int lut_m;
+    // mixed path
+    float lut_scale32;
+    int lut32_m;
+    float tri_tol_abs;  // absolute slack margin of the FP32 triangle test
+    float tri_tol_rel;  // relative (to the largest side) margin
+    float d_tiny32;     // FP32 r23^2 below which the pair is always rechecked
+    const double *sq;             // nb+1 squared edges
+    const uint16_t *lut;          // lut_m+1 lower-bound bins (FP64 path)
+    const int *ctab;              // nb*nb compact base index
+    const uint16_t *lut32;        // lut32_m+2 entries (mixed path): code | has-band flag << 8
+    const float *band;            // 2*(nb+1) guard band [lo,hi] per squared edge (FP32)
+    // dynamic LDS layout (bytes)
+    int off_sq, off_lut, off_ctab, off_lut32, off_band, off_wave, wave_bytes, lds_bytes;
+};
+
+}  // namespace nck

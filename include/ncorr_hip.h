@@ -1,0 +1,135 @@
+/*
+ * ncorr_hip.h -- C-ABI of the MI355X three-point triangle-count library
+ * (libncorr_hip.so).  Plain C types only; no HIP or C++ types cross it.
+ *
+ * The reference has no C-ABI: its boundary is the C++ class
+ * `ncorr` (include/cudaNcorr.hpp:10-68), whose calculateD1D2D3
+ * (source/cudaNcorr.cu:220-430) uploads three double4 catalogues, runs one of
+ * countTrianglesThreeDBox / countTrianglesThreeDSurvey (source/device.cu:259-315,
+ * 381-431) and copies back a dense nb^3 double histogram.  This library is the
+ * layer under the drop-in `ncorr` class of this project (include/cudaNcorr.hpp):
+ * each entry point below names the reference code it replaces.
+ *
+ * Conventions
+ *   - Every function returns NC_OK (0) or an NC_ERR_* code; nc_last_error()
+ *     gives a message for the last failure on a context.
+ *   - Host buffers are owned by the caller and only read (inputs) or written
+ *     (outputs) during the call.  Device buffers are owned by the context or
+ *     by a catalogue handle and are reused across calls.
+ *   - One context per host thread; calls on one context are serialised on its
+ *     own HIP stream.  There is no process-global device state, so several
+ *     contexts may run side by side (unlike the reference's __constant__
+ *     parameter block, device.cu:9-18).
+ *   - Points are double4 {x, y, z, w} (w = weight), 32 bytes each, as the
+ *     reference's std::vector<double4>.
+ *   - Histograms are dense nb*nb*nb arrays indexed
+ *     [i_smallest*nb*nb + i_middle*nb + i_largest] (device.cu:198), filled only
+ *     on i_smallest <= i_middle <= i_largest.
+ */
+#ifndef NCORR_HIP_H
+#define NCORR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nc_status {
+    NC_OK = 0,
+    NC_ERR_ARG = 1,     /* bad argument (null pointer, bad size, bad box) */
+    NC_ERR_BINS = 2,    /* edges not >= 0 and strictly increasing, or fewer than 2 */
+    NC_ERR_DOMAIN = 3,  /* a point's cell key falls outside the cell grid */
+    NC_ERR_NOMEM = 4,   /* device or host allocation failed */
+    NC_ERR_HIP = 5,     /* a HIP runtime call failed */
+    NC_ERR_LIMIT = 6    /* a size exceeds what the kernels support */
+};
+
+/* Arithmetic of the triplet kernel. */
+enum nc_path {
+    NC_PATH_PLAIN = 0, /* every separation, bin and triangle test in FP64 as the reference */
+    NC_PATH_MIXED = 1  /* FP32 separations on re-centred coordinates, FP64 recheck in a guard band */
+};
+
+typedef struct nc_context nc_context;
+typedef struct nc_catalog nc_catalog;
+
+/* Geometry and binning of one count: the arguments of the reference's threeD
+ * constructor ncorr(box, 0, bins, isSurvey, ori) (cudaNcorr.hpp:14,
+ * cudaNcorr.cu:118-157).  `ori` is not needed: the threeD kernels never read it. */
+typedef struct {
+    double box[3];        /* periodic box (box mode) or padded survey bounding box */
+    int survey;           /* 0: periodic box, 1: survey (open boundaries) */
+    int nedges;           /* number of bin edges = nb + 1 */
+    const double *edges;  /* host pointer, bin edges in r (not squared) */
+} nc_params;
+
+/* Device time of the last count call on a context, from HIP events on the
+ * context's stream. */
+typedef struct {
+    float ms_build;      /* cell-list builds (key, scan, scatter) */
+    float ms_count;      /* triplet kernel(s) */
+    float ms_total;      /* first build kernel to end of the count */
+    int64_t n_kernel;    /* triplet kernel launches in that span */
+} nc_timing;
+
+/* Create a context on HIP device `device` (replaces the implicit CUDA context
+ * and the __constant__ parameter block of the reference, device.cu:9-18). */
+int nc_create(nc_context **ctx, int device);
+int nc_destroy(nc_context *ctx);
+const char *nc_last_error(const nc_context *ctx);
+
+/* Upload a host catalogue of n double4 points into a device-resident handle.
+ * Replaces the per-cell cudaMalloc/cudaMemcpy of cudaNcorr.cu:279-313: the
+ * points are copied once, as one buffer, and gridded on the device. */
+int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog **cat);
+/* Wrap a device buffer of n double4 points (not copied; must outlive the handle). */
+int nc_catalog_wrap_device(nc_context *ctx, const double *d_xyzw, int64_t n, nc_catalog **cat);
+int nc_catalog_destroy(nc_catalog *cat);
+int64_t nc_catalog_size(const nc_catalog *cat);
+
+/* Dense histogram length nb^3 and compact (sorted-bin) length C(nb+2,3) for a
+ * number of edges; 0 if nedges < 2. */
+int64_t nc_hist_size(int nedges);
+int64_t nc_compact_size(int nedges);
+
+/* The threeD count (replaces ncorr::calculateD1D2D3, cudaNcorr.cu:220-430, and
+ * the kernels device.cu:259-315 / 381-431).  Ordered triplets (p1 in c1, p2 in
+ * c2, p3 in c3); each binned triplet adds (w1*w2)*w3.
+ *   hist   [nb^3] double, required: the reference's triCountsAll_.
+ *   counts [nb^3] uint64, optional (NULL): exact triplet counts (unit weights);
+ *          filled only when every weight of c1, c2, c3 is 1.
+ * When every weight is 1 the kernel counts in integers and hist holds those
+ * counts exactly (below 2^53), as the reference's double sums of 1.0. */
+int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalog *c2, nc_catalog *c3,
+             int path, double *hist, uint64_t *counts);
+
+/* Host-buffer convenience form of nc_count: uploads three double4 arrays,
+ * counts, and frees them (the reference's calculateD1D2D3 signature). */
+int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, int64_t n1, const double *p2,
+                  int64_t n2, const double *p3, int64_t n3, int path, double *hist, uint64_t *counts);
+
+/* Batched count (no counterpart in the reference, which makes one
+ * calculateD1D2D3 call per term and per catalogue, main.cu:78-95): nterms
+ * terms (c1[t], c2[t], c3[t]) with shared params run in one persistent launch
+ * that pulls (term, primary-tile) items from a device work queue.  The launch
+ * sequence is captured in a HIP graph and replayed while the same terms are
+ * passed again.  hist: nterms*nb^3 doubles; counts: nterms*nb^3 uint64 or
+ * NULL (a term's counts are filled when all its weights are 1). */
+int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1,
+                     nc_catalog *const *c2, nc_catalog *const *c3, int path, double *hist, uint64_t *counts);
+
+/* Device timing of the last nc_count / nc_count_batched on ctx. */
+int nc_last_timing(const nc_context *ctx, nc_timing *out);
+
+/* Tuning knobs of the triplet kernel (see nc_autotune).  Values <= 0 keep the
+ * current setting.  waves_per_group in {1,2,4,8}; chunk = primaries per work
+ * item; wgs_per_cu = persistent workgroups per compute unit. */
+int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu);
+int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NCORR_HIP_H */

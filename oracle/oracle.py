@@ -1,0 +1,207 @@
+"""CPU oracle for the three-point triangle count -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker / the timed CPU
+baseline.  The product library never calls it.
+
+Two independent restatements of the reference's threeD count
+(``source/device.cu:154-201``, ``259-315``, ``381-431``;
+``source/cudaNcorr.cu:37-86``, ``118-157``):
+
+* ``count``: ctypes wrapper of ``oracle.c`` -- the cell-list loop structure
+  of the reference (27 neighbour entries for p2, 27 for p3), OpenMP over
+  primaries.  This is the oracle the HIP library is compared with.
+* ``brute_force``: an O(n1*n2*n3) enumeration with no cell list at all, used to
+  pin ``count`` on small inputs.  Survey mode enumerates all triples directly;
+  box mode uses the nearest periodic image of p2 and p3 relative to p1, which
+  is what the reference's one-wrap 27-cell search finds when the box is a
+  whole number (>= 3) of cells.  With ``exact=True`` every squared separation
+  is evaluated as fma(dz,dz,fma(dy,dy,dx*dx)) with exact rational arithmetic
+  and a single rounding per operation, so the canonical evaluation order is
+  checked independently of any FMA hardware.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from fractions import Fraction
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ERRORS = {1: "invalid argument", 2: "invalid bin edges", 3: "point outside the cell grid", 4: "out of memory"}
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path} (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.nc_oracle_count.argtypes = [dp, ctypes.c_long, dp, ctypes.c_long, dp, ctypes.c_long, dp, ctypes.c_int,
+                                        dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_uint64), dp]
+        lib.nc_oracle_count.restype = ctypes.c_int
+        lib.nc_oracle_sensitivity.argtypes = [dp, ctypes.c_long, dp, ctypes.c_long, dp, ctypes.c_int, dp, ctypes.c_int]
+        lib.nc_oracle_sensitivity.restype = ctypes.c_long
+        lib.nc_oracle_compact.argtypes = [dp, ctypes.c_int, dp, dp, ctypes.c_int]
+        lib.nc_oracle_compact.restype = ctypes.c_int
+        _LIB = lib
+    return _LIB
+
+
+def _pts(p):
+    a = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(-1, 4))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0]
+
+
+def count(p1, p2, p3, edges, box, survey=False, contract=0, nthreads=0):
+    """Ordered-triplet histogram of the reference semantics.
+
+    p1, p2, p3: (n,4) float64 arrays of x,y,z,w.  edges: nb+1 bin edges in r.
+    box: (3,) box size (survey mode: the padded bounding box of
+    ``survey::prep_cat_survey``).  Returns (counts uint64 (nb,nb,nb),
+    wsum float64 (nb,nb,nb)) indexed [i_smallest, i_middle, i_largest].
+    """
+    lib = _lib()
+    a1, q1, n1 = _pts(p1)
+    a2, q2, n2 = _pts(p2)
+    a3, q3, n3 = _pts(p3)
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    nb = e.size - 1
+    nh = max(nb, 0) ** 3
+    cnt = np.zeros(max(nh, 1), dtype=np.uint64)
+    ws = np.zeros(max(nh, 1), dtype=np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib.nc_oracle_count(q1, n1, q2, n2, q3, n3, e.ctypes.data_as(dp), e.size, b.ctypes.data_as(dp),
+                             int(bool(survey)), int(contract), int(nthreads),
+                             cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ws.ctypes.data_as(dp))
+    del a1, a2, a3
+    if rc != 0:
+        raise ValueError(f"oracle: {ERRORS.get(rc, rc)}")
+    return cnt[:nh].reshape(nb, nb, nb), ws[:nh].reshape(nb, nb, nb)
+
+
+def sensitivity(p1, p2, edges, box, survey=False):
+    """Number of (p1,p2) pairs whose bin could depend on the FMA contraction order."""
+    lib = _lib()
+    a1, q1, n1 = _pts(p1)
+    a2, q2, n2 = _pts(p2)
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    r = lib.nc_oracle_sensitivity(q1, n1, q2, n2, e.ctypes.data_as(dp), e.size, b.ctypes.data_as(dp), int(bool(survey)))
+    del a1, a2
+    if r < 0:
+        raise ValueError(f"oracle: {ERRORS.get(-r, r)}")
+    return int(r)
+
+
+def compact(edges, dense):
+    """Output compaction of cudaNcorr.cu:72-86 (sorted bins with mid_k <= mid_i + mid_j)."""
+    lib = _lib()
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    d = np.ascontiguousarray(np.asarray(dense, dtype=np.float64).reshape(-1))
+    dp = ctypes.POINTER(ctypes.c_double)
+    n = lib.nc_oracle_compact(e.ctypes.data_as(dp), e.size, None, None, 0)
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    lib.nc_oracle_compact(e.ctypes.data_as(dp), e.size, d.ctypes.data_as(dp), out.ctypes.data_as(dp), n)
+    return out[:n]
+
+
+# ---------------------------------------------------------------------------
+# brute force
+
+
+def _fma_exact(a: float, b: float, c: float) -> float:
+    """fma with one rounding: exact rational a*b+c rounded to nearest double."""
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def sqrsep_exact(dx: float, dy: float, dz: float) -> float:
+    """fma(dz,dz, fma(dy,dy, dx*dx)) with exact intermediate rounding."""
+    return _fma_exact(dz, dz, _fma_exact(dy, dy, dx * dx))
+
+
+def _sep2(a, b, exact):
+    d = a[..., None, :3] - b[None, :, :3]
+    if exact:
+        out = np.empty(d.shape[:-1])
+        for idx in np.ndindex(out.shape):
+            out[idx] = sqrsep_exact(*map(float, d[idx]))
+        return out
+    return (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+
+
+def _images(p, center, box):
+    """Per-axis nearest periodic image of every point of p relative to center."""
+    q = p.copy()
+    for ax in range(3):
+        L = box[ax]
+        best = q[:, ax].copy()
+        for s in (-L, L):
+            cand = p[:, ax] + s
+            better = np.abs(center[ax] - cand) < np.abs(center[ax] - best)
+            best = np.where(better, cand, best)
+        q[:, ax] = best
+    return q
+
+
+def brute_force(p1, p2, p3, edges, box=None, exact=False):
+    """Independent O(n^3) restatement (see module docstring).
+
+    box=None: survey mode (no periodic images).  Otherwise the periodic box,
+    which must be a whole number >= 3 of cells for the reference to agree.
+    Returns (counts uint64 (nb,nb,nb), wsum float64 (nb,nb,nb)).
+    """
+    p1 = np.asarray(p1, dtype=np.float64).reshape(-1, 4)
+    p2 = np.asarray(p2, dtype=np.float64).reshape(-1, 4)
+    p3 = np.asarray(p3, dtype=np.float64).reshape(-1, 4)
+    e = np.asarray(edges, dtype=np.float64)
+    sq = e * e
+    nb = e.size - 1
+    cnt = np.zeros((nb, nb, nb), dtype=np.uint64)
+    ws = np.zeros((nb, nb, nb), dtype=np.float64)
+    lo, hi = sq[0], sq[-1]
+    for a in p1:
+        q2 = p2 if box is None else _images(p2, a, box)
+        q3 = p3 if box is None else _images(p3, a, box)
+        r12 = _sep2(a[None], q2, exact)[0]
+        r13 = _sep2(a[None], q3, exact)[0]
+        j = np.nonzero((r12 > lo) & (r12 < hi))[0]
+        k = np.nonzero((r13 > lo) & (r13 < hi))[0]
+        if j.size == 0 or k.size == 0:
+            continue
+        r23 = _sep2(q2[j], q3[k], exact)
+        A = np.broadcast_to(r12[j][:, None], r23.shape)
+        B = np.broadcast_to(r13[k][None, :], r23.shape)
+        ok = (r23 > lo) & (r23 < hi)
+        s = np.sort(np.stack([A, B, r23], -1), axis=-1)
+        f = np.sqrt(s.astype(np.float32))
+        ok &= (f[..., 0] + f[..., 1]) > f[..., 2]
+        bins = np.searchsorted(sq, s, side="right") - 1
+        w = (a[3] * q2[j, 3])[:, None] * q3[k, 3][None, :]
+        for jj, kk in zip(*np.nonzero(ok)):
+            i1, i2, i3 = bins[jj, kk]
+            cnt[i1, i2, i3] += 1
+            ws[i1, i2, i3] += w[jj, kk]
+    return cnt, ws
+
+
+def output_bins(edges):
+    """Bin mids of the compact output (cudaNcorr.cu:145-154), as an (m,3) array."""
+    e = np.asarray(edges, dtype=np.float64)
+    mid = [(e[i + 1] + e[i]) / 2.0 for i in range(e.size - 1)]
+    nb = len(mid)
+    out = []
+    for i in range(nb):
+        for j in range(i, nb):
+            for k in range(j, nb):
+                if mid[k] <= mid[i] + mid[j]:
+                    out.append((mid[i], mid[j], mid[k]))
+    return np.array(out, dtype=np.float64).reshape(-1, 3)
