@@ -1,0 +1,67 @@
+"""The C++ drop-in class on the GPU: ncorr::calculateD1D2D3 followed by
+getTriCountsOut, called exactly as source/main.cu:31-45 and 72-95 do, checked
+against the oracle's compacted histogram (cudaNcorr.cu:72-86)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cudancorr_amd.gen import edges_linear, prep_cat_survey, survey_cap, uniform_box
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBDIR = os.path.join(ROOT, "cudancorr_amd", "lib")
+
+
+@pytest.fixture(scope="module")
+def dropin(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("dropin") / "dropin_main")
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_main.cpp")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), src, "-L" + LIBDIR,
+                    "-lncorr", "-lncorr_hip", "-Wl,-rpath," + LIBDIR, "-o", exe], check=True)
+    return exe
+
+
+def _run(exe, tmp, cats, names, survey, precision, box, edges):
+    paths = {}
+    for n, c in cats.items():
+        p = str(tmp / f"{n}.bin")
+        np.ascontiguousarray(c, dtype=np.float64).tofile(p)
+        paths[n] = p
+    args = [exe, "count", paths[names[0]], paths[names[1]], paths[names[2]], str(int(survey)), str(precision)]
+    args += [repr(float(b)) for b in box] + [repr(float(e)) for e in edges]
+    print("dropin", names, "survey", survey, "precision", precision, flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    return np.array([float(x) for x in out.stdout.split()])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dropin_box_ddd(dropin, tmp_path, precision):
+    p = uniform_box(8000, 150.0, seed=31)
+    edges = edges_linear(2.0, 20.0, 10)
+    box = [150.0] * 3
+    got = _run(dropin, tmp_path, {"D": p}, ("D", "D", "D"), False, precision, box, edges)
+    c, _ = oracle.count(p, p, p, edges, box)
+    want = oracle.compact(edges, c.astype(np.float64))
+    assert got.size == 150
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("term", ["DDD", "DDR", "DRR", "RRR"])
+def test_dropin_survey_terms(dropin, tmp_path, term):
+    d, r = survey_cap(3000, 4500, seed=12)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 40.0, 20)
+    got = _run(dropin, tmp_path, {"D": d, "R": r}, tuple(term), True, 1, box, edges)
+    cats = {"D": d, "R": r}
+    _, ws = oracle.count(cats[term[0]], cats[term[1]], cats[term[2]], edges, box, survey=True)
+    want = oracle.compact(edges, ws)
+    nz = want != 0
+    assert np.array_equal(got[~nz], want[~nz])
+    assert np.max(np.abs(got[nz] - want[nz]) / want[nz]) <= 1e-9
