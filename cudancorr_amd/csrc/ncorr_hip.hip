@@ -78,9 +78,9 @@ __device__ __forceinline__ void wave_sync() {
 // Exact bin of a squared separation d with r2min < d < r2max: lut[j] is a
 // lower bound of the bin of every d with (int)(d*scale) == j, and the loop
 // steps up over the squared edges (half-open bins, device.cu:181-191).
-__device__ __forceinline__ int bin_of(double d, double scale, const uint16_t *lut, const double *sq) {
+__device__ __forceinline__ int bin_of(double d, double scale, const uint16_t *lut, const double *sq, int nb) {
     int k = lut[(int)(d * scale)];
-    while (d >= sq[k + 1]) ++k;
+    while (k < nb - 1 && d >= sq[k + 1]) ++k;
     return k;
 }
 
@@ -228,24 +228,38 @@ __global__ void k_unit_weight(const double4 *__restrict__ raw, long long n, unsi
 // S2 x S3 that tests r23 only.
 //
 // Plain list entry (FP64, 40 B):   Elem {x,y,z image position, k, f} + weight
-// Mixed list entry (52 B):         float4 {x,y,z relative to p1, f} + k
-//                                  + double4 {x,y,z image position, weight}
+// Mixed list entry (56 B):         float4 {x,y,z relative to p1, s = |r|^2}
+//                                  + int2 {k, f} + double4 {image position, weight}
 // k = exact bin of |p1-q|^2, f = sqrtf(float(|p1-q|^2)) as in device.cu:173.
 
-constexpr int kQueueCap = 256;  // deferred exact rechecks per wave (mixed path)
+constexpr int kQueueCap = 128;  // deferred exact rechecks per wave (mixed path)
 
 struct LdsTabs {
     unsigned long long *hist;
     const double *sq;
     const uint16_t *lut;
     const int *ctab;
-    const uint16_t *lut32;
-    const float *band;
+    const uint2 *lut32;
+    const uint16_t *t3;
 };
 
 // Bytes of one list entry in a wave's scratch slot.
 __host__ __device__ constexpr size_t list_entry_bytes(bool mixed) {
-    return mixed ? sizeof(float4) + sizeof(int) + sizeof(double4) : sizeof(Elem) + sizeof(double);
+    return mixed ? sizeof(float4) + sizeof(int2) + sizeof(double4) : sizeof(Elem) + sizeof(double);
+}
+
+// Arrays of a mixed-path list inside a wave's scratch slot.
+struct MList {
+    float4 *R;   // x, y, z relative to p1 (FP32), s = |r|^2 (FP32)
+    int2 *KF;    // exact bin of |p1-q|^2, sqrtf(float(|p1-q|^2)) bits
+    double4 *D;  // image position (FP64) and weight
+};
+__device__ __forceinline__ MList mlist(const char *base, int cap) {
+    MList m;
+    m.R = reinterpret_cast<float4 *>(const_cast<char *>(base));
+    m.KF = reinterpret_cast<int2 *>(const_cast<char *>(base) + (size_t)cap * sizeof(float4));
+    m.D = reinterpret_cast<double4 *>(const_cast<char *>(base) + (size_t)cap * (sizeof(float4) + sizeof(int2)));
+    return m;
 }
 
 // Compact the neighbours of primary (px,py,pz) in catalogue (X,Y,Z,Wt,st)
@@ -278,15 +292,14 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
             const unsigned long long m = __ballot(pass);
             const int pos = n + lane_rank(m);
             if (pass && pos < A.cap) {
-                const int k = bin_of(d, A.lut_scale, lut, sq);
+                const int k = bin_of(d, A.lut_scale, lut, sq, A.nb);
                 const float f = __builtin_sqrtf((float)d);
                 if constexpr (MIXED) {
-                    float4 *R = reinterpret_cast<float4 *>(base);
-                    int *K = reinterpret_cast<int *>(base + (size_t)A.cap * sizeof(float4));
-                    double4 *D = reinterpret_cast<double4 *>(base + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
-                    R[pos] = make_float4((float)(qx - px), (float)(qy - py), (float)(qz - pz), f);
-                    K[pos] = k;
-                    D[pos] = make_double4(qx, qy, qz, qw);
+                    const MList m = mlist(base, A.cap);
+                    const float rx = (float)(qx - px), ry = (float)(qy - py), rz = (float)(qz - pz);
+                    m.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
+                    m.KF[pos] = make_int2(k, __float_as_int(f));
+                    m.D[pos] = make_double4(qx, qy, qz, qw);
                 } else {
                     Elem e;
                     e.x = qx;
@@ -304,15 +317,35 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
     return n;
 }
 
-// Add one accepted (p2=a, p3=b) pair with bins (ka, kb, k) to the LDS
-// histogram over sorted bins.  SYM: the pair also stands for (p2=b, p3=a).
-template <bool SYM, bool WEIGHTED>
-__device__ __forceinline__ void accumulate(const KArgs &A, const LdsTabs &t, int ka, int kb, int k, double w1a,
-                                           double wa, double w1b, double wb) {
+// Sorted histogram slot of bins (ka, kb, k), k in [-1, nb] (slots of k
+// outside [0, nb) are never added).  FAST: one LDS table lookup; otherwise the
+// compact C(nb+2,3) index through ctab.
+template <bool FAST>
+__device__ __forceinline__ int sorted_slot(const KArgs &A, const LdsTabs &t, int ka, int kb, int k) {
+    if constexpr (FAST) return t.t3[__mul24(__mul24(ka, A.nb) + kb, A.nb + 2) + k + 1];
     const int klo = min(min(ka, kb), k);
     const int khi = max(max(ka, kb), k);
     const int kmid = ka + kb + k - klo - khi;
-    const int idx = t.ctab[klo * A.nb + kmid] + (khi - kmid);
+    return t.ctab[klo * A.nb + kmid] + (khi - kmid);
+}
+
+// Triangle test inputs are non-negative floats: their bit patterns order as
+// unsigned integers, so min/med/max run on the integer ALU (v_min3_u32,
+// v_med3_u32, v_max3_u32) without NaN canonicalisation.
+struct Sides {
+    float lo, mid, hi;
+};
+__device__ __forceinline__ Sides sort3(float a, float b, float c) {
+    const unsigned ua = __float_as_uint(a), ub = __float_as_uint(b), uc = __float_as_uint(c);
+    const unsigned lo = min(min(ua, ub), uc);
+    const unsigned hi = max(max(ua, ub), uc);
+    const unsigned mid = max(min(ua, ub), min(max(ua, ub), uc));
+    return {__uint_as_float(lo), __uint_as_float(mid), __uint_as_float(hi)};
+}
+
+// Histogram update of one accepted pair at a precomputed slot.
+template <bool SYM, bool WEIGHTED>
+__device__ __forceinline__ void hist_add(const LdsTabs &t, int idx, double w1a, double wa, double w1b, double wb) {
     if constexpr (WEIGHTED) {
         double v = w1a * wb;  // (w1*w2)*w3, device.cu:305
         if (SYM) v += w1b * wa;
@@ -322,41 +355,75 @@ __device__ __forceinline__ void accumulate(const KArgs &A, const LdsTabs &t, int
     }
 }
 
+// Bin of a squared separation (exact for r2min < d < r2max; any value in
+// [0, nb] otherwise).  FAST: the lookup table is fine enough that one step
+// over the squared edges always suffices (checked on the host).
+template <bool FAST>
+__device__ __forceinline__ int bin_any(const KArgs &A, const LdsTabs &t, double d) {
+    const int j = (int)fmin(d * A.lut_scale, (double)A.lut_m);
+    int k = t.lut[j];
+    if constexpr (FAST) {
+        k += d >= t.sq[k + 1] ? 1 : 0;
+    } else {
+        while (k < A.nb - 1 && d >= t.sq[k + 1]) ++k;
+    }
+    return k;
+}
+
 // Exact FP64 evaluation of one pair (device.cu:299-307 restated): r23 from
-// the image positions, strict range filter, exact bin, FP32 triangle test.
-template <bool SYM, bool WEIGHTED>
+// the image positions, strict range filter, exact bin, FP32 triangle test
+// with the correctly rounded sqrtf.
+template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, double ax, double ay, double az, int ka,
                                            float fa, double w1a, double wa, double bx, double by, double bz, int kb,
                                            float fb, double w1b, double wb, bool ok) {
     const double d = sqrsep(ax - bx, ay - by, az - bz);
     if (ok && d > A.r2min && d < A.r2max) {
-        const int k = bin_of(d, A.lut_scale, t.lut, t.sq);
+        const int k = bin_any<false>(A, t, d);
         const float f = __builtin_sqrtf((float)d);
         const float lo = fminf(fminf(fa, fb), f);
         const float hi = fmaxf(fmaxf(fa, fb), f);
         const float mid = __builtin_amdgcn_fmed3f(fa, fb, f);
         if (lo + mid > hi)  // the reference rejects sqrtf(r1)+sqrtf(r2) <= sqrtf(r3), device.cu:173
-            accumulate<SYM, WEIGHTED>(A, t, ka, kb, k, w1a, wa, w1b, wb);
+            hist_add<SYM, WEIGHTED>(t, sorted_slot<FAST>(A, t, ka, kb, k), w1a, wa, w1b, wb);
     }
 }
 
-// Plain path: all pairs of S2 (broadcast from LDS, a) x S3 (one per lane, b).
-template <bool SYM, bool WEIGHTED>
+// Plain path: all pairs of S2 (broadcast from LDS, a) x S3 (one per lane, b),
+// U entries of S2 per iteration in phases (LDS reads of all U pairs, then
+// arithmetic, then histogram updates) so their latencies overlap.  The
+// triangle test uses the hardware sqrt (<= 1 ulp) and falls back to the
+// correctly rounded one when the slack is within 8 ulp of the largest side.
+template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
                                 int n3, char *wlds, int lane) {
+    constexpr int U = 2;
     const int C = A.chunk_elems;
+    const int Cp = C + 4;
     const Elem *L2 = reinterpret_cast<const Elem *>(l2);
     const double *W2 = reinterpret_cast<const double *>(l2 + (size_t)A.cap * sizeof(Elem));
     const Elem *L3 = reinterpret_cast<const Elem *>(l3);
     const double *W3 = reinterpret_cast<const double *>(l3 + (size_t)A.cap * sizeof(Elem));
     Elem *la = reinterpret_cast<Elem *>(wlds);
-    double *lw = reinterpret_cast<double *>(wlds + (size_t)C * sizeof(Elem));
+    double *lw = reinterpret_cast<double *>(wlds + (size_t)Cp * sizeof(Elem));
+    const double r2min = A.r2min, r2max = A.r2max, scale = A.lut_scale;
+    const float near_rel = 8.0f * 1.1920929e-7f;
     for (int a0 = 0; a0 < n2; a0 += C) {
         const int a1 = min(a0 + C, n2);
         wave_sync();
-        for (int i = lane; i < a1 - a0; i += kWave) {
-            la[i] = L2[a0 + i];
-            if (WEIGHTED) lw[i] = W2[a0 + i];
+        for (int i = lane; i < a1 - a0 + 4; i += kWave) {
+            Elem e;
+            double w = 0.0;
+            if (i < a1 - a0) {
+                e = L2[a0 + i];
+                if (WEIGHTED) w = W2[a0 + i];
+            } else {  // padding: far away, never in range
+                e.x = e.y = e.z = 1e150;
+                e.k = 0;
+                e.f = 0.0f;
+            }
+            la[i] = e;
+            if (WEIGHTED) lw[i] = w;
         }
         wave_sync();
         for (int b0 = SYM ? a0 : 0; b0 < n3; b0 += kWave) {
@@ -368,18 +435,57 @@ __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, con
                 eb = L3[b];
                 if (WEIGHTED) wb = W3[b];
             } else {
-                eb.x = eb.y = eb.z = 0.0;
+                eb.x = eb.y = eb.z = -1e150;
                 eb.k = 0;
                 eb.f = 0.0f;
             }
             const double w1b = w1 * wb;
             const int alim = SYM ? min(a1, min(n3, b0 + kWave) - 1) : a1;
-            for (int a = a0; a < alim; ++a) {
-                const Elem ea = la[a - a0];
-                double wa = 0.0;
-                if (WEIGHTED) wa = lw[a - a0];
-                exact_pair<SYM, WEIGHTED>(A, t, ea.x, ea.y, ea.z, ea.k, ea.f, w1 * wa, wa, eb.x, eb.y, eb.z, eb.k, eb.f,
-                                          w1b, wb, vb && (!SYM || a < b));
+            for (int a = a0; a < alim; a += U) {
+                Elem ea[U];
+                double wa[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ea[u] = la[a - a0 + u];
+                    wa[u] = WEIGHTED ? lw[a - a0 + u] : 0.0;
+                }
+                double d[U];
+                bool inr[U];
+                int k[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool ok = vb && (!SYM || a + u < b);
+                    d[u] = sqrsep(ea[u].x - eb.x, ea[u].y - eb.y, ea[u].z - eb.z);
+                    inr[u] = ok && d[u] > r2min && d[u] < r2max;
+                    const double dj = inr[u] ? d[u] : 0.0;
+                    k[u] = t.lut[(int)(dj * scale)];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if constexpr (FAST) {
+                        k[u] += d[u] >= t.sq[k[u] + 1] ? 1 : 0;
+                    } else {
+                        while (k[u] < A.nb - 1 && d[u] >= t.sq[k[u] + 1]) ++k[u];
+                    }
+                }
+                bool acc[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const float fd = (float)d[u];
+                    const Sides s = sort3(ea[u].f, eb.f, __builtin_amdgcn_sqrtf(fd));
+                    const float sum = s.lo + s.mid;
+                    const bool near = fabsf(sum - s.hi) <= near_rel * s.hi || fd < 1e-30f;
+                    acc[u] = inr[u] && !near && sum > s.hi;
+                    if (inr[u] && near) {  // rare: redo the test with the correctly rounded sqrtf
+                        const Sides e = sort3(ea[u].f, eb.f, __builtin_sqrtf(fd));
+                        acc[u] = e.lo + e.mid > e.hi;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (acc[u])
+                        hist_add<SYM, WEIGHTED>(t, sorted_slot<FAST>(A, t, ea[u].k, eb.k, k[u]), w1 * wa[u], wa[u],
+                                                w1b, wb);
             }
         }
     }
@@ -387,127 +493,140 @@ __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, con
 
 // Mixed path, deferred exact rechecks: every queued (a, b) pair is evaluated
 // with the FP64 image positions, one pair per lane.
-template <bool SYM, bool WEIGHTED>
+template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
                             const uint2 *q, int qn, int lane) {
-    const float4 *R2 = reinterpret_cast<const float4 *>(l2);
-    const int *K2 = reinterpret_cast<const int *>(l2 + (size_t)A.cap * sizeof(float4));
-    const double4 *D2 = reinterpret_cast<const double4 *>(l2 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
-    const float4 *R3 = reinterpret_cast<const float4 *>(l3);
-    const int *K3 = reinterpret_cast<const int *>(l3 + (size_t)A.cap * sizeof(float4));
-    const double4 *D3 = reinterpret_cast<const double4 *>(l3 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
+    const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
     wave_sync();
     for (int i = lane; i < qn; i += kWave) {
         const uint2 e = q[i];
-        const double4 pa = D2[e.x], pb = D3[e.y];
+        const double4 pa = m2.D[e.x], pb = m3.D[e.y];
+        const int2 ka = m2.KF[e.x], kb = m3.KF[e.y];
         const double wa = WEIGHTED ? pa.w : 0.0, wb = WEIGHTED ? pb.w : 0.0;
-        exact_pair<SYM, WEIGHTED>(A, t, pa.x, pa.y, pa.z, K2[e.x], R2[e.x].w, w1 * wa, wa, pb.x, pb.y, pb.z, K3[e.y],
-                                  R3[e.y].w, w1 * wb, wb, true);
+        exact_pair<SYM, WEIGHTED, FAST>(A, t, pa.x, pa.y, pa.z, ka.x, __int_as_float(ka.y), w1 * wa, wa, pb.x, pb.y,
+                                        pb.z, kb.x, __int_as_float(kb.y), w1 * wb, wb, true);
     }
     wave_sync();
 }
 
-// Mixed path: r23^2 in FP32 from coordinates re-centred on p1.  A lookup over
-// d*scale32 gives the bin directly when the cell is clear of every guard band;
-// a cell holding one band compares against it; d inside a band, a triangle
-// slack within its error bound, or a near-zero separation sends the pair to
-// the exact FP64 queue.
-template <bool SYM, bool WEIGHTED>
+// Mixed path.  With r = q - p1 re-centred in FP32 and s = |r|^2,
+//   r23^2 = s_a + s_b - 2 r_a.r_b
+// and the triangle (p1, a, b) is degenerate only when r_a and r_b are nearly
+// collinear, i.e. when s_a*s_b - (r_a.r_b)^2 is small.  Per pair: one dot
+// product, one lookup over r23^2*scale32 that returns the bin below the
+// (at most one) edge near that value and the edge itself, and the
+// collinearity bound.  A pair whose r23^2 lies within the guard band g32 of
+// that edge, or whose collinearity measure is within the error bound (see
+// make_mixed_tables), is queued for the exact FP64 evaluation.  U entries of
+// S2 per iteration in phases (LDS reads, arithmetic, updates, queue pushes).
+template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
                                 int n3, char *wlds, int lane) {
+    constexpr int U = 4;
     const int C = A.chunk_elems;
-    const float4 *R2 = reinterpret_cast<const float4 *>(l2);
-    const int *K2 = reinterpret_cast<const int *>(l2 + (size_t)A.cap * sizeof(float4));
-    const double4 *D2 = reinterpret_cast<const double4 *>(l2 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
-    const float4 *R3 = reinterpret_cast<const float4 *>(l3);
-    const int *K3 = reinterpret_cast<const int *>(l3 + (size_t)A.cap * sizeof(float4));
-    const double4 *D3 = reinterpret_cast<const double4 *>(l3 + (size_t)A.cap * (sizeof(float4) + sizeof(int)));
+    const int Cp = C + 4;
+    const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
     float4 *la = reinterpret_cast<float4 *>(wlds);
-    int *lk = reinterpret_cast<int *>(wlds + (size_t)C * sizeof(float4));
-    double *lw = reinterpret_cast<double *>(wlds + (size_t)C * (sizeof(float4) + sizeof(int)));
-    uint2 *q = reinterpret_cast<uint2 *>(wlds + (size_t)C * (sizeof(float4) + sizeof(int) + sizeof(double)));
+    int *lk = reinterpret_cast<int *>(wlds + (size_t)Cp * sizeof(float4));
+    double *lw = reinterpret_cast<double *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int)));
+    uint2 *q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
     const float s32 = A.lut_scale32;
     const float jmax = (float)(A.lut32_m + 1);
-    const float tol_abs = A.tri_tol_abs, tol_rel = A.tri_tol_rel, d_tiny = A.d_tiny32;
-    const int nb = A.nb;
+    const float g32 = A.g32, c1 = A.tri_c1, c2 = A.tri_c2;
+    const unsigned nb = (unsigned)A.nb;
     int qn = 0;
     for (int a0 = 0; a0 < n2; a0 += C) {
         const int a1 = min(a0 + C, n2);
         wave_sync();
-        for (int i = lane; i < a1 - a0; i += kWave) {
-            la[i] = R2[a0 + i];
-            lk[i] = K2[a0 + i];
-            if (WEIGHTED) lw[i] = D2[a0 + i].w;
+        for (int i = lane; i < a1 - a0 + 4; i += kWave) {
+            float4 r = make_float4(3e18f, 3e18f, 3e18f, 2.7e37f);  // padding: never in range
+            int k = 0;
+            double w = 0.0;
+            if (i < a1 - a0) {
+                r = m2.R[a0 + i];
+                k = m2.KF[a0 + i].x;
+                if (WEIGHTED) w = m2.D[a0 + i].w;
+            }
+            la[i] = r;
+            lk[i] = k;
+            if (WEIGHTED) lw[i] = w;
         }
         wave_sync();
         for (int b0 = SYM ? a0 : 0; b0 < n3; b0 += kWave) {
             const int b = b0 + lane;
             const bool vb = b < n3;
-            float4 rb = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 rb = make_float4(-3e18f, -3e18f, -3e18f, 2.7e37f);
             int kb = 0;
             double wb = 0.0;
             if (vb) {
-                rb = R3[b];
-                kb = K3[b];
-                if (WEIGHTED) wb = D3[b].w;
+                rb = m3.R[b];
+                kb = m3.KF[b].x;
+                if (WEIGHTED) wb = m3.D[b].w;
             }
             const double w1b = w1 * wb;
             const int alim = SYM ? min(a1, min(n3, b0 + kWave) - 1) : a1;
-            for (int a = a0; a < alim; ++a) {
-                const float4 ra = la[a - a0];
-                const bool ok = vb && (!SYM || a < b);
-                const float dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
-                const float d = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                bool amb = false, acc = false;
-                int k = 0;
-                if (ok) {
-                    const unsigned j = (unsigned)fminf(d * s32, jmax);
-                    const unsigned e = t.lut32[j];
-                    int c = (int)(e & 0xffu);
-                    bool amb_b = false;
-                    if (e & 0x100u) {
-                        const float blo = t.band[2 * c], bhi = t.band[2 * c + 1];
-                        amb_b = d >= blo && d <= bhi;
-                        c += d > bhi ? 1 : 0;
-                    }
-                    const bool valid = c >= 1 && c <= nb;
-                    if (valid || amb_b) {
-                        const float f = __builtin_amdgcn_sqrtf(d);
-                        const float lo = fminf(fminf(ra.w, rb.w), f);
-                        const float hi = fmaxf(fmaxf(ra.w, rb.w), f);
-                        const float mid = __builtin_amdgcn_fmed3f(ra.w, rb.w, f);
-                        const float slack = (lo + mid) - hi;
-                        const bool amb_t = fabsf(slack) <= __builtin_fmaf(tol_rel, hi, tol_abs) || d < d_tiny;
-                        amb = amb_b || amb_t;
-                        acc = !amb && valid && slack > 0.0f;
-                        k = c - 1;
-                    }
+            for (int a = a0; a < alim; a += U) {
+                float4 ra[U];
+                int ka[U];
+                double wa[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ra[u] = la[a - a0 + u];
+                    ka[u] = lk[a - a0 + u];
+                    wa[u] = WEIGHTED ? lw[a - a0 + u] : 0.0;
                 }
-                if (acc) {
-                    double wa = 0.0;
-                    if (WEIGHTED) wa = lw[a - a0];
-                    accumulate<SYM, WEIGHTED>(A, t, lk[a - a0], kb, k, w1 * wa, wa, w1b, wb);
+                float d[U], dot[U], ss[U];
+                uint2 e[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
+                    ss[u] = ra[u].w + rb.w;
+                    d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
+                    e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u] * s32, 0.0f, jmax)];
                 }
-                const unsigned long long m = __ballot(amb);
-                if (m) {
-                    const int pos = qn + lane_rank(m);
-                    if (amb) q[pos] = make_uint2((unsigned)a, (unsigned)b);
-                    qn += __popcll(m);
-                    if (qn > kQueueCap - kWave) {
-                        drain_queue<SYM, WEIGHTED>(A, t, w1, l2, l3, q, qn, lane);
-                        qn = 0;
+                bool acc[U], amb[U];
+                int slot[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool ok = vb && (!SYM || a + u < b);
+                    const float E = __uint_as_float(e[u].y);
+                    const int k = (int)e[u].x + (d[u] > E ? 1 : 0);
+                    const bool amb_b = fabsf(d[u] - E) <= g32;
+                    const bool valid = (unsigned)k < nb;
+                    const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
+                    const bool amb_t = delta <= __builtin_fmaf(c1, ss[u], c2);
+                    amb[u] = ok && (amb_b || (valid && amb_t));
+                    acc[u] = ok && valid && !amb_b && !amb_t;
+                    slot[u] = sorted_slot<FAST>(A, t, ka[u], kb, k);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (acc[u]) hist_add<SYM, WEIGHTED>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const unsigned long long m = __ballot(amb[u]);
+                    if (m) {
+                        const int pos = qn + lane_rank(m);
+                        if (amb[u]) q[pos] = make_uint2((unsigned)(a + u), (unsigned)b);
+                        qn += __popcll(m);
+                        if (qn > kQueueCap - kWave) {
+                            drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, q, qn, lane);
+                            qn = 0;
+                        }
                     }
                 }
             }
         }
     }
-    if (qn > 0) drain_queue<SYM, WEIGHTED>(A, t, w1, l2, l3, q, qn, lane);
+    if (qn > 0) drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, q, qn, lane);
 }
 
+// Merge the workgroup's LDS histogram into the term's global compact
+// histogram and clear it.
 __device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long long *hist) {
     __syncthreads();
     unsigned long long *out = A.out + (long long)term * A.nbc;
-    for (int i = threadIdx.x; i < A.nbc; i += blockDim.x) {
+    for (int i = threadIdx.x; i < A.nslots; i += blockDim.x) {
         const unsigned long long v = hist[i];
         if (v != 0ull) {
             if (weighted)
@@ -520,13 +639,22 @@ __device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long
     __syncthreads();
 }
 
+template <bool MIXED, bool FAST, bool SYM, bool WEIGHTED>
+__device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
+                                          const char *l3, int n3, char *wlds, int lane) {
+    if constexpr (MIXED)
+        pair_loop_mixed<SYM, WEIGHTED, FAST>(A, t, w1, l2, n2, l3, n3, wlds, lane);
+    else
+        pair_loop_plain<SYM, WEIGHTED, FAST>(A, t, w1, l2, n2, l3, n3, wlds, lane);
+}
+
 // Persistent triplet kernel.  Workgroups pull (term, primary range) chunks
 // from the queue A.chunks with one atomic per chunk; the waves of a workgroup
 // take the primaries of the chunk one at a time.  The LDS histogram of the
 // workgroup is merged into the term's global histogram whenever the workgroup
 // moves to another term and at exit.
-template <bool MIXED>
-__global__ __launch_bounds__(512) void k_triplets(KArgs A) {
+template <bool MIXED, bool FAST>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_triplets(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
@@ -535,22 +663,28 @@ __global__ __launch_bounds__(512) void k_triplets(KArgs A) {
     double *sq = reinterpret_cast<double *>(lds + A.off_sq);
     uint16_t *lut = reinterpret_cast<uint16_t *>(lds + A.off_lut);
     int *ctab = reinterpret_cast<int *>(lds + A.off_ctab);
-    uint16_t *lut32 = reinterpret_cast<uint16_t *>(lds + A.off_lut32);
-    float *band = reinterpret_cast<float *>(lds + A.off_band);
+    uint2 *lut32 = reinterpret_cast<uint2 *>(lds + A.off_lut32);
+    uint16_t *t3 = reinterpret_cast<uint16_t *>(lds + A.off_t3);
     t.sq = sq;
     t.lut = lut;
     t.ctab = ctab;
     t.lut32 = lut32;
-    t.band = band;
+    t.t3 = t3;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int i = tid; i < A.nbc; i += nthr) t.hist[i] = 0ull;
-    for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
-    for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
-    for (int i = tid; i < A.nb * A.nb; i += nthr) ctab[i] = A.ctab[i];
+    for (int i = tid; i < A.nslots; i += nthr) t.hist[i] = 0ull;
     if constexpr (MIXED) {
+        // FP64 tables are only read by the rare exact rechecks: global memory
+        t.sq = A.sq;
+        t.lut = A.lut;
         for (int i = tid; i <= A.lut32_m + 1; i += nthr) lut32[i] = A.lut32[i];
-        for (int i = tid; i < 2 * (A.nb + 1); i += nthr) band[i] = A.band[i];
+    } else {
+        for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
+        for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
     }
+    if constexpr (FAST)
+        for (int i = tid; i < A.nb * A.nb * (A.nb + 2); i += nthr) t3[i] = A.t3[i];
+    else
+        for (int i = tid; i < A.nb * A.nb; i += nthr) ctab[i] = A.ctab[i];
     const int lane = tid & (kWave - 1), wave = tid / kWave, nw = nthr / kWave;
     char *wlds = lds + A.off_wave + wave * A.wave_bytes;
     char *slot = A.scratch + (long long)(blockIdx.x * nw + wave) * A.slot_bytes;
@@ -589,10 +723,10 @@ __global__ __launch_bounds__(512) void k_triplets(KArgs A) {
                 continue;
             }
             const int n2 =
-                build_list<MIXED>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, sq, lut);
+                build_list<MIXED>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq, t.lut);
             const int n3 = T.sym ? n2
                                  : build_list<MIXED>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3,
-                                                     lane, sq, lut);
+                                                     lane, t.sq, t.lut);
             if (n2 > A.cap || n3 > A.cap) {
                 if (lane == 0) atomicOr(A.err, kErrOverflow);
                 continue;
@@ -600,30 +734,16 @@ __global__ __launch_bounds__(512) void k_triplets(KArgs A) {
             if (n2 == 0 || n3 == 0) continue;
             wave_sync();
             const char *lb = T.sym ? l2 : l3;
-            if constexpr (MIXED) {
-                if (T.sym) {
-                    if (T.weighted)
-                        pair_loop_mixed<true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                    else
-                        pair_loop_mixed<true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                } else {
-                    if (T.weighted)
-                        pair_loop_mixed<false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                    else
-                        pair_loop_mixed<false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                }
+            if (T.sym) {
+                if (T.weighted)
+                    pair_loop<MIXED, FAST, true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                else
+                    pair_loop<MIXED, FAST, true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
             } else {
-                if (T.sym) {
-                    if (T.weighted)
-                        pair_loop_plain<true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                    else
-                        pair_loop_plain<true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                } else {
-                    if (T.weighted)
-                        pair_loop_plain<false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                    else
-                        pair_loop_plain<false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
-                }
+                if (T.weighted)
+                    pair_loop<MIXED, FAST, false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                else
+                    pair_loop<MIXED, FAST, false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
             }
             wave_sync();
         }
@@ -632,8 +752,10 @@ __global__ __launch_bounds__(512) void k_triplets(KArgs A) {
     if (cur >= 0) flush_hist(A, cur, cur_weighted, t.hist);
 }
 
-template __global__ void k_triplets<false>(KArgs);
-template __global__ void k_triplets<true>(KArgs);
+template __global__ void k_triplets<false, false>(KArgs);
+template __global__ void k_triplets<false, true>(KArgs);
+template __global__ void k_triplets<true, false>(KArgs);
+template __global__ void k_triplets<true, true>(KArgs);
 
 // ---------------------------------------------------------------------------
 // host side
@@ -674,13 +796,14 @@ struct Geometry {
     double lut_scale;
     int lut_m;
     std::vector<int> ctab;
+    bool lut_single_step = false;  // one step over the edges always suffices after the lookup
     // mixed path: guard bands and the FP32 lookup (valid when mixed_ok)
     bool mixed_ok = false;
-    std::vector<uint16_t> lut32;
-    std::vector<float> band;
+    std::vector<uint2> lut32;
     float lut_scale32 = 0.f;
     int lut32_m = 0;
-    float tri_tol_abs = 0.f, tri_tol_rel = 0.f, d_tiny32 = 0.f;
+    float g32 = 0.f, tri_c1 = 0.f, tri_c2 = 0.f;
+    std::vector<uint16_t> t3;  // (ka*nb+kb)*(nb+2)+k+1 -> compact slot
 };
 
 }  // namespace
@@ -726,77 +849,91 @@ int fail(nc_context *ctx, int code, const std::string &msg) {
 
 int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
 
-// Guard bands and FP32 lookup of the mixed path.
+// Guard band, collinearity bound and FP32 lookup of the mixed path.
 //
-// Error bound.  With u = 2^-24, coordinates re-centred on p1 and rounded to
-// FP32 (|component| < rmax, error <= u*rmax each), differences rounded once
-// and the FMA chain rounded three times, the FP32 squared separation d32 of a
-// pair with true squared separation d obeys
-//     |d32 - d| <= u*(4*sqrt(3)*rmax*sqrt(d) + 5*d) + 12*u^2*rmax^2,
-// and the reference's FP64 value is within a few 1e-16*d of d.  Every squared
-// edge E gets the band E -/+ 2x that bound (rounded outward to FP32): a d32
-// outside every band is on the same side of every edge as the FP64 value.
-// The triangle-test inputs differ from the reference's in the r23 side only;
-// sqrt(d32) is within 32*u*rmax + 16*u*hi of sqrtf(float(d64)) unless d32 is
-// tiny, so slacks within that margin, and d32 < (64*u*rmax)^2, are rechecked.
+// Let u = 2^-24, R = rmax, A, B the FP64 image positions of two list entries,
+// P the primary.  Stored: r = fl32(A - P) per component (|error| <= u*R, as
+// |A - P| < R), s = fl32(|r|^2).  Per pair: dot = fl32(r_a . r_b) and
+// d32 = fl32(s_a + s_b - 2*dot).  Then, with S = |A-P|^2 and D = (A-P).(B-P)
+// exact,  |s - S| <= 6.5uR^2,  |dot - D| <= 6.5uR^2,  and
+//     |d32 - |A-B|^2| <= 32uR^2,
+// while the reference's FP64 |A-B|^2 is within ~1e-15 R^2 of the exact value.
+// Every squared edge E (rounded to FP32, error <= u*R^2) gets the guard band
+// g32 = 80uR^2 (2x the sum): d32 outside it is on the same side of E as the
+// reference's value.
+//
+// Triangle test.  The reference rejects fl(f1 + f2) <= f3 on sorted
+// f = sqrtf(float(r^2)); with exact lengths a = |A-P|, b = |B-P|, c = |A-B|,
+// it accepts whenever min(a + b - c, c - |a - b|) > 20uR.  Both slacks are at
+// least ab*sin^2(t)/(2(a+b)) = (S_a S_b - D^2)/(2ab(a+b)) and ab(a+b) <=
+// R(S_a + S_b), so  S_a S_b - D^2 > 40uR^2 (S_a + S_b)  is sufficient.  The
+// FP32 value s_a*s_b - dot^2 is within 28uR^4 of the exact one, hence a pair
+// is accepted when  s_a*s_b - dot^2 > 80uR^2 (s_a + s_b) + 64uR^4  (both
+// terms doubled) and otherwise re-evaluated exactly.
 //
 // Lookup: j = min(fl32(d32*s32), M+1) over [0, 1.25*rmax^2); any d32 mapped
-// to j lies in [(j-1)/s32, (j+2)/s32].  Entry j holds the number of bands
-// entirely below that interval (bin+1 code) and a flag when one band meets it;
-// tables where one interval meets two bands are rejected (M is doubled).
+// to j lies in [(j-1)/s32, (j+2)/s32].  Entry j = {bin below the one band that
+// meets that interval (or below all of it), the band's edge E as float bits,
+// +inf if no band meets it}.  Tables where an interval meets two bands are
+// rejected (M is doubled); bins narrower than the bands disable the path.
 static void make_mixed_tables(Geometry &G) {
     G.mixed_ok = false;
     const double u = std::ldexp(1.0, -24);
     const double R = G.edges.back();
     const int ne = G.nb + 1;
-    std::vector<float> lo(ne), hi(ne);
-    for (int k = 0; k < ne; ++k) {
-        const double E = G.sq[k];
-        const double g = 2.0 * (u * (6.93 * R * std::sqrt(E) + 5.0 * E) + 12.0 * u * u * R * R) + 1e-12 * E;
-        float l = (float)(E - g), h = (float)(E + g);
-        if ((double)l > E - g) l = std::nextafter(l, -INFINITY);
-        if ((double)h < E + g) h = std::nextafter(h, INFINITY);
-        lo[k] = l;
-        hi[k] = h;
-    }
+    const double g = 80.0 * u * R * R;
+    std::vector<float> E(ne);
+    for (int k = 0; k < ne; ++k) E[k] = (float)G.sq[k];
     for (int k = 0; k + 1 < ne; ++k)
-        if (!(hi[k] < lo[k + 1])) return;  // bins narrower than the guard bands
+        if (!((double)E[k] + g < (double)E[k + 1] - g)) return;  // bins narrower than the guard band
     const double r2max = G.sq[G.nb];
-    for (int M = 2048; M <= 16384; M *= 2) {
+    for (int M = 512; M <= 16384; M *= 2) {
         const float s32 = (float)((double)M / (1.25 * r2max));
         const double s = (double)s32;
-        std::vector<uint16_t> lut(M + 2);
+        std::vector<uint2> lut(M + 2);
         bool ok = true;
         for (int j = 0; j <= M + 1 && ok; ++j) {
             const double a = j == 0 ? -INFINITY : (double)(j - 1) / s * (1.0 - 1e-6);
             const double b = j == M + 1 ? INFINITY : (double)(j + 2) / s * (1.0 + 1e-6);
             int below = 0, meet = 0, which = -1;
             for (int k = 0; k < ne; ++k) {
-                if ((double)hi[k] < a)
+                if ((double)E[k] + g < a)
                     ++below;
-                else if ((double)lo[k] <= b) {
+                else if ((double)E[k] - g <= b) {
                     ++meet;
                     which = k;
                 }
             }
             if (meet > 1 || (meet == 1 && which != below)) ok = false;
-            lut[j] = (uint16_t)(below | (meet ? 0x100 : 0));
+            const float edge = meet ? E[which] : INFINITY;
+            uint32_t eb;
+            std::memcpy(&eb, &edge, 4);
+            lut[j] = make_uint2((uint32_t)(below - 1), eb);
         }
         if (!ok) continue;
         G.lut32 = std::move(lut);
         G.lut32_m = M;
         G.lut_scale32 = s32;
-        G.band.resize(2 * ne);
-        for (int k = 0; k < ne; ++k) {
-            G.band[2 * k] = lo[k];
-            G.band[2 * k + 1] = hi[k];
-        }
-        G.tri_tol_abs = (float)(32.0 * u * R);
-        G.tri_tol_rel = (float)(16.0 * u);
-        G.d_tiny32 = (float)((64.0 * u * R) * (64.0 * u * R));
-        G.mixed_ok = G.nb < 255;
+        G.g32 = (float)g;
+        G.tri_c1 = (float)(80.0 * u * R * R);
+        G.tri_c2 = (float)(64.0 * u * R * R * R * R);
+        G.mixed_ok = true;
         return;
     }
+}
+
+// Slot table of the FAST variants: (ka*nb + kb)*(nb+2) + k + 1 -> compact
+// index of the sorted bins, for ka, kb in [0, nb) and k in [-1, nb].
+static void make_slot_table(Geometry &G) {
+    const int nb = G.nb;
+    G.t3.assign((size_t)nb * nb * (nb + 2), 0);
+    for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < nb; ++k) {
+                const int lo = std::min(std::min(a, b), k), hi = std::max(std::max(a, b), k);
+                const int mid = a + b + k - lo - hi;
+                G.t3[((size_t)a * nb + b) * (nb + 2) + k + 1] = (uint16_t)(G.ctab[lo * nb + mid] + (hi - mid));
+            }
 }
 
 // Geometry of the reference's threeD constructor (cudaNcorr.cu:118-157) plus
@@ -824,16 +961,26 @@ int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
         G.ncells *= G.nc[a];
     }
     if (G.ncells > (1LL << 27)) return fail(ctx, NC_ERR_LIMIT, "cell grid larger than 2^27 cells");
-    // lower-bound bin lookup over d*scale (see bin_of)
-    G.lut_m = 2048;
+    // lower-bound bin lookup over d*scale (see bin_of / bin_any): the table
+    // is refined until one step over the squared edges always suffices.
     const double r2max = G.sq[G.nb];
-    G.lut_scale = (double)G.lut_m / r2max;
-    G.lut.assign(G.lut_m + 1, 0);
-    for (int j = 0; j <= G.lut_m; ++j) {
-        int k = 0;
-        for (int e = 1; e < G.nb; ++e)
-            if (G.sq[e] * G.lut_scale * (1.0 + 1e-12) <= (double)(j - 1)) k = e;
-        G.lut[j] = (uint16_t)k;
+    for (int M = 2048;; M *= 2) {
+        G.lut_m = M;
+        G.lut_scale = (double)M / r2max;
+        G.lut.assign(M + 1, 0);
+        int worst = 0;
+        for (int j = 0; j <= M; ++j) {
+            int k = 0;
+            for (int e = 1; e < G.nb; ++e)
+                if (G.sq[e] * G.lut_scale * (1.0 + 1e-12) <= (double)(j - 1)) k = e;
+            G.lut[j] = (uint16_t)k;
+            int steps = 0;
+            for (int e = k + 1; e < G.nb; ++e)
+                if (G.sq[e] <= (double)(j + 1) / G.lut_scale * (1.0 + 1e-9)) ++steps;
+            worst = std::max(worst, steps);
+        }
+        G.lut_single_step = worst <= 1;
+        if (G.lut_single_step || M >= 8192) break;
     }
     // compact index of sorted (i <= j <= k) bins in lexicographic order
     G.ctab.assign((size_t)G.nb * G.nb, 0);
@@ -844,6 +991,7 @@ int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
             c += G.nb - j;
         }
     make_mixed_tables(G);
+    if (G.nbc <= 65535) make_slot_table(G);
     return NC_OK;
 }
 
@@ -908,22 +1056,59 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const int nterms = (int)terms.size();
     const size_t nh = (size_t)G.nb * G.nb * G.nb;
     const bool mixed = path == NC_PATH_MIXED && G.mixed_ok;
-    // --- tables (copied into LDS by the kernel)
     auto al16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
-    const size_t off_sq = 0;
-    const size_t off_lut = al16(off_sq + sizeof(double) * G.sq.size());
-    const size_t off_ctab = al16(off_lut + sizeof(uint16_t) * G.lut.size());
-    const size_t off_lut32 = al16(off_ctab + sizeof(int) * G.ctab.size());
-    const size_t off_band = al16(off_lut32 + (mixed ? sizeof(uint16_t) * G.lut32.size() : 0));
-    const size_t tab_bytes = al16(off_band + (mixed ? sizeof(float) * G.band.size() : 0));
+    const int W = ctx->waves_per_group;
+    const int block = W * kWave;
+    auto wave_lds = [&](int c) {
+        const size_t cp = (size_t)c + 4;
+        return mixed ? al16(cp * (sizeof(float4) + sizeof(int) + sizeof(double)) + kQueueCap * sizeof(uint2))
+                     : al16(cp * (sizeof(Elem) + sizeof(double)));
+    };
+    // --- tables.  Global copies of all; the kernel copies into LDS what its
+    // variant reads in the pair loop: plain sq + lut, mixed lut32, and the
+    // slot table t3 (FAST) or ctab.  The LDS histogram holds the C(nb+2,3)
+    // sorted bins.
+    const size_t g_sq = 0;
+    const size_t g_lut = al16(g_sq + sizeof(double) * G.sq.size());
+    const size_t g_ctab = al16(g_lut + sizeof(uint16_t) * G.lut.size());
+    const size_t g_lut32 = al16(g_ctab + sizeof(int) * G.ctab.size());
+    const size_t g_t3 = al16(g_lut32 + sizeof(uint2) * G.lut32.size());
+    const size_t tab_bytes = al16(g_t3 + sizeof(uint16_t) * G.t3.size());
     std::vector<char> tabs(tab_bytes);
-    std::memcpy(tabs.data() + off_sq, G.sq.data(), sizeof(double) * G.sq.size());
-    std::memcpy(tabs.data() + off_lut, G.lut.data(), sizeof(uint16_t) * G.lut.size());
-    std::memcpy(tabs.data() + off_ctab, G.ctab.data(), sizeof(int) * G.ctab.size());
-    if (mixed) {
-        std::memcpy(tabs.data() + off_lut32, G.lut32.data(), sizeof(uint16_t) * G.lut32.size());
-        std::memcpy(tabs.data() + off_band, G.band.data(), sizeof(float) * G.band.size());
-    }
+    std::memcpy(tabs.data() + g_sq, G.sq.data(), sizeof(double) * G.sq.size());
+    std::memcpy(tabs.data() + g_lut, G.lut.data(), sizeof(uint16_t) * G.lut.size());
+    std::memcpy(tabs.data() + g_ctab, G.ctab.data(), sizeof(int) * G.ctab.size());
+    if (!G.lut32.empty()) std::memcpy(tabs.data() + g_lut32, G.lut32.data(), sizeof(uint2) * G.lut32.size());
+    if (!G.t3.empty()) std::memcpy(tabs.data() + g_t3, G.t3.data(), sizeof(uint16_t) * G.t3.size());
+    const size_t hist_bytes = al16((size_t)G.nbc * 8);
+    auto lds_tables = [&](bool fast_, size_t &o_sq, size_t &o_lut, size_t &o_ctab, size_t &o_lut32, size_t &o_t3) {
+        size_t o = 0;
+        o_sq = o_lut = o_ctab = o_lut32 = o_t3 = 0;
+        if (!mixed) {
+            o_sq = o;
+            o = al16(o + sizeof(double) * G.sq.size());
+            o_lut = o;
+            o = al16(o + sizeof(uint16_t) * G.lut.size());
+        } else {
+            o_lut32 = o;
+            o = al16(o + sizeof(uint2) * G.lut32.size());
+        }
+        if (fast_) {
+            o_t3 = o;
+            o = al16(o + sizeof(uint16_t) * G.t3.size());
+        } else {
+            o_ctab = o;
+            o = al16(o + sizeof(int) * G.ctab.size());
+        }
+        return o;
+    };
+    size_t off_sq, off_lut, off_ctab, off_lut32, off_t3;
+    // FAST variant: slot table lookup (and, plain path, a single-step bin
+    // lookup), used when the table fits in LDS next to everything else.
+    const bool fast = !G.t3.empty() && (mixed || G.lut_single_step) &&
+                      hist_bytes + lds_tables(true, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64) <=
+                          ctx->lds_max;
+    const size_t lds_tab_bytes = lds_tables(fast, off_sq, off_lut, off_ctab, off_lut32, off_t3);
     NC_HIP(ctx, dgrow(ctx->tables, tab_bytes));
     NC_HIP(ctx, hipMemcpyAsync(ctx->tables.p, tabs.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
     NC_HIP(ctx, dgrow(ctx->errw, 16));
@@ -978,20 +1163,14 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const int cap = (int)((std::max(1u, host_flags[1]) + 15u) & ~15u);
 
     // --- launch shape
-    const int W = ctx->waves_per_group;
-    const int block = W * kWave;
     int C = ctx->chunk_elems;
-    const size_t hist_bytes = al16((size_t)G.nbc * 8);
-    const size_t lds_tab = al16(hist_bytes + tab_bytes);
-    auto wave_lds = [&](int c) {
-        return mixed ? al16((size_t)c * (sizeof(float4) + sizeof(int) + sizeof(double)) + kQueueCap * sizeof(uint2))
-                     : al16((size_t)c * (sizeof(Elem) + sizeof(double)));
-    };
+    const size_t lds_tab = al16(hist_bytes + lds_tab_bytes);
     while (C > 16 && lds_tab + W * wave_lds(C) > ctx->lds_max) C /= 2;
     const size_t wave_bytes = wave_lds(C);
     const size_t lds_bytes = lds_tab + W * wave_bytes;
     if (lds_bytes > ctx->lds_max) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
-    const void *kfn = mixed ? (const void *)k_triplets<true> : (const void *)k_triplets<false>;
+    const void *kfn = mixed ? (fast ? (const void *)k_triplets<true, true> : (const void *)k_triplets<true, false>)
+                            : (fast ? (const void *)k_triplets<false, true> : (const void *)k_triplets<false, false>);
     int per_cu = ctx->wgs_per_cu;
     if (per_cu <= 0) {
         int occ = 0;
@@ -1071,31 +1250,36 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.nbc = G.nbc;
     A.lut_scale = G.lut_scale;
     A.lut_m = G.lut_m;
-    A.sq = (const double *)((char *)ctx->tables.p + off_sq);
-    A.lut = (const uint16_t *)((char *)ctx->tables.p + off_lut);
-    A.ctab = (const int *)((char *)ctx->tables.p + off_ctab);
-    A.lut32 = (const uint16_t *)((char *)ctx->tables.p + off_lut32);
-    A.band = (const float *)((char *)ctx->tables.p + off_band);
+    A.sq = (const double *)((char *)ctx->tables.p + g_sq);
+    A.lut = (const uint16_t *)((char *)ctx->tables.p + g_lut);
+    A.ctab = (const int *)((char *)ctx->tables.p + g_ctab);
+    A.lut32 = (const uint2 *)((char *)ctx->tables.p + g_lut32);
+    A.t3 = (const uint16_t *)((char *)ctx->tables.p + g_t3);
     A.lut_scale32 = G.lut_scale32;
     A.lut32_m = G.lut32_m;
-    A.tri_tol_abs = G.tri_tol_abs;
-    A.tri_tol_rel = G.tri_tol_rel;
-    A.d_tiny32 = G.d_tiny32;
+    A.g32 = G.g32;
+    A.tri_c1 = G.tri_c1;
+    A.tri_c2 = G.tri_c2;
+    A.nslots = G.nbc;
     A.off_sq = (int)hist_bytes;
     A.off_lut = (int)(hist_bytes + off_lut);
     A.off_ctab = (int)(hist_bytes + off_ctab);
     A.off_lut32 = (int)(hist_bytes + off_lut32);
-    A.off_band = (int)(hist_bytes + off_band);
+    A.off_t3 = (int)(hist_bytes + off_t3);
     A.off_wave = (int)lds_tab;
     A.wave_bytes = (int)wave_bytes;
     A.lds_bytes = (int)lds_bytes;
 
     NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (A.nchunks > 0) {
-        if (mixed)
-            hipLaunchKernelGGL(k_triplets<true>, dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+        if (mixed && fast)
+            hipLaunchKernelGGL((k_triplets<true, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+        else if (mixed)
+            hipLaunchKernelGGL((k_triplets<true, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+        else if (fast)
+            hipLaunchKernelGGL((k_triplets<false, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
         else
-            hipLaunchKernelGGL(k_triplets<false>, dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+            hipLaunchKernelGGL((k_triplets<false, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
         NC_HIP(ctx, hipGetLastError());
     }
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));

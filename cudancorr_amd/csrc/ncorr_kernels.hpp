@@ -64,21 +64,22 @@ struct KArgs {
     int survey;
     double r2min, r2max;
     int nb, nbc;
+    int nslots;                   // LDS histogram slots (= nbc, compact sorted bins)
     double lut_scale;
     int lut_m;
     // mixed path
     float lut_scale32;
     int lut32_m;
-    float tri_tol_abs;  // absolute slack margin of the FP32 triangle test
-    float tri_tol_rel;  // relative (to the largest side) margin
-    float d_tiny32;     // FP32 r23^2 below which the pair is always rechecked
+    float g32;     // guard band: |fp32 r23^2 - fp64 r23^2| bound (incl. edge rounding)
+    float tri_c1;  // collinearity threshold: s_a*s_b - dot^2 <= tri_c1*(s_a+s_b) + tri_c2 => recheck
+    float tri_c2;
     const double *sq;             // nb+1 squared edges
     const uint16_t *lut;          // lut_m+1 lower-bound bins (FP64 path)
     const int *ctab;              // nb*nb compact base index
-    const uint16_t *lut32;        // lut32_m+2 entries (mixed path): code | has-band flag << 8
-    const float *band;            // 2*(nb+1) guard band [lo,hi] per squared edge (FP32)
+    const uint2 *lut32;           // lut32_m+2 entries (mixed path): {bin below the edge, edge value as float bits}
+    const uint16_t *t3;           // nb*nb*(nb+2): (ka*nb+kb)*(nb+2)+k+1 -> compact slot
     // dynamic LDS layout (bytes)
-    int off_sq, off_lut, off_ctab, off_lut32, off_band, off_wave, wave_bytes, lds_bytes;
+    int off_sq, off_lut, off_ctab, off_lut32, off_t3, off_wave, wave_bytes, lds_bytes;
 };
 
 }  // namespace nck
