@@ -51,6 +51,38 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_calibration(nbytes):
+    """Device-to-device copy of a known size (the runtime's copy kernel), run
+    under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to get the counters' scale."""
+    hip = _hip()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    device_sync(hip)
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    device_sync(hip)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per triplet-kernel launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), if a summary exists."""
+    path = os.path.join(ROOT, "profiles", "k_triplets_mixed_batched_boxes_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(cats, edges, box, budget_s=15.0):
     """Time the CPU oracle (the reference's 27x27-cell loop, FP64, OpenMP over
     primaries) on a bounded sample: the first primaries of the largest box,
@@ -91,7 +123,12 @@ def main():
     ap.add_argument("--boxes", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true", help="skip the 64-call plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -176,7 +213,7 @@ def main():
         "plain": plain,
         "speedup_vs_plain": (plain["wall_ms"] / (wall_m * 1e3)) if plain else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profiles(),
                      "kernel": "k_triplets (mixed, batched)", "kernel_ms": count_ms_m},
         "cpu_baseline": cpu,
     }
