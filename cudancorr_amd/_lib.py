@@ -77,9 +77,9 @@ def lib():
     L.nc_count_batched.argtypes = [vp, ctypes.POINTER(NcParams), ctypes.c_int, ctypes.POINTER(vp),
                                    ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, dp, up]
     L.nc_last_timing.argtypes = [vp, ctypes.POINTER(NcTiming)]
-    L.nc_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.nc_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.nc_get_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
-                                ctypes.POINTER(ctypes.c_int)]
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     _LIB = L
     return L
 
@@ -223,10 +223,10 @@ class Context:
         self._check(lib().nc_last_timing(self.h, ctypes.byref(t)))
         return t
 
-    def set_tuning(self, waves_per_group=0, chunk=0, wgs_per_cu=-1):
-        self._check(lib().nc_set_tuning(self.h, waves_per_group, chunk, wgs_per_cu))
+    def set_tuning(self, waves_per_group=0, chunk=0, wgs_per_cu=-1, list_chunk=0):
+        self._check(lib().nc_set_tuning(self.h, waves_per_group, chunk, wgs_per_cu, list_chunk))
 
     def get_tuning(self):
-        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        self._check(lib().nc_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().nc_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value

@@ -232,7 +232,7 @@ __global__ void k_unit_weight(const double4 *__restrict__ raw, long long n, unsi
 //                                  + int2 {k, f} + double4 {image position, weight}
 // k = exact bin of |p1-q|^2, f = sqrtf(float(|p1-q|^2)) as in device.cu:173.
 
-constexpr int kQueueCap = 128;  // deferred exact rechecks per wave (mixed path)
+constexpr int kQueueCap = 320;  // deferred exact rechecks per wave (mixed path), >= 4*64 + 64
 
 struct LdsTabs {
     unsigned long long *hist;
@@ -389,11 +389,67 @@ __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, dou
     }
 }
 
-// Plain path: all pairs of S2 (broadcast from LDS, a) x S3 (one per lane, b),
-// U entries of S2 per iteration in phases (LDS reads of all U pairs, then
-// arithmetic, then histogram updates) so their latencies overlap.  The
-// triangle test uses the hardware sqrt (<= 1 ulp) and falls back to the
-// correctly rounded one when the slack is within 8 ulp of the largest side.
+// ---------------------------------------------------------------------------
+// Pair traversal.
+//
+// The pairs of S2 x S3 are visited in groups of U pairs per lane.  Two orders:
+//  * rectangular: entry a of S2 broadcast from the wave's LDS chunk, entry b
+//    of S3 held per lane (all pairs of an a-range and a 64-entry b-block);
+//  * circulant (SYM only, the 64-entry diagonal blocks): lane l of a block of
+//    m entries pairs with entry (l + t) mod m, t = 1 .. m/2, gathered from the
+//    LDS chunk -- every unordered pair of the block exactly once, no lane
+//    idle except for the half step t = m/2 of an even block.
+// When p2 and p3 are the same catalogue (SYM) the a-ranges of a b-block are
+// the entries before it (rectangular) and the block itself (circulant), so
+// every unordered pair is visited once and counted twice.
+
+// Plain path, one group of U pairs: a-entries ea[u] (broadcast or gathered),
+// b-entry eb per lane.  The triangle test uses the hardware sqrt (<= 1 ulp)
+// and redoes it with the correctly rounded sqrtf when the slack is within
+// 8 ulp of the largest side.
+template <int U, bool SYM, bool WEIGHTED, bool FAST>
+__device__ __forceinline__ void plain_group(const KArgs &A, const LdsTabs &t, const Elem (&ea)[U],
+                                            const double (&wa)[U], const bool (&ok)[U], const Elem &eb, double wb,
+                                            double w1, double w1b) {
+    const double r2min = A.r2min, r2max = A.r2max, scale = A.lut_scale;
+    const float near_rel = 8.0f * 1.1920929e-7f;
+    double d[U];
+    bool inr[U];
+    int k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        d[u] = sqrsep(ea[u].x - eb.x, ea[u].y - eb.y, ea[u].z - eb.z);
+        inr[u] = ok[u] && d[u] > r2min && d[u] < r2max;
+        const double dj = inr[u] ? d[u] : 0.0;
+        k[u] = t.lut[(int)(dj * scale)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if constexpr (FAST) {
+            k[u] += d[u] >= t.sq[k[u] + 1] ? 1 : 0;
+        } else {
+            while (k[u] < A.nb - 1 && d[u] >= t.sq[k[u] + 1]) ++k[u];
+        }
+    }
+    bool acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const float fd = (float)d[u];
+        const Sides s = sort3(ea[u].f, eb.f, __builtin_amdgcn_sqrtf(fd));
+        const float sum = s.lo + s.mid;
+        const bool near = fabsf(sum - s.hi) <= near_rel * s.hi || fd < 1e-30f;
+        acc[u] = inr[u] && !near && sum > s.hi;
+        if (inr[u] && near) {  // rare: redo the test with the correctly rounded sqrtf
+            const Sides e = sort3(ea[u].f, eb.f, __builtin_sqrtf(fd));
+            acc[u] = e.lo + e.mid > e.hi;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (acc[u])
+            hist_add<SYM, WEIGHTED>(t, sorted_slot<FAST>(A, t, ea[u].k, eb.k, k[u]), w1 * wa[u], wa[u], w1b, wb);
+}
+
 template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
                                 int n3, char *wlds, int lane) {
@@ -406,8 +462,6 @@ __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, con
     const double *W3 = reinterpret_cast<const double *>(l3 + (size_t)A.cap * sizeof(Elem));
     Elem *la = reinterpret_cast<Elem *>(wlds);
     double *lw = reinterpret_cast<double *>(wlds + (size_t)Cp * sizeof(Elem));
-    const double r2min = A.r2min, r2max = A.r2max, scale = A.lut_scale;
-    const float near_rel = 8.0f * 1.1920929e-7f;
     for (int a0 = 0; a0 < n2; a0 += C) {
         const int a1 = min(a0 + C, n2);
         wave_sync();
@@ -440,52 +494,39 @@ __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, con
                 eb.f = 0.0f;
             }
             const double w1b = w1 * wb;
-            const int alim = SYM ? min(a1, min(n3, b0 + kWave) - 1) : a1;
-            for (int a = a0; a < alim; a += U) {
+            // rectangular rows: every a of [a0, rect_end) with every b of the block
+            const int rect_end = SYM ? min(b0, a1) : a1;
+            for (int a = a0; a < rect_end; a += U) {
                 Elem ea[U];
                 double wa[U];
+                bool ok[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     ea[u] = la[a - a0 + u];
                     wa[u] = WEIGHTED ? lw[a - a0 + u] : 0.0;
+                    ok[u] = vb && a + u < rect_end;
                 }
-                double d[U];
-                bool inr[U];
-                int k[U];
+                plain_group<U, SYM, WEIGHTED, FAST>(A, t, ea, wa, ok, eb, wb, w1, w1b);
+            }
+            if (SYM && b0 < a1) {  // diagonal block: circulant over its m entries
+                const int m = min(kWave, n3 - b0);
+                const int tmax = m / 2;
+                for (int s = 1; s <= tmax; s += U) {
+                    Elem ea[U];
+                    double wa[U];
+                    bool ok[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const bool ok = vb && (!SYM || a + u < b);
-                    d[u] = sqrsep(ea[u].x - eb.x, ea[u].y - eb.y, ea[u].z - eb.z);
-                    inr[u] = ok && d[u] > r2min && d[u] < r2max;
-                    const double dj = inr[u] ? d[u] : 0.0;
-                    k[u] = t.lut[(int)(dj * scale)];
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if constexpr (FAST) {
-                        k[u] += d[u] >= t.sq[k[u] + 1] ? 1 : 0;
-                    } else {
-                        while (k[u] < A.nb - 1 && d[u] >= t.sq[k[u] + 1]) ++k[u];
+                    for (int u = 0; u < U; ++u) {
+                        const int st = s + u;
+                        int off = lane + st;
+                        off -= off >= m ? m : 0;
+                        ok[u] = vb && st <= tmax && (2 * st != m || lane < st);
+                        const int li = b0 - a0 + (ok[u] ? off : 0);
+                        ea[u] = la[li];
+                        wa[u] = WEIGHTED ? lw[li] : 0.0;
                     }
+                    plain_group<U, SYM, WEIGHTED, FAST>(A, t, ea, wa, ok, eb, wb, w1, w1b);
                 }
-                bool acc[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const float fd = (float)d[u];
-                    const Sides s = sort3(ea[u].f, eb.f, __builtin_amdgcn_sqrtf(fd));
-                    const float sum = s.lo + s.mid;
-                    const bool near = fabsf(sum - s.hi) <= near_rel * s.hi || fd < 1e-30f;
-                    acc[u] = inr[u] && !near && sum > s.hi;
-                    if (inr[u] && near) {  // rare: redo the test with the correctly rounded sqrtf
-                        const Sides e = sort3(ea[u].f, eb.f, __builtin_sqrtf(fd));
-                        acc[u] = e.lo + e.mid > e.hi;
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (acc[u])
-                        hist_add<SYM, WEIGHTED>(t, sorted_slot<FAST>(A, t, ea[u].k, eb.k, k[u]), w1 * wa[u], wa[u],
-                                                w1b, wb);
             }
         }
     }
@@ -509,16 +550,73 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
     wave_sync();
 }
 
-// Mixed path.  With r = q - p1 re-centred in FP32 and s = |r|^2,
-//   r23^2 = s_a + s_b - 2 r_a.r_b
-// and the triangle (p1, a, b) is degenerate only when r_a and r_b are nearly
-// collinear, i.e. when s_a*s_b - (r_a.r_b)^2 is small.  Per pair: one dot
-// product, one lookup over r23^2*scale32 that returns the bin below the
-// (at most one) edge near that value and the edge itself, and the
-// collinearity bound.  A pair whose r23^2 lies within the guard band g32 of
-// that edge, or whose collinearity measure is within the error bound (see
-// make_mixed_tables), is queued for the exact FP64 evaluation.  U entries of
-// S2 per iteration in phases (LDS reads, arithmetic, updates, queue pushes).
+// Per-wave state of the mixed pair loop.
+struct MixedCtx {
+    const char *l2, *l3;
+    uint2 *q;  // deferred rechecks (LDS)
+    int qn;
+    float s32, jmax, g32, c1, c2;
+    unsigned nb;
+    double w1;
+};
+
+// Mixed path, one group of U pairs.  With r = q - p1 re-centred in FP32 and
+// s = |r|^2:  r23^2 = s_a + s_b - 2 r_a.r_b, and the triangle (p1, a, b) is
+// degenerate only when r_a and r_b are nearly collinear, i.e. when
+// s_a*s_b - (r_a.r_b)^2 is small.  Per pair: one dot product, one lookup over
+// r23^2*scale32 that returns the bin below the (at most one) edge near that
+// value and the edge itself, and the collinearity bound.  A pair whose r23^2
+// lies within the guard band g32 of that edge, or whose collinearity measure
+// is within the error bound (make_mixed_tables), is queued for the exact FP64
+// evaluation.  Phases: lookups of all U pairs, then arithmetic, histogram
+// updates, queue pushes.
+template <int U, bool SYM, bool WEIGHTED, bool FAST>
+__device__ __forceinline__ void mixed_group(const KArgs &A, const LdsTabs &t, MixedCtx &mc, const float4 (&ra)[U],
+                                            const int (&ka)[U], const double (&wa)[U], const int (&ai)[U],
+                                            const bool (&ok)[U], const float4 &rb, int kb, double wb, double w1b,
+                                            int b, int lane) {
+    float d[U], dot[U], ss[U];
+    uint2 e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
+        ss[u] = ra[u].w + rb.w;
+        d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
+        e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u] * mc.s32, 0.0f, mc.jmax)];
+    }
+    bool acc[U], amb[U];
+    int slot[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const float E = __uint_as_float(e[u].y);
+        const int k = (int)e[u].x + (d[u] > E ? 1 : 0);
+        const bool amb_b = fabsf(d[u] - E) <= mc.g32;
+        const bool valid = (unsigned)k < mc.nb;
+        const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
+        const bool amb_t = delta <= __builtin_fmaf(mc.c1, ss[u], mc.c2);
+        amb[u] = ok[u] && (amb_b || (valid && amb_t));
+        acc[u] = ok[u] && valid && !amb_b && !amb_t;
+        slot[u] = sorted_slot<FAST>(A, t, ka[u], kb, k);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (acc[u]) hist_add<SYM, WEIGHTED>(t, slot[u], mc.w1 * wa[u], wa[u], w1b, wb);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const unsigned long long m = __ballot(amb[u]);
+        if (m) {
+            const int pos = mc.qn + lane_rank(m);
+            if (amb[u]) mc.q[pos] = make_uint2((unsigned)ai[u], (unsigned)b);
+            mc.qn += __popcll(m);
+        }
+    }
+    // one group adds at most U*64 entries: drain once the queue could overflow
+    if (mc.qn > kQueueCap - U * kWave) {
+        drain_queue<SYM, WEIGHTED, FAST>(A, t, mc.w1, mc.l2, mc.l3, mc.q, mc.qn, lane);
+        mc.qn = 0;
+    }
+}
+
 template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
                                 int n3, char *wlds, int lane) {
@@ -529,12 +627,18 @@ __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, con
     float4 *la = reinterpret_cast<float4 *>(wlds);
     int *lk = reinterpret_cast<int *>(wlds + (size_t)Cp * sizeof(float4));
     double *lw = reinterpret_cast<double *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int)));
-    uint2 *q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
-    const float s32 = A.lut_scale32;
-    const float jmax = (float)(A.lut32_m + 1);
-    const float g32 = A.g32, c1 = A.tri_c1, c2 = A.tri_c2;
-    const unsigned nb = (unsigned)A.nb;
-    int qn = 0;
+    MixedCtx mc;
+    mc.l2 = l2;
+    mc.l3 = l3;
+    mc.q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
+    mc.qn = 0;
+    mc.s32 = A.lut_scale32;
+    mc.jmax = (float)(A.lut32_m + 1);
+    mc.g32 = A.g32;
+    mc.c1 = A.tri_c1;
+    mc.c2 = A.tri_c2;
+    mc.nb = (unsigned)A.nb;
+    mc.w1 = w1;
     for (int a0 = 0; a0 < n2; a0 += C) {
         const int a1 = min(a0 + C, n2);
         wave_sync();
@@ -564,61 +668,48 @@ __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, con
                 if (WEIGHTED) wb = m3.D[b].w;
             }
             const double w1b = w1 * wb;
-            const int alim = SYM ? min(a1, min(n3, b0 + kWave) - 1) : a1;
-            for (int a = a0; a < alim; a += U) {
+            const int rect_end = SYM ? min(b0, a1) : a1;
+            for (int a = a0; a < rect_end; a += U) {
                 float4 ra[U];
-                int ka[U];
+                int ka[U], ai[U];
                 double wa[U];
+                bool ok[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     ra[u] = la[a - a0 + u];
                     ka[u] = lk[a - a0 + u];
                     wa[u] = WEIGHTED ? lw[a - a0 + u] : 0.0;
+                    ai[u] = a + u;
+                    ok[u] = vb && a + u < rect_end;
                 }
-                float d[U], dot[U], ss[U];
-                uint2 e[U];
+                mixed_group<U, SYM, WEIGHTED, FAST>(A, t, mc, ra, ka, wa, ai, ok, rb, kb, wb, w1b, b, lane);
+            }
+            if (SYM && b0 < a1) {  // diagonal block: circulant over its m entries
+                const int m = min(kWave, n3 - b0);
+                const int tmax = m / 2;
+                for (int s = 1; s <= tmax; s += U) {
+                    float4 ra[U];
+                    int ka[U], ai[U];
+                    double wa[U];
+                    bool ok[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
-                    ss[u] = ra[u].w + rb.w;
-                    d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
-                    e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u] * s32, 0.0f, jmax)];
-                }
-                bool acc[U], amb[U];
-                int slot[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const bool ok = vb && (!SYM || a + u < b);
-                    const float E = __uint_as_float(e[u].y);
-                    const int k = (int)e[u].x + (d[u] > E ? 1 : 0);
-                    const bool amb_b = fabsf(d[u] - E) <= g32;
-                    const bool valid = (unsigned)k < nb;
-                    const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
-                    const bool amb_t = delta <= __builtin_fmaf(c1, ss[u], c2);
-                    amb[u] = ok && (amb_b || (valid && amb_t));
-                    acc[u] = ok && valid && !amb_b && !amb_t;
-                    slot[u] = sorted_slot<FAST>(A, t, ka[u], kb, k);
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (acc[u]) hist_add<SYM, WEIGHTED>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const unsigned long long m = __ballot(amb[u]);
-                    if (m) {
-                        const int pos = qn + lane_rank(m);
-                        if (amb[u]) q[pos] = make_uint2((unsigned)(a + u), (unsigned)b);
-                        qn += __popcll(m);
-                        if (qn > kQueueCap - kWave) {
-                            drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, q, qn, lane);
-                            qn = 0;
-                        }
+                    for (int u = 0; u < U; ++u) {
+                        const int st = s + u;
+                        int off = lane + st;
+                        off -= off >= m ? m : 0;
+                        ok[u] = vb && st <= tmax && (2 * st != m || lane < st);
+                        const int li = b0 - a0 + (ok[u] ? off : 0);
+                        ra[u] = la[li];
+                        ka[u] = lk[li];
+                        wa[u] = WEIGHTED ? lw[li] : 0.0;
+                        ai[u] = a0 + li;
                     }
+                    mixed_group<U, SYM, WEIGHTED, FAST>(A, t, mc, ra, ka, wa, ai, ok, rb, kb, wb, w1b, b, lane);
                 }
             }
         }
     }
-    if (qn > 0) drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, q, qn, lane);
+    if (mc.qn > 0) drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, mc.q, mc.qn, lane);
 }
 
 // Merge the workgroup's LDS histogram into the term's global compact
@@ -810,6 +901,7 @@ struct Geometry {
 
 struct nc_catalog {
     nc_context *ctx = nullptr;
+    uint64_t uid = 0;  // unique per handle (graph plans key on it)
     int64_t n = 0;
     const double *d_raw = nullptr;
     bool owns = false;
@@ -817,8 +909,24 @@ struct nc_catalog {
     DBuf raw, key, rank, cnt, start, x, y, z, w;
 };
 
+// Captured launch sequence of nc_count_batched: replayed while the same
+// terms, params, path and tuning are passed again.
+struct BatchPlan {
+    bool valid = false;
+    std::vector<uint64_t> uids;
+    std::vector<double> pkey;
+    std::vector<int> tkey;
+    hipGraphExec_t build_exec = nullptr, count_exec = nullptr;
+    unsigned long long *h_out = nullptr;
+    unsigned *h_err = nullptr;
+    std::vector<int> weighted;
+    int nb = 0, nbc = 0, nkernel = 0;
+};
+
 struct nc_context {
     int device = 0;
+    BatchPlan plan;
+    uint64_t next_uid = 1;
     hipStream_t stream = nullptr;
     std::string err;
     int ncu = 0;
@@ -826,8 +934,8 @@ struct nc_context {
     DBuf tables, terms, chunks, out, counter, errw, scratch, bound;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     nc_timing timing{};
-    int waves_per_group = 4;
-    int chunk = 16;
+    int waves_per_group = 8;
+    int chunk = 64;
     int wgs_per_cu = 0;
     int chunk_elems = 128;
 };
@@ -1041,8 +1149,75 @@ struct Term {
     nc_catalog *c1, *c2, *c3;
 };
 
+void plan_drop(nc_context *ctx) {
+    BatchPlan &p = ctx->plan;
+    if (p.build_exec) (void)hipGraphExecDestroy(p.build_exec);
+    if (p.count_exec) (void)hipGraphExecDestroy(p.count_exec);
+    if (p.h_out) (void)hipHostFree(p.h_out);
+    if (p.h_err) (void)hipHostFree(p.h_err);
+    p = BatchPlan();
+}
+
+// Compact sorted-bin results -> dense nb^3 histogram (device.cu:198 layout).
+void convert_out(int nb, int nbc, int nterms, const std::vector<int> &weighted, const unsigned long long *res,
+                 double *hist, uint64_t *counts) {
+    const size_t nh = (size_t)nb * nb * nb;
+    for (int i = 0; i < nterms; ++i) {
+        double *h = hist + (size_t)i * nh;
+        uint64_t *cn = counts ? counts + (size_t)i * nh : nullptr;
+        std::fill(h, h + nh, 0.0);
+        if (cn) std::fill(cn, cn + nh, (uint64_t)0);
+        const unsigned long long *r = res + (size_t)i * nbc;
+        int c = 0;
+        for (int a = 0; a < nb; ++a)
+            for (int b = a; b < nb; ++b)
+                for (int k = b; k < nb; ++k, ++c) {
+                    const size_t di = (size_t)k + (size_t)b * nb + (size_t)a * nb * nb;
+                    if (weighted[i]) {
+                        double v;
+                        std::memcpy(&v, &r[c], 8);
+                        h[di] = v;
+                    } else {
+                        h[di] = (double)r[c];
+                        if (cn) cn[di] = r[c];
+                    }
+                }
+    }
+}
+
+// Replay a captured plan: cell-list builds graph, triplet graph, copy-back.
+int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
+    BatchPlan &p = ctx->plan;
+    retry = false;
+    NC_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    NC_HIP(ctx, hipGraphLaunch(p.build_exec, ctx->stream));
+    NC_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    NC_HIP(ctx, hipGraphLaunch(p.count_exec, ctx->stream));
+    NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    const int nterms = (int)p.weighted.size();
+    NC_HIP(ctx, hipMemcpyAsync(p.h_out, ctx->out.p, sizeof(unsigned long long) * (size_t)nterms * p.nbc,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(p.h_err, ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms_b = 0.f, ms_c = 0.f;
+    NC_HIP(ctx, hipEventElapsedTime(&ms_b, ctx->ev[0], ctx->ev[1]));
+    NC_HIP(ctx, hipEventElapsedTime(&ms_c, ctx->ev[1], ctx->ev[3]));
+    ctx->timing.ms_build = ms_b;
+    ctx->timing.ms_count = ms_c;
+    ctx->timing.ms_total = ms_b + ms_c;
+    ctx->timing.n_kernel = p.nkernel;
+    if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
+    if (*p.h_err & kErrOverflow) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
+        plan_drop(ctx);
+        retry = true;
+        return NC_OK;
+    }
+    convert_out(p.nb, p.nbc, nterms, p.weighted, p.h_out, hist, counts);
+    return NC_OK;
+}
+
 int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &terms, int path, double *hist,
-              uint64_t *counts) {
+              uint64_t *counts, bool use_graph) {
     if (!ctx) return NC_ERR_ARG;
     if (!P || !hist) return fail(ctx, NC_ERR_ARG, "null params or output");
     if (path != NC_PATH_PLAIN && path != NC_PATH_MIXED) return fail(ctx, NC_ERR_ARG, "unknown path");
@@ -1053,6 +1228,19 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     int rc = make_geometry(ctx, P, G);
     if (rc) return rc;
     NC_HIP(ctx, hipSetDevice(ctx->device));
+    // --- graph plan: key = catalogues, params, path, tuning
+    std::vector<uint64_t> uids;
+    for (const Term &t : terms) uids.insert(uids.end(), {t.c1->uid, t.c2->uid, t.c3->uid});
+    std::vector<double> pkey(P->box, P->box + 3);
+    pkey.push_back((double)G.survey);
+    pkey.insert(pkey.end(), G.edges.begin(), G.edges.end());
+    const std::vector<int> tkey = {path, ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
+    if (use_graph && ctx->plan.valid && ctx->plan.uids == uids && ctx->plan.pkey == pkey && ctx->plan.tkey == tkey) {
+        bool retry = false;
+        rc = plan_replay(ctx, hist, counts, retry);
+        if (rc || !retry) return rc;
+    }
+    plan_drop(ctx);
     const int nterms = (int)terms.size();
     const size_t nh = (size_t)G.nb * G.nb * G.nb;
     const bool mixed = path == NC_PATH_MIXED && G.mixed_ok;
@@ -1114,7 +1302,6 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     NC_HIP(ctx, dgrow(ctx->errw, 16));
     NC_HIP(ctx, dgrow(ctx->counter, 16));
     NC_HIP(ctx, dgrow(ctx->bound, 16));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->errw.p, 0u, 4LL);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->bound.p, 0u, 4LL);
     for (const Term &t : terms)
         for (nc_catalog *c : {t.c1, t.c2, t.c3})
@@ -1135,10 +1322,16 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             if (!found) grids.push_back({cs[r], r == 0});
         }
     }
-    for (CatGrid &g : grids) {
-        rc = build_grid(ctx, G, g.cat, G.survey && g.is_p1);
-        if (rc) return rc;
-    }
+    auto enqueue_builds = [&]() -> int {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->errw.p, 0u, 4LL);
+        for (CatGrid &g : grids) {
+            const int r = build_grid(ctx, G, g.cat, G.survey && g.is_p1);
+            if (r) return r;
+        }
+        return NC_OK;
+    };
+    rc = enqueue_builds();
+    if (rc) return rc;
     NC_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     GridGeo gg;
     for (int a = 0; a < 3; ++a) {
@@ -1165,7 +1358,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // --- launch shape
     int C = ctx->chunk_elems;
     const size_t lds_tab = al16(hist_bytes + lds_tab_bytes);
-    while (C > 16 && lds_tab + W * wave_lds(C) > ctx->lds_max) C /= 2;
+    // chunks are whole 64-entry blocks (the circulant diagonal blocks need it)
+    C = std::max(64, C - C % 64);
+    while (C > 64 && lds_tab + W * wave_lds(C) > ctx->lds_max) C -= 64;
     const size_t wave_bytes = wave_lds(C);
     const size_t lds_bytes = lds_tab + W * wave_bytes;
     if (lds_bytes > ctx->lds_max) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
@@ -1222,10 +1417,6 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (!chunks.empty())
         NC_HIP(ctx, hipMemcpyAsync(ctx->chunks.p, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice,
                                    ctx->stream));
-    hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->out.p, 0ull,
-                       (long long)nterms * G.nbc);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
-
     KArgs A;
     std::memset(&A, 0, sizeof(A));
     A.terms = (const TermDesc *)ctx->terms.p;
@@ -1270,18 +1461,24 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.wave_bytes = (int)wave_bytes;
     A.lds_bytes = (int)lds_bytes;
 
+    auto enqueue_count = [&]() -> int {
+        hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->out.p, 0ull,
+                           (long long)nterms * G.nbc);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
+        if (A.nchunks > 0) {
+            if (mixed && fast)
+                hipLaunchKernelGGL((k_triplets<true, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+            else if (mixed)
+                hipLaunchKernelGGL((k_triplets<true, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+            else if (fast)
+                hipLaunchKernelGGL((k_triplets<false, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+            else
+                hipLaunchKernelGGL((k_triplets<false, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+        }
+        return hipGetLastError() == hipSuccess ? NC_OK : NC_ERR_HIP;
+    };
     NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    if (A.nchunks > 0) {
-        if (mixed && fast)
-            hipLaunchKernelGGL((k_triplets<true, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
-        else if (mixed)
-            hipLaunchKernelGGL((k_triplets<true, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
-        else if (fast)
-            hipLaunchKernelGGL((k_triplets<false, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
-        else
-            hipLaunchKernelGGL((k_triplets<false, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
-        NC_HIP(ctx, hipGetLastError());
-    }
+    if (enqueue_count() != NC_OK) return fail(ctx, NC_ERR_HIP, "triplet kernel launch failed");
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     std::vector<unsigned long long> res((size_t)nterms * G.nbc);
     NC_HIP(ctx, hipMemcpyAsync(res.data(), ctx->out.p, sizeof(unsigned long long) * res.size(), hipMemcpyDeviceToHost,
@@ -1298,27 +1495,36 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
 
-    // --- compact sorted bins -> dense nb^3 (device.cu:198 layout)
-    for (int i = 0; i < nterms; ++i) {
-        double *h = hist + (size_t)i * nh;
-        uint64_t *cn = counts ? counts + (size_t)i * nh : nullptr;
-        std::fill(h, h + nh, 0.0);
-        if (cn) std::fill(cn, cn + nh, (uint64_t)0);
-        const unsigned long long *r = res.data() + (size_t)i * G.nbc;
-        int c = 0;
-        for (int a = 0; a < G.nb; ++a)
-            for (int b = a; b < G.nb; ++b)
-                for (int k = b; k < G.nb; ++k, ++c) {
-                    const size_t di = (size_t)k + (size_t)b * G.nb + (size_t)a * G.nb * G.nb;
-                    if (weighted[i]) {
-                        double v;
-                        std::memcpy(&v, &r[c], 8);
-                        h[di] = v;
-                    } else {
-                        h[di] = (double)r[c];
-                        if (cn) cn[di] = r[c];
-                    }
-                }
+    convert_out(G.nb, G.nbc, nterms, weighted, res.data(), hist, counts);
+
+    // --- capture the launch sequence for replay (builds, then count); any
+    // failure leaves the eager result valid and no plan cached
+    if (use_graph) {
+        BatchPlan p;
+        p.uids = uids;
+        p.pkey = pkey;
+        p.tkey = tkey;
+        p.weighted = weighted;
+        p.nb = G.nb;
+        p.nbc = G.nbc;
+        p.nkernel = A.nchunks > 0 ? 1 : 0;
+        bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
+                      hipSuccess &&
+                  hipHostMalloc((void **)&p.h_err, 16) == hipSuccess;
+        for (int part = 0; part < 2 && ok; ++part) {
+            hipGraph_t g = nullptr;
+            ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (!ok) break;
+            const int erc = part == 0 ? enqueue_builds() : enqueue_count();
+            ok = hipStreamEndCapture(ctx->stream, &g) == hipSuccess && erc == NC_OK && g != nullptr;
+            if (ok) ok = hipGraphInstantiate(part == 0 ? &p.build_exec : &p.count_exec, g, nullptr, nullptr, 0) ==
+                         hipSuccess;
+            if (g) (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+        p.valid = ok;
+        ctx->plan = p;
+        if (!ok) plan_drop(ctx);
     }
     return NC_OK;
 }
@@ -1345,7 +1551,7 @@ int nc_create(nc_context **out, int device) {
         return NC_ERR_HIP;
     }
     ctx->ncu = prop.multiProcessorCount;
-    ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 65536) - 256;
+    ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
     for (auto &e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) {
             nc_destroy(ctx);
@@ -1359,6 +1565,7 @@ int nc_destroy(nc_context *ctx) {
     if (!ctx) return NC_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    plan_drop(ctx);
     for (DBuf *b : {&ctx->tables, &ctx->terms, &ctx->chunks, &ctx->out, &ctx->counter, &ctx->errw, &ctx->scratch,
                     &ctx->bound})
         dfree(*b);
@@ -1383,6 +1590,7 @@ int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog
     nc_catalog *c = new (std::nothrow) nc_catalog();
     if (!c) return fail(ctx, NC_ERR_NOMEM, "host allocation");
     c->ctx = ctx;
+    c->uid = ctx->next_uid++;
     c->n = n;
     c->owns = true;
     for (int64_t i = 0; i < n; ++i)
@@ -1412,6 +1620,7 @@ int nc_catalog_wrap_device(nc_context *ctx, const double *d_xyzw, int64_t n, nc_
     nc_catalog *c = new (std::nothrow) nc_catalog();
     if (!c) return fail(ctx, NC_ERR_NOMEM, "host allocation");
     c->ctx = ctx;
+    c->uid = ctx->next_uid++;
     c->n = n;
     c->owns = false;
     c->d_raw = d_xyzw;
@@ -1441,6 +1650,8 @@ int nc_catalog_destroy(nc_catalog *c) {
     if (c->ctx) {
         (void)hipSetDevice(c->ctx->device);
         (void)hipStreamSynchronize(c->ctx->stream);
+        const std::vector<uint64_t> &u = c->ctx->plan.uids;
+        if (std::find(u.begin(), u.end(), c->uid) != u.end()) plan_drop(c->ctx);
     }
     for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->cnt, &c->start, &c->x, &c->y, &c->z, &c->w}) dfree(*b);
     delete c;
@@ -1452,7 +1663,7 @@ int64_t nc_catalog_size(const nc_catalog *c) { return c ? c->n : -1; }
 int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalog *c2, nc_catalog *c3, int path,
              double *hist, uint64_t *counts) {
     std::vector<Term> t{{c1, c2, c3}};
-    return run_count(ctx, params, t, path, hist, counts);
+    return run_count(ctx, params, t, path, hist, counts, false);
 }
 
 int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, int64_t n1, const double *p2, int64_t n2,
@@ -1489,7 +1700,7 @@ int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_ca
     if (nterms < 1 || !c1 || !c2 || !c3) return fail(ctx, NC_ERR_ARG, "bad term list");
     std::vector<Term> t(nterms);
     for (int i = 0; i < nterms; ++i) t[i] = {c1[i], c2[i], c3[i]};
-    return run_count(ctx, params, t, path, hist, counts);
+    return run_count(ctx, params, t, path, hist, counts, true);
 }
 
 int nc_last_timing(const nc_context *ctx, nc_timing *out) {
@@ -1498,8 +1709,13 @@ int nc_last_timing(const nc_context *ctx, nc_timing *out) {
     return NC_OK;
 }
 
-int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu) {
+int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu, int list_chunk) {
     if (!ctx) return NC_ERR_ARG;
+    if (list_chunk > 0) {
+        if (list_chunk % 64 != 0 || list_chunk > 4096)
+            return fail(ctx, NC_ERR_ARG, "list_chunk must be a multiple of 64, at most 4096");
+        ctx->chunk_elems = list_chunk;
+    }
     if (waves_per_group > 0) {
         if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4 && waves_per_group != 8)
             return fail(ctx, NC_ERR_ARG, "waves_per_group must be 1, 2, 4 or 8");
@@ -1510,8 +1726,9 @@ int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_c
     return NC_OK;
 }
 
-int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu) {
+int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu, int *list_chunk) {
     if (!ctx) return NC_ERR_ARG;
+    if (list_chunk) *list_chunk = ctx->chunk_elems;
     if (waves_per_group) *waves_per_group = ctx->waves_per_group;
     if (chunk) *chunk = ctx->chunk;
     if (wgs_per_cu) *wgs_per_cu = ctx->wgs_per_cu;

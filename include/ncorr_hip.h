@@ -122,11 +122,14 @@ int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_ca
 /* Device timing of the last nc_count / nc_count_batched on ctx. */
 int nc_last_timing(const nc_context *ctx, nc_timing *out);
 
-/* Tuning knobs of the triplet kernel (see nc_autotune).  Values <= 0 keep the
- * current setting.  waves_per_group in {1,2,4,8}; chunk = primaries per work
- * item; wgs_per_cu = persistent workgroups per compute unit. */
-int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu);
-int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu);
+/* Tuning knobs of the triplet kernel.  Values <= 0 keep the current setting
+ * (wgs_per_cu: 0 = as many as fit, < 0 keeps).
+ *   waves_per_group  in {1, 2, 4, 8}
+ *   chunk            primaries per work item of the device queue
+ *   wgs_per_cu       persistent workgroups per compute unit
+ *   list_chunk       neighbour-list entries staged in LDS per wave (multiple of 64) */
+int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu, int list_chunk);
+int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu, int *list_chunk);
 
 #ifdef __cplusplus
 }

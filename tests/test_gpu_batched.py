@@ -1,0 +1,79 @@
+"""nc_count_batched: one persistent launch over many terms, captured as HIP
+graphs after the first call and replayed while the same terms come back.
+
+Every replay is checked against per-term eager counts (nc_count, plain FP64),
+with host work and device synchronisations between replays; the four survey
+terms DDD/DDR/DRR/RRR in one batch are checked against the oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN
+from cudancorr_amd.gen import edges_linear, neyman_scott_box, prep_cat_survey, survey_cap
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    hip = ctypes.CDLL("libamdhip64.so")
+    assert hip.hipDeviceSynchronize() == 0
+
+
+def test_batched_boxes_replays(ctx):
+    print("batched boxes", flush=True)
+    L = 300.0
+    edges = edges_linear(0.0, 30.0, 15)
+    box = [L, L, L]
+    pts = [neyman_scott_box(n, 40, 4.0, L, seed=100 + n) for n in (150, 400, 900)]
+    cats = [ctx.catalog(p) for p in pts]
+    want = np.stack([ctx.count(c, c, c, edges, box, path=NC_PATH_PLAIN)[1] for c in cats])
+    terms = [(c, c, c) for c in cats]
+    for rep in range(4):
+        hist, cnt = ctx.count_batched(terms, edges, box, path=NC_PATH_MIXED)
+        assert np.array_equal(cnt, want), f"call {rep}"
+        assert np.array_equal(hist, want.astype(np.float64))
+        _ = np.sort(np.random.default_rng(rep).random(100000))  # host work between replays
+        _sync()
+    # a different term list (reordered, with a repeat) gets its own plan
+    terms2 = [terms[2], terms[0], terms[2]]
+    for rep in range(2):
+        _, cnt2 = ctx.count_batched(terms2, edges, box, path=NC_PATH_MIXED)
+        assert np.array_equal(cnt2, want[[2, 0, 2]])
+    # plain batched equals mixed batched
+    _, cntp = ctx.count_batched(terms, edges, box, path=NC_PATH_PLAIN)
+    assert np.array_equal(cntp, want)
+    # destroying a catalogue of the cached plan drops the plan
+    cats[1].close()
+    terms3 = [terms[0], terms[2]]
+    _, cnt3 = ctx.count_batched(terms3, edges, box, path=NC_PATH_MIXED)
+    assert np.array_equal(cnt3, want[[0, 2]])
+    for c in cats:
+        c.close()
+
+
+def test_batched_survey_four_terms(ctx):
+    print("batched survey", flush=True)
+    d, r = survey_cap(3000, 4500, seed=21)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 40.0, 20)
+    D, R = ctx.catalog(d), ctx.catalog(r)
+    terms = [(D, D, D), (D, D, R), (D, R, R), (R, R, R)]
+    arrs = {id(D): d, id(R): r}
+    for rep in range(3):
+        hist, cnt = ctx.count_batched(terms, edges, box, survey=True, path=NC_PATH_MIXED)
+        for i, (a, b, c) in enumerate(terms):
+            wc, ws = oracle.count(arrs[id(a)], arrs[id(b)], arrs[id(c)], edges, box, survey=True)
+            if i == 3:
+                assert np.array_equal(cnt[i], wc)
+            else:
+                nz = ws != 0
+                assert np.array_equal(hist[i][~nz], ws[~nz])
+                assert np.max(np.abs(hist[i][nz] - ws[nz]) / ws[nz]) <= 1e-9
+        _sync()
+    D.close()
+    R.close()
