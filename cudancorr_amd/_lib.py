@@ -24,7 +24,7 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "invalid bin edges", 3: "point outs
 EXPORTS = (
     "nc_create", "nc_destroy", "nc_last_error", "nc_catalog_create", "nc_catalog_wrap_device",
     "nc_catalog_destroy", "nc_catalog_size", "nc_hist_size", "nc_compact_size", "nc_count", "nc_count_host",
-    "nc_count_batched", "nc_last_timing", "nc_set_tuning", "nc_get_tuning",
+    "nc_count_batched", "nc_last_timing", "nc_set_tuning", "nc_get_tuning", "nc_autotune",
 )
 
 
@@ -36,6 +36,11 @@ class NcParams(ctypes.Structure):
 class NcTiming(ctypes.Structure):
     _fields_ = [("ms_build", ctypes.c_float), ("ms_count", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("n_kernel", ctypes.c_int64)]
+
+
+class NcTuning(ctypes.Structure):
+    _fields_ = [("waves_per_group", ctypes.c_int), ("chunk", ctypes.c_int), ("wgs_per_cu", ctypes.c_int),
+                ("list_chunk", ctypes.c_int), ("ms", ctypes.c_float), ("from_cache", ctypes.c_int)]
 
 
 class NcError(RuntimeError):
@@ -80,6 +85,8 @@ def lib():
     L.nc_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.nc_get_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.nc_autotune.argtypes = [vp, ctypes.POINTER(NcParams), ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                              ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(NcTuning)]
     _LIB = L
     return L
 
@@ -217,6 +224,18 @@ class Context:
         self._check(lib().nc_count_batched(self.h, ctypes.byref(p), T, c1, c2, c3, int(path), _dp(hist),
                                            cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
         return hist.reshape(T, nb, nb, nb), cnt.reshape(T, nb, nb, nb)
+
+    def autotune(self, terms, edges, box, survey=False, path=NC_PATH_MIXED):
+        """nc_autotune over (Catalog, Catalog, Catalog) terms; returns a dict."""
+        T = len(terms)
+        arr = ctypes.c_void_p * T
+        c1 = arr(*[t[0].h for t in terms])
+        c2 = arr(*[t[1].h for t in terms])
+        c3 = arr(*[t[2].h for t in terms])
+        p, e = self.params(edges, box, survey)
+        best = NcTuning()
+        self._check(lib().nc_autotune(self.h, ctypes.byref(p), T, c1, c2, c3, int(path), ctypes.byref(best)))
+        return {f: getattr(best, f) for f, _ in NcTuning._fields_}
 
     def timing(self) -> NcTiming:
         t = NcTiming()

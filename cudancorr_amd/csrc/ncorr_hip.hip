@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -228,8 +229,10 @@ __global__ void k_unit_weight(const double4 *__restrict__ raw, long long n, unsi
 // S2 x S3 that tests r23 only.
 //
 // Plain list entry (FP64, 40 B):   Elem {x,y,z image position, k, f} + weight
-// Mixed list entry (56 B):         float4 {x,y,z relative to p1, s = |r|^2}
-//                                  + int2 {k, f} + double4 {image position, weight}
+// Mixed list entry (28 B):         float4 {x,y,z relative to p1, s = |r|^2}
+//                                  + int2 {k | neighbour entry << 8, f} + catalogue index
+// (the FP64 image position of an entry is rebuilt from the catalogue index and
+// its neighbour entry's shift, exactly as the list build computed it)
 // k = exact bin of |p1-q|^2, f = sqrtf(float(|p1-q|^2)) as in device.cu:173.
 
 constexpr int kQueueCap = 320;  // deferred exact rechecks per wave (mixed path), >= 4*64 + 64
@@ -245,22 +248,27 @@ struct LdsTabs {
 
 // Bytes of one list entry in a wave's scratch slot.
 __host__ __device__ constexpr size_t list_entry_bytes(bool mixed) {
-    return mixed ? sizeof(float4) + sizeof(int2) + sizeof(double4) : sizeof(Elem) + sizeof(double);
+    return mixed ? sizeof(float4) + sizeof(int2) + sizeof(int) : sizeof(Elem) + sizeof(double);
 }
 
 // Arrays of a mixed-path list inside a wave's scratch slot.
 struct MList {
-    float4 *R;   // x, y, z relative to p1 (FP32), s = |r|^2 (FP32)
-    int2 *KF;    // exact bin of |p1-q|^2, sqrtf(float(|p1-q|^2)) bits
-    double4 *D;  // image position (FP64) and weight
+    float4 *R;  // x, y, z relative to p1 (FP32), s = |r|^2 (FP32)
+    int2 *KF;   // exact bin of |p1-q|^2 | neighbour entry << 8, sqrtf(float(|p1-q|^2)) bits
+    int *I;     // index of the point in the cell-sorted catalogue
 };
 __device__ __forceinline__ MList mlist(const char *base, int cap) {
     MList m;
     m.R = reinterpret_cast<float4 *>(const_cast<char *>(base));
     m.KF = reinterpret_cast<int2 *>(const_cast<char *>(base) + (size_t)cap * sizeof(float4));
-    m.D = reinterpret_cast<double4 *>(const_cast<char *>(base) + (size_t)cap * (sizeof(float4) + sizeof(int2)));
+    m.I = reinterpret_cast<int *>(const_cast<char *>(base) + (size_t)cap * (sizeof(float4) + sizeof(int2)));
     return m;
 }
+
+// Catalogue arrays behind a mixed list (for weights and exact rechecks).
+struct CatRef {
+    const double *x, *y, *z, *w;
+};
 
 // Compact the neighbours of primary (px,py,pz) in catalogue (X,Y,Z,Wt,st)
 // whose squared separation lies in (r2min, r2max) into the list at `base`.
@@ -298,8 +306,8 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                     const MList m = mlist(base, A.cap);
                     const float rx = (float)(qx - px), ry = (float)(qy - py), rz = (float)(qz - pz);
                     m.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
-                    m.KF[pos] = make_int2(k, __float_as_int(f));
-                    m.D[pos] = make_double4(qx, qy, qz, qw);
+                    m.KF[pos] = make_int2(k | (s << 8), __float_as_int(f));
+                    m.I[pos] = j;
                 } else {
                     Elem e;
                     e.x = qx;
@@ -533,19 +541,25 @@ __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, con
 }
 
 // Mixed path, deferred exact rechecks: every queued (a, b) pair is evaluated
-// with the FP64 image positions, one pair per lane.
+// with the FP64 image positions (catalogue point + periodic shift of its
+// neighbour entry, rebuilt as in build_list), one pair per lane.
 template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
-                            const uint2 *q, int qn, int lane) {
+                            const CatRef &c2, const CatRef &c3, int cx, int cy, int cz, const uint2 *q, int qn,
+                            int lane) {
     const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
     wave_sync();
     for (int i = lane; i < qn; i += kWave) {
         const uint2 e = q[i];
-        const double4 pa = m2.D[e.x], pb = m3.D[e.y];
         const int2 ka = m2.KF[e.x], kb = m3.KF[e.y];
-        const double wa = WEIGHTED ? pa.w : 0.0, wb = WEIGHTED ? pb.w : 0.0;
-        exact_pair<SYM, WEIGHTED, FAST>(A, t, pa.x, pa.y, pa.z, ka.x, __int_as_float(ka.y), w1 * wa, wa, pb.x, pb.y,
-                                        pb.z, kb.x, __int_as_float(kb.y), w1 * wb, wb, true);
+        const int ja = m2.I[e.x], jb = m3.I[e.y];
+        double sax, say, saz, sbx, sby, sbz;
+        neighbour(A, cx, cy, cz, ka.x >> 8, sax, say, saz);
+        neighbour(A, cx, cy, cz, kb.x >> 8, sbx, sby, sbz);
+        const double wa = WEIGHTED ? c2.w[ja] : 0.0, wb = WEIGHTED ? c3.w[jb] : 0.0;
+        exact_pair<SYM, WEIGHTED, FAST>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
+                                        __int_as_float(ka.y), w1 * wa, wa, c3.x[jb] + sbx, c3.y[jb] + sby,
+                                        c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true);
     }
     wave_sync();
 }
@@ -553,7 +567,9 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
 // Per-wave state of the mixed pair loop.
 struct MixedCtx {
     const char *l2, *l3;
-    uint2 *q;  // deferred rechecks (LDS)
+    CatRef cat2, cat3;
+    int cx, cy, cz;  // primary cell (neighbour-entry shifts)
+    uint2 *q;        // deferred rechecks (LDS)
     int qn;
     float s32, jmax, g32, c1, c2;
     unsigned nb;
@@ -612,14 +628,16 @@ __device__ __forceinline__ void mixed_group(const KArgs &A, const LdsTabs &t, Mi
     }
     // one group adds at most U*64 entries: drain once the queue could overflow
     if (mc.qn > kQueueCap - U * kWave) {
-        drain_queue<SYM, WEIGHTED, FAST>(A, t, mc.w1, mc.l2, mc.l3, mc.q, mc.qn, lane);
+        drain_queue<SYM, WEIGHTED, FAST>(A, t, mc.w1, mc.l2, mc.l3, mc.cat2, mc.cat3, mc.cx, mc.cy, mc.cz, mc.q,
+                                         mc.qn, lane);
         mc.qn = 0;
     }
 }
 
 template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
-                                int n3, char *wlds, int lane) {
+                                int n3, char *wlds, int lane, const CatRef &c2, const CatRef &c3, int cx, int cy,
+                                int cz) {
     constexpr int U = 4;
     const int C = A.chunk_elems;
     const int Cp = C + 4;
@@ -630,6 +648,11 @@ __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, con
     MixedCtx mc;
     mc.l2 = l2;
     mc.l3 = l3;
+    mc.cat2 = c2;
+    mc.cat3 = c3;
+    mc.cx = cx;
+    mc.cy = cy;
+    mc.cz = cz;
     mc.q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
     mc.qn = 0;
     mc.s32 = A.lut_scale32;
@@ -648,8 +671,8 @@ __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, con
             double w = 0.0;
             if (i < a1 - a0) {
                 r = m2.R[a0 + i];
-                k = m2.KF[a0 + i].x;
-                if (WEIGHTED) w = m2.D[a0 + i].w;
+                k = m2.KF[a0 + i].x & 0xff;
+                if (WEIGHTED) w = c2.w[m2.I[a0 + i]];
             }
             la[i] = r;
             lk[i] = k;
@@ -664,8 +687,8 @@ __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, con
             double wb = 0.0;
             if (vb) {
                 rb = m3.R[b];
-                kb = m3.KF[b].x;
-                if (WEIGHTED) wb = m3.D[b].w;
+                kb = m3.KF[b].x & 0xff;
+                if (WEIGHTED) wb = c3.w[m3.I[b]];
             }
             const double w1b = w1 * wb;
             const int rect_end = SYM ? min(b0, a1) : a1;
@@ -709,7 +732,8 @@ __device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, con
             }
         }
     }
-    if (mc.qn > 0) drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, mc.q, mc.qn, lane);
+    if (mc.qn > 0)
+        drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, mc.cat2, mc.cat3, mc.cx, mc.cy, mc.cz, mc.q, mc.qn, lane);
 }
 
 // Merge the workgroup's LDS histogram into the term's global compact
@@ -732,9 +756,10 @@ __device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long
 
 template <bool MIXED, bool FAST, bool SYM, bool WEIGHTED>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
-                                          const char *l3, int n3, char *wlds, int lane) {
+                                          const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
+                                          const CatRef &c3, int cx, int cy, int cz) {
     if constexpr (MIXED)
-        pair_loop_mixed<SYM, WEIGHTED, FAST>(A, t, w1, l2, n2, l3, n3, wlds, lane);
+        pair_loop_mixed<SYM, WEIGHTED, FAST>(A, t, w1, l2, n2, l3, n3, wlds, lane, c2, c3, cx, cy, cz);
     else
         pair_loop_plain<SYM, WEIGHTED, FAST>(A, t, w1, l2, n2, l3, n3, wlds, lane);
 }
@@ -825,16 +850,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (n2 == 0 || n3 == 0) continue;
             wave_sync();
             const char *lb = T.sym ? l2 : l3;
+            const CatRef c2 = {T.x2, T.y2, T.z2, T.w2};
+            const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3};
             if (T.sym) {
                 if (T.weighted)
-                    pair_loop<MIXED, FAST, true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    pair_loop<MIXED, FAST, true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
                 else
-                    pair_loop<MIXED, FAST, true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    pair_loop<MIXED, FAST, true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
             } else {
                 if (T.weighted)
-                    pair_loop<MIXED, FAST, false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    pair_loop<MIXED, FAST, false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
                 else
-                    pair_loop<MIXED, FAST, false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane);
+                    pair_loop<MIXED, FAST, false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
             }
             wave_sync();
         }
@@ -938,6 +965,8 @@ struct nc_context {
     int chunk = 64;
     int wgs_per_cu = 0;
     int chunk_elems = 128;
+    int sample_stride = 1;  // autotuner: queue only every n-th work item
+    std::string dev_name;
 };
 
 namespace {
@@ -1383,6 +1412,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     std::vector<TermDesc> td(nterms);
     std::vector<Chunk> chunks;
     std::vector<int> weighted(nterms);
+    long long nitems = 0;
     for (int i = 0; i < nterms; ++i) {
         const Term &t = terms[i];
         TermDesc &d = td[i];
@@ -1406,7 +1436,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         d.pad = 0;
         weighted[i] = d.weighted;
         if (t.c2->n == 0 || t.c3->n == 0) continue;
-        for (int b = 0; b < d.n1; b += ctx->chunk) chunks.push_back({i, b, std::min(d.n1, b + ctx->chunk), 0});
+        for (int b = 0; b < d.n1; b += ctx->chunk, ++nitems)
+            if (nitems % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(d.n1, b + ctx->chunk), 0});
     }
     NC_HIP(ctx, dgrow(ctx->terms, sizeof(TermDesc) * td.size()));
     NC_HIP(ctx, dgrow(ctx->chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size())));
@@ -1551,6 +1582,7 @@ int nc_create(nc_context **out, int device) {
         return NC_ERR_HIP;
     }
     ctx->ncu = prop.multiProcessorCount;
+    ctx->dev_name = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
     ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
     for (auto &e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) {
@@ -1735,4 +1767,85 @@ int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *
     return NC_OK;
 }
 
+int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1, nc_catalog *const *c2,
+                nc_catalog *const *c3, int path, nc_tuning *best) {
+    if (!ctx) return NC_ERR_ARG;
+    if (nterms < 1 || !c1 || !c2 || !c3 || !params || !best) return fail(ctx, NC_ERR_ARG, "bad autotune arguments");
+    std::vector<Term> t(nterms);
+    int64_t npts = 0, nprim = 0;
+    for (int i = 0; i < nterms; ++i) {
+        t[i] = {c1[i], c2[i], c3[i]};
+        if (!c1[i] || !c2[i] || !c3[i]) return fail(ctx, NC_ERR_ARG, "null catalogue");
+        npts += c1[i]->n + c2[i]->n + c3[i]->n;
+        nprim += c1[i]->n;
+    }
+    // cache key: device, path, survey, bins, sizes
+    char key[256];
+    std::snprintf(key, sizeof(key), "%s path=%d survey=%d nb=%d terms=%d points=%lld", ctx->dev_name.c_str(), path,
+                  params->survey ? 1 : 0, params->nedges - 1, nterms, (long long)npts);
+    const char *cache = std::getenv("NC_TUNE_CACHE");
+    if (cache && *cache) {
+        if (FILE *f = std::fopen(cache, "r")) {
+            char line[512];
+            while (std::fgets(line, sizeof(line), f)) {
+                const size_t kl = std::strlen(key);
+                int w, c, pc, lc;
+                float ms;
+                if (std::strncmp(line, key, kl) == 0 && line[kl] == '|' &&
+                    std::sscanf(line + kl + 1, "%d %d %d %d %f", &w, &c, &pc, &lc, &ms) == 5) {
+                    std::fclose(f);
+                    ctx->waves_per_group = w;
+                    ctx->chunk = c;
+                    ctx->wgs_per_cu = pc;
+                    ctx->chunk_elems = lc;
+                    *best = {w, c, pc, lc, ms, 1};
+                    return NC_OK;
+                }
+            }
+            std::fclose(f);
+        }
+    }
+    const int saved[4] = {ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
+    std::vector<double> hist((size_t)nterms * nc_hist_size(params->nedges));
+    // sample: about 64 work items per workgroup slot of the chip at the largest chunk
+    const long long items = std::max<long long>(1, nprim / 16);
+    ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
+    nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
+    int rc = NC_OK;
+    for (int w : {4, 8})
+        for (int c : {16, 64})
+            for (int lc : {64, 128, 256}) {
+                ctx->waves_per_group = w;
+                ctx->chunk = c;
+                ctx->wgs_per_cu = 0;
+                ctx->chunk_elems = lc;
+                rc = run_count(ctx, params, t, path, hist.data(), nullptr, false);
+                if (rc) break;
+                const float ms = ctx->timing.ms_count;
+                if (ms < b.ms) b = {w, c, 0, lc, ms, 0};
+            }
+    ctx->sample_stride = 1;
+    plan_drop(ctx);
+    if (rc) {
+        ctx->waves_per_group = saved[0];
+        ctx->chunk = saved[1];
+        ctx->wgs_per_cu = saved[2];
+        ctx->chunk_elems = saved[3];
+        return rc;
+    }
+    ctx->waves_per_group = b.waves_per_group;
+    ctx->chunk = b.chunk;
+    ctx->wgs_per_cu = b.wgs_per_cu;
+    ctx->chunk_elems = b.list_chunk;
+    *best = b;
+    if (cache && *cache)
+        if (FILE *f = std::fopen(cache, "a")) {
+            std::fprintf(f, "%s|%d %d %d %d %.4f\n", key, b.waves_per_group, b.chunk, b.wgs_per_cu, b.list_chunk,
+                         b.ms);
+            std::fclose(f);
+        }
+    return NC_OK;
+}
+
 }  // extern "C"
+

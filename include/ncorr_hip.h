@@ -131,6 +131,21 @@ int nc_last_timing(const nc_context *ctx, nc_timing *out);
 int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu, int list_chunk);
 int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu, int *list_chunk);
 
+/* Autotuner (no counterpart in the reference): times the triplet kernel on a
+ * strided sample of the work items of these terms for every candidate of
+ * waves_per_group {4, 8} x chunk {16, 64} x list_chunk {64, 128, 256} (wgs_per_cu
+ * = as many as fit), keeps the fastest in ctx and returns it in *best (ms =
+ * device time of the sample).  With the environment variable NC_TUNE_CACHE
+ * naming a file, results are looked up in and appended to that file, keyed by
+ * device, path, survey flag, bin count and catalogue sizes. */
+typedef struct {
+    int waves_per_group, chunk, wgs_per_cu, list_chunk;
+    float ms;
+    int from_cache;
+} nc_tuning;
+int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1, nc_catalog *const *c2,
+                nc_catalog *const *c3, int path, nc_tuning *best);
+
 #ifdef __cplusplus
 }
 #endif

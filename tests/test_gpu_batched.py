@@ -77,3 +77,24 @@ def test_batched_survey_four_terms(ctx):
         _sync()
     D.close()
     R.close()
+
+
+def test_autotune_keeps_results(ctx, tmp_path, monkeypatch):
+    print("autotune", flush=True)
+    monkeypatch.setenv("NC_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    L = 250.0
+    edges = edges_linear(0.0, 25.0, 10)
+    box = [L, L, L]
+    cats = [ctx.catalog(neyman_scott_box(n, 30, 4.0, L, seed=7 + n)) for n in (200, 500)]
+    terms = [(c, c, c) for c in cats]
+    before = ctx.get_tuning()
+    _, want = ctx.count_batched(terms, edges, box, path=NC_PATH_MIXED)
+    best = ctx.autotune(terms, edges, box, path=NC_PATH_MIXED)
+    assert best["waves_per_group"] in (4, 8) and best["list_chunk"] in (64, 128, 256) and best["from_cache"] == 0
+    _, got = ctx.count_batched(terms, edges, box, path=NC_PATH_MIXED)
+    assert np.array_equal(got, want)
+    again = ctx.autotune(terms, edges, box, path=NC_PATH_MIXED)
+    assert again["from_cache"] == 1 and again["waves_per_group"] == best["waves_per_group"]
+    ctx.set_tuning(*before)
+    for c in cats:
+        c.close()
