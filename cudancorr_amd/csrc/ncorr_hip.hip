@@ -398,87 +398,108 @@ __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, dou
 }
 
 // ---------------------------------------------------------------------------
-// Pair traversal.
+// Pair traversal (shared by both paths).
 //
-// The pairs of S2 x S3 are visited in groups of U pairs per lane.  Two orders:
-//  * rectangular: entry a of S2 broadcast from the wave's LDS chunk, entry b
-//    of S3 held per lane (all pairs of an a-range and a 64-entry b-block);
-//  * circulant (SYM only, the 64-entry diagonal blocks): lane l of a block of
-//    m entries pairs with entry (l + t) mod m, t = 1 .. m/2, gathered from the
-//    LDS chunk -- every unordered pair of the block exactly once, no lane
-//    idle except for the half step t = m/2 of an even block.
-// When p2 and p3 are the same catalogue (SYM) the a-ranges of a b-block are
-// the entries before it (rectangular) and the block itself (circulant), so
-// every unordered pair is visited once and counted twice.
-
-// Plain path, one group of U pairs: a-entries ea[u] (broadcast or gathered),
-// b-entry eb per lane.  The triangle test uses the hardware sqrt (<= 1 ulp)
-// and redoes it with the correctly rounded sqrtf when the slack is within
-// 8 ulp of the largest side.
-template <int U, bool SYM, bool WEIGHTED, bool FAST>
-__device__ __forceinline__ void plain_group(const KArgs &A, const LdsTabs &t, const Elem (&ea)[U],
-                                            const double (&wa)[U], const bool (&ok)[U], const Elem &eb, double wb,
-                                            double w1, double w1b) {
-    const double r2min = A.r2min, r2max = A.r2max, scale = A.lut_scale;
-    const float near_rel = 8.0f * 1.1920929e-7f;
-    double d[U];
-    bool inr[U];
-    int k[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        d[u] = sqrsep(ea[u].x - eb.x, ea[u].y - eb.y, ea[u].z - eb.z);
-        inr[u] = ok[u] && d[u] > r2min && d[u] < r2max;
-        const double dj = inr[u] ? d[u] : 0.0;
-        k[u] = t.lut[(int)(dj * scale)];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        if constexpr (FAST) {
-            k[u] += d[u] >= t.sq[k[u] + 1] ? 1 : 0;
-        } else {
-            while (k[u] < A.nb - 1 && d[u] >= t.sq[k[u] + 1]) ++k[u];
+// Lanes hold one entry each of a 64-entry block of one list; "rows" are
+// entries staged in the wave's LDS chunk, broadcast to all lanes
+// (rectangular passes) or gathered per lane (circulant passes: lane l of a
+// block of m entries pairs with entry (l + s) mod m, s = 1 .. m/2, so every
+// unordered pair of the block is visited once with no lane idle except in the
+// half step s = m/2 of an even block).
+//   SYM (p2 and p3 the same catalogue, one list of n entries):
+//     full region F = the first 64*floor(n/64) entries, tail T = the rest;
+//     (1) chunks of F as rows x full blocks of F as lanes, pairs a < b: the
+//         rows before the block (rectangular) and the block itself
+//         (circulant);
+//     (2) T as rows x the full blocks as lanes (rectangular), then T's own
+//         pairs (circulant).
+//   ASYM (S2 x S3): chunks of S2 as rows x full blocks of S3 as lanes; then
+//     the S3 tail as rows x blocks of S2 as lanes (roles swapped, so only the
+//     last S2 block has idle lanes).
+// Every pair (SYM: unordered pair, counted twice) is visited exactly once.
+template <class Pol>
+__device__ __forceinline__ void traverse(Pol &P, int n2, int n3) {
+    const int C = P.C;
+    if constexpr (Pol::kSym) {
+        // row segments: chunks of F, then T; lane blocks: the full blocks
+        // from the segment on (F) or all of them and T itself (T)
+        const int n = n2;
+        const int nf = n & ~(kWave - 1);
+        for (int a0 = 0, a1; a0 < n; a0 = a1) {
+            a1 = a0 < nf ? min(a0 + C, nf) : n;
+            P.stage(false, a0, a1);
+            const int bend = a0 < nf ? nf : n;
+            for (int b0 = a0 < nf ? a0 : 0; b0 < bend; b0 += kWave) {
+                P.load_lanes(true, b0, n);
+                P.template rect<false>(a0, b0 < a0 ? a1 : min(b0, a1));
+                if (b0 >= a0 && b0 < a1) P.circ(b0, min(kWave, n - b0));
+            }
+        }
+    } else {
+        const int nf3 = n3 & ~(kWave - 1);
+        if (nf3 > 0)
+            for (int a0 = 0; a0 < n2; a0 += C) {
+                const int a1 = min(a0 + C, n2);
+                P.stage(false, a0, a1);
+                for (int b0 = 0; b0 < nf3; b0 += kWave) {
+                    P.load_lanes(true, b0, nf3);
+                    P.template rect<false>(a0, a1);
+                }
+            }
+        if (nf3 < n3) {
+            P.stage(true, nf3, n3);
+            for (int b0 = 0; b0 < n2; b0 += kWave) {
+                P.load_lanes(false, b0, n2);
+                P.template rect<true>(nf3, n3);
+            }
         }
     }
-    bool acc[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const float fd = (float)d[u];
-        const Sides s = sort3(ea[u].f, eb.f, __builtin_amdgcn_sqrtf(fd));
-        const float sum = s.lo + s.mid;
-        const bool near = fabsf(sum - s.hi) <= near_rel * s.hi || fd < 1e-30f;
-        acc[u] = inr[u] && !near && sum > s.hi;
-        if (inr[u] && near) {  // rare: redo the test with the correctly rounded sqrtf
-            const Sides e = sort3(ea[u].f, eb.f, __builtin_sqrtf(fd));
-            acc[u] = e.lo + e.mid > e.hi;
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-        if (acc[u])
-            hist_add<SYM, WEIGHTED>(t, sorted_slot<FAST>(A, t, ea[u].k, eb.k, k[u]), w1 * wa[u], wa[u], w1b, wb);
 }
 
+// Plain path (FP64 everywhere).  Rows: Elem + weight in LDS; lanes: Elem +
+// weight in registers.  Per group of U pairs: separations, bin lookups, the
+// FP32 triangle test with the hardware sqrt (<= 1 ulp) -- redone with the
+// correctly rounded sqrtf when the slack is within 8 ulp of the largest side
+// -- and the histogram updates.
 template <bool SYM, bool WEIGHTED, bool FAST>
-__device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
-                                int n3, char *wlds, int lane) {
-    constexpr int U = 2;
-    const int C = A.chunk_elems;
-    const int Cp = C + 4;
-    const Elem *L2 = reinterpret_cast<const Elem *>(l2);
-    const double *W2 = reinterpret_cast<const double *>(l2 + (size_t)A.cap * sizeof(Elem));
-    const Elem *L3 = reinterpret_cast<const Elem *>(l3);
-    const double *W3 = reinterpret_cast<const double *>(l3 + (size_t)A.cap * sizeof(Elem));
-    Elem *la = reinterpret_cast<Elem *>(wlds);
-    double *lw = reinterpret_cast<double *>(wlds + (size_t)Cp * sizeof(Elem));
-    for (int a0 = 0; a0 < n2; a0 += C) {
-        const int a1 = min(a0 + C, n2);
+struct PlainPol {
+    static constexpr bool kSym = SYM;
+    static constexpr int U = 2;
+    const KArgs &A;
+    const LdsTabs &t;
+    const Elem *L2, *L3;
+    const double *W2, *W3;
+    Elem *la;
+    double *lw;
+    double w1;
+    int C, lane, row_first;
+    Elem eb;
+    double wb, w1b;
+    bool vb;
+
+    __device__ PlainPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2, const char *l3, char *wlds,
+                        int lane_)
+        : A(A_), t(t_), w1(w1_), lane(lane_) {
+        C = A.chunk_elems;
+        L2 = reinterpret_cast<const Elem *>(l2);
+        W2 = reinterpret_cast<const double *>(l2 + (size_t)A.cap * sizeof(Elem));
+        L3 = reinterpret_cast<const Elem *>(l3);
+        W3 = reinterpret_cast<const double *>(l3 + (size_t)A.cap * sizeof(Elem));
+        la = reinterpret_cast<Elem *>(wlds);
+        lw = reinterpret_cast<double *>(wlds + (size_t)(C + 4) * sizeof(Elem));
+        row_first = 0;
+    }
+
+    __device__ void stage(bool from3, int first, int last) {
+        const Elem *L = from3 ? L3 : L2;
+        const double *W = from3 ? W3 : W2;
         wave_sync();
-        for (int i = lane; i < a1 - a0 + 4; i += kWave) {
+        for (int i = lane; i < last - first + 4; i += kWave) {
             Elem e;
             double w = 0.0;
-            if (i < a1 - a0) {
-                e = L2[a0 + i];
-                if (WEIGHTED) w = W2[a0 + i];
+            if (i < last - first) {
+                e = L[first + i];
+                if (WEIGHTED) w = W[first + i];
             } else {  // padding: far away, never in range
                 e.x = e.y = e.z = 1e150;
                 e.k = 0;
@@ -487,62 +508,113 @@ __device__ void pair_loop_plain(const KArgs &A, const LdsTabs &t, double w1, con
             la[i] = e;
             if (WEIGHTED) lw[i] = w;
         }
+        row_first = first;
         wave_sync();
-        for (int b0 = SYM ? a0 : 0; b0 < n3; b0 += kWave) {
-            const int b = b0 + lane;
-            const bool vb = b < n3;
-            Elem eb;
-            double wb = 0.0;
-            if (vb) {
-                eb = L3[b];
-                if (WEIGHTED) wb = W3[b];
+    }
+
+    __device__ void load_lanes(bool from3, int b0, int end) {
+        const int b = b0 + lane;
+        vb = b < end;
+        wb = 0.0;
+        if (vb) {
+            eb = (from3 ? L3 : L2)[b];
+            if (WEIGHTED) wb = (from3 ? W3 : W2)[b];
+        } else {
+            eb.x = eb.y = eb.z = -1e150;
+            eb.k = 0;
+            eb.f = 0.0f;
+        }
+        w1b = w1 * wb;
+    }
+
+    // U pairs: rows ea[u] (weights wa[u]) with the lane entry.  SWAP: the rows
+    // are the p3 entries, so the weight product is (w1*w_lane)*w_row.
+    template <bool SWAP>
+    __device__ __forceinline__ void group(const Elem (&ea)[U], const double (&wa)[U], const bool (&ok)[U]) {
+        const double r2min = A.r2min, r2max = A.r2max, scale = A.lut_scale;
+        const float near_rel = 8.0f * 1.1920929e-7f;
+        double d[U];
+        bool inr[U];
+        int k[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            d[u] = sqrsep(ea[u].x - eb.x, ea[u].y - eb.y, ea[u].z - eb.z);
+            inr[u] = ok[u] && d[u] > r2min && d[u] < r2max;
+            const double dj = inr[u] ? d[u] : 0.0;
+            k[u] = t.lut[(int)(dj * scale)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if constexpr (FAST) {
+                k[u] += d[u] >= t.sq[k[u] + 1] ? 1 : 0;
             } else {
-                eb.x = eb.y = eb.z = -1e150;
-                eb.k = 0;
-                eb.f = 0.0f;
-            }
-            const double w1b = w1 * wb;
-            // rectangular rows: every a of [a0, rect_end) with every b of the block
-            const int rect_end = SYM ? min(b0, a1) : a1;
-            for (int a = a0; a < rect_end; a += U) {
-                Elem ea[U];
-                double wa[U];
-                bool ok[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    ea[u] = la[a - a0 + u];
-                    wa[u] = WEIGHTED ? lw[a - a0 + u] : 0.0;
-                    ok[u] = vb && a + u < rect_end;
-                }
-                plain_group<U, SYM, WEIGHTED, FAST>(A, t, ea, wa, ok, eb, wb, w1, w1b);
-            }
-            if (SYM && b0 < a1) {  // diagonal block: circulant over its m entries
-                const int m = min(kWave, n3 - b0);
-                const int tmax = m / 2;
-                for (int s = 1; s <= tmax; s += U) {
-                    Elem ea[U];
-                    double wa[U];
-                    bool ok[U];
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int st = s + u;
-                        int off = lane + st;
-                        off -= off >= m ? m : 0;
-                        ok[u] = vb && st <= tmax && (2 * st != m || lane < st);
-                        const int li = b0 - a0 + (ok[u] ? off : 0);
-                        ea[u] = la[li];
-                        wa[u] = WEIGHTED ? lw[li] : 0.0;
-                    }
-                    plain_group<U, SYM, WEIGHTED, FAST>(A, t, ea, wa, ok, eb, wb, w1, w1b);
-                }
+                while (k[u] < A.nb - 1 && d[u] >= t.sq[k[u] + 1]) ++k[u];
             }
         }
+        bool acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float fd = (float)d[u];
+            const Sides s = sort3(ea[u].f, eb.f, __builtin_amdgcn_sqrtf(fd));
+            const float sum = s.lo + s.mid;
+            const bool near = fabsf(sum - s.hi) <= near_rel * s.hi || fd < 1e-30f;
+            acc[u] = inr[u] && !near && sum > s.hi;
+            if (inr[u] && near) {  // rare: redo the test with the correctly rounded sqrtf
+                const Sides e = sort3(ea[u].f, eb.f, __builtin_sqrtf(fd));
+                acc[u] = e.lo + e.mid > e.hi;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (acc[u]) {
+                const int slot = sorted_slot<FAST>(A, t, ea[u].k, eb.k, k[u]);
+                if (SWAP)
+                    hist_add<SYM, WEIGHTED>(t, slot, w1b, wb, w1 * wa[u], wa[u]);
+                else
+                    hist_add<SYM, WEIGHTED>(t, slot, w1 * wa[u], wa[u], w1b, wb);
+            }
     }
-}
 
-// Mixed path, deferred exact rechecks: every queued (a, b) pair is evaluated
-// with the FP64 image positions (catalogue point + periodic shift of its
-// neighbour entry, rebuilt as in build_list), one pair per lane.
+    template <bool SWAP>
+    __device__ void rect(int r0, int r1) {
+        for (int a = r0; a < r1; a += U) {
+            Elem ea[U];
+            double wa[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ea[u] = la[a - row_first + u];
+                wa[u] = WEIGHTED ? lw[a - row_first + u] : 0.0;
+                ok[u] = vb && a + u < r1;
+            }
+            group<SWAP>(ea, wa, ok);
+        }
+    }
+
+    __device__ void circ(int b0, int m) {
+        const int smax = m / 2;
+        for (int s = 1; s <= smax; s += U) {
+            Elem ea[U];
+            double wa[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int st = s + u;
+                int off = lane + st;
+                off -= off >= m ? m : 0;
+                ok[u] = vb && st <= smax && (2 * st != m || lane < st);
+                const int li = b0 - row_first + (ok[u] ? off : 0);
+                ea[u] = la[li];
+                wa[u] = WEIGHTED ? lw[li] : 0.0;
+            }
+            group<false>(ea, wa, ok);
+        }
+    }
+};
+
+// Mixed path, deferred exact rechecks: every queued (S2 entry, S3 entry) pair
+// is evaluated with the FP64 image positions (catalogue point + periodic
+// shift of its neighbour entry, rebuilt as in build_list), one pair per lane.
 template <bool SYM, bool WEIGHTED, bool FAST>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
                             const CatRef &c2, const CatRef &c3, int cx, int cy, int cz, const uint2 *q, int qn,
@@ -564,177 +636,194 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
     wave_sync();
 }
 
-// Per-wave state of the mixed pair loop.
-struct MixedCtx {
-    const char *l2, *l3;
-    CatRef cat2, cat3;
-    int cx, cy, cz;  // primary cell (neighbour-entry shifts)
-    uint2 *q;        // deferred rechecks (LDS)
-    int qn;
-    float s32, jmax, g32, c1, c2;
-    unsigned nb;
-    double w1;
-};
-
-// Mixed path, one group of U pairs.  With r = q - p1 re-centred in FP32 and
-// s = |r|^2:  r23^2 = s_a + s_b - 2 r_a.r_b, and the triangle (p1, a, b) is
-// degenerate only when r_a and r_b are nearly collinear, i.e. when
-// s_a*s_b - (r_a.r_b)^2 is small.  Per pair: one dot product, one lookup over
-// r23^2*scale32 that returns the bin below the (at most one) edge near that
-// value and the edge itself, and the collinearity bound.  A pair whose r23^2
-// lies within the guard band g32 of that edge, or whose collinearity measure
-// is within the error bound (make_mixed_tables), is queued for the exact FP64
-// evaluation.  Phases: lookups of all U pairs, then arithmetic, histogram
-// updates, queue pushes.
-template <int U, bool SYM, bool WEIGHTED, bool FAST>
-__device__ __forceinline__ void mixed_group(const KArgs &A, const LdsTabs &t, MixedCtx &mc, const float4 (&ra)[U],
-                                            const int (&ka)[U], const double (&wa)[U], const int (&ai)[U],
-                                            const bool (&ok)[U], const float4 &rb, int kb, double wb, double w1b,
-                                            int b, int lane) {
-    float d[U], dot[U], ss[U];
-    uint2 e[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
-        ss[u] = ra[u].w + rb.w;
-        d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
-        e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u] * mc.s32, 0.0f, mc.jmax)];
-    }
-    bool acc[U], amb[U];
-    int slot[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const float E = __uint_as_float(e[u].y);
-        const int k = (int)e[u].x + (d[u] > E ? 1 : 0);
-        const bool amb_b = fabsf(d[u] - E) <= mc.g32;
-        const bool valid = (unsigned)k < mc.nb;
-        const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
-        const bool amb_t = delta <= __builtin_fmaf(mc.c1, ss[u], mc.c2);
-        amb[u] = ok[u] && (amb_b || (valid && amb_t));
-        acc[u] = ok[u] && valid && !amb_b && !amb_t;
-        slot[u] = sorted_slot<FAST>(A, t, ka[u], kb, k);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-        if (acc[u]) hist_add<SYM, WEIGHTED>(t, slot[u], mc.w1 * wa[u], wa[u], w1b, wb);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const unsigned long long m = __ballot(amb[u]);
-        if (m) {
-            const int pos = mc.qn + lane_rank(m);
-            if (amb[u]) mc.q[pos] = make_uint2((unsigned)ai[u], (unsigned)b);
-            mc.qn += __popcll(m);
-        }
-    }
-    // one group adds at most U*64 entries: drain once the queue could overflow
-    if (mc.qn > kQueueCap - U * kWave) {
-        drain_queue<SYM, WEIGHTED, FAST>(A, t, mc.w1, mc.l2, mc.l3, mc.cat2, mc.cat3, mc.cx, mc.cy, mc.cz, mc.q,
-                                         mc.qn, lane);
-        mc.qn = 0;
-    }
-}
-
+// Mixed path.  With r = q - p1 re-centred in FP32 and s = |r|^2:
+// r23^2 = s_a + s_b - 2 r_a.r_b, and the triangle (p1, a, b) is degenerate
+// only when r_a and r_b are nearly collinear, i.e. when s_a*s_b - (r_a.r_b)^2
+// is small.  Per pair: one dot product, one lookup over r23^2*scale32 that
+// returns the bin below the (at most one) edge near that value and the edge
+// itself, and the collinearity bound.  A pair whose r23^2 lies within the
+// guard band g32 of that edge, or whose collinearity measure is within the
+// error bound (make_mixed_tables), is queued for the exact FP64 evaluation.
+// Rows: float4 {r, s} + bin + weight in LDS; lanes in registers.
 template <bool SYM, bool WEIGHTED, bool FAST>
-__device__ void pair_loop_mixed(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2, const char *l3,
-                                int n3, char *wlds, int lane, const CatRef &c2, const CatRef &c3, int cx, int cy,
-                                int cz) {
-    constexpr int U = 4;
-    const int C = A.chunk_elems;
-    const int Cp = C + 4;
-    const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
-    float4 *la = reinterpret_cast<float4 *>(wlds);
-    int *lk = reinterpret_cast<int *>(wlds + (size_t)Cp * sizeof(float4));
-    double *lw = reinterpret_cast<double *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int)));
-    MixedCtx mc;
-    mc.l2 = l2;
-    mc.l3 = l3;
-    mc.cat2 = c2;
-    mc.cat3 = c3;
-    mc.cx = cx;
-    mc.cy = cy;
-    mc.cz = cz;
-    mc.q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
-    mc.qn = 0;
-    mc.s32 = A.lut_scale32;
-    mc.jmax = (float)(A.lut32_m + 1);
-    mc.g32 = A.g32;
-    mc.c1 = A.tri_c1;
-    mc.c2 = A.tri_c2;
-    mc.nb = (unsigned)A.nb;
-    mc.w1 = w1;
-    for (int a0 = 0; a0 < n2; a0 += C) {
-        const int a1 = min(a0 + C, n2);
+struct MixedPol {
+    static constexpr bool kSym = SYM;
+    static constexpr int U = 4;
+    const KArgs &A;
+    const LdsTabs &t;
+    MList m2, m3;
+    CatRef c2, c3;
+    const char *l2, *l3;
+    int cx, cy, cz;
+    double w1;
+    int lane;
+    float4 *la;
+    int *lk;
+    double *lw;
+    uint2 *q;
+    int qn, C, row_first;
+    float s32, jmax, g32, c1, c2t;
+    unsigned nb;
+    float4 rb;
+    int kb, bidx;
+    double wb, w1b;
+    bool vb;
+
+    __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
+                        int lane_, const CatRef &cat2, const CatRef &cat3, int cx_, int cy_, int cz_)
+        : A(A_), t(t_), c2(cat2), c3(cat3), l2(l2_), l3(l3_), cx(cx_), cy(cy_), cz(cz_), w1(w1_), lane(lane_) {
+        C = A.chunk_elems;
+        const int Cp = C + 4;
+        m2 = mlist(l2, A.cap);
+        m3 = mlist(l3, A.cap);
+        la = reinterpret_cast<float4 *>(wlds);
+        lk = reinterpret_cast<int *>(wlds + (size_t)Cp * sizeof(float4));
+        lw = reinterpret_cast<double *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int)));
+        q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
+        qn = 0;
+        row_first = 0;
+        s32 = A.lut_scale32;
+        jmax = (float)(A.lut32_m + 1);
+        g32 = A.g32;
+        c1 = A.tri_c1;
+        c2t = A.tri_c2;
+        nb = (unsigned)A.nb;
+    }
+
+    __device__ void stage(bool from3, int first, int last) {
+        const MList &m = from3 ? m3 : m2;
+        const CatRef &c = from3 ? c3 : c2;
         wave_sync();
-        for (int i = lane; i < a1 - a0 + 4; i += kWave) {
+        for (int i = lane; i < last - first + 4; i += kWave) {
             float4 r = make_float4(3e18f, 3e18f, 3e18f, 2.7e37f);  // padding: never in range
             int k = 0;
             double w = 0.0;
-            if (i < a1 - a0) {
-                r = m2.R[a0 + i];
-                k = m2.KF[a0 + i].x & 0xff;
-                if (WEIGHTED) w = c2.w[m2.I[a0 + i]];
+            if (i < last - first) {
+                r = m.R[first + i];
+                k = m.KF[first + i].x & 0xff;
+                if (WEIGHTED) w = c.w[m.I[first + i]];
             }
             la[i] = r;
             lk[i] = k;
             if (WEIGHTED) lw[i] = w;
         }
+        row_first = first;
         wave_sync();
-        for (int b0 = SYM ? a0 : 0; b0 < n3; b0 += kWave) {
-            const int b = b0 + lane;
-            const bool vb = b < n3;
-            float4 rb = make_float4(-3e18f, -3e18f, -3e18f, 2.7e37f);
-            int kb = 0;
-            double wb = 0.0;
-            if (vb) {
-                rb = m3.R[b];
-                kb = m3.KF[b].x & 0xff;
-                if (WEIGHTED) wb = c3.w[m3.I[b]];
-            }
-            const double w1b = w1 * wb;
-            const int rect_end = SYM ? min(b0, a1) : a1;
-            for (int a = a0; a < rect_end; a += U) {
-                float4 ra[U];
-                int ka[U], ai[U];
-                double wa[U];
-                bool ok[U];
+    }
+
+    __device__ void load_lanes(bool from3, int b0, int end) {
+        const MList &m = from3 ? m3 : m2;
+        bidx = b0 + lane;
+        vb = bidx < end;
+        rb = make_float4(-3e18f, -3e18f, -3e18f, 2.7e37f);
+        kb = 0;
+        wb = 0.0;
+        if (vb) {
+            rb = m.R[bidx];
+            kb = m.KF[bidx].x & 0xff;
+            if (WEIGHTED) wb = (from3 ? c3 : c2).w[m.I[bidx]];
+        }
+        w1b = w1 * wb;
+    }
+
+    __device__ void drain() {
+        drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane);
+        qn = 0;
+    }
+
+    // U pairs: rows (ra, ka, wa, list index ai) with the lane entry.  SWAP:
+    // the rows are S3 entries and the lanes S2 entries.
+    template <bool SWAP>
+    __device__ __forceinline__ void group(const float4 (&ra)[U], const int (&ka)[U], const double (&wa)[U],
+                                          const int (&ai)[U], const bool (&ok)[U]) {
+        float d[U], dot[U], ss[U];
+        uint2 e[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    ra[u] = la[a - a0 + u];
-                    ka[u] = lk[a - a0 + u];
-                    wa[u] = WEIGHTED ? lw[a - a0 + u] : 0.0;
-                    ai[u] = a + u;
-                    ok[u] = vb && a + u < rect_end;
-                }
-                mixed_group<U, SYM, WEIGHTED, FAST>(A, t, mc, ra, ka, wa, ai, ok, rb, kb, wb, w1b, b, lane);
-            }
-            if (SYM && b0 < a1) {  // diagonal block: circulant over its m entries
-                const int m = min(kWave, n3 - b0);
-                const int tmax = m / 2;
-                for (int s = 1; s <= tmax; s += U) {
-                    float4 ra[U];
-                    int ka[U], ai[U];
-                    double wa[U];
-                    bool ok[U];
+        for (int u = 0; u < U; ++u) {
+            dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
+            ss[u] = ra[u].w + rb.w;
+            d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
+            e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u] * s32, 0.0f, jmax)];
+        }
+        bool acc[U], amb[U];
+        int slot[U];
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int st = s + u;
-                        int off = lane + st;
-                        off -= off >= m ? m : 0;
-                        ok[u] = vb && st <= tmax && (2 * st != m || lane < st);
-                        const int li = b0 - a0 + (ok[u] ? off : 0);
-                        ra[u] = la[li];
-                        ka[u] = lk[li];
-                        wa[u] = WEIGHTED ? lw[li] : 0.0;
-                        ai[u] = a0 + li;
-                    }
-                    mixed_group<U, SYM, WEIGHTED, FAST>(A, t, mc, ra, ka, wa, ai, ok, rb, kb, wb, w1b, b, lane);
-                }
+        for (int u = 0; u < U; ++u) {
+            const float E = __uint_as_float(e[u].y);
+            const int k = (int)e[u].x + (d[u] > E ? 1 : 0);
+            const bool amb_b = fabsf(d[u] - E) <= g32;
+            const bool valid = (unsigned)k < nb;
+            const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
+            const bool amb_t = delta <= __builtin_fmaf(c1, ss[u], c2t);
+            amb[u] = ok[u] && (amb_b || (valid && amb_t));
+            acc[u] = ok[u] && valid && !amb_b && !amb_t;
+            slot[u] = sorted_slot<FAST>(A, t, ka[u], kb, k);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (acc[u]) {
+                if (SWAP)
+                    hist_add<SYM, WEIGHTED>(t, slot[u], w1b, wb, w1 * wa[u], wa[u]);
+                else
+                    hist_add<SYM, WEIGHTED>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
+            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned long long m = __ballot(amb[u]);
+            if (m) {
+                const int pos = qn + lane_rank(m);
+                if (amb[u])
+                    q[pos] = SWAP ? make_uint2((unsigned)bidx, (unsigned)ai[u])
+                                  : make_uint2((unsigned)ai[u], (unsigned)bidx);
+                qn += __popcll(m);
             }
         }
+        // one group adds at most U*64 entries: drain once the queue could overflow
+        if (qn > kQueueCap - U * kWave) drain();
     }
-    if (mc.qn > 0)
-        drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, mc.cat2, mc.cat3, mc.cx, mc.cy, mc.cz, mc.q, mc.qn, lane);
-}
+
+    template <bool SWAP>
+    __device__ void rect(int r0, int r1) {
+        for (int a = r0; a < r1; a += U) {
+            float4 ra[U];
+            int ka[U], ai[U];
+            double wa[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ra[u] = la[a - row_first + u];
+                ka[u] = lk[a - row_first + u];
+                wa[u] = WEIGHTED ? lw[a - row_first + u] : 0.0;
+                ai[u] = a + u;
+                ok[u] = vb && a + u < r1;
+            }
+            group<SWAP>(ra, ka, wa, ai, ok);
+        }
+    }
+
+    __device__ void circ(int b0, int m) {
+        const int smax = m / 2;
+        for (int s = 1; s <= smax; s += U) {
+            float4 ra[U];
+            int ka[U], ai[U];
+            double wa[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int st = s + u;
+                int off = lane + st;
+                off -= off >= m ? m : 0;
+                ok[u] = vb && st <= smax && (2 * st != m || lane < st);
+                const int li = b0 - row_first + (ok[u] ? off : 0);
+                ra[u] = la[li];
+                ka[u] = lk[li];
+                wa[u] = WEIGHTED ? lw[li] : 0.0;
+                ai[u] = row_first + li;
+            }
+            group<false>(ra, ka, wa, ai, ok);
+        }
+    }
+};
 
 // Merge the workgroup's LDS histogram into the term's global compact
 // histogram and clear it.
@@ -758,10 +847,14 @@ template <bool MIXED, bool FAST, bool SYM, bool WEIGHTED>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
                                           const CatRef &c3, int cx, int cy, int cz) {
-    if constexpr (MIXED)
-        pair_loop_mixed<SYM, WEIGHTED, FAST>(A, t, w1, l2, n2, l3, n3, wlds, lane, c2, c3, cx, cy, cz);
-    else
-        pair_loop_plain<SYM, WEIGHTED, FAST>(A, t, w1, l2, n2, l3, n3, wlds, lane);
+    if constexpr (MIXED) {
+        MixedPol<SYM, WEIGHTED, FAST> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz);
+        traverse(P, n2, n3);
+        if (P.qn > 0) P.drain();
+    } else {
+        PlainPol<SYM, WEIGHTED, FAST> P(A, t, w1, l2, l3, wlds, lane);
+        traverse(P, n2, n3);
+    }
 }
 
 // Persistent triplet kernel.  Workgroups pull (term, primary range) chunks
@@ -1271,7 +1364,6 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     }
     plan_drop(ctx);
     const int nterms = (int)terms.size();
-    const size_t nh = (size_t)G.nb * G.nb * G.nb;
     const bool mixed = path == NC_PATH_MIXED && G.mixed_ok;
     auto al16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
     const int W = ctx->waves_per_group;
