@@ -70,6 +70,18 @@ __device__ __forceinline__ int lane_rank(unsigned long long m) {
 }
 
 // Order LDS / global accesses of the lanes of one wave (wave-private data).
+// Wave-uniform copies of values the compiler cannot prove uniform (loads
+// through generic pointers may read per-lane private memory, so their results
+// count as divergent even at a uniform address).  Keeps the loops that depend
+// on them under scalar control.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -283,7 +295,7 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
         double shx, shy, shz;
         const int cell = neighbour(A, cx, cy, cz, s, shx, shy, shz);
         if (cell < 0) continue;
-        const int beg = (int)st[cell], end = (int)st[cell + 1];
+        const int beg = uniform((int)st[cell]), end = uniform((int)st[cell + 1]);
         for (int j0 = beg; j0 < end; j0 += kWave) {
             const int j = j0 + lane;
             const bool v = j < end;
@@ -304,7 +316,9 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                 const float f = __builtin_sqrtf((float)d);
                 if constexpr (MIXED) {
                     const MList m = mlist(base, A.cap);
-                    const float rx = (float)(qx - px), ry = (float)(qy - py), rz = (float)(qz - pz);
+                    const double sg = A.sigma32;  // power of 2: the scaling is exact
+                    const float rx = (float)((qx - px) * sg), ry = (float)((qy - py) * sg),
+                                rz = (float)((qz - pz) * sg);
                     m.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
                     m.KF[pos] = make_int2(k | (s << 8), __float_as_int(f));
                     m.I[pos] = j;
@@ -325,16 +339,34 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
     return n;
 }
 
-// Sorted histogram slot of bins (ka, kb, k), k in [-1, nb] (slots of k
-// outside [0, nb) are never added).  FAST: one LDS table lookup; otherwise the
-// compact C(nb+2,3) index through ctab.
+// Byte offset in the LDS histogram of the sorted slot of bins (ka, kb, k),
+// k in [-1, nb] (slots of k outside [0, nb) are never added).  FAST: one LDS
+// table lookup; otherwise the compact C(nb+2,3) index through ctab.
 template <bool FAST>
 __device__ __forceinline__ int sorted_slot(const KArgs &A, const LdsTabs &t, int ka, int kb, int k) {
     if constexpr (FAST) return t.t3[__mul24(__mul24(ka, A.nb) + kb, A.nb + 2) + k + 1];
     const int klo = min(min(ka, kb), k);
     const int khi = max(max(ka, kb), k);
     const int kmid = ka + kb + k - klo - khi;
-    return t.ctab[klo * A.nb + kmid] + (khi - kmid);
+    return (t.ctab[klo * A.nb + kmid] + (khi - kmid)) * (int)sizeof(unsigned long long);
+}
+
+// Bins as held by the pair loops: with the slot table, the row entry carries
+// the byte offset 2*(ka*nb*(nb+2) + 1) and the lane entry 2*kb*(nb+2), so the
+// slot offset of (ka, kb, k) is the u16 at byte row + lane + 2k of t3 (the
+// table is symmetric in ka and kb); without it the plain bins.
+template <bool FAST>
+__device__ __forceinline__ int row_bin(const KArgs &A, int k) {
+    return FAST ? 2 * (__mul24(k, A.nb * (A.nb + 2)) + 1) : k;
+}
+template <bool FAST>
+__device__ __forceinline__ int lane_bin(const KArgs &A, int k) {
+    return FAST ? 2 * __mul24(k, A.nb + 2) : k;
+}
+template <bool FAST>
+__device__ __forceinline__ int pair_slot(const KArgs &A, const LdsTabs &t, int kr, int kl, int k) {
+    if constexpr (FAST) return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(t.t3) + kr + kl + 2 * k);
+    return sorted_slot<false>(A, t, kr, kl, k);
 }
 
 // Triangle test inputs are non-negative floats: their bit patterns order as
@@ -351,15 +383,16 @@ __device__ __forceinline__ Sides sort3(float a, float b, float c) {
     return {__uint_as_float(lo), __uint_as_float(mid), __uint_as_float(hi)};
 }
 
-// Histogram update of one accepted pair at a precomputed slot.
+// Histogram update of one accepted pair at a precomputed slot (byte offset).
 template <bool SYM, bool WEIGHTED>
-__device__ __forceinline__ void hist_add(const LdsTabs &t, int idx, double w1a, double wa, double w1b, double wb) {
+__device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1a, double wa, double w1b, double wb) {
+    char *h = reinterpret_cast<char *>(t.hist) + off;
     if constexpr (WEIGHTED) {
         double v = w1a * wb;  // (w1*w2)*w3, device.cu:305
         if (SYM) v += w1b * wa;
-        atomicAdd(reinterpret_cast<double *>(&t.hist[idx]), v);
+        atomicAdd(reinterpret_cast<double *>(h), v);
     } else {
-        atomicAdd(&t.hist[idx], SYM ? 2ull : 1ull);
+        atomicAdd(reinterpret_cast<unsigned long long *>(h), SYM ? 2ull : 1ull);
     }
 }
 
@@ -499,6 +532,7 @@ struct PlainPol {
             double w = 0.0;
             if (i < last - first) {
                 e = L[first + i];
+                e.k = row_bin<FAST>(A, e.k);
                 if (WEIGHTED) w = W[first + i];
             } else {  // padding: far away, never in range
                 e.x = e.y = e.z = 1e150;
@@ -518,6 +552,7 @@ struct PlainPol {
         wb = 0.0;
         if (vb) {
             eb = (from3 ? L3 : L2)[b];
+            eb.k = lane_bin<FAST>(A, eb.k);
             if (WEIGHTED) wb = (from3 ? W3 : W2)[b];
         } else {
             eb.x = eb.y = eb.z = -1e150;
@@ -539,7 +574,7 @@ struct PlainPol {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             d[u] = sqrsep(ea[u].x - eb.x, ea[u].y - eb.y, ea[u].z - eb.z);
-            inr[u] = ok[u] && d[u] > r2min && d[u] < r2max;
+            inr[u] = ok[u] & (d[u] > r2min) & (d[u] < r2max);
             const double dj = inr[u] ? d[u] : 0.0;
             k[u] = t.lut[(int)(dj * scale)];
         }
@@ -557,9 +592,9 @@ struct PlainPol {
             const float fd = (float)d[u];
             const Sides s = sort3(ea[u].f, eb.f, __builtin_amdgcn_sqrtf(fd));
             const float sum = s.lo + s.mid;
-            const bool near = fabsf(sum - s.hi) <= near_rel * s.hi || fd < 1e-30f;
-            acc[u] = inr[u] && !near && sum > s.hi;
-            if (inr[u] && near) {  // rare: redo the test with the correctly rounded sqrtf
+            const bool near = (fabsf(sum - s.hi) <= near_rel * s.hi) | (fd < 1e-30f);
+            acc[u] = inr[u] & !near & (sum > s.hi);
+            if (inr[u] & near) {  // rare: redo the test with the correctly rounded sqrtf
                 const Sides e = sort3(ea[u].f, eb.f, __builtin_sqrtf(fd));
                 acc[u] = e.lo + e.mid > e.hi;
             }
@@ -567,7 +602,7 @@ struct PlainPol {
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
-                const int slot = sorted_slot<FAST>(A, t, ea[u].k, eb.k, k[u]);
+                const int slot = pair_slot<FAST>(A, t, ea[u].k, eb.k, k[u]);
                 if (SWAP)
                     hist_add<SYM, WEIGHTED>(t, slot, w1b, wb, w1 * wa[u], wa[u]);
                 else
@@ -662,7 +697,7 @@ struct MixedPol {
     double *lw;
     uint2 *q;
     int qn, C, row_first;
-    float s32, jmax, g32, c1, c2t;
+    float jmax, g32, c1, c2t;
     unsigned nb;
     float4 rb;
     int kb, bidx;
@@ -682,7 +717,6 @@ struct MixedPol {
         q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
         qn = 0;
         row_first = 0;
-        s32 = A.lut_scale32;
         jmax = (float)(A.lut32_m + 1);
         g32 = A.g32;
         c1 = A.tri_c1;
@@ -700,7 +734,7 @@ struct MixedPol {
             double w = 0.0;
             if (i < last - first) {
                 r = m.R[first + i];
-                k = m.KF[first + i].x & 0xff;
+                k = row_bin<FAST>(A, m.KF[first + i].x & 0xff);
                 if (WEIGHTED) w = c.w[m.I[first + i]];
             }
             la[i] = r;
@@ -720,7 +754,7 @@ struct MixedPol {
         wb = 0.0;
         if (vb) {
             rb = m.R[bidx];
-            kb = m.KF[bidx].x & 0xff;
+            kb = lane_bin<FAST>(A, m.KF[bidx].x & 0xff);
             if (WEIGHTED) wb = (from3 ? c3 : c2).w[m.I[bidx]];
         }
         w1b = w1 * wb;
@@ -743,7 +777,7 @@ struct MixedPol {
             dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
             ss[u] = ra[u].w + rb.w;
             d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
-            e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u] * s32, 0.0f, jmax)];
+            e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u], 0.0f, jmax)];
         }
         bool acc[U], amb[U];
         int slot[U];
@@ -755,9 +789,25 @@ struct MixedPol {
             const bool valid = (unsigned)k < nb;
             const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
             const bool amb_t = delta <= __builtin_fmaf(c1, ss[u], c2t);
-            amb[u] = ok[u] && (amb_b || (valid && amb_t));
-            acc[u] = ok[u] && valid && !amb_b && !amb_t;
-            slot[u] = sorted_slot<FAST>(A, t, ka[u], kb, k);
+            // bitwise, so the predicates stay lane masks in scalar registers
+            amb[u] = ok[u] & (amb_b | (valid & amb_t));
+            acc[u] = ok[u] & valid & !amb_b & !amb_t;
+            slot[u] = pair_slot<FAST>(A, t, ka[u], kb, k);
+        }
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) any = any | amb[u];
+        if (__ballot(any)) {  // rare
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned long long m = __ballot(amb[u]);
+                if (amb[u])
+                    q[qn + lane_rank(m)] = SWAP ? make_uint2((unsigned)bidx, (unsigned)ai[u])
+                                                : make_uint2((unsigned)ai[u], (unsigned)bidx);
+                qn += __popcll(m);
+            }
+            // one group adds at most U*64 entries: drain once the queue could overflow
+            if (qn > kQueueCap - U * kWave) drain();
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -767,19 +817,6 @@ struct MixedPol {
                 else
                     hist_add<SYM, WEIGHTED>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
             }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const unsigned long long m = __ballot(amb[u]);
-            if (m) {
-                const int pos = qn + lane_rank(m);
-                if (amb[u])
-                    q[pos] = SWAP ? make_uint2((unsigned)bidx, (unsigned)ai[u])
-                                  : make_uint2((unsigned)ai[u], (unsigned)bidx);
-                qn += __popcll(m);
-            }
-        }
-        // one group adds at most U*64 entries: drain once the queue could overflow
-        if (qn > kQueueCap - U * kWave) drain();
     }
 
     template <bool SWAP>
@@ -907,7 +944,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             s_next = 0;
         }
         __syncthreads();
-        const int ci = s_chunk;
+        // LDS reads count as divergent to the compiler: make the chunk index
+        // (and everything derived from it) wave-uniform explicitly
+        const int ci = uniform(s_chunk);
         if (ci >= A.nchunks) break;
         const Chunk ch = A.chunks[ci];
         const TermDesc T = A.terms[ch.term];
@@ -917,11 +956,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             cur_weighted = T.weighted;
         }
         for (;;) {
-            int p = 0;
-            if (lane == 0) p = atomicAdd(&s_next, 1);
-            p = __builtin_amdgcn_readfirstlane(p) + ch.begin;
+            // every lane takes part (lane 0 adds 1, the others 0): a branch on
+            // lane == 0 here gets threaded through the loop by the optimiser,
+            // which leaves the whole primary loop under divergent control
+            const int p = uniform(atomicAdd(&s_next, lane == 0 ? 1 : 0)) + ch.begin;
             if (p >= ch.end) break;
-            const double px = T.x1[p], py = T.y1[p], pz = T.z1[p], pw = T.w1[p];
+            const double px = uniform(T.x1[p]), py = uniform(T.y1[p]), pz = uniform(T.z1[p]),
+                         pw = uniform(T.w1[p]);
             bool bad = false;
             const bool clamp = !A.survey;  // device.cu:78-80 vs 84-87
             const int cx = axis_cell(px, A.cs[0], A.nc[0], clamp, bad);
@@ -936,25 +977,27 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             const int n3 = T.sym ? n2
                                  : build_list<MIXED>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3,
                                                      lane, t.sq, t.lut);
-            if (n2 > A.cap || n3 > A.cap) {
+            // wave-uniform lengths keep the pair loops' control in scalar registers
+            const int n2u = uniform(n2), n3u = uniform(n3);
+            if (n2u > A.cap || n3u > A.cap) {
                 if (lane == 0) atomicOr(A.err, kErrOverflow);
                 continue;
             }
-            if (n2 == 0 || n3 == 0) continue;
+            if (n2u == 0 || n3u == 0) continue;
             wave_sync();
             const char *lb = T.sym ? l2 : l3;
             const CatRef c2 = {T.x2, T.y2, T.z2, T.w2};
             const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3};
             if (T.sym) {
                 if (T.weighted)
-                    pair_loop<MIXED, FAST, true, true>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, FAST, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
                 else
-                    pair_loop<MIXED, FAST, true, false>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, FAST, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
             } else {
                 if (T.weighted)
-                    pair_loop<MIXED, FAST, false, true>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, FAST, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
                 else
-                    pair_loop<MIXED, FAST, false, false>(A, t, pw, l2, n2, lb, n3, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, FAST, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
             }
             wave_sync();
         }
@@ -1011,10 +1054,11 @@ struct Geometry {
     // mixed path: guard bands and the FP32 lookup (valid when mixed_ok)
     bool mixed_ok = false;
     std::vector<uint2> lut32;
+    double sigma32 = 0.0;
     float lut_scale32 = 0.f;
     int lut32_m = 0;
     float g32 = 0.f, tri_c1 = 0.f, tri_c2 = 0.f;
-    std::vector<uint16_t> t3;  // (ka*nb+kb)*(nb+2)+k+1 -> compact slot
+    std::vector<uint16_t> t3;  // (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
 };
 
 }  // namespace
@@ -1101,11 +1145,18 @@ int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
 // is accepted when  s_a*s_b - dot^2 > 80uR^2 (s_a + s_b) + 64uR^4  (both
 // terms doubled) and otherwise re-evaluated exactly.
 //
-// Lookup: j = min(fl32(d32*s32), M+1) over [0, 1.25*rmax^2); any d32 mapped
-// to j lies in [(j-1)/s32, (j+2)/s32].  Entry j = {bin below the one band that
-// meets that interval (or below all of it), the band's edge E as float bits,
-// +inf if no band meets it}.  Tables where an interval meets two bands are
-// rejected (M is doubled); bins narrower than the bands disable the path.
+// Scaling: s32 = sigma^2 is a power of 4 and the lists hold sigma*(A - P)
+// (rounded once, from the exact FP64 product), so every FP32 quantity above
+// is the unscaled one times a power of 2, with the same rounding; the
+// thresholds g32, c1, c2 and the edges are stored scaled (by s32, s32, s32^2,
+// s32).
+//
+// Lookup: j = min(d32', M+1) with d32' = s32*d32 over [0, 1.25*rmax^2*s32);
+// any d32 mapped to j lies in [(j-1)/s32, (j+2)/s32].  Entry j = {bin below
+// the one band that meets that interval (or below all of it), the band's edge
+// s32*E as float bits, +inf if no band meets it}.  Tables where an interval
+// meets two bands are rejected (s32 is multiplied by 4); bins narrower than
+// the bands disable the path.
 static void make_mixed_tables(Geometry &G) {
     G.mixed_ok = false;
     const double u = std::ldexp(1.0, -24);
@@ -1117,9 +1168,12 @@ static void make_mixed_tables(Geometry &G) {
     for (int k = 0; k + 1 < ne; ++k)
         if (!((double)E[k] + g < (double)E[k + 1] - g)) return;  // bins narrower than the guard band
     const double r2max = G.sq[G.nb];
-    for (int M = 512; M <= 16384; M *= 2) {
-        const float s32 = (float)((double)M / (1.25 * r2max));
-        const double s = (double)s32;
+    // s32 = 4^kk from the smallest power of 4 with M >= 512 on, x4 per retry
+    const int kk0 = (int)std::ceil(std::log2(512.0 / (1.25 * r2max)) / 2.0);
+    for (int kk = kk0; kk <= kk0 + 3; ++kk) {
+        const double s = std::ldexp(1.0, 2 * kk);
+        const int M = (int)std::ceil(1.25 * r2max * s);
+        if (M > 16384) break;
         std::vector<uint2> lut(M + 2);
         bool ok = true;
         for (int j = 0; j <= M + 1 && ok; ++j) {
@@ -1135,7 +1189,7 @@ static void make_mixed_tables(Geometry &G) {
                 }
             }
             if (meet > 1 || (meet == 1 && which != below)) ok = false;
-            const float edge = meet ? E[which] : INFINITY;
+            const float edge = meet ? (float)((double)E[which] * s) : INFINITY;  // exact: s is a power of 4
             uint32_t eb;
             std::memcpy(&eb, &edge, 4);
             lut[j] = make_uint2((uint32_t)(below - 1), eb);
@@ -1143,17 +1197,16 @@ static void make_mixed_tables(Geometry &G) {
         if (!ok) continue;
         G.lut32 = std::move(lut);
         G.lut32_m = M;
-        G.lut_scale32 = s32;
-        G.g32 = (float)g;
-        G.tri_c1 = (float)(80.0 * u * R * R);
-        G.tri_c2 = (float)(64.0 * u * R * R * R * R);
+        G.sigma32 = std::ldexp(1.0, kk);
+        G.lut_scale32 = (float)s;
+        G.g32 = (float)(g * s);
+        G.tri_c1 = (float)(80.0 * u * R * R * s);
+        G.tri_c2 = (float)(64.0 * u * R * R * R * R * s * s);
         G.mixed_ok = true;
         return;
     }
 }
 
-// Slot table of the FAST variants: (ka*nb + kb)*(nb+2) + k + 1 -> compact
-// index of the sorted bins, for ka, kb in [0, nb) and k in [-1, nb].
 static void make_slot_table(Geometry &G) {
     const int nb = G.nb;
     G.t3.assign((size_t)nb * nb * (nb + 2), 0);
@@ -1162,7 +1215,8 @@ static void make_slot_table(Geometry &G) {
             for (int k = 0; k < nb; ++k) {
                 const int lo = std::min(std::min(a, b), k), hi = std::max(std::max(a, b), k);
                 const int mid = a + b + k - lo - hi;
-                G.t3[((size_t)a * nb + b) * (nb + 2) + k + 1] = (uint16_t)(G.ctab[lo * nb + mid] + (hi - mid));
+                G.t3[((size_t)a * nb + b) * (nb + 2) + k + 1] =
+                    (uint16_t)((G.ctab[lo * nb + mid] + (hi - mid)) * sizeof(unsigned long long));
             }
 }
 
@@ -1221,7 +1275,7 @@ int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
             c += G.nb - j;
         }
     make_mixed_tables(G);
-    if (G.nbc <= 65535) make_slot_table(G);
+    if ((size_t)G.nbc * sizeof(unsigned long long) <= 65535) make_slot_table(G);  // byte offsets fit 16 bits
     return NC_OK;
 }
 
@@ -1569,6 +1623,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.ctab = (const int *)((char *)ctx->tables.p + g_ctab);
     A.lut32 = (const uint2 *)((char *)ctx->tables.p + g_lut32);
     A.t3 = (const uint16_t *)((char *)ctx->tables.p + g_t3);
+    A.sigma32 = G.sigma32;
     A.lut_scale32 = G.lut_scale32;
     A.lut32_m = G.lut32_m;
     A.g32 = G.g32;

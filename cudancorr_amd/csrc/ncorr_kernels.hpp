@@ -68,16 +68,20 @@ struct KArgs {
     double lut_scale;
     int lut_m;
     // mixed path
-    float lut_scale32;
+    // Quantities in units scaled by sigma^2 = lut_scale32, a power of 4: list
+    // coordinates are stored as sigma*(q - p1), so r23^2 arrives pre-scaled
+    // and its lookup index is the value itself.
+    double sigma32;     // power of 2
+    float lut_scale32;  // sigma32^2
     int lut32_m;
-    float g32;     // guard band: |fp32 r23^2 - fp64 r23^2| bound (incl. edge rounding)
-    float tri_c1;  // collinearity threshold: s_a*s_b - dot^2 <= tri_c1*(s_a+s_b) + tri_c2 => recheck
+    float g32;     // guard band: |fp32 r23^2 - fp64 r23^2| bound (incl. edge rounding), scaled
+    float tri_c1;  // collinearity threshold: s_a*s_b - dot^2 <= tri_c1*(s_a+s_b) + tri_c2 => recheck, scaled
     float tri_c2;
     const double *sq;             // nb+1 squared edges
     const uint16_t *lut;          // lut_m+1 lower-bound bins (FP64 path)
     const int *ctab;              // nb*nb compact base index
-    const uint2 *lut32;           // lut32_m+2 entries (mixed path): {bin below the edge, edge value as float bits}
-    const uint16_t *t3;           // nb*nb*(nb+2): (ka*nb+kb)*(nb+2)+k+1 -> compact slot
+    const uint2 *lut32;           // lut32_m+2 entries (mixed path): {bin below the edge, scaled edge as float bits}
+    const uint16_t *t3;           // nb*nb*(nb+2): (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
     // dynamic LDS layout (bytes)
     int off_sq, off_lut, off_ctab, off_lut32, off_t3, off_wave, wave_bytes, lds_bytes;
 };

@@ -6,7 +6,10 @@ pair_loop_mixed) decides a pair without FP64 arithmetic when
   * s_a*s_b - dot^2 > 80uR^2 (s_a + s_b) + 64uR^4  (triangle far from degenerate),
 with u = 2^-24, R = rmax, r = fl32(q - p1), s = fl32(|r|^2),
 dot = fl32(r_a . r_b), d32 = fl32(s_a + s_b - 2 dot).  Every other pair is
-re-evaluated in FP64.  This test emulates that FP32 arithmetic with numpy and
+re-evaluated in FP64.  The kernel works in units scaled by a power of 4
+(coordinates times sigma = 2^kk, thresholds and edges times sigma^2 or
+sigma^4); the emulation does the same and the decisions are checked to be
+identical for every scale.  This test emulates that FP32 arithmetic with numpy and
 checks, on random and adversarial pairs (near bin edges, near-collinear
 triples, survey-sized coordinates), that every decided pair gets the bin and
 the triangle outcome of the reference's FP64 evaluation (device.cu:154-201).
@@ -27,15 +30,23 @@ def fma32(a, b, c):
     return f32(np.float64(a) * np.float64(b) + np.float64(c))
 
 
-def mixed_decisions(P, A, B, edges):
+def kernel_scale_exponent(edges):
+    """kk of the kernel's first table attempt: the smallest power of 4, s32 =
+    4^kk, with 1.25*rmax^2*s32 >= 512."""
+    return int(np.ceil(np.log2(512.0 / (1.25 * float(edges[-1]) ** 2)) / 2.0))
+
+
+def mixed_decisions(P, A, B, edges, kk=0):
     R = float(edges[-1])
+    sg = 2.0 ** kk
+    s32 = sg * sg
     sq = edges * edges
-    E32 = f32(sq)
-    g = 80.0 * U * R * R
-    c1 = np.float32(80.0 * U * R * R)
-    c2 = np.float32(64.0 * U * R ** 4)
-    ra = f32(A - P)
-    rb = f32(B - P)
+    E32 = f32(f32(sq) * s32)
+    g = 80.0 * U * R * R * s32
+    c1 = np.float32(80.0 * U * R * R * s32)
+    c2 = np.float32(64.0 * U * R ** 4 * s32 * s32)
+    ra = f32((A - P) * sg)
+    rb = f32((B - P) * sg)
     sa = fma32(ra[:, 2], ra[:, 2], fma32(ra[:, 1], ra[:, 1], f32(ra[:, 0] * ra[:, 0])))
     sb = fma32(rb[:, 2], rb[:, 2], fma32(rb[:, 1], rb[:, 1], f32(rb[:, 0] * rb[:, 0])))
     dot = fma32(ra[:, 2], rb[:, 2], fma32(ra[:, 1], rb[:, 1], f32(ra[:, 0] * rb[:, 0])))
@@ -74,7 +85,11 @@ def check(P, A, B, edges):
     db = np.sum((B - P) ** 2, 1)
     keep = (da > sq[0]) & (da < sq[-1]) & (db > sq[0]) & (db < sq[-1])
     P, A, B = P[keep], A[keep], B[keep]
-    decided, accepted, k32 = mixed_decisions(P, A, B, edges)
+    decided, accepted, k32 = mixed_decisions(P, A, B, edges, kernel_scale_exponent(edges))
+    # the scaling is exact: unscaled and other scales decide identically
+    for kk in (0, -7, 9):
+        d2, a2, k2 = mixed_decisions(P, A, B, edges, kk)
+        assert np.array_equal(d2, decided) and np.array_equal(a2, accepted) and np.array_equal(k2, k32)
     inr, k, tri = reference(P, A, B, edges)
     # decided pairs: out of range iff the reference's value is out of range
     assert np.array_equal(accepted, decided & inr)
