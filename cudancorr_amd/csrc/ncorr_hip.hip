@@ -285,57 +285,102 @@ struct CatRef {
 // Compact the neighbours of primary (px,py,pz) in catalogue (X,Y,Z,Wt,st)
 // whose squared separation lies in (r2min, r2max) into the list at `base`.
 // Returns the list length (entries beyond cap are not stored).
+//
+// The 27 neighbour entries are visited in the reference's x-major order
+// (cudaNcorr.cu:37-48); cells are numbered z-fastest, so the three entries
+// dz = -1, 0, 1 of one (dx, dy) are consecutive cells and, when none of them
+// wraps (box) or leaves the grid (survey: those are skipped), one contiguous
+// point range with one image shift.  Such a range is scanned as a whole, two
+// 64-point blocks per step (both loads in flight together); the list order is
+// the reference's entry order either way.  Each list entry records a
+// neighbour entry s whose shift is its own (the middle one for a merged range).
 template <bool MIXED>
 __device__ int build_list(const KArgs &A, const double *__restrict__ X, const double *__restrict__ Y,
                           const double *__restrict__ Z, const double *__restrict__ Wt, const uint32_t *__restrict__ st,
                           double px, double py, double pz, int cx, int cy, int cz, char *__restrict__ base, int lane,
                           const double *sq, const uint16_t *lut) {
     int n = 0;
-    for (int s = 0; s < 27; ++s) {
-        double shx, shy, shz;
-        const int cell = neighbour(A, cx, cy, cz, s, shx, shy, shz);
-        if (cell < 0) continue;
-        const int beg = uniform((int)st[cell]), end = uniform((int)st[cell + 1]);
-        for (int j0 = beg; j0 < end; j0 += kWave) {
-            const int j = j0 + lane;
-            const bool v = j < end;
-            double qx = 0.0, qy = 0.0, qz = 0.0, qw = 0.0;
-            if (v) {
-                // image position: point + shift (device.cu:282-284)
-                qx = X[j] + shx;
-                qy = Y[j] + shy;
-                qz = Z[j] + shz;
-                qw = Wt[j];
+    auto keep = [&](int j, bool v, double qx, double qy, double qz, double qw, int s) {
+        const double d = sqrsep(px - qx, py - qy, pz - qz);
+        const bool pass = v & (d > A.r2min) & (d < A.r2max);
+        const unsigned long long m = __ballot(pass);
+        const int pos = n + lane_rank(m);
+        if (pass & (pos < A.cap)) {
+            const int k = bin_of(d, A.lut_scale, lut, sq, A.nb);
+            const float f = __builtin_sqrtf((float)d);
+            if constexpr (MIXED) {
+                const MList ml = mlist(base, A.cap);
+                const double sg = A.sigma32;  // power of 2: the scaling is exact
+                const float rx = (float)((qx - px) * sg), ry = (float)((qy - py) * sg), rz = (float)((qz - pz) * sg);
+                ml.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
+                ml.KF[pos] = make_int2(k | (s << 8), __float_as_int(f));
+                ml.I[pos] = j;
+            } else {
+                Elem e;
+                e.x = qx;
+                e.y = qy;
+                e.z = qz;
+                e.k = k;
+                e.f = f;
+                reinterpret_cast<Elem *>(base)[pos] = e;
+                reinterpret_cast<double *>(base + (size_t)A.cap * sizeof(Elem))[pos] = qw;
             }
-            const double d = sqrsep(px - qx, py - qy, pz - qz);
-            const bool pass = v && d > A.r2min && d < A.r2max;
-            const unsigned long long m = __ballot(pass);
-            const int pos = n + lane_rank(m);
-            if (pass && pos < A.cap) {
-                const int k = bin_of(d, A.lut_scale, lut, sq, A.nb);
-                const float f = __builtin_sqrtf((float)d);
-                if constexpr (MIXED) {
-                    const MList m = mlist(base, A.cap);
-                    const double sg = A.sigma32;  // power of 2: the scaling is exact
-                    const float rx = (float)((qx - px) * sg), ry = (float)((qy - py) * sg),
-                                rz = (float)((qz - pz) * sg);
-                    m.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
-                    m.KF[pos] = make_int2(k | (s << 8), __float_as_int(f));
-                    m.I[pos] = j;
-                } else {
-                    Elem e;
-                    e.x = qx;
-                    e.y = qy;
-                    e.z = qz;
-                    e.k = k;
-                    e.f = f;
-                    reinterpret_cast<Elem *>(base)[pos] = e;
-                    reinterpret_cast<double *>(base + (size_t)A.cap * sizeof(Elem))[pos] = qw;
-                }
-            }
-            n += __popcll(m);
         }
-    }
+        n += __popcll(m);
+    };
+    // points [beg, end) of the sorted catalogue, image shift (shx, shy, shz)
+    auto scan = [&](int beg, int end, double shx, double shy, double shz, int s) {
+        for (int j0 = beg; j0 < end; j0 += 2 * kWave) {
+            const int ja = j0 + lane, jb = ja + kWave;
+            const bool va = ja < end, vb = jb < end;
+            double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0, bx = 0.0, by = 0.0, bz = 0.0, bw = 0.0;
+            // image position: point + shift (device.cu:282-284)
+            if (va) {
+                ax = X[ja] + shx;
+                ay = Y[ja] + shy;
+                az = Z[ja] + shz;
+                aw = Wt[ja];
+            }
+            if (vb) {
+                bx = X[jb] + shx;
+                by = Y[jb] + shy;
+                bz = Z[jb] + shz;
+                bw = Wt[jb];
+            }
+            keep(ja, va, ax, ay, az, aw, s);
+            if (j0 + kWave < end) keep(jb, vb, bx, by, bz, bw, s);
+        }
+    };
+    const int nz = A.nc[2];
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int s0 = (dx + 1) * 9 + (dy + 1) * 3;  // entry of dz = -1
+            int ex = cx + dx, ey = cy + dy;
+            double shx = 0.0, shy = 0.0;
+            if (A.survey) {
+                if (ex < 0 || ex >= A.nc[0] || ey < 0 || ey >= A.nc[1]) continue;
+                const int row = nz * (ey + A.nc[1] * ex);
+                const int z0 = max(cz - 1, 0), z1 = min(cz + 1, nz - 1);
+                scan(uniform((int)st[row + z0]), uniform((int)st[row + z1 + 1]), 0.0, 0.0, 0.0, s0 + 1);
+                continue;
+            }
+            if (ex == A.nc[0]) { ex = 0; shx = A.box[0]; }
+            if (ey == A.nc[1]) { ey = 0; shy = A.box[1]; }
+            if (ex == -1) { ex = A.nc[0] - 1; shx = -A.box[0]; }
+            if (ey == -1) { ey = A.nc[1] - 1; shy = -A.box[1]; }
+            const int row = nz * (ey + A.nc[1] * ex);
+            if (cz >= 1 && cz + 1 < nz) {
+                scan(uniform((int)st[row + cz - 1]), uniform((int)st[row + cz + 2]), shx, shy, 0.0, s0 + 1);
+                continue;
+            }
+            for (int dz = -1; dz <= 1; ++dz) {  // a z wrap: one cell at a time
+                int ez = cz + dz;
+                double shz = 0.0;
+                if (ez == nz) { ez = 0; shz = A.box[2]; }
+                if (ez == -1) { ez = nz - 1; shz = -A.box[2]; }
+                scan(uniform((int)st[row + ez]), uniform((int)st[row + ez + 1]), shx, shy, shz, s0 + dz + 1);
+            }
+        }
     return n;
 }
 
