@@ -384,34 +384,51 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
     return n;
 }
 
-// Byte offset in the LDS histogram of the sorted slot of bins (ka, kb, k),
-// k in [-1, nb] (slots of k outside [0, nb) are never added).  FAST: one LDS
-// table lookup; otherwise the compact C(nb+2,3) index through ctab.
-template <bool FAST>
+// LDS histogram layouts of the triplet kernel (HM):
+//   kHistCtab   compact C(nb+2,3) sorted slots (u64 counts / f64 sums), slot
+//               index through ctab; the plain path's FP64 bin lookup may take
+//               several steps;
+//   kHistT3     compact sorted slots, slot byte offset from the t3 table;
+//   kHistDense  dense nb^3 u32 counts indexed (ka, kb, k) unsorted, folded
+//               into the sorted slots when flushed (unit-weight terms only).
+// kHistT3 and kHistDense also take the single-step FP64 bin lookup.
+constexpr int kHistCtab = 0, kHistT3 = 1, kHistDense = 2;
+
+// Byte offset in the LDS histogram of the slot of bins (ka, kb, k), k in
+// [-1, nb] (slots of k outside [0, nb) are never added).
+template <int HM>
 __device__ __forceinline__ int sorted_slot(const KArgs &A, const LdsTabs &t, int ka, int kb, int k) {
-    if constexpr (FAST) return t.t3[__mul24(__mul24(ka, A.nb) + kb, A.nb + 2) + k + 1];
+    if constexpr (HM == kHistT3) return t.t3[__mul24(__mul24(ka, A.nb) + kb, A.nb + 2) + k + 1];
+    if constexpr (HM == kHistDense) return 4 * (__mul24(__mul24(ka, A.nb) + kb, A.nb) + k);
     const int klo = min(min(ka, kb), k);
     const int khi = max(max(ka, kb), k);
     const int kmid = ka + kb + k - klo - khi;
     return (t.ctab[klo * A.nb + kmid] + (khi - kmid)) * (int)sizeof(unsigned long long);
 }
 
-// Bins as held by the pair loops: with the slot table, the row entry carries
-// the byte offset 2*(ka*nb*(nb+2) + 1) and the lane entry 2*kb*(nb+2), so the
-// slot offset of (ka, kb, k) is the u16 at byte row + lane + 2k of t3 (the
-// table is symmetric in ka and kb); without it the plain bins.
-template <bool FAST>
+// Bins as held by the pair loops, pre-multiplied so that the slot of
+// (ka, kb, k) costs one add: kHistT3, row 2*(ka*nb*(nb+2) + 1) and lane
+// 2*kb*(nb+2) bytes into t3 (symmetric in ka and kb), slot = the u16 at byte
+// row + lane + 2k; kHistDense, row 4*ka*nb^2 and lane 4*kb*nb, slot byte
+// offset row + lane + 4k; kHistCtab, the plain bins.
+template <int HM>
 __device__ __forceinline__ int row_bin(const KArgs &A, int k) {
-    return FAST ? 2 * (__mul24(k, A.nb * (A.nb + 2)) + 1) : k;
+    if constexpr (HM == kHistT3) return 2 * (__mul24(k, A.nb * (A.nb + 2)) + 1);
+    if constexpr (HM == kHistDense) return 4 * __mul24(k, A.nb * A.nb);
+    return k;
 }
-template <bool FAST>
+template <int HM>
 __device__ __forceinline__ int lane_bin(const KArgs &A, int k) {
-    return FAST ? 2 * __mul24(k, A.nb + 2) : k;
+    if constexpr (HM == kHistT3) return 2 * __mul24(k, A.nb + 2);
+    if constexpr (HM == kHistDense) return 4 * __mul24(k, A.nb);
+    return k;
 }
-template <bool FAST>
+template <int HM>
 __device__ __forceinline__ int pair_slot(const KArgs &A, const LdsTabs &t, int kr, int kl, int k) {
-    if constexpr (FAST) return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(t.t3) + kr + kl + 2 * k);
-    return sorted_slot<false>(A, t, kr, kl, k);
+    if constexpr (HM == kHistT3)
+        return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(t.t3) + kr + kl + 2 * k);
+    if constexpr (HM == kHistDense) return kr + kl + 4 * k;
+    return sorted_slot<kHistCtab>(A, t, kr, kl, k);
 }
 
 // Triangle test inputs are non-negative floats: their bit patterns order as
@@ -429,10 +446,13 @@ __device__ __forceinline__ Sides sort3(float a, float b, float c) {
 }
 
 // Histogram update of one accepted pair at a precomputed slot (byte offset).
-template <bool SYM, bool WEIGHTED>
+template <bool SYM, bool WEIGHTED, int HM>
 __device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1a, double wa, double w1b, double wb) {
     char *h = reinterpret_cast<char *>(t.hist) + off;
-    if constexpr (WEIGHTED) {
+    if constexpr (HM == kHistDense) {
+        static_assert(!WEIGHTED, "dense counts are for unit weights");
+        atomicAdd(reinterpret_cast<unsigned *>(h), SYM ? 2u : 1u);
+    } else if constexpr (WEIGHTED) {
         double v = w1a * wb;  // (w1*w2)*w3, device.cu:305
         if (SYM) v += w1b * wa;
         atomicAdd(reinterpret_cast<double *>(h), v);
@@ -442,13 +462,13 @@ __device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1a, 
 }
 
 // Bin of a squared separation (exact for r2min < d < r2max; any value in
-// [0, nb] otherwise).  FAST: the lookup table is fine enough that one step
+// [0, nb] otherwise).  STEP1: the lookup table is fine enough that one step
 // over the squared edges always suffices (checked on the host).
-template <bool FAST>
+template <bool STEP1>
 __device__ __forceinline__ int bin_any(const KArgs &A, const LdsTabs &t, double d) {
     const int j = (int)fmin(d * A.lut_scale, (double)A.lut_m);
     int k = t.lut[j];
-    if constexpr (FAST) {
+    if constexpr (STEP1) {
         k += d >= t.sq[k + 1] ? 1 : 0;
     } else {
         while (k < A.nb - 1 && d >= t.sq[k + 1]) ++k;
@@ -459,7 +479,7 @@ __device__ __forceinline__ int bin_any(const KArgs &A, const LdsTabs &t, double 
 // Exact FP64 evaluation of one pair (device.cu:299-307 restated): r23 from
 // the image positions, strict range filter, exact bin, FP32 triangle test
 // with the correctly rounded sqrtf.
-template <bool SYM, bool WEIGHTED, bool FAST>
+template <bool SYM, bool WEIGHTED, int HM>
 __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, double ax, double ay, double az, int ka,
                                            float fa, double w1a, double wa, double bx, double by, double bz, int kb,
                                            float fb, double w1b, double wb, bool ok) {
@@ -471,7 +491,7 @@ __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, dou
         const float hi = fmaxf(fmaxf(fa, fb), f);
         const float mid = __builtin_amdgcn_fmed3f(fa, fb, f);
         if (lo + mid > hi)  // the reference rejects sqrtf(r1)+sqrtf(r2) <= sqrtf(r3), device.cu:173
-            hist_add<SYM, WEIGHTED>(t, sorted_slot<FAST>(A, t, ka, kb, k), w1a, wa, w1b, wb);
+            hist_add<SYM, WEIGHTED, HM>(t, sorted_slot<HM>(A, t, ka, kb, k), w1a, wa, w1b, wb);
     }
 }
 
@@ -539,7 +559,7 @@ __device__ __forceinline__ void traverse(Pol &P, int n2, int n3) {
 // FP32 triangle test with the hardware sqrt (<= 1 ulp) -- redone with the
 // correctly rounded sqrtf when the slack is within 8 ulp of the largest side
 // -- and the histogram updates.
-template <bool SYM, bool WEIGHTED, bool FAST>
+template <bool SYM, bool WEIGHTED, int HM>
 struct PlainPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 2;
@@ -577,7 +597,7 @@ struct PlainPol {
             double w = 0.0;
             if (i < last - first) {
                 e = L[first + i];
-                e.k = row_bin<FAST>(A, e.k);
+                e.k = row_bin<HM>(A, e.k);
                 if (WEIGHTED) w = W[first + i];
             } else {  // padding: far away, never in range
                 e.x = e.y = e.z = 1e150;
@@ -597,7 +617,7 @@ struct PlainPol {
         wb = 0.0;
         if (vb) {
             eb = (from3 ? L3 : L2)[b];
-            eb.k = lane_bin<FAST>(A, eb.k);
+            eb.k = lane_bin<HM>(A, eb.k);
             if (WEIGHTED) wb = (from3 ? W3 : W2)[b];
         } else {
             eb.x = eb.y = eb.z = -1e150;
@@ -625,7 +645,7 @@ struct PlainPol {
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if constexpr (FAST) {
+            if constexpr (HM != kHistCtab) {
                 k[u] += d[u] >= t.sq[k[u] + 1] ? 1 : 0;
             } else {
                 while (k[u] < A.nb - 1 && d[u] >= t.sq[k[u] + 1]) ++k[u];
@@ -647,11 +667,11 @@ struct PlainPol {
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
-                const int slot = pair_slot<FAST>(A, t, ea[u].k, eb.k, k[u]);
+                const int slot = pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]);
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED>(t, slot, w1b, wb, w1 * wa[u], wa[u]);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1b, wb, w1 * wa[u], wa[u]);
                 else
-                    hist_add<SYM, WEIGHTED>(t, slot, w1 * wa[u], wa[u], w1b, wb);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1 * wa[u], wa[u], w1b, wb);
             }
     }
 
@@ -695,7 +715,7 @@ struct PlainPol {
 // Mixed path, deferred exact rechecks: every queued (S2 entry, S3 entry) pair
 // is evaluated with the FP64 image positions (catalogue point + periodic
 // shift of its neighbour entry, rebuilt as in build_list), one pair per lane.
-template <bool SYM, bool WEIGHTED, bool FAST>
+template <bool SYM, bool WEIGHTED, int HM>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
                             const CatRef &c2, const CatRef &c3, int cx, int cy, int cz, const uint2 *q, int qn,
                             int lane) {
@@ -709,7 +729,7 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
         neighbour(A, cx, cy, cz, ka.x >> 8, sax, say, saz);
         neighbour(A, cx, cy, cz, kb.x >> 8, sbx, sby, sbz);
         const double wa = WEIGHTED ? c2.w[ja] : 0.0, wb = WEIGHTED ? c3.w[jb] : 0.0;
-        exact_pair<SYM, WEIGHTED, FAST>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
+        exact_pair<SYM, WEIGHTED, HM>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
                                         __int_as_float(ka.y), w1 * wa, wa, c3.x[jb] + sbx, c3.y[jb] + sby,
                                         c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true);
     }
@@ -725,7 +745,7 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
 // guard band g32 of that edge, or whose collinearity measure is within the
 // error bound (make_mixed_tables), is queued for the exact FP64 evaluation.
 // Rows: float4 {r, s} + bin + weight in LDS; lanes in registers.
-template <bool SYM, bool WEIGHTED, bool FAST>
+template <bool SYM, bool WEIGHTED, int HM>
 struct MixedPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 4;
@@ -779,7 +799,7 @@ struct MixedPol {
             double w = 0.0;
             if (i < last - first) {
                 r = m.R[first + i];
-                k = row_bin<FAST>(A, m.KF[first + i].x & 0xff);
+                k = row_bin<HM>(A, m.KF[first + i].x & 0xff);
                 if (WEIGHTED) w = c.w[m.I[first + i]];
             }
             la[i] = r;
@@ -799,14 +819,14 @@ struct MixedPol {
         wb = 0.0;
         if (vb) {
             rb = m.R[bidx];
-            kb = lane_bin<FAST>(A, m.KF[bidx].x & 0xff);
+            kb = lane_bin<HM>(A, m.KF[bidx].x & 0xff);
             if (WEIGHTED) wb = (from3 ? c3 : c2).w[m.I[bidx]];
         }
         w1b = w1 * wb;
     }
 
     __device__ void drain() {
-        drain_queue<SYM, WEIGHTED, FAST>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane);
+        drain_queue<SYM, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane);
         qn = 0;
     }
 
@@ -837,7 +857,7 @@ struct MixedPol {
             // bitwise, so the predicates stay lane masks in scalar registers
             amb[u] = ok[u] & (amb_b | (valid & amb_t));
             acc[u] = ok[u] & valid & !amb_b & !amb_t;
-            slot[u] = pair_slot<FAST>(A, t, ka[u], kb, k);
+            slot[u] = pair_slot<HM>(A, t, ka[u], kb, k);
         }
         bool any = false;
 #pragma unroll
@@ -858,9 +878,9 @@ struct MixedPol {
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED>(t, slot[u], w1b, wb, w1 * wa[u], wa[u]);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1b, wb, w1 * wa[u], wa[u]);
                 else
-                    hist_add<SYM, WEIGHTED>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
             }
     }
 
@@ -908,33 +928,50 @@ struct MixedPol {
 };
 
 // Merge the workgroup's LDS histogram into the term's global compact
-// histogram and clear it.
-__device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long long *hist) {
+// histogram and clear it.  Dense counts go to the sorted slot of their bins.
+template <int HM>
+__device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long long *hist,
+                           unsigned long long *pairs) {
     __syncthreads();
     unsigned long long *out = A.out + (long long)term * A.nbc;
-    for (int i = threadIdx.x; i < A.nslots; i += blockDim.x) {
-        const unsigned long long v = hist[i];
-        if (v != 0ull) {
-            if (weighted)
-                unsafeAtomicAdd(reinterpret_cast<double *>(&out[i]), __longlong_as_double((long long)v));
-            else
-                atomicAdd(&out[i], v);
-            hist[i] = 0ull;
+    if constexpr (HM == kHistDense) {
+        unsigned *h32 = reinterpret_cast<unsigned *>(hist);
+        const int nb = A.nb, nb3 = nb * nb * nb;
+        for (int i = threadIdx.x; i < nb3; i += blockDim.x) {
+            const unsigned v = h32[i];
+            if (v != 0u) {
+                const int ka = i / (nb * nb), kb = (i / nb) % nb, k = i % nb;
+                const int klo = min(min(ka, kb), k), khi = max(max(ka, kb), k), kmid = ka + kb + k - klo - khi;
+                atomicAdd(&out[A.ctab[klo * nb + kmid] + (khi - kmid)], (unsigned long long)v);
+                h32[i] = 0u;
+            }
+        }
+        if (threadIdx.x == 0) *pairs = 0ull;
+    } else {
+        for (int i = threadIdx.x; i < A.nslots; i += blockDim.x) {
+            const unsigned long long v = hist[i];
+            if (v != 0ull) {
+                if (weighted)
+                    unsafeAtomicAdd(reinterpret_cast<double *>(&out[i]), __longlong_as_double((long long)v));
+                else
+                    atomicAdd(&out[i], v);
+                hist[i] = 0ull;
+            }
         }
     }
     __syncthreads();
 }
 
-template <bool MIXED, bool FAST, bool SYM, bool WEIGHTED>
+template <bool MIXED, int HM, bool SYM, bool WEIGHTED>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
                                           const CatRef &c3, int cx, int cy, int cz) {
     if constexpr (MIXED) {
-        MixedPol<SYM, WEIGHTED, FAST> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz);
+        MixedPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz);
         traverse(P, n2, n3);
         if (P.qn > 0) P.drain();
     } else {
-        PlainPol<SYM, WEIGHTED, FAST> P(A, t, w1, l2, l3, wlds, lane);
+        PlainPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane);
         traverse(P, n2, n3);
     }
 }
@@ -943,12 +980,15 @@ __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, doub
 // from the queue A.chunks with one atomic per chunk; the waves of a workgroup
 // take the primaries of the chunk one at a time.  The LDS histogram of the
 // workgroup is merged into the term's global histogram whenever the workgroup
-// moves to another term and at exit.
-template <bool MIXED, bool FAST>
+// moves to another term and at exit.  Dense u32 counts are also merged once
+// the pairs added since the last merge reach A.flush_pairs <= 2^31: one chunk
+// adds at most chunk * cap^2 < 2^31 (checked on the host), so no count wraps.
+template <bool MIXED, int HM>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_triplets(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
+    __shared__ unsigned long long s_pairs;
     LdsTabs t;
     t.hist = reinterpret_cast<unsigned long long *>(lds);
     double *sq = reinterpret_cast<double *>(lds + A.off_sq);
@@ -963,6 +1003,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     t.t3 = t3;
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int i = tid; i < A.nslots; i += nthr) t.hist[i] = 0ull;
+    if (tid == 0) s_pairs = 0ull;
     if constexpr (MIXED) {
         // FP64 tables are only read by the rare exact rechecks: global memory
         t.sq = A.sq;
@@ -972,9 +1013,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
         for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
     }
-    if constexpr (FAST)
+    if constexpr (HM == kHistT3)
         for (int i = tid; i < A.nb * A.nb * (A.nb + 2); i += nthr) t3[i] = A.t3[i];
-    else
+    else if constexpr (HM == kHistCtab)
         for (int i = tid; i < A.nb * A.nb; i += nthr) ctab[i] = A.ctab[i];
     const int lane = tid & (kWave - 1), wave = tid / kWave, nw = nthr / kWave;
     char *wlds = lds + A.off_wave + wave * A.wave_bytes;
@@ -982,6 +1023,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     char *l2 = slot;
     char *l3 = slot + (size_t)A.cap * list_entry_bytes(MIXED);
     int cur = -1, cur_weighted = 0;
+    unsigned long long wpairs = 0ull;  // pairs this wave added in the current chunk
     __syncthreads();
     for (;;) {
         if (tid == 0) {
@@ -995,8 +1037,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (ci >= A.nchunks) break;
         const Chunk ch = A.chunks[ci];
         const TermDesc T = A.terms[ch.term];
-        if (ch.term != cur) {
-            if (cur >= 0) flush_hist(A, cur, cur_weighted, t.hist);
+        const bool full = HM == kHistDense && uniform((int)(s_pairs >= A.flush_pairs)) != 0;
+        if (ch.term != cur || full) {
+            if (cur >= 0) flush_hist<HM>(A, cur, cur_weighted, t.hist, &s_pairs);
             cur = ch.term;
             cur_weighted = T.weighted;
         }
@@ -1033,28 +1076,46 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
             const char *lb = T.sym ? l2 : l3;
             const CatRef c2 = {T.x2, T.y2, T.z2, T.w2};
             const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3};
+            if constexpr (HM == kHistDense) {
+                if (T.weighted) {  // never planned so
+                    if (lane == 0) atomicOr(A.err, kErrMode);
+                    continue;
+                }
+                wpairs += T.sym ? (unsigned long long)n2u * (unsigned long long)(n2u - 1)
+                                : (unsigned long long)n2u * (unsigned long long)n3u;
+            }
             if (T.sym) {
-                if (T.weighted)
-                    pair_loop<MIXED, FAST, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
-                else
-                    pair_loop<MIXED, FAST, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                if (T.weighted) {
+                    if constexpr (HM != kHistDense)
+                        pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                } else {
+                    pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                }
             } else {
-                if (T.weighted)
-                    pair_loop<MIXED, FAST, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
-                else
-                    pair_loop<MIXED, FAST, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                if (T.weighted) {
+                    if constexpr (HM != kHistDense)
+                        pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                } else {
+                    pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                }
             }
             wave_sync();
         }
+        if constexpr (HM == kHistDense) {
+            atomicAdd(&s_pairs, lane == 0 ? wpairs : 0ull);  // every lane: no branch on lane == 0
+            wpairs = 0ull;
+        }
         __syncthreads();
     }
-    if (cur >= 0) flush_hist(A, cur, cur_weighted, t.hist);
+    if (cur >= 0) flush_hist<HM>(A, cur, cur_weighted, t.hist, &s_pairs);
 }
 
-template __global__ void k_triplets<false, false>(KArgs);
-template __global__ void k_triplets<false, true>(KArgs);
-template __global__ void k_triplets<true, false>(KArgs);
-template __global__ void k_triplets<true, true>(KArgs);
+template __global__ void k_triplets<false, kHistCtab>(KArgs);
+template __global__ void k_triplets<false, kHistT3>(KArgs);
+template __global__ void k_triplets<false, kHistDense>(KArgs);
+template __global__ void k_triplets<true, kHistCtab>(KArgs);
+template __global__ void k_triplets<true, kHistT3>(KArgs);
+template __global__ void k_triplets<true, kHistDense>(KArgs);
 
 // ---------------------------------------------------------------------------
 // host side
@@ -1129,7 +1190,7 @@ struct BatchPlan {
     unsigned long long *h_out = nullptr;
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
-    int nb = 0, nbc = 0, nkernel = 0;
+    int nb = 0, nbc = 0, nkernel = 0, hist_layout = 0;
 };
 
 struct nc_context {
@@ -1147,6 +1208,9 @@ struct nc_context {
     int chunk = 64;
     int wgs_per_cu = 0;
     int chunk_elems = 128;
+    // dense counts are merged once this many pairs were added (< 2^31 keeps
+    // u32 counts from wrapping; NC_DENSE_FLUSH_PAIRS lowers it for tests)
+    unsigned long long dense_flush = 1ull << 31;
     int sample_stride = 1;  // autotuner: queue only every n-th work item
     std::string dev_name;
 };
@@ -1427,12 +1491,14 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     ctx->timing.ms_count = ms_c;
     ctx->timing.ms_total = ms_b + ms_c;
     ctx->timing.n_kernel = p.nkernel;
+    ctx->timing.hist_layout = p.hist_layout;
     if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (*p.h_err & kErrOverflow) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
         plan_drop(ctx);
         retry = true;
         return NC_OK;
     }
+    if (*p.h_err & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
     convert_out(p.nb, p.nbc, nterms, p.weighted, p.h_out, hist, counts);
     return NC_OK;
 }
@@ -1474,8 +1540,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     };
     // --- tables.  Global copies of all; the kernel copies into LDS what its
     // variant reads in the pair loop: plain sq + lut, mixed lut32, and the
-    // slot table t3 (FAST) or ctab.  The LDS histogram holds the C(nb+2,3)
-    // sorted bins.
+    // slot table t3 (kHistT3) or ctab (kHistCtab).  The LDS histogram holds the
+    // C(nb+2,3) sorted bins, or nb^3 dense u32 counts (kHistDense).
     const size_t g_sq = 0;
     const size_t g_lut = al16(g_sq + sizeof(double) * G.sq.size());
     const size_t g_ctab = al16(g_lut + sizeof(uint16_t) * G.lut.size());
@@ -1488,8 +1554,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     std::memcpy(tabs.data() + g_ctab, G.ctab.data(), sizeof(int) * G.ctab.size());
     if (!G.lut32.empty()) std::memcpy(tabs.data() + g_lut32, G.lut32.data(), sizeof(uint2) * G.lut32.size());
     if (!G.t3.empty()) std::memcpy(tabs.data() + g_t3, G.t3.data(), sizeof(uint16_t) * G.t3.size());
-    const size_t hist_bytes = al16((size_t)G.nbc * 8);
-    auto lds_tables = [&](bool fast_, size_t &o_sq, size_t &o_lut, size_t &o_ctab, size_t &o_lut32, size_t &o_t3) {
+    const size_t nb3 = (size_t)G.nb * G.nb * G.nb;
+    auto hist_size = [&](int hm) { return hm == kHistDense ? al16(nb3 * 4) : al16((size_t)G.nbc * 8); };
+    auto lds_tables = [&](int hm, size_t &o_sq, size_t &o_lut, size_t &o_ctab, size_t &o_lut32, size_t &o_t3) {
         size_t o = 0;
         o_sq = o_lut = o_ctab = o_lut32 = o_t3 = 0;
         if (!mixed) {
@@ -1501,22 +1568,16 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             o_lut32 = o;
             o = al16(o + sizeof(uint2) * G.lut32.size());
         }
-        if (fast_) {
+        if (hm == kHistT3) {
             o_t3 = o;
             o = al16(o + sizeof(uint16_t) * G.t3.size());
-        } else {
+        } else if (hm == kHistCtab) {
             o_ctab = o;
             o = al16(o + sizeof(int) * G.ctab.size());
         }
         return o;
     };
     size_t off_sq, off_lut, off_ctab, off_lut32, off_t3;
-    // FAST variant: slot table lookup (and, plain path, a single-step bin
-    // lookup), used when the table fits in LDS next to everything else.
-    const bool fast = !G.t3.empty() && (mixed || G.lut_single_step) &&
-                      hist_bytes + lds_tables(true, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64) <=
-                          ctx->lds_max;
-    const size_t lds_tab_bytes = lds_tables(fast, off_sq, off_lut, off_ctab, off_lut32, off_t3);
     NC_HIP(ctx, dgrow(ctx->tables, tab_bytes));
     NC_HIP(ctx, hipMemcpyAsync(ctx->tables.p, tabs.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
     NC_HIP(ctx, dgrow(ctx->errw, 16));
@@ -1575,6 +1636,25 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     const int cap = (int)((std::max(1u, host_flags[1]) + 15u) & ~15u);
 
+    // --- histogram layout: dense u32 counts when every term has unit weights,
+    // they fit and no count can wrap between merges (one chunk adds at most
+    // chunk * cap^2 pairs); else the slot table; else ctab.  The plain path
+    // needs the single-step bin lookup for either of the first two.
+    bool all_unit = true;
+    for (const Term &t : terms) all_unit = all_unit && t.c1->unit_w && t.c2->unit_w && t.c3->unit_w;
+    auto fits = [&](int hm) {
+        return hist_size(hm) + lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64) <=
+               ctx->lds_max;
+    };
+    int hm = kHistCtab;
+    if ((mixed || G.lut_single_step) && all_unit && (double)ctx->chunk * (double)cap * (double)cap < 2147483648.0 &&
+        fits(kHistDense))
+        hm = kHistDense;
+    else if ((mixed || G.lut_single_step) && !G.t3.empty() && fits(kHistT3))
+        hm = kHistT3;
+    const size_t hist_bytes = hist_size(hm);
+    const size_t lds_tab_bytes = lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3);
+
     // --- launch shape
     int C = ctx->chunk_elems;
     const size_t lds_tab = al16(hist_bytes + lds_tab_bytes);
@@ -1584,8 +1664,12 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const size_t wave_bytes = wave_lds(C);
     const size_t lds_bytes = lds_tab + W * wave_bytes;
     if (lds_bytes > ctx->lds_max) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
-    const void *kfn = mixed ? (fast ? (const void *)k_triplets<true, true> : (const void *)k_triplets<true, false>)
-                            : (fast ? (const void *)k_triplets<false, true> : (const void *)k_triplets<false, false>);
+    const void *kfns[2][3] = {
+        {(const void *)k_triplets<false, kHistCtab>, (const void *)k_triplets<false, kHistT3>,
+         (const void *)k_triplets<false, kHistDense>},
+        {(const void *)k_triplets<true, kHistCtab>, (const void *)k_triplets<true, kHistT3>,
+         (const void *)k_triplets<true, kHistDense>}};
+    const void *kfn = kfns[mixed ? 1 : 0][hm];
     int per_cu = ctx->wgs_per_cu;
     if (per_cu <= 0) {
         int occ = 0;
@@ -1674,7 +1758,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.g32 = G.g32;
     A.tri_c1 = G.tri_c1;
     A.tri_c2 = G.tri_c2;
-    A.nslots = G.nbc;
+    A.nslots = hm == kHistDense ? (int)((nb3 + 1) / 2) : G.nbc;
+    A.flush_pairs = ctx->dense_flush;
     A.off_sq = (int)hist_bytes;
     A.off_lut = (int)(hist_bytes + off_lut);
     A.off_ctab = (int)(hist_bytes + off_ctab);
@@ -1689,14 +1774,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
                            (long long)nterms * G.nbc);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
         if (A.nchunks > 0) {
-            if (mixed && fast)
-                hipLaunchKernelGGL((k_triplets<true, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
-            else if (mixed)
-                hipLaunchKernelGGL((k_triplets<true, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
-            else if (fast)
-                hipLaunchKernelGGL((k_triplets<false, true>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
-            else
-                hipLaunchKernelGGL((k_triplets<false, false>), dim3(grid), dim3(block), lds_bytes, ctx->stream, A);
+            void *args[] = {&A};
+            if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
+                return NC_ERR_HIP;
         }
         return hipGetLastError() == hipSuccess ? NC_OK : NC_ERR_HIP;
     };
@@ -1715,8 +1795,10 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.ms_count = ms_c;
     ctx->timing.ms_total = ms_b + ms_c;
     ctx->timing.n_kernel = A.nchunks > 0 ? 1 : 0;
+    ctx->timing.hist_layout = hm;
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
+    if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
 
     convert_out(G.nb, G.nbc, nterms, weighted, res.data(), hist, counts);
 
@@ -1731,6 +1813,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.nb = G.nb;
         p.nbc = G.nbc;
         p.nkernel = A.nchunks > 0 ? 1 : 0;
+        p.hist_layout = hm;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
                       hipSuccess &&
                   hipHostMalloc((void **)&p.h_err, 16) == hipSuccess;
@@ -1776,6 +1859,10 @@ int nc_create(nc_context **out, int device) {
     ctx->ncu = prop.multiProcessorCount;
     ctx->dev_name = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
     ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
+    if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
+        const unsigned long long v = std::strtoull(e, nullptr, 10);
+        if (v > 0 && v <= (1ull << 31)) ctx->dense_flush = v;
+    }
     for (auto &e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) {
             nc_destroy(ctx);

@@ -17,6 +17,7 @@ namespace nck {
 constexpr int kWave = 64;
 constexpr unsigned kErrDomain = 1u;
 constexpr unsigned kErrOverflow = 2u;
+constexpr unsigned kErrMode = 4u;  // a weighted term in a dense-count launch
 constexpr int kMaxWavesPerGroup = 8;
 
 // One compacted neighbour of a primary: the image position used by the
@@ -64,7 +65,8 @@ struct KArgs {
     int survey;
     double r2min, r2max;
     int nb, nbc;
-    int nslots;                   // LDS histogram slots (= nbc, compact sorted bins)
+    int nslots;                   // u64 words of the LDS histogram (nbc compact slots, or nb^3 u32 dense counts)
+    unsigned long long flush_pairs;  // dense counts: merge once this many pairs were added (<= 2^31)
     double lut_scale;
     int lut_m;
     // mixed path

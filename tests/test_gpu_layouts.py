@@ -1,0 +1,82 @@
+"""Workgroup histogram layouts of the triplet kernel, each against the oracle.
+
+Unit-weight launches count into dense nb^3 32-bit words (layout 2) unless a
+chunk could add 2^31 pairs before a merge; then, and for weighted terms, into
+the sorted slots through the slot table (layout 1).  Every layout must give the
+same, bit-exact counts; the dense counts must survive being merged after every
+chunk (NC_DENSE_FLUSH_PAIRS set low in a fresh context).
+"""
+import numpy as np
+import pytest
+
+from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, Context
+from cudancorr_amd.gen import edges_linear, neyman_scott_box, prep_cat_survey, survey_cap
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+PATHS = [NC_PATH_PLAIN, NC_PATH_MIXED]
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _box_case():
+    p = neyman_scott_box(400, 40, 3.0, 220.0, seed=31)
+    edges = edges_linear(0.0, 24.0, 12)
+    box = np.array([220.0] * 3)
+    want, _ = oracle.count(p, p, p, edges, box)
+    return p, edges, box, want
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_dense_and_slot_table_layouts(ctx, path):
+    _log(f"layouts path={path}")
+    p, edges, box, want = _box_case()
+    assert want.sum() > 100_000
+    before = ctx.get_tuning()
+    try:
+        _, got = ctx.count(p, p, p, edges, box, path=path)
+        assert ctx.timing().hist_layout == 2
+        assert np.array_equal(got, want)
+        # chunks of 2^20 primaries could add more than 2^31 pairs: slot table
+        ctx.set_tuning(-1, 1 << 20, -1, 0)
+        _, got = ctx.count(p, p, p, edges, box, path=path)
+        assert ctx.timing().hist_layout == 1
+        assert np.array_equal(got, want)
+    finally:
+        ctx.set_tuning(*before)
+
+
+def test_weighted_terms_use_sorted_slots(ctx):
+    _log("layouts weighted")
+    d, r = survey_cap(1500, 2500, seed=8)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.1
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 40.0, 16)
+    _, ws = oracle.count(d, d, r, edges, box, survey=True)
+    hist, _ = ctx.count(d, d, r, edges, box, survey=True, path=NC_PATH_MIXED)
+    assert ctx.timing().hist_layout == 1
+    nz = ws != 0
+    assert np.array_equal(hist[~nz], ws[~nz])
+    assert np.max(np.abs(hist[nz] - ws[nz]) / ws[nz]) <= 1e-9
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_dense_counts_merged_every_chunk(monkeypatch, path):
+    _log(f"layouts dense flush path={path}")
+    p, edges, box, want = _box_case()
+    monkeypatch.setenv("NC_DENSE_FLUSH_PAIRS", "1")
+    with Context(0) as c2:
+        c2.set_tuning(-1, 4, -1, 0)  # many small chunks, each followed by a merge
+        _, got = c2.count(p, p, p, edges, box, path=path)
+        assert c2.timing().hist_layout == 2
+        assert np.array_equal(got, want)
+        # batched (graph capture and replays) with the same merges
+        cat = c2.catalog(p)
+        for _ in range(3):
+            _, cnt = c2.count_batched([(cat, cat, cat), (cat, cat, cat)], edges, box, path=path)
+            assert np.array_equal(cnt[0], want) and np.array_equal(cnt[1], want)
+        cat.close()
