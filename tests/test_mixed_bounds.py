@@ -36,13 +36,13 @@ def kernel_scale_exponent(edges):
     return int(np.ceil(np.log2(512.0 / (1.25 * float(edges[-1]) ** 2)) / 2.0))
 
 
-def mixed_decisions(P, A, B, edges, kk=0):
+def mixed_decisions(P, A, B, edges, kk=0, guard=80.0):
     R = float(edges[-1])
     sg = 2.0 ** kk
     s32 = sg * sg
     sq = edges * edges
     E32 = f32(f32(sq) * s32)
-    g = 80.0 * U * R * R * s32
+    g = guard * U * R * R * s32
     c1 = np.float32(80.0 * U * R * R * s32)
     c2 = np.float32(64.0 * U * R ** 4 * s32 * s32)
     ra = f32((A - P) * sg)
@@ -78,17 +78,17 @@ def reference(P, A, B, edges):
     return inr, k, tri
 
 
-def check(P, A, B, edges):
+def check(P, A, B, edges, guard=80.0):
     sq = edges * edges
     # only list entries are paired: both within (rmin, rmax) of the primary
     da = np.sum((A - P) ** 2, 1)
     db = np.sum((B - P) ** 2, 1)
     keep = (da > sq[0]) & (da < sq[-1]) & (db > sq[0]) & (db < sq[-1])
     P, A, B = P[keep], A[keep], B[keep]
-    decided, accepted, k32 = mixed_decisions(P, A, B, edges, kernel_scale_exponent(edges))
+    decided, accepted, k32 = mixed_decisions(P, A, B, edges, kernel_scale_exponent(edges), guard)
     # the scaling is exact: unscaled and other scales decide identically
     for kk in (0, -7, 9):
-        d2, a2, k2 = mixed_decisions(P, A, B, edges, kk)
+        d2, a2, k2 = mixed_decisions(P, A, B, edges, kk, guard)
         assert np.array_equal(d2, decided) and np.array_equal(a2, accepted) and np.array_equal(k2, k32)
     inr, k, tri = reference(P, A, B, edges)
     # decided pairs: out of range iff the reference's value is out of range
@@ -156,3 +156,24 @@ def test_near_collinear(scale):
     B = P + v * tb[:, None] + w * h[:, None]
     frac, acc = check(P, A, B, edges)
     assert acc > 0
+
+
+@pytest.mark.parametrize("scale", [5.0, 1500.0])
+def test_pairs_on_log_edges(scale):
+    """Log-spaced edges and |a - b| within ~1e-6 of an edge while a and b sit
+    far from the primary, where r23^2 = s_a + s_b - 2 dot loses the most to
+    cancellation (errors far above the spread of the pairs around the edge)."""
+    rng = np.random.default_rng(4)
+    edges = np.logspace(0.0, np.log10(30.0), 16)
+    n = 400_000
+    P = rng.uniform(0, 60, (n, 3)) + scale
+    A = around(rng, n, P, 29.0, lo=15.0)
+    e = edges[rng.integers(1, edges.size - 1, n)]
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    B = A + v * (e * (1.0 + rng.normal(0, 1e-6, n)))[:, None]
+    frac, _ = check(P, A, B, edges)
+    assert frac < 1.0
+    # the same pairs without the guard band: some get a wrong bin
+    with pytest.raises(AssertionError):
+        check(P, A, B, edges, guard=0.0)
