@@ -71,14 +71,15 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per triplet-kernel launch measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), if a summary exists."""
+def from_profiles(key):
+    """A figure of the committed rocprofv3 summary of this workload's triplet
+    kernel: traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel) or valu_busy_pct (VALUBusy counter), None if absent."""
     path = os.path.join(ROOT, "profiles", "k_triplets_mixed_batched_boxes_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -220,7 +221,8 @@ def main():
         "tuning": {"mixed": tune_m, "plain": tune_p},
         "speedup_vs_plain": (plain["wall_ms"] / (wall_m * 1e3)) if plain else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profiles(),
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": from_profiles("traffic_bytes_per_launch"),
+                     "valu_busy_pct": from_profiles("valu_busy_pct"),
                      "kernel": "k_triplets (mixed, batched)", "kernel_ms": count_ms_m},
         "cpu_baseline": cpu,
     }

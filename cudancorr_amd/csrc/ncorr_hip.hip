@@ -335,17 +335,19 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
             const bool va = ja < end, vb = jb < end;
             double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0, bx = 0.0, by = 0.0, bz = 0.0, bw = 0.0;
             // image position: point + shift (device.cu:282-284)
+            // (weights only for the plain list: the mixed list reads them from
+            // the catalogue where they are needed)
             if (va) {
                 ax = X[ja] + shx;
                 ay = Y[ja] + shy;
                 az = Z[ja] + shz;
-                aw = Wt[ja];
+                if (!MIXED) aw = Wt[ja];
             }
             if (vb) {
                 bx = X[jb] + shx;
                 by = Y[jb] + shy;
                 bz = Z[jb] + shz;
-                bw = Wt[jb];
+                if (!MIXED) bw = Wt[jb];
             }
             keep(ja, va, ax, ay, az, aw, s);
             if (j0 + kWave < end) keep(jb, vb, bx, by, bz, bw, s);
