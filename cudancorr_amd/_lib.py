@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(LIB_DIR, "libncorr_hip.so")
 
 NC_PATH_PLAIN = 0
 NC_PATH_MIXED = 1
+NC_BIN_THREED = 0
+NC_BIN_RPPI = 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "invalid bin edges", 3: "point outside the cell grid",
           4: "out of memory", 5: "HIP error", 6: "size limit"}
@@ -23,14 +25,15 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "invalid bin edges", 3: "point outs
 # Every entry point declared in include/ncorr_hip.h.
 EXPORTS = (
     "nc_create", "nc_destroy", "nc_last_error", "nc_catalog_create", "nc_catalog_wrap_device",
-    "nc_catalog_destroy", "nc_catalog_size", "nc_hist_size", "nc_compact_size", "nc_count", "nc_count_host",
-    "nc_count_batched", "nc_last_timing", "nc_set_tuning", "nc_get_tuning", "nc_autotune",
+    "nc_catalog_destroy", "nc_catalog_size", "nc_hist_size", "nc_compact_size", "nc_hist_len", "nc_count",
+    "nc_count_host", "nc_count_batched", "nc_last_timing", "nc_set_tuning", "nc_get_tuning", "nc_autotune",
 )
 
 
 class NcParams(ctypes.Structure):
     _fields_ = [("box", ctypes.c_double * 3), ("survey", ctypes.c_int), ("nedges", ctypes.c_int),
-                ("edges", ctypes.POINTER(ctypes.c_double))]
+                ("edges", ctypes.POINTER(ctypes.c_double)), ("bin_type", ctypes.c_int),
+                ("num_pi_bins", ctypes.c_int), ("ori", ctypes.c_double * 3)]
 
 
 class NcTiming(ctypes.Structure):
@@ -77,6 +80,8 @@ def lib():
     L.nc_catalog_size.restype = i64
     L.nc_hist_size.argtypes = [ctypes.c_int]
     L.nc_hist_size.restype = i64
+    L.nc_hist_len.argtypes = [ctypes.POINTER(NcParams)]
+    L.nc_hist_len.restype = i64
     L.nc_compact_size.argtypes = [ctypes.c_int]
     L.nc_compact_size.restype = i64
     L.nc_count.argtypes = [vp, ctypes.POINTER(NcParams), vp, vp, vp, ctypes.c_int, dp, up]
@@ -157,22 +162,32 @@ class Context:
             pass
 
     @staticmethod
-    def params(edges, box, survey=False):
+    def params(edges, box, survey=False, npi=0, ori=(0.0, 0.0, 0.0)):
+        """nc_params for threeD binning, or rppi binning when npi > 0 (npi
+        unit-width pi bins; survey line of sight from ori)."""
         e = np.ascontiguousarray(edges, dtype=np.float64)
         p = NcParams()
         for i in range(3):
             p.box[i] = float(box[i])
+            p.ori[i] = float(ori[i])
         p.survey = int(bool(survey))
         p.nedges = e.size
         p.edges = _dp(e)
+        p.bin_type = NC_BIN_RPPI if npi > 0 else NC_BIN_THREED
+        p.num_pi_bins = int(npi)
         return p, e
+
+    @staticmethod
+    def _shape(nb, npi):
+        return (nb, nb, nb, npi, npi) if npi > 0 else (nb, nb, nb)
 
     def catalog(self, points) -> Catalog:
         return Catalog(self, points)
 
-    def count(self, c1, c2, c3, edges, box, survey=False, path=NC_PATH_PLAIN):
+    def count(self, c1, c2, c3, edges, box, survey=False, path=NC_PATH_PLAIN, npi=0, ori=(0.0, 0.0, 0.0)):
         """nc_count on catalogues (Catalog or point arrays).  Returns
-        (hist float64 (nb,nb,nb), counts uint64 (nb,nb,nb))."""
+        (hist float64, counts uint64), shaped (nb,nb,nb) or, for rppi (npi >
+        0), (nb,nb,nb,npi,npi)."""
         owned = []
         cats = []
         seen = {}
@@ -185,56 +200,58 @@ class Context:
                     seen[key] = Catalog(self, c)
                     owned.append(seen[key])
                 cats.append(seen[key])
-        p, e = self.params(edges, box, survey)
-        nb = e.size - 1
-        hist = np.zeros(max(nb, 1) ** 3, dtype=np.float64)
-        cnt = np.zeros(max(nb, 1) ** 3, dtype=np.uint64)
+        p, e = self.params(edges, box, survey, npi, ori)
+        shape = self._shape(e.size - 1, npi)
+        hist = np.zeros(max(int(np.prod(shape)), 1), dtype=np.float64)
+        cnt = np.zeros(hist.size, dtype=np.uint64)
         try:
             self._check(lib().nc_count(self.h, ctypes.byref(p), cats[0].h, cats[1].h, cats[2].h, int(path),
                                        _dp(hist), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
         finally:
             for c in owned:
                 c.close()
-        return hist.reshape(nb, nb, nb), cnt.reshape(nb, nb, nb)
+        n = int(np.prod(shape))
+        return hist[:n].reshape(shape), cnt[:n].reshape(shape)
 
-    def count_host(self, p1, p2, p3, edges, box, survey=False, path=NC_PATH_PLAIN):
+    def count_host(self, p1, p2, p3, edges, box, survey=False, path=NC_PATH_PLAIN, npi=0, ori=(0.0, 0.0, 0.0)):
         """nc_count_host: the drop-in host-buffer form."""
         a1, a2, a3 = _points(p1), None, None
         a2 = a1 if p2 is p1 else _points(p2)
         a3 = a2 if p3 is p2 else (a1 if p3 is p1 else _points(p3))
-        p, e = self.params(edges, box, survey)
-        nb = e.size - 1
-        hist = np.zeros(max(nb, 1) ** 3, dtype=np.float64)
-        cnt = np.zeros(max(nb, 1) ** 3, dtype=np.uint64)
+        p, e = self.params(edges, box, survey, npi, ori)
+        shape = self._shape(e.size - 1, npi)
+        n = int(np.prod(shape))
+        hist = np.zeros(max(n, 1), dtype=np.float64)
+        cnt = np.zeros(hist.size, dtype=np.uint64)
         self._check(lib().nc_count_host(self.h, ctypes.byref(p), _dp(a1), a1.shape[0], _dp(a2), a2.shape[0],
                                         _dp(a3), a3.shape[0], int(path), _dp(hist),
                                         cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
-        return hist.reshape(nb, nb, nb), cnt.reshape(nb, nb, nb)
+        return hist[:n].reshape(shape), cnt[:n].reshape(shape)
 
-    def count_batched(self, terms, edges, box, survey=False, path=NC_PATH_MIXED):
+    def count_batched(self, terms, edges, box, survey=False, path=NC_PATH_MIXED, npi=0, ori=(0.0, 0.0, 0.0)):
         """nc_count_batched over a list of (Catalog, Catalog, Catalog) terms.
-        Returns (hist (T,nb,nb,nb), counts (T,nb,nb,nb))."""
+        Returns (hist (T,...), counts (T,...)) with the shape of count()."""
         T = len(terms)
         arr = ctypes.c_void_p * T
         c1 = arr(*[t[0].h for t in terms])
         c2 = arr(*[t[1].h for t in terms])
         c3 = arr(*[t[2].h for t in terms])
-        p, e = self.params(edges, box, survey)
-        nb = e.size - 1
-        hist = np.zeros(T * nb ** 3, dtype=np.float64)
-        cnt = np.zeros(T * nb ** 3, dtype=np.uint64)
+        p, e = self.params(edges, box, survey, npi, ori)
+        shape = self._shape(e.size - 1, npi)
+        hist = np.zeros(T * int(np.prod(shape)), dtype=np.float64)
+        cnt = np.zeros(hist.size, dtype=np.uint64)
         self._check(lib().nc_count_batched(self.h, ctypes.byref(p), T, c1, c2, c3, int(path), _dp(hist),
                                            cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
-        return hist.reshape(T, nb, nb, nb), cnt.reshape(T, nb, nb, nb)
+        return hist.reshape((T,) + shape), cnt.reshape((T,) + shape)
 
-    def autotune(self, terms, edges, box, survey=False, path=NC_PATH_MIXED):
+    def autotune(self, terms, edges, box, survey=False, path=NC_PATH_MIXED, npi=0, ori=(0.0, 0.0, 0.0)):
         """nc_autotune over (Catalog, Catalog, Catalog) terms; returns a dict."""
         T = len(terms)
         arr = ctypes.c_void_p * T
         c1 = arr(*[t[0].h for t in terms])
         c2 = arr(*[t[1].h for t in terms])
         c3 = arr(*[t[2].h for t in terms])
-        p, e = self.params(edges, box, survey)
+        p, e = self.params(edges, box, survey, npi, ori)
         best = NcTuning()
         self._check(lib().nc_autotune(self.h, ctypes.byref(p), T, c1, c2, c3, int(path), ctypes.byref(best)))
         return {f: getattr(best, f) for f, _ in NcTuning._fields_}

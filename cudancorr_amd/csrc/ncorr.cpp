@@ -2,10 +2,11 @@
 // of libncorr_hip.so.  Host C++ only; no HIP header is included.
 //
 // Reference: source/cudaNcorr.cu.  Constructor geometry and the output bin
-// list follow cudaNcorr.cu:118-157 (bin mids cudaNcorr.cu:26-35, output filter
-// mid_k <= mid_i + mid_j at cudaNcorr.cu:148); calculateD1D2D3 replaces
-// cudaNcorr.cu:220-430 (gridding, per-cell uploads, kernel launch, copy-back)
-// with one nc_count_host call; the compaction follows cudaNcorr.cu:72-86.
+// list follow cudaNcorr.cu:118-157 (threeD) and 159-216 (rppi) (bin mids
+// cudaNcorr.cu:26-35, output filter mid_k <= mid_i + mid_j at cudaNcorr.cu:148);
+// calculateD1D2D3 replaces cudaNcorr.cu:220-430 (gridding, per-cell uploads,
+// kernel launch, copy-back) with one nc_count_host call; the compactions follow
+// cudaNcorr.cu:72-86 and 88-105.
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -64,9 +65,37 @@ ncorr::ncorr(double3 box, int bin_type, std::vector<double> &bins, bool isSurvey
 
 ncorr::ncorr(double3 box, int bin_type, std::vector<double> &bins, int numMinorBins, bool isSurvey, double3 ori)
     : binType_(static_cast<binType>(bin_type)), isSurvey_(isSurvey), bins_(bins), box_(box), ori_(ori) {
-    (void)numMinorBins;
+    numMinorBins_ = numMinorBins;
     if (binType_ != rppi) throw std::runtime_error("Invalid constructor call for bin_type.");
-    throw std::runtime_error("ncorr: rppi binning is not supported by this engine (threeD only)");
+    validate_edges(bins_);
+    if (numMinorBins < 1) throw std::invalid_argument("ncorr: need at least one pi bin");
+    numBins_ = (int)bins_.size() - 1;
+    binsMid_ = getBinsMid(bins_);
+    rpMax_ = bins_.back();
+    rpMin_ = bins_[0];
+    for (double i = 0; i <= static_cast<double>(numMinorBins); i = i + 1.0) piBins_.push_back(i);
+    numPiBins_ = (int)piBins_.size() - 1;
+    if (numPiBins_ != numMinorBins_) throw std::runtime_error("number of piBins do not match input, could be a bug");
+    piMax_ = piBins_.back();
+    const double sqrRMax = rpMax_ * rpMax_ + piMax_ * piMax_;
+    rMax_ = sqrtf((float)sqrRMax);  // FP32, as cudaNcorr.cu:187
+    rMin_ = rpMin_;
+    numBinsCounts_ = numBins_ * numBins_ * numBins_ * numPiBins_ * numPiBins_;
+    cellSize_ = isSurvey_ ? double3{rMax_, rMax_, rMax_} : double3{rpMax_, rpMax_, piMax_};
+    numCells_.x = static_cast<int>(std::ceil(box_.x / cellSize_.x));
+    numCells_.y = static_cast<int>(std::ceil(box_.y / cellSize_.y));
+    numCells_.z = static_cast<int>(std::ceil(box_.z / cellSize_.z));
+    for (int i = 0; i < numBins_; ++i)
+        for (int j = i; j < numBins_; ++j)
+            for (int k = j; k < numBins_; ++k)
+                if (binsMid_[k] <= binsMid_[i] + binsMid_[j])
+                    for (int l = 0; l < numPiBins_; ++l)
+                        for (int m = 0; m < numPiBins_; ++m) {
+                            triCountsOut_.push_back(0.0);
+                            triBinsOut_.push_back({binsMid_[i], binsMid_[j], binsMid_[k]});
+                            piBinsOut_.push_back({piBins_[l], piBins_[m]});
+                        }
+    triCountsAll_.assign(numBinsCounts_, 0.0);
 }
 
 void ncorr::setDevice(int device) {
@@ -86,13 +115,20 @@ nc_context *ncorr::context() {
 
 void ncorr::calculateD1D2D3(std::vector<double4> &p1, std::vector<double4> &p2, std::vector<double4> &p3) {
     nc_context *ctx = context();
-    nc_params P;
+    nc_params P{};
     P.box[0] = box_.x;
     P.box[1] = box_.y;
     P.box[2] = box_.z;
     P.survey = isSurvey_ ? 1 : 0;
     P.nedges = (int)bins_.size();
     P.edges = bins_.data();
+    if (binType_ == rppi) {
+        P.bin_type = NC_BIN_RPPI;
+        P.num_pi_bins = numPiBins_;
+        P.ori[0] = ori_.x;
+        P.ori[1] = ori_.y;
+        P.ori[2] = ori_.z;
+    }
     triCountsAll_.assign(numBinsCounts_, 0.0);
     const double *d1 = reinterpret_cast<const double *>(p1.data());
     const double *d2 = &p2 == &p1 ? d1 : reinterpret_cast<const double *>(p2.data());
@@ -103,7 +139,25 @@ void ncorr::calculateD1D2D3(std::vector<double4> &p1, std::vector<double4> &p2, 
           "calculateD1D2D3");
     nc_timing t;
     if (nc_last_timing(ctx, &t) == NC_OK) lastMs_ = t.ms_total;
-    prepTriCountsOutThreeD();
+    if (binType_ == rppi)
+        prepTriCountsOutRppi();
+    else
+        prepTriCountsOutThreeD();
+}
+
+// cudaNcorr.cu:88-105: the threeD selection, each entry followed by its
+// npi^2 pi bins.
+void ncorr::prepTriCountsOutRppi() {
+    int iout = 0;
+    const int np = numPiBins_, nb = numBins_;
+    for (int i = 0; i < nb; ++i)
+        for (int j = i; j < nb; ++j)
+            for (int k = j; k < nb; ++k)
+                if (binsMid_[k] <= binsMid_[i] + binsMid_[j])
+                    for (int l = 0; l < np; ++l)
+                        for (int m = 0; m < np; ++m)
+                            triCountsOut_[iout++] =
+                                triCountsAll_[m + l * np + k * np * np + j * np * np * nb + i * np * np * nb * nb];
 }
 
 void ncorr::prepTriCountsOutThreeD() {

@@ -48,6 +48,15 @@ __device__ __forceinline__ double sqrsep(double dx, double dy, double dz) {
     return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
 }
 
+// sqrRpBox (device.cu:38-43): dx*dx + dy*dy, contracted in the same
+// left-to-right order as sqrsep.
+__device__ __forceinline__ double rp2(double dx, double dy) { return __builtin_fma(dy, dy, dx * dx); }
+
+// dotProduct_d3 (device.cu:23-25) in the same order.
+__device__ __forceinline__ double dot3(double ax, double ay, double az, double bx, double by, double bz) {
+    return __builtin_fma(az, bz, __builtin_fma(ay, by, ax * bx));
+}
+
 // Cell index along one axis: int(v / cs) with truncation (cudaNcorr.cu:51-56,
 // device.cu:76), optional clamp of ==n to n-1.  Out of grid => bad.
 __device__ __forceinline__ int axis_cell(double v, double cs, int n, bool clamp, bool &bad) {
@@ -258,9 +267,20 @@ struct LdsTabs {
     const uint16_t *t3;
 };
 
+// Neighbour-list flavours: threeD plain (Elem + weight), threeD mixed (MList),
+// rppi box and rppi survey (RElem).
+constexpr int kListPlain = 0, kListMixed = 1, kListRppiBox = 2, kListRppiSurvey = 3;
+
+// One rppi list entry: image position, and to the primary rp^2 and pi^2 (box)
+// or the squared 3D separation and x - ori.x (survey), and the weight.
+struct RElem {
+    double x, y, z, r, q, w;
+};
+
 // Bytes of one list entry in a wave's scratch slot.
-__host__ __device__ constexpr size_t list_entry_bytes(bool mixed) {
-    return mixed ? sizeof(float4) + sizeof(int2) + sizeof(int) : sizeof(Elem) + sizeof(double);
+__host__ __device__ constexpr size_t list_entry_bytes(int lm) {
+    return lm == kListMixed ? sizeof(float4) + sizeof(int2) + sizeof(int)
+                            : lm == kListPlain ? sizeof(Elem) + sizeof(double) : sizeof(RElem);
 }
 
 // Arrays of a mixed-path list inside a wave's scratch slot.
@@ -294,13 +314,35 @@ struct CatRef {
 // 64-point blocks per step (both loads in flight together); the list order is
 // the reference's entry order either way.  Each list entry records a
 // neighbour entry s whose shift is its own (the middle one for a merged range).
-template <bool MIXED>
+template <int LM>
 __device__ int build_list(const KArgs &A, const double *__restrict__ X, const double *__restrict__ Y,
                           const double *__restrict__ Z, const double *__restrict__ Wt, const uint32_t *__restrict__ st,
                           double px, double py, double pz, int cx, int cy, int cz, char *__restrict__ base, int lane,
                           const double *sq, const uint16_t *lut) {
+    constexpr bool MIXED = LM == kListMixed;
     int n = 0;
     auto keep = [&](int j, bool v, double qx, double qy, double qz, double qw, int s) {
+        if constexpr (LM == kListRppiBox || LM == kListRppiSurvey) {
+            // rppi: box keeps pi^2 < piMax^2 and rpMin^2 < rp^2 < rpMax^2
+            // (device.cu:343-346), survey 0 < s^2 < sqrRMax (device.cu:454)
+            double r, q;
+            bool pass;
+            if constexpr (LM == kListRppiBox) {
+                const double dz = pz - qz;
+                q = dz * dz;
+                r = rp2(px - qx, py - qy);
+                pass = v & (q < A.pimax2) & (r < A.r2max) & (r > A.r2min);
+            } else {
+                r = sqrsep(px - qx, py - qy, pz - qz);
+                q = qx - A.ori_x;
+                pass = v & (r < A.smax2) & (r > 0.0);
+            }
+            const unsigned long long m = __ballot(pass);
+            const int pos = n + lane_rank(m);
+            if (pass & (pos < A.cap)) reinterpret_cast<RElem *>(base)[pos] = RElem{qx, qy, qz, r, q, qw};
+            n += __popcll(m);
+            return;
+        }
         const double d = sqrsep(px - qx, py - qy, pz - qz);
         const bool pass = v & (d > A.r2min) & (d < A.r2max);
         const unsigned long long m = __ballot(pass);
@@ -395,6 +437,7 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
 //               into the sorted slots when flushed (unit-weight terms only).
 // kHistT3 and kHistDense also take the single-step FP64 bin lookup.
 constexpr int kHistCtab = 0, kHistT3 = 1, kHistDense = 2;
+constexpr int kHistGlobal = 3;  // rppi: the dense histogram in global memory, no LDS copy
 
 // Byte offset in the LDS histogram of the slot of bins (ka, kb, k), k in
 // [-1, nb] (slots of k outside [0, nb) are never added).
@@ -1023,7 +1066,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     char *wlds = lds + A.off_wave + wave * A.wave_bytes;
     char *slot = A.scratch + (long long)(blockIdx.x * nw + wave) * A.slot_bytes;
     char *l2 = slot;
-    char *l3 = slot + (size_t)A.cap * list_entry_bytes(MIXED);
+    char *l3 = slot + (size_t)A.cap * list_entry_bytes(MIXED ? kListMixed : kListPlain);
     int cur = -1, cur_weighted = 0;
     unsigned long long wpairs = 0ull;  // pairs this wave added in the current chunk
     __syncthreads();
@@ -1063,9 +1106,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
                 continue;
             }
             const int n2 =
-                build_list<MIXED>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq, t.lut);
+                build_list<MIXED ? kListMixed : kListPlain>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz,
+                                                            l2, lane, t.sq, t.lut);
             const int n3 = T.sym ? n2
-                                 : build_list<MIXED>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3,
+                                 : build_list<MIXED ? kListMixed : kListPlain>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py,
+                                                                               pz, cx, cy, cz, l3,
                                                      lane, t.sq, t.lut);
             // wave-uniform lengths keep the pair loops' control in scalar registers
             const int n2u = uniform(n2), n3u = uniform(n3);
@@ -1112,6 +1157,261 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (cur >= 0) flush_hist<HM>(A, cur, cur_weighted, t.hist, &s_pairs);
 }
 
+// ---------------------------------------------------------------------------
+// rppi binning (projected separation rp, line-of-sight separation pi).
+
+// getTriangleIndexRppi (device.cu:203-255): sort the sides by rp^2 carrying
+// their pi^2, FP32 triangle test on rp, rp bins, pi bins floor(sqrtf(pi^2))
+// of the two smallest-rp sides, the reference's flat index.  -1: rejected;
+// -2: the flat index falls past the histogram (a pi index equal to npi on the
+// last bins; the reference writes out of bounds there).
+__device__ __forceinline__ long long rppi_index(const KArgs &A, const LdsTabs &t, double r1, double r2, double r3,
+                                                double q1, double q2, double q3) {
+    double x;
+    if (r1 > r2) {
+        x = r1; r1 = r2; r2 = x;
+        x = q1; q1 = q2; q2 = x;
+    }
+    if (r1 > r3) {
+        x = r1; r1 = r3; r3 = x;
+        x = q1; q1 = q3; q3 = x;
+    }
+    if (r2 > r3) {
+        x = r2; r2 = r3; r3 = x;
+        x = q2; q2 = q3; q3 = x;
+    }
+    if (__builtin_sqrtf((float)r1) + __builtin_sqrtf((float)r2) <= __builtin_sqrtf((float)r3)) return -1;
+    const long long k1 = bin_any<false>(A, t, r1), k2 = bin_any<false>(A, t, r2), k3 = bin_any<false>(A, t, r3);
+    const long long p1 = (long long)floorf(__builtin_sqrtf((float)q1));
+    const long long p2 = (long long)floorf(__builtin_sqrtf((float)q2));
+    const long long np = A.npi, nb = A.nb;
+    const long long idx = p2 + p1 * np + (k3 + k2 * nb + k1 * nb * nb) * np * np;
+    return idx < (long long)A.nbc ? idx : -2;
+}
+
+// Pair evaluation of the rppi kernels on the plain FP64 path: rows from the
+// wave's LDS chunk (broadcast), lane entries in registers.  Box (SURVEY
+// false): rp^2 and pi^2 of the third side from the image positions; with
+// p2 and p3 the same catalogue each unordered pair is evaluated once and
+// counted for both orders (evaluated twice when two rp^2 tie, since the sort
+// then keeps the input order of the tied sides).  Survey: every ordered pair,
+// as the line of sight (p1 + p2 + p3 - 3 ori) depends on the order of the
+// FP64 sum.  Counts go straight to the term's dense global histogram.
+template <bool SURVEY, bool SYM, bool WEIGHTED>
+struct RppiPol {
+    static constexpr bool kSym = SYM;
+    static constexpr int U = 2;
+    const KArgs &A;
+    const LdsTabs &t;
+    const RElem *L2, *L3;
+    RElem *la;
+    unsigned long long *out;
+    double px, py, pz, ppx, w1;
+    int C, lane, row_first;
+    RElem eb;
+    double bsx, bsy, bsz;  // p1 - lane entry (survey)
+    bool vb;
+
+    __device__ RppiPol(const KArgs &A_, const LdsTabs &t_, const char *l2, const char *l3, char *wlds, int lane_,
+                       unsigned long long *out_, double px_, double py_, double pz_, double w1_)
+        : A(A_), t(t_), out(out_), px(px_), py(py_), pz(pz_), w1(w1_), lane(lane_) {
+        C = A.chunk_elems;
+        L2 = reinterpret_cast<const RElem *>(l2);
+        L3 = reinterpret_cast<const RElem *>(l3);
+        la = reinterpret_cast<RElem *>(wlds);
+        ppx = px - A.ori_x;
+        row_first = 0;
+    }
+
+    __device__ void stage(bool from3, int first, int last) {
+        const RElem *L = from3 ? L3 : L2;
+        wave_sync();
+        for (int i = lane; i < last - first + 4; i += kWave) {
+            RElem e{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (i < last - first) e = L[first + i];
+            la[i] = e;
+        }
+        row_first = first;
+        wave_sync();
+    }
+
+    __device__ void load_lanes(bool from3, int b0, int end) {
+        const int b = b0 + lane;
+        vb = b < end;
+        eb = RElem{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (vb) eb = (from3 ? L3 : L2)[b];
+        bsx = px - eb.x;
+        bsy = py - eb.y;
+        bsz = pz - eb.z;
+    }
+
+    __device__ __forceinline__ void add(long long idx, double v, unsigned long long n) {
+        if (idx < 0) return;
+        if (WEIGHTED)
+            unsafeAtomicAdd(reinterpret_cast<double *>(&out[idx]), v);
+        else
+            atomicAdd(&out[idx], n);
+    }
+
+    // one (p2, p3) pair; sa: the row holds p2 (else p3).  s2a: p1 - p2.
+    __device__ __forceinline__ void pair(const RElem &e2, const RElem &e3, double s2x, double s2y, double s2z,
+                                         double s3x, double s3y, double s3z, bool ok) {
+        const double dx = e2.x - e3.x, dy = e2.y - e3.y, dz = e2.z - e3.z;  // p2 - p3
+        if constexpr (!SURVEY) {
+            const double q3 = dz * dz, r3 = rp2(dx, dy);  // device.cu:357-361
+            if (!(ok && q3 < A.pimax2 && r3 < A.r2max && r3 > A.r2min)) return;
+            const long long i23 = rppi_index(A, t, e2.r, e3.r, r3, e2.q, e3.q, q3);
+            if (SYM) {
+                const bool tie = e2.r == e3.r || e2.r == r3 || e3.r == r3;
+                if (tie) {
+                    const long long i32 = rppi_index(A, t, e3.r, e2.r, r3, e3.q, e2.q, q3);
+                    add(i23, (w1 * e2.w) * e3.w, 1ull);
+                    add(i32, (w1 * e3.w) * e2.w, 1ull);
+                } else {
+                    add(i23, (w1 * e2.w) * e3.w + (w1 * e3.w) * e2.w, 2ull);
+                }
+            } else {
+                add(i23, (w1 * e2.w) * e3.w, 1ull);
+            }
+        } else {
+            const double s23 = sqrsep(dx, dy, dz);  // device.cu:465-466
+            if (!(ok && s23 < A.smax2 && s23 > 0.0)) return;
+            // getLOS (device.cu:51-60): pos = p - ori with ori = (ori.x, 0, 0)
+            const double lx = (ppx + e2.q) + e3.q, ly = (py + e2.y) + e3.y, lz = (pz + e2.z) + e3.z;
+            const double sl = dot3(lx, ly, lz, lx, ly, lz);
+            const double lim = A.pimax2 * sl;
+            double t1 = dot3(lx, ly, lz, s2x, s2y, s2z);
+            t1 = t1 * t1;
+            double t2 = dot3(lx, ly, lz, s3x, s3y, s3z);
+            t2 = t2 * t2;
+            double t3 = dot3(lx, ly, lz, dx, dy, dz);
+            t3 = t3 * t3;
+            if (lim < t1 || lim < t2 || lim < t3) return;  // device.cu:470-475
+            const double q1 = t1 / sl, q2 = t2 / sl, q3 = t3 / sl;  // getSqrRpSqrPi, device.cu:68-71
+            const double r1 = e2.r - q1, r2 = e3.r - q2, r3 = s23 - q3;
+            if (!(r1 < A.r2max && r2 < A.r2max && r3 < A.r2max && r1 > A.r2min && r2 > A.r2min && r3 > A.r2min &&
+                  q1 < A.pimax2 && q2 < A.pimax2))
+                return;
+            add(rppi_index(A, t, r1, r2, r3, q1, q2, q3), (w1 * e2.w) * e3.w, 1ull);
+        }
+    }
+
+    template <bool SWAP>
+    __device__ __forceinline__ void row(const RElem &ea, bool ok) {
+        if (SWAP)  // row = p3, lane = p2
+            pair(eb, ea, bsx, bsy, bsz, px - ea.x, py - ea.y, pz - ea.z, ok);
+        else
+            pair(ea, eb, px - ea.x, py - ea.y, pz - ea.z, bsx, bsy, bsz, ok);
+    }
+
+    template <bool SWAP>
+    __device__ void rect(int r0, int r1) {
+        for (int a = r0; a < r1; ++a) row<SWAP>(la[a - row_first], vb);
+    }
+
+    __device__ void circ(int b0, int m) {
+        const int smax = m / 2;
+        for (int st = 1; st <= smax; ++st) {
+            int off = lane + st;
+            off -= off >= m ? m : 0;
+            const bool ok = vb && (2 * st != m || lane < st);
+            row<false>(la[b0 - row_first + (ok ? off : 0)], ok);
+        }
+    }
+};
+
+// Persistent rppi kernel (countTrianglesRppiBox / countTrianglesRppiSurvey,
+// device.cu:317-379, 434-505): the work queue, list build and traversal of
+// k_triplets with the rppi filters and pair arithmetic.
+template <bool SURVEY>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_rppi(KArgs A) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    __shared__ int s_chunk;
+    __shared__ int s_next;
+    LdsTabs t;
+    t.hist = nullptr;
+    double *sq = reinterpret_cast<double *>(lds + A.off_sq);
+    uint16_t *lut = reinterpret_cast<uint16_t *>(lds + A.off_lut);
+    t.sq = sq;
+    t.lut = lut;
+    t.ctab = nullptr;
+    t.lut32 = nullptr;
+    t.t3 = nullptr;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
+    for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
+    constexpr int LM = SURVEY ? kListRppiSurvey : kListRppiBox;
+    const int lane = tid & (kWave - 1), wave = tid / kWave, nw = nthr / kWave;
+    char *wlds = lds + A.off_wave + wave * A.wave_bytes;
+    char *slot = A.scratch + (long long)(blockIdx.x * nw + wave) * A.slot_bytes;
+    char *l2 = slot;
+    char *l3 = slot + (size_t)A.cap * list_entry_bytes(LM);
+    __syncthreads();
+    for (;;) {
+        if (tid == 0) {
+            s_chunk = (int)atomicAdd(A.counter, 1u);
+            s_next = 0;
+        }
+        __syncthreads();
+        const int ci = uniform(s_chunk);
+        if (ci >= A.nchunks) break;
+        const Chunk ch = A.chunks[ci];
+        const TermDesc T = A.terms[ch.term];
+        unsigned long long *out = A.out + (long long)ch.term * A.nbc;
+        for (;;) {
+            const int p = uniform(atomicAdd(&s_next, lane == 0 ? 1 : 0)) + ch.begin;
+            if (p >= ch.end) break;
+            const double px = uniform(T.x1[p]), py = uniform(T.y1[p]), pz = uniform(T.z1[p]),
+                         pw = uniform(T.w1[p]);
+            bool bad = false;
+            const int cx = axis_cell(px, A.cs[0], A.nc[0], !SURVEY, bad);
+            const int cy = axis_cell(py, A.cs[1], A.nc[1], !SURVEY, bad);
+            const int cz = axis_cell(pz, A.cs[2], A.nc[2], !SURVEY, bad);
+            if (bad) {
+                if (lane == 0) atomicOr(A.err, kErrDomain);
+                continue;
+            }
+            // survey: every ordered pair (the LOS sum is order dependent)
+            const bool sym = !SURVEY && T.sym;
+            const int n2 = build_list<LM>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
+                                          t.lut);
+            const int n3 = T.sym ? n2
+                                 : build_list<LM>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3, lane,
+                                                  t.sq, t.lut);
+            const int n2u = uniform(n2), n3u = uniform(n3);
+            if (n2u > A.cap || n3u > A.cap) {
+                if (lane == 0) atomicOr(A.err, kErrOverflow);
+                continue;
+            }
+            if (n2u == 0 || n3u == 0) continue;
+            wave_sync();
+            const char *lb = T.sym ? l2 : l3;
+            if (sym) {
+                if (T.weighted) {
+                    RppiPol<SURVEY, true, true> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
+                    traverse(P, n2u, n3u);
+                } else {
+                    RppiPol<SURVEY, true, false> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
+                    traverse(P, n2u, n3u);
+                }
+            } else {
+                if (T.weighted) {
+                    RppiPol<SURVEY, false, true> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
+                    traverse(P, n2u, n3u);
+                } else {
+                    RppiPol<SURVEY, false, false> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
+                    traverse(P, n2u, n3u);
+                }
+            }
+            wave_sync();
+        }
+        __syncthreads();
+    }
+}
+
+template __global__ void k_rppi<false>(KArgs);
+template __global__ void k_rppi<true>(KArgs);
+
 template __global__ void k_triplets<false, kHistCtab>(KArgs);
 template __global__ void k_triplets<false, kHistT3>(KArgs);
 template __global__ void k_triplets<false, kHistDense>(KArgs);
@@ -1153,6 +1453,9 @@ struct Geometry {
     long long ncells;
     int survey;
     int nb, nbc;
+    int rppi = 0, npi = 0;      // rppi binning: npi unit-width pi bins
+    double ori_x = 0.0;         // survey origin (x only reaches the device in the reference)
+    long long out_words = 0;    // output words per term: nbc compact bins, or nb^3 * npi^2 (rppi)
     std::vector<double> edges, sq;
     std::vector<uint16_t> lut;
     double lut_scale;
@@ -1192,7 +1495,8 @@ struct BatchPlan {
     unsigned long long *h_out = nullptr;
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
-    int nb = 0, nbc = 0, nkernel = 0, hist_layout = 0;
+    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0;
+    long long nbc = 0;  // output words per term
 };
 
 struct nc_context {
@@ -1343,13 +1647,30 @@ int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
     G.nb = ne - 1;
     if (G.nb > 40) return fail(ctx, NC_ERR_LIMIT, "at most 40 bins per side are supported");
     G.nbc = (int)choose_compact(G.nb);
+    G.out_words = G.nbc;
     G.sq.resize(ne);
     for (int i = 0; i < ne; ++i) G.sq[i] = G.edges[i] * G.edges[i];  // cudaNcorr.cu:124-127
     G.survey = P->survey ? 1 : 0;
+    if (P->bin_type != NC_BIN_THREED && P->bin_type != NC_BIN_RPPI) return fail(ctx, NC_ERR_ARG, "unknown bin type");
+    double cell[3];
+    for (int a = 0; a < 3; ++a) cell[a] = G.edges[ne - 1];  // cellSize = rmax, cudaNcorr.cu:137
+    if (P->bin_type == NC_BIN_RPPI) {
+        // rppi constructor, cudaNcorr.cu:159-216: pi bins 0..npi, cells
+        // (rpMax, rpMax, piMax) in a box, sqrtf(rpMax^2 + piMax^2) (FP32) in a survey
+        if (P->num_pi_bins < 1 || P->num_pi_bins > 4096) return fail(ctx, NC_ERR_BINS, "need 1..4096 pi bins");
+        G.rppi = 1;
+        G.npi = P->num_pi_bins;
+        G.ori_x = P->ori[0];
+        const double rpmax = G.edges[ne - 1], pimax = (double)G.npi;
+        const double rmax = (double)sqrtf((float)(rpmax * rpmax + pimax * pimax));
+        for (int a = 0; a < 3; ++a) cell[a] = G.survey ? rmax : (a == 2 ? pimax : rpmax);
+        G.out_words = (long long)G.nb * G.nb * G.nb * G.npi * G.npi;
+        if (G.out_words > (1LL << 27)) return fail(ctx, NC_ERR_LIMIT, "rppi histogram larger than 2^27 bins");
+    }
     G.ncells = 1;
     for (int a = 0; a < 3; ++a) {
         G.box[a] = P->box[a];
-        G.cs[a] = G.edges[ne - 1];  // cellSize = rmax, cudaNcorr.cu:137
+        G.cs[a] = cell[a];
         const double q = std::ceil(G.box[a] / G.cs[a]);
         if (!(q >= 1.0 && q <= 1048576.0)) return fail(ctx, NC_ERR_ARG, "box must be positive and finite");
         G.nc[a] = (int)q;
@@ -1385,6 +1706,7 @@ int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
             G.ctab[i * G.nb + j] = c;
             c += G.nb - j;
         }
+    if (G.rppi) return NC_OK;  // plain FP64 path only, histogram in global memory
     make_mixed_tables(G);
     if ((size_t)G.nbc * sizeof(unsigned long long) <= 65535) make_slot_table(G);  // byte offsets fit 16 bits
     return NC_OK;
@@ -1446,8 +1768,25 @@ void plan_drop(nc_context *ctx) {
 }
 
 // Compact sorted-bin results -> dense nb^3 histogram (device.cu:198 layout).
-void convert_out(int nb, int nbc, int nterms, const std::vector<int> &weighted, const unsigned long long *res,
-                 double *hist, uint64_t *counts) {
+void convert_out(int nb, long long nbc, int rppi, int nterms, const std::vector<int> &weighted,
+                 const unsigned long long *res, double *hist, uint64_t *counts) {
+    if (rppi) {  // already the dense layout
+        for (int i = 0; i < nterms; ++i) {
+            const unsigned long long *r = res + (size_t)i * nbc;
+            double *h = hist + (size_t)i * nbc;
+            uint64_t *cn = counts ? counts + (size_t)i * nbc : nullptr;
+            for (long long k = 0; k < nbc; ++k) {
+                if (weighted[i]) {
+                    std::memcpy(&h[k], &r[k], 8);
+                } else {
+                    h[k] = (double)r[k];
+                    if (cn) cn[k] = r[k];
+                }
+            }
+            if (weighted[i] && cn) std::fill(cn, cn + nbc, (uint64_t)0);
+        }
+        return;
+    }
     const size_t nh = (size_t)nb * nb * nb;
     for (int i = 0; i < nterms; ++i) {
         double *h = hist + (size_t)i * nh;
@@ -1501,7 +1840,7 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
         return NC_OK;
     }
     if (*p.h_err & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
-    convert_out(p.nb, p.nbc, nterms, p.weighted, p.h_out, hist, counts);
+    convert_out(p.nb, p.nbc, p.rppi, nterms, p.weighted, p.h_out, hist, counts);
     return NC_OK;
 }
 
@@ -1522,6 +1861,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     for (const Term &t : terms) uids.insert(uids.end(), {t.c1->uid, t.c2->uid, t.c3->uid});
     std::vector<double> pkey(P->box, P->box + 3);
     pkey.push_back((double)G.survey);
+    pkey.push_back((double)G.rppi);
+    pkey.push_back((double)G.npi);
+    pkey.push_back(G.ori_x);
     pkey.insert(pkey.end(), G.edges.begin(), G.edges.end());
     const std::vector<int> tkey = {path, ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
     if (use_graph && ctx->plan.valid && ctx->plan.uids == uids && ctx->plan.pkey == pkey && ctx->plan.tkey == tkey) {
@@ -1535,8 +1877,10 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     auto al16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
     const int W = ctx->waves_per_group;
     const int block = W * kWave;
+    const int lmode = G.rppi ? (G.survey ? kListRppiSurvey : kListRppiBox) : mixed ? kListMixed : kListPlain;
     auto wave_lds = [&](int c) {
         const size_t cp = (size_t)c + 4;
+        if (G.rppi) return al16(cp * sizeof(RElem));
         return mixed ? al16(cp * (sizeof(float4) + sizeof(int) + sizeof(double)) + kQueueCap * sizeof(uint2))
                      : al16(cp * (sizeof(Elem) + sizeof(double)));
     };
@@ -1557,7 +1901,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (!G.lut32.empty()) std::memcpy(tabs.data() + g_lut32, G.lut32.data(), sizeof(uint2) * G.lut32.size());
     if (!G.t3.empty()) std::memcpy(tabs.data() + g_t3, G.t3.data(), sizeof(uint16_t) * G.t3.size());
     const size_t nb3 = (size_t)G.nb * G.nb * G.nb;
-    auto hist_size = [&](int hm) { return hm == kHistDense ? al16(nb3 * 4) : al16((size_t)G.nbc * 8); };
+    auto hist_size = [&](int hm) {
+        return hm == kHistGlobal ? (size_t)0 : hm == kHistDense ? al16(nb3 * 4) : al16((size_t)G.nbc * 8);
+    };
     auto lds_tables = [&](int hm, size_t &o_sq, size_t &o_lut, size_t &o_ctab, size_t &o_lut32, size_t &o_t3) {
         size_t o = 0;
         o_sq = o_lut = o_ctab = o_lut32 = o_t3 = 0;
@@ -1649,8 +1995,10 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
                ctx->lds_max;
     };
     int hm = kHistCtab;
-    if ((mixed || G.lut_single_step) && all_unit && (double)ctx->chunk * (double)cap * (double)cap < 2147483648.0 &&
-        fits(kHistDense))
+    if (G.rppi)
+        hm = kHistGlobal;
+    else if ((mixed || G.lut_single_step) && all_unit &&
+             (double)ctx->chunk * (double)cap * (double)cap < 2147483648.0 && fits(kHistDense))
         hm = kHistDense;
     else if ((mixed || G.lut_single_step) && !G.t3.empty() && fits(kHistT3))
         hm = kHistT3;
@@ -1671,7 +2019,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
          (const void *)k_triplets<false, kHistDense>},
         {(const void *)k_triplets<true, kHistCtab>, (const void *)k_triplets<true, kHistT3>,
          (const void *)k_triplets<true, kHistDense>}};
-    const void *kfn = kfns[mixed ? 1 : 0][hm];
+    const void *kfn = G.rppi ? (G.survey ? (const void *)k_rppi<true> : (const void *)k_rppi<false>)
+                             : kfns[mixed ? 1 : 0][hm];
     int per_cu = ctx->wgs_per_cu;
     if (per_cu <= 0) {
         int occ = 0;
@@ -1680,7 +2029,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     }
     const int grid = ctx->ncu * per_cu;
     const long long nwaves = (long long)grid * W;
-    const long long slot_bytes = (long long)((2 * (size_t)cap * list_entry_bytes(mixed) + 255) & ~(size_t)255);
+    const long long slot_bytes = (long long)((2 * (size_t)cap * list_entry_bytes(lmode) + 255) & ~(size_t)255);
     if ((double)nwaves * (double)slot_bytes > 64.0 * (1ULL << 30))
         return fail(ctx, NC_ERR_LIMIT, "neighbour lists need more than 64 GiB of scratch");
     NC_HIP(ctx, dgrow(ctx->scratch, (size_t)(nwaves * slot_bytes)));
@@ -1718,7 +2067,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     }
     NC_HIP(ctx, dgrow(ctx->terms, sizeof(TermDesc) * td.size()));
     NC_HIP(ctx, dgrow(ctx->chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size())));
-    NC_HIP(ctx, dgrow(ctx->out, sizeof(unsigned long long) * (size_t)nterms * G.nbc));
+    NC_HIP(ctx, dgrow(ctx->out, sizeof(unsigned long long) * (size_t)nterms * G.out_words));
     if (!td.empty())
         NC_HIP(ctx, hipMemcpyAsync(ctx->terms.p, td.data(), sizeof(TermDesc) * td.size(), hipMemcpyHostToDevice,
                                    ctx->stream));
@@ -1746,7 +2095,11 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.r2min = G.sq[0];
     A.r2max = G.sq[G.nb];
     A.nb = G.nb;
-    A.nbc = G.nbc;
+    A.nbc = (int)G.out_words;
+    A.npi = G.npi;
+    A.pimax2 = (double)G.npi * (double)G.npi;
+    A.smax2 = G.sq[G.nb] + A.pimax2;  // rpMax*rpMax + piMax*piMax, cudaNcorr.cu:186
+    A.ori_x = G.ori_x;
     A.lut_scale = G.lut_scale;
     A.lut_m = G.lut_m;
     A.sq = (const double *)((char *)ctx->tables.p + g_sq);
@@ -1760,7 +2113,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.g32 = G.g32;
     A.tri_c1 = G.tri_c1;
     A.tri_c2 = G.tri_c2;
-    A.nslots = hm == kHistDense ? (int)((nb3 + 1) / 2) : G.nbc;
+    A.nslots = hm == kHistDense ? (int)((nb3 + 1) / 2) : hm == kHistGlobal ? 0 : G.nbc;
     A.flush_pairs = ctx->dense_flush;
     A.off_sq = (int)hist_bytes;
     A.off_lut = (int)(hist_bytes + off_lut);
@@ -1773,7 +2126,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 
     auto enqueue_count = [&]() -> int {
         hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->out.p, 0ull,
-                           (long long)nterms * G.nbc);
+                           (long long)nterms * G.out_words);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
         if (A.nchunks > 0) {
             void *args[] = {&A};
@@ -1785,7 +2138,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (enqueue_count() != NC_OK) return fail(ctx, NC_ERR_HIP, "triplet kernel launch failed");
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-    std::vector<unsigned long long> res((size_t)nterms * G.nbc);
+    std::vector<unsigned long long> res((size_t)nterms * G.out_words);
     NC_HIP(ctx, hipMemcpyAsync(res.data(), ctx->out.p, sizeof(unsigned long long) * res.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
     NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1802,7 +2155,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
     if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
 
-    convert_out(G.nb, G.nbc, nterms, weighted, res.data(), hist, counts);
+    convert_out(G.nb, G.out_words, G.rppi, nterms, weighted, res.data(), hist, counts);
 
     // --- capture the launch sequence for replay (builds, then count); any
     // failure leaves the eager result valid and no plan cached
@@ -1813,7 +2166,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.tkey = tkey;
         p.weighted = weighted;
         p.nb = G.nb;
-        p.nbc = G.nbc;
+        p.nbc = G.out_words;
+        p.rppi = G.rppi;
         p.nkernel = A.nchunks > 0 ? 1 : 0;
         p.hist_layout = hm;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
@@ -1894,6 +2248,13 @@ const char *nc_last_error(const nc_context *ctx) { return ctx ? ctx->err.c_str()
 int64_t nc_hist_size(int nedges) { return nedges < 2 ? 0 : (int64_t)(nedges - 1) * (nedges - 1) * (nedges - 1); }
 
 int64_t nc_compact_size(int nedges) { return nedges < 2 ? 0 : choose_compact(nedges - 1); }
+
+int64_t nc_hist_len(const nc_params *p) {
+    if (!p || p->nedges < 2) return 0;
+    const int64_t nb = p->nedges - 1;
+    if (p->bin_type == NC_BIN_RPPI) return p->num_pi_bins < 1 ? 0 : nb * nb * nb * p->num_pi_bins * p->num_pi_bins;
+    return p->bin_type == NC_BIN_THREED ? nb * nb * nb : 0;
+}
 
 int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog **cat) {
     if (!ctx || !cat || n < 0 || (n > 0 && !xyzw)) return ctx ? fail(ctx, NC_ERR_ARG, "bad catalogue") : NC_ERR_ARG;
@@ -2087,7 +2448,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
         }
     }
     const int saved[4] = {ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
-    std::vector<double> hist((size_t)nterms * nc_hist_size(params->nedges));
+    std::vector<double> hist((size_t)nterms * nc_hist_len(params));
     // sample: about 64 work items per workgroup slot of the chip at the largest chunk
     const long long items = std::max<long long>(1, nprim / 16);
     ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));

@@ -66,6 +66,12 @@ struct KArgs {
     double r2min, r2max;
     int nb, nbc;
     int nslots;                   // u64 words of the LDS histogram (nbc compact slots, or nb^3 u32 dense counts)
+    // rppi binning (cudaNcorr.cu:159-216): npi unit-width pi bins; nbc is then
+    // the dense nb^3 * npi^2 output length per term
+    int npi;
+    double pimax2;  // npi^2
+    double smax2;   // rpMax^2 + piMax^2 (survey list filter)
+    double ori_x;   // survey origin, x only (cudaNcorr.cu:254)
     unsigned long long flush_pairs;  // dense counts: merge once this many pairs were added (<= 2^31)
     double lut_scale;
     int lut_m;

@@ -24,10 +24,10 @@ class ncorr {
     // bin_type != threeD, and std::invalid_argument for edges that are not
     // >= 0 and strictly increasing (undefined behaviour in the reference).
     ncorr(double3 box, int bin_type, std::vector<double> &bins, bool isSurvey, double3 ori = double3{0.0, 0.0, 0.0});
-    // rppi constructor (cudaNcorr.cu:159-216).  rppi binning is not provided
-    // by this engine: throws std::runtime_error("Invalid constructor call for
-    // bin_type.") for bin_type != rppi, as the reference, and otherwise a
-    // std::runtime_error saying rppi is not supported.
+    // rppi constructor (cudaNcorr.cu:159-216): bins are rp edges, pi bins are
+    // 0, 1, ..., numMinorBins.  Throws std::runtime_error("Invalid constructor
+    // call for bin_type.") for bin_type != rppi, as the reference, and
+    // std::invalid_argument for bad edges or numMinorBins < 1.
     ncorr(double3 box, int bin_type, std::vector<double> &bins, int numMinorBins, bool isSurvey,
           double3 ori = double3{0.0, 0.0, 0.0});
 
@@ -43,7 +43,8 @@ class ncorr {
     // Extensions.
     void setPrecision(precision p) { precision_ = p; }
     void setDevice(int device);
-    // Dense nb^3 histogram of the last call (the reference's triCountsAll_).
+    // Dense histogram of the last call (the reference's triCountsAll_: nb^3,
+    // or nb^3 * npi^2 for rppi).
     std::vector<double> getTriCountsAll() const { return triCountsAll_; }
     // Device milliseconds of the last call: cell-list build + triplet kernel.
     double lastDeviceMs() const { return lastMs_; }
@@ -53,7 +54,10 @@ class ncorr {
     bool isSurvey_;
     std::vector<double> bins_, binsMid_;
     double rMin_, rMax_;
+    double rpMin_ = 0.0, rpMax_ = 0.0, piMax_ = 0.0;
+    std::vector<double> piBins_;
     int numBins_, numBinsCounts_;
+    int numMinorBins_ = 0, numPiBins_ = 0;
     double3 box_, cellSize_, ori_;
     int3 numCells_;
     std::vector<double3> triBinsOut_;
@@ -67,6 +71,7 @@ class ncorr {
 
     std::vector<double> getBinsMid(const std::vector<double> &bins);
     void prepTriCountsOutThreeD();
+    void prepTriCountsOutRppi();
     nc_context *context();
 };
 

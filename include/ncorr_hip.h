@@ -54,14 +54,26 @@ enum nc_path {
 typedef struct nc_context nc_context;
 typedef struct nc_catalog nc_catalog;
 
-/* Geometry and binning of one count: the arguments of the reference's threeD
- * constructor ncorr(box, 0, bins, isSurvey, ori) (cudaNcorr.hpp:14,
- * cudaNcorr.cu:118-157).  `ori` is not needed: the threeD kernels never read it. */
+/* Binning of a count (the reference's ncorr::binType, cudaNcorr.hpp:12). */
+enum nc_bin_type {
+    NC_BIN_THREED = 0, /* three sides binned in r */
+    NC_BIN_RPPI = 1    /* three sides binned in projected rp, two in line-of-sight pi */
+};
+
+/* Geometry and binning of one count: the arguments of the reference's
+ * constructors ncorr(box, 0, bins, isSurvey, ori) (threeD, cudaNcorr.hpp:14,
+ * cudaNcorr.cu:118-157) and ncorr(box, 1, bins, numMinorBins, isSurvey, ori)
+ * (rppi, cudaNcorr.hpp:15, cudaNcorr.cu:159-216).  A zeroed tail (bin_type 0)
+ * is the threeD count. */
 typedef struct {
     double box[3];        /* periodic box (box mode) or padded survey bounding box */
     int survey;           /* 0: periodic box, 1: survey (open boundaries) */
     int nedges;           /* number of bin edges = nb + 1 */
-    const double *edges;  /* host pointer, bin edges in r (not squared) */
+    const double *edges;  /* host pointer, bin edges in r (rppi: in rp), not squared */
+    int bin_type;         /* nc_bin_type */
+    int num_pi_bins;      /* rppi: unit-width pi bins 0..num_pi_bins (numMinorBins) */
+    double ori[3];        /* survey origin of the rppi line of sight; as in the
+                             reference only ori[0] is used (cudaNcorr.cu:254) */
 } nc_params;
 
 /* Device time of the last count call on a context, from HIP events on the
@@ -96,15 +108,22 @@ int64_t nc_catalog_size(const nc_catalog *cat);
  * number of edges; 0 if nedges < 2. */
 int64_t nc_hist_size(int nedges);
 int64_t nc_compact_size(int nedges);
+/* Dense histogram length per term for params: nb^3 (threeD) or nb^3 * npi^2
+ * (rppi, cudaNcorr.cu:189); 0 if invalid. */
+int64_t nc_hist_len(const nc_params *params);
 
-/* The threeD count (replaces ncorr::calculateD1D2D3, cudaNcorr.cu:220-430, and
- * the kernels device.cu:259-315 / 381-431).  Ordered triplets (p1 in c1, p2 in
- * c2, p3 in c3); each binned triplet adds (w1*w2)*w3.
- *   hist   [nb^3] double, required: the reference's triCountsAll_.
- *   counts [nb^3] uint64, optional (NULL): exact triplet counts (unit weights);
- *          filled only when every weight of c1, c2, c3 is 1.
+/* The count (replaces ncorr::calculateD1D2D3, cudaNcorr.cu:220-430, and the
+ * kernels device.cu:259-315 / 381-431 (threeD), 317-379 / 434-505 (rppi)).
+ * Ordered triplets (p1 in c1, p2 in c2, p3 in c3); each binned triplet adds
+ * (w1*w2)*w3.
+ *   hist   [nc_hist_len] double, required: the reference's triCountsAll_
+ *          (threeD [i_smallest*nb^2 + i_middle*nb + i_largest]; rppi
+ *          [((i1*nb + i2)*nb + i3)*npi^2 + pi1*npi + pi2], device.cu:250-252).
+ *   counts [nc_hist_len] uint64, optional (NULL): exact triplet counts (unit
+ *          weights); filled only when every weight of c1, c2, c3 is 1.
  * When every weight is 1 the kernel counts in integers and hist holds those
- * counts exactly (below 2^53), as the reference's double sums of 1.0. */
+ * counts exactly (below 2^53), as the reference's double sums of 1.0.  rppi
+ * runs the FP64 path whatever `path` asks for. */
 int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalog *c2, nc_catalog *c3,
              int path, double *hist, uint64_t *counts);
 

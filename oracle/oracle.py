@@ -50,6 +50,13 @@ def _lib():
         lib.nc_oracle_sensitivity.restype = ctypes.c_long
         lib.nc_oracle_compact.argtypes = [dp, ctypes.c_int, dp, dp, ctypes.c_int]
         lib.nc_oracle_compact.restype = ctypes.c_int
+        lib.nc_oracle_count_rppi.argtypes = [dp, ctypes.c_long, dp, ctypes.c_long, dp, ctypes.c_long, dp,
+                                             ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int,
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), dp,
+                                             ctypes.POINTER(ctypes.c_long)]
+        lib.nc_oracle_count_rppi.restype = ctypes.c_int
+        lib.nc_oracle_compact_rppi.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
+        lib.nc_oracle_compact_rppi.restype = ctypes.c_int
         _LIB = lib
     return _LIB
 
@@ -111,6 +118,53 @@ def compact(edges, dense):
     n = lib.nc_oracle_compact(e.ctypes.data_as(dp), e.size, None, None, 0)
     out = np.zeros(max(n, 1), dtype=np.float64)
     lib.nc_oracle_compact(e.ctypes.data_as(dp), e.size, d.ctypes.data_as(dp), out.ctypes.data_as(dp), n)
+    return out[:n]
+
+
+def count_rppi(p1, p2, p3, edges, npi, box, survey=False, ori=(0.0, 0.0, 0.0), contract=0, nthreads=0):
+    """Ordered-triplet rppi histogram of the reference semantics
+    (device.cu:203-255, 317-379, 434-505; cudaNcorr.cu:159-216).
+
+    edges: nb+1 rp edges; npi: number of unit-width pi bins.  Returns (counts
+    uint64 (nb,nb,nb,npi,npi), wsum float64 (same), dropped) indexed
+    [rp bin of the smallest, middle, largest rp side, pi bin of the smallest,
+    of the middle]; dropped = triplets whose flat index falls past the
+    histogram (the reference writes out of bounds there).
+    """
+    lib = _lib()
+    a1, q1, n1 = _pts(p1)
+    a2, q2, n2 = _pts(p2)
+    a3, q3, n3 = _pts(p3)
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    o = np.ascontiguousarray(ori, dtype=np.float64)
+    nb = e.size - 1
+    nh = max(nb, 0) ** 3 * npi * npi
+    cnt = np.zeros(max(nh, 1), dtype=np.uint64)
+    ws = np.zeros(max(nh, 1), dtype=np.float64)
+    lost = ctypes.c_long(0)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib.nc_oracle_count_rppi(q1, n1, q2, n2, q3, n3, e.ctypes.data_as(dp), e.size, int(npi),
+                                  b.ctypes.data_as(dp), int(bool(survey)), o.ctypes.data_as(dp), int(contract),
+                                  int(nthreads), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                  ws.ctypes.data_as(dp), ctypes.byref(lost))
+    del a1, a2, a3
+    if rc != 0:
+        raise ValueError(f"oracle: {ERRORS.get(rc, rc)}")
+    shape = (nb, nb, nb, npi, npi)
+    return cnt[:nh].reshape(shape), ws[:nh].reshape(shape), int(lost.value)
+
+
+def compact_rppi(edges, npi, dense):
+    """Output compaction of cudaNcorr.cu:88-105."""
+    lib = _lib()
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    d = np.ascontiguousarray(np.asarray(dense, dtype=np.float64).reshape(-1))
+    dp = ctypes.POINTER(ctypes.c_double)
+    n = lib.nc_oracle_compact_rppi(e.ctypes.data_as(dp), e.size, int(npi), None, None, 0)
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    lib.nc_oracle_compact_rppi(e.ctypes.data_as(dp), e.size, int(npi), d.ctypes.data_as(dp),
+                               out.ctypes.data_as(dp), n)
     return out[:n]
 
 
@@ -205,3 +259,119 @@ def output_bins(edges):
                 if mid[k] <= mid[i] + mid[j]:
                     out.append((mid[i], mid[j], mid[k]))
     return np.array(out, dtype=np.float64).reshape(-1, 3)
+
+
+def _sort3_carry(r, q):
+    """The compare-and-swap network of device.cu:215-229 on the last axis of r
+    (3 values), applying the same swaps to q."""
+    r = r.copy()
+    q = q.copy()
+    for i, j in ((0, 1), (0, 2), (1, 2)):
+        sw = r[..., i] > r[..., j]
+        ri, rj = r[..., i].copy(), r[..., j].copy()
+        qi, qj = q[..., i].copy(), q[..., j].copy()
+        r[..., i] = np.where(sw, rj, ri)
+        r[..., j] = np.where(sw, ri, rj)
+        q[..., i] = np.where(sw, qj, qi)
+        q[..., j] = np.where(sw, qi, qj)
+    return r, q
+
+
+def brute_force_rppi(p1, p2, p3, edges, npi, box=None, ori_x=0.0):
+    """Independent O(n^3) restatement of the rppi count with no cell list, in
+    the no-contraction evaluation order (the oracle's contract=1).
+
+    box=None: survey mode (LOS of each triplet from (ori_x, 0, 0)); the cell
+    size sqrtf(rpmax^2 + npi^2) must not round below the true value.  Box
+    mode: nearest periodic image relative to p1; the box must be a whole
+    number >= 3 of cells (rpmax, rpmax, npi).  Returns (counts, wsum,
+    dropped) as count_rppi.
+    """
+    p1 = np.asarray(p1, dtype=np.float64).reshape(-1, 4)
+    p2 = np.asarray(p2, dtype=np.float64).reshape(-1, 4)
+    p3 = np.asarray(p3, dtype=np.float64).reshape(-1, 4)
+    e = np.asarray(edges, dtype=np.float64)
+    sq = e * e
+    nb = e.size - 1
+    nh = nb ** 3 * npi * npi
+    cnt = np.zeros(nh, dtype=np.uint64)
+    ws = np.zeros(nh, dtype=np.float64)
+    lost = 0
+    rlo, rhi = sq[0], sq[-1]
+    pimax2 = float(npi) * float(npi)
+    smax2 = e[-1] * e[-1] + pimax2
+
+    def rp2(d):
+        return d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]
+
+    def s2(d):
+        return (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+
+    def dot(l, d):
+        return (l[..., 0] * d[..., 0] + l[..., 1] * d[..., 1]) + l[..., 2] * d[..., 2]
+
+    for a in p1:
+        q2 = p2 if box is None else _images(p2, a, box)
+        q3 = p3 if box is None else _images(p3, a, box)
+        d12 = a[None, :3] - q2[:, :3]
+        d13 = a[None, :3] - q3[:, :3]
+        if box is not None:
+            z12 = d12[:, 2] * d12[:, 2]
+            z13 = d13[:, 2] * d13[:, 2]
+            r12 = rp2(d12)
+            r13 = rp2(d13)
+            j = np.nonzero((z12 < pimax2) & (r12 < rhi) & (r12 > rlo))[0]
+            k = np.nonzero((z13 < pimax2) & (r13 < rhi) & (r13 > rlo))[0]
+            if j.size == 0 or k.size == 0:
+                continue
+            d23 = q2[j][:, None, :3] - q3[k][None, :, :3]
+            z23 = d23[..., 2] * d23[..., 2]
+            r23 = rp2(d23)
+            ok = (z23 < pimax2) & (r23 < rhi) & (r23 > rlo)
+            R = np.stack([np.broadcast_to(r12[j][:, None], r23.shape), np.broadcast_to(r13[k][None, :], r23.shape),
+                          r23], -1)
+            Q = np.stack([np.broadcast_to(z12[j][:, None], r23.shape), np.broadcast_to(z13[k][None, :], r23.shape),
+                          z23], -1)
+        else:
+            s12 = s2(d12)
+            s13 = s2(d13)
+            j = np.nonzero((s12 < smax2) & (s12 > 0.0))[0]
+            k = np.nonzero((s13 < smax2) & (s13 > 0.0))[0]
+            if j.size == 0 or k.size == 0:
+                continue
+            b2 = q2[j][:, None, :3]
+            b3 = q3[k][None, :, :3]
+            d23 = b2 - b3
+            s23 = s2(d23)
+            ok = (s23 < smax2) & (s23 > 0.0)
+            o = np.array([ori_x, 0.0, 0.0])
+            los = ((a[:3] - o) + (b2 - o)) + (b3 - o)
+            sl = dot(los, los)
+            lim = pimax2 * sl
+            t1 = dot(los, a[:3] - b2) ** 2
+            t2 = dot(los, a[:3] - b3) ** 2
+            t3 = dot(los, d23) ** 2
+            ok &= ~(lim < t1) & ~(lim < t2) & ~(lim < t3)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                Q = np.stack([t1 / sl, t2 / sl, t3 / sl], -1)
+            S = np.stack([np.broadcast_to(s12[j][:, None], s23.shape), np.broadcast_to(s13[k][None, :], s23.shape),
+                          s23], -1)
+            R = S - Q
+            ok &= np.all((R < rhi) & (R > rlo), axis=-1) & (Q[..., 0] < pimax2) & (Q[..., 1] < pimax2)
+        Rs, Qs = _sort3_carry(R, Q)
+        f = np.sqrt(Rs.astype(np.float32))
+        ok &= (f[..., 0] + f[..., 1]) > f[..., 2]
+        bins = np.searchsorted(sq, Rs, side="right") - 1
+        pib = np.floor(np.sqrt(Qs.astype(np.float32))).astype(np.int64)
+        idx = (pib[..., 1] + pib[..., 0] * npi + bins[..., 2] * npi * npi + bins[..., 1] * nb * npi * npi +
+               bins[..., 0] * nb * nb * npi * npi)
+        w = (a[3] * q2[j, 3])[:, None] * q3[k, 3][None, :]
+        for jj, kk in zip(*np.nonzero(ok)):
+            t = int(idx[jj, kk])
+            if t >= nh:
+                lost += 1
+                continue
+            cnt[t] += 1
+            ws[t] += w[jj, kk]
+    shape = (nb, nb, nb, npi, npi)
+    return cnt.reshape(shape), ws.reshape(shape), lost

@@ -7,6 +7,8 @@
 //   dropin_main count <cat1> <cat2> <cat3> <survey> <precision> <bx> <by> <bz> <edge0> <edge1> ...
 //       catN: binary files of n x double4; the same path passed twice means the
 //       same std::vector object.  Prints getTriCountsOut(), one value per line.
+//   dropin_main rppi <cat1> <cat2> <cat3> <survey> <npi> <ori_x> <bx> <by> <bz> <edge0> <edge1> ...
+//       the same with rppi binning (rp edges, npi pi bins).
 #include <cstdio>
 #include <fstream>
 #include <map>
@@ -64,7 +66,21 @@ static int mode_ctor() {
     CHECK(throws<std::runtime_error>([&] { ncorr c(box, 1, e, false, {0, 0, 0}); }));
     CHECK(throws<std::runtime_error>([&] { ncorr c(box, 2, e, false, {0, 0, 0}); }));
     CHECK(throws<std::runtime_error>([&] { ncorr c(box, 0, e, 8, false, {0, 0, 0}); }));
-    CHECK(throws<std::runtime_error>([&] { ncorr c(box, 1, e, 8, false, {0, 0, 0}); }));
+    // rppi: the threeD selection of rp bins, each with npi^2 pi bins (cudaNcorr.cu:198-212)
+    for (auto &c : cases) {
+        std::vector<double> er = lin(c.lo, c.hi, c.nb);
+        const int npi = 7;
+        ncorr corr(box, 1, er, npi, false, {0.0, 0.0, 0.0});
+        CHECK(corr.getTriBinsOut().size() == c.nout * npi * npi);
+        CHECK(corr.getTriCountsOut().size() == c.nout * npi * npi);
+        auto pb = corr.getPiBinsOut();
+        CHECK(pb.size() == c.nout * npi * npi);
+        CHECK(pb[0].x == 0.0 && pb[0].y == 0.0 && pb[1].y == 1.0 && pb[npi].x == 1.0);
+        CHECK(pb[npi * npi - 1].x == npi - 1.0 && pb[npi * npi - 1].y == npi - 1.0);
+        auto tb = corr.getTriBinsOut();
+        CHECK(tb[0].z == tb[npi * npi - 1].z && tb[npi * npi].z != tb[0].z);
+    }
+    CHECK(throws<std::invalid_argument>([&] { ncorr c(box, 1, e, 0, false, {0, 0, 0}); }));
     std::vector<double> bad = {0.0, 2.0, 1.0};
     CHECK(throws<std::invalid_argument>([&] { ncorr c(box, 0, bad, false, {0, 0, 0}); }));
     std::vector<double> one = {1.0};
@@ -118,6 +134,23 @@ static int mode_count(int argc, char **argv) {
     return 0;
 }
 
+static int mode_rppi(int argc, char **argv) {
+    std::map<std::string, std::vector<double4>> cats;
+    std::string p[3] = {argv[2], argv[3], argv[4]};
+    for (auto &s : p)
+        if (!cats.count(s)) cats[s] = load(s);
+    const bool surveyMode = std::stoi(argv[5]) != 0;
+    const int npi = std::stoi(argv[6]);
+    const double3 ori = {std::stod(argv[7]), 0.0, 0.0};
+    double3 box = {std::stod(argv[8]), std::stod(argv[9]), std::stod(argv[10])};
+    std::vector<double> edges;
+    for (int i = 11; i < argc; ++i) edges.push_back(std::stod(argv[i]));
+    ncorr corr(box, 1, edges, npi, surveyMode, ori);
+    corr.calculateD1D2D3(cats[p[0]], cats[p[1]], cats[p[2]]);
+    for (double v : corr.getTriCountsOut()) std::printf("%.17g\n", v);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) return 2;
     std::string m = argv[1];
@@ -125,6 +158,7 @@ int main(int argc, char **argv) {
         if (m == "ctor") return mode_ctor();
         if (m == "survey") return mode_survey();
         if (m == "count" && argc >= 12) return mode_count(argc, argv);
+        if (m == "rppi" && argc >= 13) return mode_rppi(argc, argv);
     } catch (const std::exception &e) {
         std::printf("EXCEPTION %s\n", e.what());
         return 3;

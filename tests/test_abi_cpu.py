@@ -47,6 +47,19 @@ def test_size_helpers():
         assert L.nc_compact_size(ne) == nc
 
 
+def test_hist_len():
+    L = _lib.lib()
+    e = np.arange(0.0, 31.0, 2.0)
+    p, _ = _lib.Context.params(e, (100.0, 100.0, 100.0))
+    assert L.nc_hist_len(ctypes.byref(p)) == 15 ** 3
+    p, _ = _lib.Context.params(e, (100.0, 100.0, 100.0), npi=20)
+    assert p.bin_type == _lib.NC_BIN_RPPI
+    assert L.nc_hist_len(ctypes.byref(p)) == 15 ** 3 * 20 * 20
+    p.num_pi_bins = 0
+    assert L.nc_hist_len(ctypes.byref(p)) == 0
+    assert L.nc_hist_len(None) == 0
+
+
 def test_null_arguments_are_rejected():
     L = _lib.lib()
     assert L.nc_create(None, 0) != 0
