@@ -1,0 +1,149 @@
+"""rppi binning on the GPU (k_rppi through the C-ABI and the drop-in class)
+against the rppi oracle (device.cu:203-255, 317-379, 434-505).  Unweighted
+counts bit-exact; weighted sums within 1e-9 relative."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN
+from cudancorr_amd.gen import neyman_scott_box, prep_cat_survey, survey_cap, uniform_box
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBDIR = os.path.join(ROOT, "cudancorr_amd", "lib")
+WEIGHT_RTOL = 1e-9
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _wcheck(got, want):
+    nz = want != 0
+    assert np.array_equal(got[~nz], want[~nz])
+    assert np.max(np.abs(got[nz] - want[nz]) / np.abs(want[nz]), initial=0.0) <= WEIGHT_RTOL
+
+
+def _box_case(seed=4):
+    # box: 12 x 12 x 10 cells of (rpmax, rpmax, npi) = (8, 8, 6)
+    p = neyman_scott_box(300, 20, 2.0, 96.0, seed=seed)
+    p[:, 2] = np.mod(p[:, 2], 60.0)
+    edges = np.array([0.5, 1.0, 2.0, 3.0, 4.5, 6.0, 8.0])
+    return p, edges, 6, np.array([96.0, 96.0, 60.0])
+
+
+@pytest.mark.parametrize("path", [NC_PATH_PLAIN, NC_PATH_MIXED])
+def test_box_ddd(ctx, path):
+    _log(f"rppi box ddd path={path}")
+    p, edges, npi, box = _box_case()
+    want, _, lost = oracle.count_rppi(p, p, p, edges, npi, box)
+    hist, got = ctx.count(p, p, p, edges, box, path=path, npi=npi)
+    assert want.sum() > 100_000 and lost == 0
+    assert got.shape == (6, 6, 6, npi, npi)
+    assert np.array_equal(got, want)
+    assert np.array_equal(hist, want.astype(np.float64))
+
+
+def test_box_asymmetric_weighted(ctx):
+    _log("rppi box weighted ddr")
+    p, edges, npi, box = _box_case()
+    rng = np.random.default_rng(9)
+    d = p.copy()
+    d[:, 3] = rng.lognormal(0.0, 0.3, d.shape[0])
+    r = uniform_box(5000, 96.0, seed=17)
+    r[:, 2] = np.mod(r[:, 2], 60.0)
+    for (a, b, c) in ((d, d, r), (d, r, r), (r, d, d)):
+        cnt, ws, _ = oracle.count_rppi(a, b, c, edges, npi, box)
+        hist, _ = ctx.count(a, b, c, edges, box, npi=npi)
+        assert cnt.sum() > 10_000
+        _wcheck(hist, ws)
+
+
+def test_box_ties_on_a_lattice(ctx):
+    """Integer coordinates: many rp^2 ties between the sides of a triangle, where
+    the two orders of (p2, p3) may fall in different pi bins."""
+    _log("rppi box lattice ties")
+    g = np.arange(0, 24, 1.5)
+    X, Y, Z = np.meshgrid(g, g, g[:12], indexing="ij")
+    p = np.c_[X.ravel(), Y.ravel(), Z.ravel(), np.ones(X.size)]
+    edges = np.array([0.5, 1.6, 2.2, 3.1, 4.0])
+    box = np.array([24.0, 24.0, 18.0])  # cells (4, 4, 3): 6 x 6 x 6
+    want, _, _ = oracle.count_rppi(p, p, p, edges, 3, box)
+    _, got = ctx.count(p, p, p, edges, box, npi=3)
+    assert want.sum() > 100_000
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("term", ["DDD", "DDR", "DRR", "RRR"])
+def test_survey_terms(ctx, term):
+    _log(f"rppi survey {term}")
+    d, r = survey_cap(2500, 4000, seed=13)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.1
+    box, ori = prep_cat_survey(d, r)
+    edges = np.array([0.0, 2.0, 4.0, 6.0, 9.0])
+    npi = 12  # cells sqrtf(9^2 + 12^2) = 15
+    cats = {"D": d, "R": r}
+    a, b, c = (cats[ch] for ch in term)
+    cnt, ws, lost = oracle.count_rppi(a, b, c, edges, npi, box, survey=True, ori=ori)
+    hist, got = ctx.count(a, b, c, edges, box, survey=True, npi=npi, ori=ori)
+    assert cnt.sum() > 1000 and lost == 0
+    if term == "RRR":
+        assert np.array_equal(got, cnt)
+    else:
+        _wcheck(hist, ws)
+
+
+def test_batched_replays(ctx):
+    _log("rppi batched")
+    p, edges, npi, box = _box_case(seed=6)
+    q = uniform_box(4000, 96.0, seed=23)
+    q[:, 2] = np.mod(q[:, 2], 60.0)
+    P, Q = ctx.catalog(p), ctx.catalog(q)
+    terms = [(P, P, P), (P, P, Q), (Q, Q, Q)]
+    want = [ctx.count(*t, edges, box, npi=npi)[1] for t in terms]
+    for rep in range(3):
+        _, cnt = ctx.count_batched(terms, edges, box, path=NC_PATH_PLAIN, npi=npi)
+        for i in range(3):
+            assert np.array_equal(cnt[i], want[i]), f"replay {rep} term {i}"
+    P.close()
+    Q.close()
+
+
+@pytest.fixture(scope="module")
+def dropin(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("dropin_rppi") / "dropin_main")
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_main.cpp")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), src, "-L" + LIBDIR,
+                    "-lncorr", "-lncorr_hip", "-Wl,-rpath," + LIBDIR, "-o", exe], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("survey", [False, True])
+def test_dropin_rppi(dropin, tmp_path, survey):
+    _log(f"rppi dropin survey={survey}")
+    if survey:
+        d, r = survey_cap(2000, 10, seed=14)
+        d[:, :3] = 1000.0 + (d[:, :3] - 1000.0) * 0.1
+        box, ori = prep_cat_survey(d, r)
+        p, edges, npi = d, np.array([0.0, 2.0, 4.0, 6.0, 9.0]), 12
+    else:
+        p, edges, npi, box = _box_case(seed=8)
+        ori = (0.0, 0.0, 0.0)
+    path = str(tmp_path / "p.bin")
+    np.ascontiguousarray(p, dtype=np.float64).tofile(path)
+    args = [dropin, "rppi", path, path, path, str(int(survey)), str(npi), repr(float(ori[0]))]
+    args += [repr(float(b)) for b in box] + [repr(float(e)) for e in edges]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    got = np.array([float(x) for x in out.stdout.split()])
+    cnt, ws, _ = oracle.count_rppi(p, p, p, edges, npi, box, survey=survey, ori=ori)
+    want = oracle.compact_rppi(edges, npi, ws)
+    assert got.size == want.size
+    nz = want != 0
+    assert np.array_equal(got[~nz], want[~nz])
+    assert np.max(np.abs(got[nz] - want[nz]) / want[nz], initial=0.0) <= WEIGHT_RTOL
