@@ -16,7 +16,15 @@ unweighted histogram, counted over all sorted bins).  Counts of both paths are
 checked bit-identical every run.  Inputs are uploaded to HBM before the
 timed region; the timed region is bracketed by hipDeviceSynchronize.
 
-Prints one JSON line (see the end of main()).
+Survey workloads (--workload survey, survey-small): the four terms DDD, DDR,
+DRR, RRR of a cap/shell survey (weighted data, unit-weight randoms) as in
+source/main.cu:72-95.  The primaries are split into --slices contiguous
+slices; each slice is one nc_count_batched call over its four terms (mixed)
+or four nc_count calls (plain), and one step is all slices.  Binned triplets
+of the weighted terms come from an untimed unit-weight run of the same terms;
+weighted sums of the mixed path are compared with the plain FP64 path.
+
+Prints one JSON line (see the end of main() and run_survey()).
 """
 from __future__ import annotations
 
@@ -84,10 +92,10 @@ def from_profiles(key):
     return None
 
 
-def cpu_baseline(cats, edges, box, budget_s=15.0):
+def cpu_baseline(cats, edges, box, budget_s=15.0, survey=False):
     """Time the CPU oracle (the reference's 27x27-cell loop, FP64, OpenMP over
-    primaries) on a bounded sample: the first primaries of the largest box,
-    doubling the sample until the run takes a few seconds."""
+    primaries) on a bounded sample: random primaries of the largest catalogue
+    against all of it, the sample growing until the run takes a few seconds."""
     from oracle import oracle
 
     big = max(cats, key=lambda c: c.shape[0])
@@ -104,7 +112,7 @@ def cpu_baseline(cats, edges, box, budget_s=15.0):
         idx = rng.choice(big.shape[0], size=min(m, big.shape[0]), replace=False)
         sample = big[np.sort(idx)]
         t0 = time.perf_counter()
-        c, _ = oracle.count(sample, big, big, edges, box, nthreads=threads)
+        c, _ = oracle.count(sample, big, big, edges, box, survey=survey, nthreads=threads)
         dt = time.perf_counter() - t0
         best = (int(c.sum()), dt, sample.shape[0])
         if dt > budget_s / 4 or m >= big.shape[0]:
@@ -112,8 +120,153 @@ def cpu_baseline(cats, edges, box, budget_s=15.0):
         m *= 4
     trip, dt, n = best
     return {"value": trip / dt, "unit": "binned triplets/s", "cores": threads, "kind": "port",
-            "sample": f"{n} random primaries of the largest box (N={big.shape[0]}) against the full box, "
+            "sample": f"{n} random primaries of the largest catalogue (N={big.shape[0]}) against all of it, "
                       f"{trip} binned triplets in {dt:.2f} s"}
+
+
+def survey_workload(name):
+    """(data, randoms, edges, box, description) of a survey workload, shifted
+    by prep_cat_survey (source/survey.cu:60-128)."""
+    from cudancorr_amd.gen import SEED, WORKLOADS, prep_cat_survey, survey_cap
+
+    if name == "survey":
+        d, r = survey_cap(4_000_000, 20_000_000, seed=SEED + 4000, clustered=True, sigma=4.0)
+        edges = WORKLOADS["survey_large"]["edges"]
+        desc = ("survey cap RA 0-60, Dec 0-60 deg, comoving 1000-2000 Mpc/h: 4M Neyman-Scott data (sigma=4, "
+                "lognormal(0,0.3) f32 weights) + 20M uniform randoms (w=1), rmax=50, 25 bins/side, "
+                "DDD+DDR+DRR+RRR")
+    else:
+        d, r = survey_cap(500_000, 2_500_000, seed=SEED + 2000)
+        edges = WORKLOADS["survey_cap"]["edges"]
+        desc = ("survey cap RA 0-60, Dec 0-60 deg, comoving 1000-2000 Mpc/h: 500k uniform data "
+                "(lognormal(0,0.3) f32 weights) + 2.5M randoms (w=1), rmax=40, 20 bins/side, DDD+DDR+DRR+RRR")
+    box, _ = prep_cat_survey(d, r)
+    return d, r, edges, box, desc
+
+
+def run_survey(args):
+    from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, Context
+
+    t0 = time.perf_counter()
+    d, r, edges, box, desc = survey_workload(args.workload)
+    log(f"generated {d.shape[0]} data + {r.shape[0]} randoms in {time.perf_counter() - t0:.1f}s")
+    hip = _hip()
+    ctx = Context(0)
+    K = max(1, args.slices)
+    du = d.copy()
+    du[:, 3] = 1.0
+
+    def slices(a):
+        b = np.linspace(0, a.shape[0], K + 1).astype(int)
+        return [ctx.catalog(a[i:j]) for i, j in zip(b[:-1], b[1:])]
+
+    D, R, DU = ctx.catalog(d), ctx.catalog(r), ctx.catalog(du)
+    Ds, Rs, DUs = slices(d), slices(r), slices(du)
+    names = ["DDD", "DDR", "DRR", "RRR"]
+
+    def terms(k, unit=False):
+        dk, dd = (DUs[k], DU) if unit else (Ds[k], D)
+        return [(dk, dd, dd), (dk, dd, R), (dk, R, R), (Rs[k], R, R)]
+
+    device_sync(hip)
+
+    def run(path):
+        hist = None
+        cnt = None
+        dev = cms = 0.0
+        for _ in range(args.warmup):
+            ctx.count_batched(terms(0), edges, box, survey=True, path=path)
+        device_sync(hip)
+        t0 = time.perf_counter()
+        for k in range(K):
+            if path == NC_PATH_MIXED:
+                h, c = ctx.count_batched(terms(k), edges, box, survey=True, path=path)
+                t = ctx.timing()
+                dev += t.ms_build + t.ms_count
+                cms += t.ms_count
+            else:
+                hc = []
+                for tm in terms(k):
+                    hc.append(ctx.count(*tm, edges, box, survey=True, path=path))
+                    t = ctx.timing()
+                    dev += t.ms_build + t.ms_count
+                    cms += t.ms_count
+                h, c = np.stack([a for a, _ in hc]), np.stack([b for _, b in hc])
+            hist = h if hist is None else hist + h
+            cnt = c if cnt is None else cnt + c
+            log(f"{'mixed' if path == NC_PATH_MIXED else 'plain'} slice {k + 1}/{K}: "
+                f"{time.perf_counter() - t0:.1f}s")
+        device_sync(hip)
+        return hist, cnt, time.perf_counter() - t0, dev, cms
+
+    t0 = time.perf_counter()
+    tune_m = ctx.autotune(terms(0), edges, box, survey=True, path=NC_PATH_MIXED)
+    log(f"autotune mixed: {tune_m} ({time.perf_counter() - t0:.1f}s)")
+    hist_m, cnt_m, wall_m, dev_m, cms_m = run(NC_PATH_MIXED)
+    log(f"mixed batched: wall {wall_m:.2f} s, device {dev_m / 1e3:.2f} s (count kernels {cms_m / 1e3:.2f} s)")
+
+    # binned triplets of the weighted terms: the same terms with unit weights (untimed)
+    trip_terms = np.zeros(4, dtype=np.int64)
+    trip_terms[3] = int(cnt_m[3].sum())
+    for k in range(K):
+        _, c = ctx.count_batched(terms(k, unit=True)[:3], edges, box, survey=True, path=NC_PATH_MIXED)
+        trip_terms[:3] += c.reshape(3, -1).sum(axis=1).astype(np.int64)
+    trip = int(trip_terms.sum())
+    log(f"binned triplets per term: {dict(zip(names, trip_terms.tolist()))}")
+
+    plain = None
+    tune_p = None
+    err = {}
+    if not args.no_plain:
+        tune_p = ctx.autotune(terms(0), edges, box, survey=True, path=NC_PATH_PLAIN)
+        log(f"autotune plain: {tune_p}")
+        hist_p, cnt_p, wall_p, dev_p, cms_p = run(NC_PATH_PLAIN)
+        if not np.array_equal(cnt_p[3], cnt_m[3]):
+            raise SystemExit("plain and mixed RRR counts differ")
+        for i, n in enumerate(names[:3]):
+            a, b = hist_m[i].ravel(), hist_p[i].ravel()
+            nz = b != 0
+            if np.any(a[~nz] != 0):
+                raise SystemExit(f"{n}: mixed has weight in bins the plain path leaves empty")
+            err[n] = float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz]), initial=0.0))
+        if max(err.values()) > 1e-9:
+            raise SystemExit(f"weighted sums differ by more than 1e-9 relative: {err}")
+        plain = {"wall_ms": wall_p * 1e3, "device_ms": dev_p, "count_kernel_ms": cms_p,
+                 "triplets_per_s": trip / wall_p}
+        log(f"plain: wall {wall_p:.2f} s, device {dev_p / 1e3:.2f} s (count kernels {cms_p / 1e3:.2f} s)")
+
+    achieved = trip * FLOPS_PER_TRIPLET / (cms_m * 1e-3) / 1e12
+    cpu = None if args.no_cpu else cpu_baseline([r], edges, box, survey=True)
+    out = {
+        "metric": "binned triplets/s on 1 MI355X, batched mixed vs plain, at stated error",
+        "value": trip / wall_m,
+        "unit": "binned triplets/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": wall_m * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded survey cap)",
+        "config": {"workload": desc, "data_points": int(d.shape[0]), "random_points": int(r.shape[0]),
+                   "slices": K, "binned_triplets": trip,
+                   "binned_triplets_per_term": dict(zip(names, trip_terms.tolist()))},
+        "error": {"unweighted": "RRR bit-exact (plain vs mixed)",
+                  "weighted_rel_max_vs_plain_fp64": err or None},
+        "plain": plain,
+        "tuning": {"mixed": tune_m, "plain": tune_p},
+        "speedup_vs_plain": (plain["wall_ms"] / (wall_m * 1e3)) if plain else None,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "valu_busy_pct": None,
+                     "kernel": "k_triplets (mixed, batched)", "kernel_ms": cms_m},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for c in [D, R, DU] + Ds + Rs + DUs:
+        c.close()
+    ctx.close()
 
 
 def main():
@@ -121,6 +274,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["boxes", "survey", "survey-small"], default="boxes",
+                    help="boxes: 64 clustered periodic boxes (default); survey: 4M data + 20M randoms, "
+                         "four terms; survey-small: 500k data + 2.5M randoms")
+    ap.add_argument("--slices", type=int, default=16, help="survey: primaries split into this many launches")
     ap.add_argument("--boxes", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true", help="skip the 64-call plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -133,6 +290,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload != "boxes":
+        run_survey(args)
+        return
 
     from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, Context
     from cudancorr_amd.gen import WORKLOADS, batched_boxes

@@ -1495,7 +1495,7 @@ struct BatchPlan {
     unsigned long long *h_out = nullptr;
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
-    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0;
+    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0;
     long long nbc = 0;  // output words per term
 };
 
@@ -1506,7 +1506,8 @@ struct nc_context {
     hipStream_t stream = nullptr;
     std::string err;
     int ncu = 0;
-    size_t lds_max = 0;
+    size_t lds_max = 0;  // LDS budget of a workgroup with two per CU
+    size_t lds_hw = 0;   // the per-workgroup maximum
     DBuf tables, terms, chunks, out, counter, errw, scratch, bound;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     nc_timing timing{};
@@ -1833,6 +1834,7 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     ctx->timing.ms_total = ms_b + ms_c;
     ctx->timing.n_kernel = p.nkernel;
     ctx->timing.hist_layout = p.hist_layout;
+    ctx->timing.lds_bytes = p.lds_bytes;
     if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (*p.h_err & kErrOverflow) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
         plan_drop(ctx);
@@ -1990,18 +1992,26 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // needs the single-step bin lookup for either of the first two.
     bool all_unit = true;
     for (const Term &t : terms) all_unit = all_unit && t.c1->unit_w && t.c2->unit_w && t.c3->unit_w;
-    auto fits = [&](int hm) {
+    // The budget is first half the CU's LDS (two workgroups per CU); a layout
+    // that only fits the whole of it runs one workgroup per CU.
+    auto fits = [&](int hm, size_t budget) {
         return hist_size(hm) + lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64) <=
-               ctx->lds_max;
+               budget;
     };
-    int hm = kHistCtab;
-    if (G.rppi)
-        hm = kHistGlobal;
-    else if ((mixed || G.lut_single_step) && all_unit &&
-             (double)ctx->chunk * (double)cap * (double)cap < 2147483648.0 && fits(kHistDense))
-        hm = kHistDense;
-    else if ((mixed || G.lut_single_step) && !G.t3.empty() && fits(kHistT3))
-        hm = kHistT3;
+    auto choose = [&](size_t budget) {
+        if (G.rppi) return kHistGlobal;
+        if ((mixed || G.lut_single_step) && all_unit &&
+            (double)ctx->chunk * (double)cap * (double)cap < 2147483648.0 && fits(kHistDense, budget))
+            return kHistDense;
+        if ((mixed || G.lut_single_step) && !G.t3.empty() && fits(kHistT3, budget)) return kHistT3;
+        return kHistCtab;
+    };
+    size_t budget = ctx->lds_max;
+    int hm = choose(budget);
+    if (!fits(hm, budget) && ctx->lds_hw > budget) {
+        budget = ctx->lds_hw;
+        hm = choose(budget);
+    }
     const size_t hist_bytes = hist_size(hm);
     const size_t lds_tab_bytes = lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3);
 
@@ -2010,10 +2020,10 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const size_t lds_tab = al16(hist_bytes + lds_tab_bytes);
     // chunks are whole 64-entry blocks (the circulant diagonal blocks need it)
     C = std::max(64, C - C % 64);
-    while (C > 64 && lds_tab + W * wave_lds(C) > ctx->lds_max) C -= 64;
+    while (C > 64 && lds_tab + W * wave_lds(C) > budget) C -= 64;
     const size_t wave_bytes = wave_lds(C);
     const size_t lds_bytes = lds_tab + W * wave_bytes;
-    if (lds_bytes > ctx->lds_max) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
+    if (lds_bytes > budget) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
     const void *kfns[2][3] = {
         {(const void *)k_triplets<false, kHistCtab>, (const void *)k_triplets<false, kHistT3>,
          (const void *)k_triplets<false, kHistDense>},
@@ -2151,6 +2161,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.ms_total = ms_b + ms_c;
     ctx->timing.n_kernel = A.nchunks > 0 ? 1 : 0;
     ctx->timing.hist_layout = hm;
+    ctx->timing.lds_bytes = (int32_t)lds_bytes;
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
     if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
@@ -2170,6 +2181,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.rppi = G.rppi;
         p.nkernel = A.nchunks > 0 ? 1 : 0;
         p.hist_layout = hm;
+        p.lds_bytes = (int)lds_bytes;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
                       hipSuccess &&
                   hipHostMalloc((void **)&p.h_err, 16) == hipSuccess;
@@ -2215,6 +2227,7 @@ int nc_create(nc_context **out, int device) {
     ctx->ncu = prop.multiProcessorCount;
     ctx->dev_name = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
     ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
+    ctx->lds_hw = prop.sharedMemPerBlock - 256;
     if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
         const unsigned long long v = std::strtoull(e, nullptr, 10);
         if (v > 0 && v <= (1ull << 31)) ctx->dense_flush = v;
@@ -2461,11 +2474,20 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
                 ctx->chunk = c;
                 ctx->wgs_per_cu = 0;
                 ctx->chunk_elems = lc;
-                rc = run_count(ctx, params, t, path, hist.data(), nullptr, false);
-                if (rc) break;
+                const int r = run_count(ctx, params, t, path, hist.data(), nullptr, false);
+                if (r == NC_ERR_LIMIT) {  // this shape does not fit: try the next
+                    if (rc == NC_OK) rc = r;
+                    continue;
+                }
+                if (r != NC_OK) {
+                    rc = r;
+                    goto done;
+                }
                 const float ms = ctx->timing.ms_count;
                 if (ms < b.ms) b = {w, c, 0, lc, ms, 0};
             }
+    if (b.ms < 1e30f) rc = NC_OK;  // at least one shape ran
+done:
     ctx->sample_stride = 1;
     plan_drop(ctx);
     if (rc) {

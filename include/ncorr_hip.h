@@ -85,8 +85,10 @@ typedef struct {
     int64_t n_kernel;    /* triplet kernel launches in that span */
     int32_t hist_layout; /* workgroup histogram of the triplet kernel: 0 sorted slots via an index table,
                             1 sorted slots via a per-bin-triple slot table, 2 dense nb^3 32-bit counts
-                            (unit weights only; merged into the sorted slots before they can wrap) */
-    int32_t reserved;
+                            (unit weights only; merged into the sorted slots before they can wrap),
+                            3 the dense global histogram (rppi) */
+    int32_t lds_bytes;   /* LDS of one workgroup of the triplet kernel (above half the CU's LDS: one
+                            workgroup per CU) */
 } nc_timing;
 
 /* Create a context on HIP device `device` (replaces the implicit CUDA context

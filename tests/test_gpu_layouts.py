@@ -80,3 +80,39 @@ def test_dense_counts_merged_every_chunk(monkeypatch, path):
             _, cnt = c2.count_batched([(cat, cat, cat), (cat, cat, cat)], edges, box, path=path)
             assert np.array_equal(cnt[0], want) and np.array_equal(cnt[1], want)
         cat.close()
+
+
+HALF_CU_LDS = 81920 - 256
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_wide_binning_uses_the_whole_cu_lds(ctx, path):
+    """25 bins over rmax = 50 (the large survey's binning): the mixed path's
+    lookup and slot tables with the weighted histogram need more than half
+    the CU's LDS, so the launch runs one workgroup per CU."""
+    _log(f"layouts wide binning path={path}")
+    d, r = survey_cap(1500, 2500, seed=21)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.1
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 50.0, 25)
+    cats = {"D": ctx.catalog(d), "R": ctx.catalog(r)}
+    want = {}
+    for term in ("DDR", "RRR"):
+        a, b, c = ({"D": d, "R": r}[ch] for ch in term)
+        want[term] = oracle.count(a, b, c, edges, box, survey=True)
+    terms = [tuple(cats[ch] for ch in t) for t in ("DDR", "RRR")]
+    hist, cnt = ctx.count_batched(terms, edges, box, survey=True, path=path)
+    if path == NC_PATH_MIXED:
+        assert ctx.timing().lds_bytes > HALF_CU_LDS
+    assert want["RRR"][0].sum() > 100_000
+    assert np.array_equal(cnt[1], want["RRR"][0])
+    ws = want["DDR"][1]
+    nz = ws != 0
+    assert np.array_equal(hist[0][~nz], ws[~nz])
+    assert np.max(np.abs(hist[0][nz] - ws[nz]) / ws[nz]) <= 1e-9
+    # the unit-weight term alone: dense counts
+    _, got = ctx.count(cats["R"], cats["R"], cats["R"], edges, box, survey=True, path=path)
+    assert np.array_equal(got, want["RRR"][0])
+    for c in cats.values():
+        c.close()
