@@ -1029,7 +1029,7 @@ __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, doub
 // the pairs added since the last merge reach A.flush_pairs <= 2^31: one chunk
 // adds at most chunk * cap^2 < 2^31 (checked on the host), so no count wraps.
 template <bool MIXED, int HM>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_triplets(KArgs A) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_triplets(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
@@ -1324,7 +1324,7 @@ struct RppiPol {
 // device.cu:317-379, 434-505): the work queue, list build and traversal of
 // k_triplets with the rppi filters and pair arithmetic.
 template <bool SURVEY>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_rppi(KArgs A) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_rppi(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
@@ -1475,7 +1475,8 @@ struct Geometry {
 }  // namespace
 
 struct nc_catalog {
-    nc_context *ctx = nullptr;
+    nc_context *ctx = nullptr;  // null once the context is destroyed
+    int device = 0;
     uint64_t uid = 0;  // unique per handle (graph plans key on it)
     int64_t n = 0;
     const double *d_raw = nullptr;
@@ -1503,6 +1504,7 @@ struct nc_context {
     int device = 0;
     BatchPlan plan;
     uint64_t next_uid = 1;
+    std::vector<nc_catalog *> cats;  // live catalogues, detached by nc_destroy
     hipStream_t stream = nullptr;
     std::string err;
     int ncu = 0;
@@ -2252,6 +2254,7 @@ int nc_destroy(nc_context *ctx) {
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    for (nc_catalog *c : ctx->cats) c->ctx = nullptr;
     delete ctx;
     return NC_OK;
 }
@@ -2295,6 +2298,8 @@ int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog
         return fail(ctx, e == hipErrorOutOfMemory ? NC_ERR_NOMEM : NC_ERR_HIP, hipGetErrorString(e));
     }
     c->d_raw = (const double *)c->raw.p;
+    c->device = ctx->device;
+    ctx->cats.push_back(c);
     *cat = c;
     return NC_OK;
 }
@@ -2328,6 +2333,8 @@ int nc_catalog_wrap_device(nc_context *ctx, const double *d_xyzw, int64_t n, nc_
         }
         c->unit_w = flag == 0;
     }
+    c->device = ctx->device;
+    ctx->cats.push_back(c);
     *cat = c;
     return NC_OK;
 }
@@ -2339,6 +2346,10 @@ int nc_catalog_destroy(nc_catalog *c) {
         (void)hipStreamSynchronize(c->ctx->stream);
         const std::vector<uint64_t> &u = c->ctx->plan.uids;
         if (std::find(u.begin(), u.end(), c->uid) != u.end()) plan_drop(c->ctx);
+        auto &v = c->ctx->cats;
+        v.erase(std::remove(v.begin(), v.end(), c), v.end());
+    } else {
+        (void)hipSetDevice(c->device);
     }
     for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->cnt, &c->start, &c->x, &c->y, &c->z, &c->w}) dfree(*b);
     delete c;
@@ -2404,8 +2415,9 @@ int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_c
         ctx->chunk_elems = list_chunk;
     }
     if (waves_per_group > 0) {
-        if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4 && waves_per_group != 8)
-            return fail(ctx, NC_ERR_ARG, "waves_per_group must be 1, 2, 4 or 8");
+        if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4 && waves_per_group != 8 &&
+            waves_per_group != 16)
+            return fail(ctx, NC_ERR_ARG, "waves_per_group must be 1, 2, 4, 8 or 16");
         ctx->waves_per_group = waves_per_group;
     }
     if (chunk > 0) ctx->chunk = chunk;
@@ -2467,7 +2479,16 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
     nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
     int rc = NC_OK;
-    for (int w : {4, 8})
+    {
+        // a first run at the current shape sizes the sample to about 100 ms
+        rc = run_count(ctx, params, t, path, hist.data(), nullptr, false);
+        const double ms = ctx->timing.ms_count;
+        if (rc == NC_OK && ms > 200.0)
+            ctx->sample_stride = (int)std::min<double>((double)items, ctx->sample_stride * (ms / 100.0));
+        if (rc != NC_OK && rc != NC_ERR_LIMIT) goto done;
+        rc = NC_OK;
+    }
+    for (int w : {4, 8, 16})
         for (int c : {16, 64})
             for (int lc : {64, 128, 256}) {
                 ctx->waves_per_group = w;

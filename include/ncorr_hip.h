@@ -94,6 +94,8 @@ typedef struct {
 /* Create a context on HIP device `device` (replaces the implicit CUDA context
  * and the __constant__ parameter block of the reference, device.cu:9-18). */
 int nc_create(nc_context **ctx, int device);
+/* Catalogues of a destroyed context may still be passed to nc_catalog_destroy
+ * (in any order); nothing else. */
 int nc_destroy(nc_context *ctx);
 const char *nc_last_error(const nc_context *ctx);
 
@@ -149,7 +151,7 @@ int nc_last_timing(const nc_context *ctx, nc_timing *out);
 
 /* Tuning knobs of the triplet kernel.  Values <= 0 keep the current setting
  * (wgs_per_cu: 0 = as many as fit, < 0 keeps).
- *   waves_per_group  in {1, 2, 4, 8}
+ *   waves_per_group  in {1, 2, 4, 8, 16}
  *   chunk            primaries per work item of the device queue
  *   wgs_per_cu       persistent workgroups per compute unit
  *   list_chunk       neighbour-list entries staged in LDS per wave (multiple of 64) */
@@ -158,8 +160,8 @@ int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *
 
 /* Autotuner (no counterpart in the reference): times the triplet kernel on a
  * strided sample of the work items of these terms for every candidate of
- * waves_per_group {4, 8} x chunk {16, 64} x list_chunk {64, 128, 256} (wgs_per_cu
- * = as many as fit), keeps the fastest in ctx and returns it in *best (ms =
+ * waves_per_group {4, 8, 16} x chunk {16, 64} x list_chunk {64, 128, 256} (wgs_per_cu
+ * = as many as fit; shapes whose LDS does not fit are skipped), keeps the fastest in ctx and returns it in *best (ms =
  * device time of the sample).  With the environment variable NC_TUNE_CACHE
  * naming a file, results are looked up in and appended to that file, keyed by
  * device, path, survey flag, bin count and catalogue sizes. */

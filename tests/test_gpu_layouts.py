@@ -116,3 +116,22 @@ def test_wide_binning_uses_the_whole_cu_lds(ctx, path):
     assert np.array_equal(got, want["RRR"][0])
     for c in cats.values():
         c.close()
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("waves", [4, 16])
+def test_workgroup_sizes(ctx, path, waves):
+    """4 and 16 waves per workgroup (1024 threads) give the same counts."""
+    _log(f"layouts waves={waves} path={path}")
+    p, edges, box, want = _box_case()
+    before = ctx.get_tuning()
+    try:
+        ctx.set_tuning(waves, 16, 0, 128)
+        _, got = ctx.count(p, p, p, edges, box, path=path)
+        assert np.array_equal(got, want)
+        cat = ctx.catalog(p)
+        _, cnt = ctx.count_batched([(cat, cat, cat)], edges, box, path=path)
+        assert np.array_equal(cnt[0], want)
+        cat.close()
+    finally:
+        ctx.set_tuning(*before)
