@@ -265,6 +265,7 @@ struct LdsTabs {
     const int *ctab;
     const uint2 *lut32;
     const uint16_t *t3;
+    const uint16_t *rk;  // rppi key stream: rank of a dense rp triple
 };
 
 // Neighbour-list flavours: threeD plain (Elem + weight), threeD mixed (MList),
@@ -438,6 +439,7 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
 // kHistT3 and kHistDense also take the single-step FP64 bin lookup.
 constexpr int kHistCtab = 0, kHistT3 = 1, kHistDense = 2;
 constexpr int kHistGlobal = 3;  // rppi: the dense histogram in global memory, no LDS copy
+constexpr int kHistStream = 4;  // rppi: key stream binned per bucket in LDS (reported layout)
 
 // Byte offset in the LDS histogram of the slot of bins (ka, kb, k), k in
 // [-1, nb] (slots of k outside [0, nb) are never added).
@@ -1162,11 +1164,15 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
 
 // getTriangleIndexRppi (device.cu:203-255): sort the sides by rp^2 carrying
 // their pi^2, FP32 triangle test on rp, rp bins, pi bins floor(sqrtf(pi^2))
-// of the two smallest-rp sides, the reference's flat index.  -1: rejected;
-// -2: the flat index falls past the histogram (a pi index equal to npi on the
-// last bins; the reference writes out of bounds there).
-__device__ __forceinline__ long long rppi_index(const KArgs &A, const LdsTabs &t, double r1, double r2, double r3,
-                                                double q1, double q2, double q3) {
+// of the two smallest-rp sides.  The reference's flat index is
+// T*npi^2 + rem with the dense rp triple T = k3 + k2*nb + k1*nb^2 and
+// rem = p2 + p1*npi; a pi index equal to npi makes rem >= npi^2, which
+// carries into the next triple exactly as the flat index does, so rem is
+// returned normalised below npi^2 and T is the triple the flat index lands
+// in (T >= nb^3: past the histogram, where the reference writes out of
+// bounds).  false: rejected by the triangle test.
+__device__ __forceinline__ bool rppi_cell(const KArgs &A, const LdsTabs &t, double r1, double r2, double r3,
+                                          double q1, double q2, double q3, int &T, int &rem) {
     double x;
     if (r1 > r2) {
         x = r1; r1 = r2; r2 = x;
@@ -1180,14 +1186,115 @@ __device__ __forceinline__ long long rppi_index(const KArgs &A, const LdsTabs &t
         x = r2; r2 = r3; r3 = x;
         x = q2; q2 = q3; q3 = x;
     }
-    if (__builtin_sqrtf((float)r1) + __builtin_sqrtf((float)r2) <= __builtin_sqrtf((float)r3)) return -1;
-    const long long k1 = bin_any<false>(A, t, r1), k2 = bin_any<false>(A, t, r2), k3 = bin_any<false>(A, t, r3);
-    const long long p1 = (long long)floorf(__builtin_sqrtf((float)q1));
-    const long long p2 = (long long)floorf(__builtin_sqrtf((float)q2));
-    const long long np = A.npi, nb = A.nb;
-    const long long idx = p2 + p1 * np + (k3 + k2 * nb + k1 * nb * nb) * np * np;
-    return idx < (long long)A.nbc ? idx : -2;
+    if (__builtin_sqrtf((float)r1) + __builtin_sqrtf((float)r2) <= __builtin_sqrtf((float)r3)) return false;
+    const int k1 = bin_any<false>(A, t, r1), k2 = bin_any<false>(A, t, r2), k3 = bin_any<false>(A, t, r3);
+    const int p1 = (int)floorf(__builtin_sqrtf((float)q1));
+    const int p2 = (int)floorf(__builtin_sqrtf((float)q2));
+    const int np = A.npi, np2 = np * np, nb = A.nb;
+    T = k3 + k2 * nb + k1 * nb * nb;
+    rem = p2 + p1 * np;
+    if (rem >= np2) {  // rare: a pi index equal to npi
+        const int c = rem / np2;
+        T += c;
+        rem -= c * np2;
+    }
+    return true;
 }
+
+// Per-wave staging of the rppi key stream: up to kStage keys (and values);
+// a row adds at most 2*64, so the wave flushes once it holds more than
+// kStage - 128.  kStreamBuckets bounds the buckets of one term.  Per bucket
+// of its current term a wave holds a reserved run [cur, end) of its region
+// (taken rs_blk slots at a time, so the bucket cursors see one atomic per
+// run, not one per flush); slots it leaves unused get kRsPad.
+constexpr int kStage = 512;
+constexpr int kStreamBuckets = 128;
+constexpr unsigned kRsPad = 0xffffffffu;
+// Views of a wave's stream state, after its staged keys: the flush
+// permutation, then per bucket of the current term (nbm slots, the launch's
+// largest bucket count rounded up to 8) counts and run positions.
+struct StreamLds {
+    uint16_t *perm;  // the flush's keys in bucket order
+    unsigned *wcnt, *pre, *len1;
+    unsigned long long *pos1, *pos2, *cur, *end, *base, *cap;  // base, cap: the term's regions
+
+    __device__ void bind(char *p, int nbm) {
+        perm = reinterpret_cast<uint16_t *>(p);
+        p += (size_t)kStage * 2;
+        wcnt = reinterpret_cast<unsigned *>(p);
+        pre = wcnt + nbm;
+        len1 = pre + nbm;
+        p += (size_t)nbm * 12;
+        unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
+        pos1 = q;
+        pos2 = q + nbm;
+        cur = q + 2 * nbm;
+        end = q + 3 * nbm;
+        base = q + 4 * nbm;
+        cap = q + 5 * nbm;
+    }
+};
+__host__ __device__ constexpr size_t stream_lds_offset(bool weighted) {
+    return ((size_t)kStage * 4 + (weighted ? (size_t)kStage * 8 : 0) + (size_t)kStage * 2 + 15) & ~(size_t)15;
+}
+__host__ __device__ constexpr size_t stream_wave_bytes(bool weighted, int nbm) {
+    return stream_lds_offset(weighted) + (size_t)kStage * 2 + (size_t)nbm * 60;
+}
+
+// Exclusive prefix sum over the 64 lanes of a wave; *total: the sum.
+__device__ __forceinline__ unsigned wave_excl_scan(unsigned v, int lane, unsigned *total) {
+    unsigned x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    *total = __shfl(x, kWave - 1);
+    return x - v;
+}
+
+// Per-wave key stream state of the current term.
+struct KeyStream {
+    StreamLds L;
+    int tbase = 0, nbt = 0, shift = 0, term = -1;
+
+    // Ends the term: unused reserved slots get kRsPad; dry runs add their
+    // key counts to the bucket cursors.
+    __device__ void close(const KArgs &A, int lane) {
+        if (term < 0) return;
+        wave_sync();
+        for (int b = 0; b < nbt; ++b) {
+            const unsigned long long c = L.cur[b], e = L.end[b];
+            if (A.rs_dry) {
+                if (lane == 0 && c) atomicAdd(&A.rs_cursor[tbase + b], c);
+                continue;
+            }
+            const unsigned long long b0 = L.base[b], cap = L.cap[b];
+            const unsigned long long stop = e < cap ? e : cap;
+            for (unsigned long long i = c + lane; i < stop; i += kWave) A.rs_keys[b0 + i] = kRsPad;
+        }
+        wave_sync();
+        term = -1;
+    }
+
+    __device__ void open(const KArgs &A, int lane, int t, int4 tr) {
+        if (t == term) return;
+        close(A, lane);
+        term = t;
+        tbase = tr.x;
+        nbt = tr.y;
+        shift = tr.z;
+        for (int b = lane; b < nbt; b += kWave) {
+            L.cur[b] = L.end[b] = 0ull;
+            if (!A.rs_dry) {
+                const unsigned long long b0 = A.rs_base[tbase + b];
+                L.base[b] = b0;
+                L.cap[b] = A.rs_base[tbase + b + 1] - b0;
+            }
+        }
+        wave_sync();
+    }
+};
 
 // Pair evaluation of the rppi kernels on the plain FP64 path: rows from the
 // wave's LDS chunk (broadcast), lane entries in registers.  Box (SURVEY
@@ -1196,8 +1303,12 @@ __device__ __forceinline__ long long rppi_index(const KArgs &A, const LdsTabs &t
 // counted for both orders (evaluated twice when two rp^2 tie, since the sort
 // then keeps the input order of the tied sides).  Survey: every ordered pair,
 // as the line of sight (p1 + p2 + p3 - 3 ori) depends on the order of the
-// FP64 sum.  Counts go straight to the term's dense global histogram.
-template <bool SURVEY, bool SYM, bool WEIGHTED>
+// FP64 sum.
+//
+// Binned triplets leave either as atomics on the term's dense global
+// histogram (STREAM false) or as keys staged in LDS and flushed, grouped by
+// bucket, into the bucket regions that k_rppi_bin adds up (STREAM true).
+template <bool SURVEY, bool SYM, bool WEIGHTED, bool STREAM>
 struct RppiPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 2;
@@ -1211,9 +1322,17 @@ struct RppiPol {
     RElem eb;
     double bsx, bsy, bsz;  // p1 - lane entry (survey)
     bool vb;
+    int nb3, np2;
+    // key stream
+    const uint16_t *rank;
+    unsigned *sk;
+    double *sv;
+    uint16_t *srk;
+    StreamLds S;
+    int cnt, tbase, nbt, shift;
 
     __device__ RppiPol(const KArgs &A_, const LdsTabs &t_, const char *l2, const char *l3, char *wlds, int lane_,
-                       unsigned long long *out_, double px_, double py_, double pz_, double w1_)
+                       unsigned long long *out_, double px_, double py_, double pz_, double w1_, int4 term)
         : A(A_), t(t_), out(out_), px(px_), py(py_), pz(pz_), w1(w1_), lane(lane_) {
         C = A.chunk_elems;
         L2 = reinterpret_cast<const RElem *>(l2);
@@ -1221,6 +1340,24 @@ struct RppiPol {
         la = reinterpret_cast<RElem *>(wlds);
         ppx = px - A.ori_x;
         row_first = 0;
+        nb3 = A.nb * A.nb * A.nb;
+        np2 = A.npi * A.npi;
+        cnt = 0;
+        tbase = term.x;
+        nbt = term.y;
+        shift = term.z;
+        if (STREAM) {
+            rank = t.rk;
+            char *b = wlds + A.off_stage;
+            sk = reinterpret_cast<unsigned *>(b);
+            b += (size_t)kStage * 4;
+            if (WEIGHTED) {
+                sv = reinterpret_cast<double *>(b);
+                b += (size_t)kStage * 8;
+            }
+            srk = reinterpret_cast<uint16_t *>(b);
+            S.bind(wlds + A.off_stage + A.rs_lds_off, A.rs_nbm);
+        }
     }
 
     __device__ void stage(bool from3, int first, int last) {
@@ -1245,55 +1382,163 @@ struct RppiPol {
         bsz = pz - eb.z;
     }
 
-    __device__ __forceinline__ void add(long long idx, double v, unsigned long long n) {
-        if (idx < 0) return;
+    // dense flat index of (T, rem) on the term's histogram
+    __device__ __forceinline__ void add_direct(int T, int rem, double v, unsigned long long n) {
+        const long long idx = (long long)T * np2 + rem;
         if (WEIGHTED)
             unsafeAtomicAdd(reinterpret_cast<double *>(&out[idx]), v);
         else
             atomicAdd(&out[idx], n);
     }
 
-    // one (p2, p3) pair; sa: the row holds p2 (else p3).  s2a: p1 - p2.
+    // Called by every lane of the wave (has: this lane holds a key).
+    __device__ __forceinline__ void emit(bool has, int T, int rem, int n, double v) {
+        const bool keep = has & (T < nb3);  // past the histogram: dropped
+        if constexpr (!STREAM) {
+            if (keep) add_direct(T, rem, v, (unsigned long long)n);
+        } else {
+            const unsigned long long m = __ballot(keep);
+            if (m != 0) {
+                if (keep) {
+                    const int pos = cnt + lane_rank(m);
+                    sk[pos] = ((unsigned)rank[T] * (unsigned)np2 + (unsigned)rem) | (n == 2 ? 0x80000000u : 0u);
+                    if (WEIGHTED) sv[pos] = v;
+                }
+                cnt += __popcll(m);
+            }
+        }
+    }
+
+    // The staged keys, grouped by bucket (a counting sort in LDS), go to the
+    // bucket regions with one cursor atomic per bucket; keys past a region's
+    // capacity are added to the dense histogram directly.
+    __device__ void flush() {
+        if constexpr (STREAM) {
+            const int n = cnt;
+            wave_sync();
+            for (int b = lane; b < nbt; b += kWave) S.wcnt[b] = 0u;
+            wave_sync();
+            for (int i = lane; i < n; i += kWave)
+                srk[i] = (uint16_t)atomicAdd(&S.wcnt[(sk[i] & 0x7fffffffu) >> shift], 1u);
+            wave_sync();
+            // per bucket: the keys of this flush fill the rest of the wave's
+            // run (pos1, len1), then a new run of whole rs_blk blocks (pos2)
+            for (int b = lane; b < nbt; b += kWave) {
+                const unsigned c = S.wcnt[b];
+                if (c == 0u) continue;
+                if (A.rs_dry) {
+                    S.cur[b] += c;
+                    continue;
+                }
+                const unsigned long long cur = S.cur[b], end = S.end[b];
+                const unsigned long long avail = end - cur;
+                S.pos1[b] = cur;
+                if (avail >= c) {
+                    S.len1[b] = c;
+                    S.cur[b] = cur + c;
+                } else {
+                    const unsigned long long more = c - avail, blk = (unsigned long long)A.rs_blk;
+                    const unsigned long long take = (more + blk - 1) / blk * blk;
+                    const unsigned long long p2 = atomicAdd(&A.rs_cursor[tbase + b], take);
+                    S.len1[b] = (unsigned)avail;
+                    S.pos2[b] = p2;
+                    S.cur[b] = p2 + more;
+                    S.end[b] = p2 + take;
+                }
+            }
+            if (!A.rs_dry) {
+                // bucket order, so that neighbouring lanes store to neighbouring slots
+                unsigned tot = 0u;
+                const unsigned v0 = lane < nbt ? S.wcnt[lane] : 0u;
+                const unsigned p0 = wave_excl_scan(v0, lane, &tot);
+                if (lane < nbt) S.pre[lane] = p0;
+                if (nbt > kWave) {
+                    unsigned tot1 = 0u;
+                    const unsigned v1 = lane + kWave < nbt ? S.wcnt[lane + kWave] : 0u;
+                    const unsigned p1 = wave_excl_scan(v1, lane, &tot1);
+                    if (lane + kWave < nbt) S.pre[lane + kWave] = tot + p1;
+                }
+                wave_sync();
+                for (int i = lane; i < n; i += kWave)
+                    S.perm[S.pre[(sk[i] & 0x7fffffffu) >> shift] + srk[i]] = (uint16_t)i;
+            }
+            wave_sync();
+            if (!A.rs_dry) {
+                for (int j = lane; j < n; j += kWave) {
+                    const int i = S.perm[j];
+                    const unsigned key = sk[i];
+                    const int b = (int)((key & 0x7fffffffu) >> shift);
+                    const unsigned long long b0 = S.base[b];
+                    const unsigned r = (unsigned)j - S.pre[b], l1 = S.len1[b];
+                    const unsigned long long dst = r < l1 ? S.pos1[b] + r : S.pos2[b] + (r - l1);
+                    if (dst < S.cap[b]) {
+                        A.rs_keys[b0 + dst] = key;
+                        if (WEIGHTED) A.rs_vals[b0 + dst] = sv[i];
+                    } else {  // region full: the reference's own atomic
+                        const unsigned c = key & 0x7fffffffu;
+                        const unsigned rt = c / (unsigned)np2;
+                        add_direct(A.rs_inv[rt], (int)(c - rt * (unsigned)np2), WEIGHTED ? sv[i] : 0.0,
+                                   1ull + (key >> 31));
+                    }
+                }
+            }
+            wave_sync();
+            cnt = 0;
+        }
+    }
+
+    // one (p2, p3) pair; s2a: p1 - p2.  All lanes call it (ok: the lane's pair is real).
     __device__ __forceinline__ void pair(const RElem &e2, const RElem &e3, double s2x, double s2y, double s2z,
                                          double s3x, double s3y, double s3z, bool ok) {
         const double dx = e2.x - e3.x, dy = e2.y - e3.y, dz = e2.z - e3.z;  // p2 - p3
+        int T0 = 0, rem0 = 0, T1 = 0, rem1 = 0, n0 = 1;
+        bool h0 = false, h1 = false;
+        double v0 = 0.0, v1 = 0.0;
         if constexpr (!SURVEY) {
             const double q3 = dz * dz, r3 = rp2(dx, dy);  // device.cu:357-361
-            if (!(ok && q3 < A.pimax2 && r3 < A.r2max && r3 > A.r2min)) return;
-            const long long i23 = rppi_index(A, t, e2.r, e3.r, r3, e2.q, e3.q, q3);
-            if (SYM) {
-                const bool tie = e2.r == e3.r || e2.r == r3 || e3.r == r3;
-                if (tie) {
-                    const long long i32 = rppi_index(A, t, e3.r, e2.r, r3, e3.q, e2.q, q3);
-                    add(i23, (w1 * e2.w) * e3.w, 1ull);
-                    add(i32, (w1 * e3.w) * e2.w, 1ull);
+            if (ok && q3 < A.pimax2 && r3 < A.r2max && r3 > A.r2min) {
+                h0 = rppi_cell(A, t, e2.r, e3.r, r3, e2.q, e3.q, q3, T0, rem0);
+                if (SYM) {
+                    const bool tie = e2.r == e3.r || e2.r == r3 || e3.r == r3;
+                    if (tie) {
+                        h1 = rppi_cell(A, t, e3.r, e2.r, r3, e3.q, e2.q, q3, T1, rem1);
+                        v0 = (w1 * e2.w) * e3.w;
+                        v1 = (w1 * e3.w) * e2.w;
+                    } else {
+                        n0 = 2;
+                        v0 = (w1 * e2.w) * e3.w + (w1 * e3.w) * e2.w;
+                    }
                 } else {
-                    add(i23, (w1 * e2.w) * e3.w + (w1 * e3.w) * e2.w, 2ull);
+                    v0 = (w1 * e2.w) * e3.w;
                 }
-            } else {
-                add(i23, (w1 * e2.w) * e3.w, 1ull);
             }
         } else {
             const double s23 = sqrsep(dx, dy, dz);  // device.cu:465-466
-            if (!(ok && s23 < A.smax2 && s23 > 0.0)) return;
-            // getLOS (device.cu:51-60): pos = p - ori with ori = (ori.x, 0, 0)
-            const double lx = (ppx + e2.q) + e3.q, ly = (py + e2.y) + e3.y, lz = (pz + e2.z) + e3.z;
-            const double sl = dot3(lx, ly, lz, lx, ly, lz);
-            const double lim = A.pimax2 * sl;
-            double t1 = dot3(lx, ly, lz, s2x, s2y, s2z);
-            t1 = t1 * t1;
-            double t2 = dot3(lx, ly, lz, s3x, s3y, s3z);
-            t2 = t2 * t2;
-            double t3 = dot3(lx, ly, lz, dx, dy, dz);
-            t3 = t3 * t3;
-            if (lim < t1 || lim < t2 || lim < t3) return;  // device.cu:470-475
-            const double q1 = t1 / sl, q2 = t2 / sl, q3 = t3 / sl;  // getSqrRpSqrPi, device.cu:68-71
-            const double r1 = e2.r - q1, r2 = e3.r - q2, r3 = s23 - q3;
-            if (!(r1 < A.r2max && r2 < A.r2max && r3 < A.r2max && r1 > A.r2min && r2 > A.r2min && r3 > A.r2min &&
-                  q1 < A.pimax2 && q2 < A.pimax2))
-                return;
-            add(rppi_index(A, t, r1, r2, r3, q1, q2, q3), (w1 * e2.w) * e3.w, 1ull);
+            if (ok && s23 < A.smax2 && s23 > 0.0) {
+                // getLOS (device.cu:51-60): pos = p - ori with ori = (ori.x, 0, 0)
+                const double lx = (ppx + e2.q) + e3.q, ly = (py + e2.y) + e3.y, lz = (pz + e2.z) + e3.z;
+                const double sl = dot3(lx, ly, lz, lx, ly, lz);
+                const double lim = A.pimax2 * sl;
+                double t1 = dot3(lx, ly, lz, s2x, s2y, s2z);
+                t1 = t1 * t1;
+                double t2 = dot3(lx, ly, lz, s3x, s3y, s3z);
+                t2 = t2 * t2;
+                double t3 = dot3(lx, ly, lz, dx, dy, dz);
+                t3 = t3 * t3;
+                if (!(lim < t1 || lim < t2 || lim < t3)) {  // device.cu:470-475
+                    const double q1 = t1 / sl, q2 = t2 / sl, q3 = t3 / sl;  // getSqrRpSqrPi, device.cu:68-71
+                    const double r1 = e2.r - q1, r2 = e3.r - q2, r3 = s23 - q3;
+                    if (r1 < A.r2max && r2 < A.r2max && r3 < A.r2max && r1 > A.r2min && r2 > A.r2min &&
+                        r3 > A.r2min && q1 < A.pimax2 && q2 < A.pimax2) {
+                        h0 = rppi_cell(A, t, r1, r2, r3, q1, q2, q3, T0, rem0);
+                        v0 = (w1 * e2.w) * e3.w;
+                    }
+                }
+            }
         }
+        emit(h0, T0, rem0, n0, v0);
+        if (SYM && !SURVEY) emit(h1, T1, rem1, 1, v1);
+        if (STREAM && cnt > kStage - 2 * kWave) flush();
     }
 
     template <bool SWAP>
@@ -1320,10 +1565,19 @@ struct RppiPol {
     }
 };
 
+template <bool SURVEY, bool SYM, bool WEIGHTED, bool STREAM>
+__device__ __forceinline__ void rppi_primary(const KArgs &A, const LdsTabs &t, const char *l2, const char *lb,
+                                             char *wlds, int lane, unsigned long long *out, double px, double py,
+                                             double pz, double pw, int4 term, int n2u, int n3u) {
+    RppiPol<SURVEY, SYM, WEIGHTED, STREAM> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term);
+    traverse(P, n2u, n3u);
+    if (STREAM && P.cnt > 0) P.flush();
+}
+
 // Persistent rppi kernel (countTrianglesRppiBox / countTrianglesRppiSurvey,
 // device.cu:317-379, 434-505): the work queue, list build and traversal of
 // k_triplets with the rppi filters and pair arithmetic.
-template <bool SURVEY>
+template <bool SURVEY, bool STREAM>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_rppi(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
@@ -1337,15 +1591,24 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     t.ctab = nullptr;
     t.lut32 = nullptr;
     t.t3 = nullptr;
+    t.rk = nullptr;
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
     for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
+    if (STREAM) {
+        uint16_t *rk = reinterpret_cast<uint16_t *>(lds + A.off_rank);
+        t.rk = rk;
+        const int nb3 = A.nb * A.nb * A.nb;
+        for (int i = tid; i < nb3; i += nthr) rk[i] = A.rs_rank[i];
+    }
     constexpr int LM = SURVEY ? kListRppiSurvey : kListRppiBox;
     const int lane = tid & (kWave - 1), wave = tid / kWave, nw = nthr / kWave;
     char *wlds = lds + A.off_wave + wave * A.wave_bytes;
     char *slot = A.scratch + (long long)(blockIdx.x * nw + wave) * A.slot_bytes;
     char *l2 = slot;
     char *l3 = slot + (size_t)A.cap * list_entry_bytes(LM);
+    KeyStream ks;
+    if (STREAM) ks.L.bind(wlds + A.off_stage + A.rs_lds_off, A.rs_nbm);
     __syncthreads();
     for (;;) {
         if (tid == 0) {
@@ -1357,6 +1620,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         if (ci >= A.nchunks) break;
         const Chunk ch = A.chunks[ci];
         const TermDesc T = A.terms[ch.term];
+        const int4 term = STREAM ? A.rs_term[ch.term] : make_int4(0, 0, 0, 0);
+        if (STREAM) ks.open(A, lane, ch.term, term);
         unsigned long long *out = A.out + (long long)ch.term * A.nbc;
         for (;;) {
             const int p = uniform(atomicAdd(&s_next, lane == 0 ? 1 : 0)) + ch.begin;
@@ -1387,30 +1652,78 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             wave_sync();
             const char *lb = T.sym ? l2 : l3;
             if (sym) {
-                if (T.weighted) {
-                    RppiPol<SURVEY, true, true> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
-                    traverse(P, n2u, n3u);
-                } else {
-                    RppiPol<SURVEY, true, false> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
-                    traverse(P, n2u, n3u);
-                }
+                if (T.weighted)
+                    rppi_primary<SURVEY, true, true, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term, n2u,
+                                                             n3u);
+                else
+                    rppi_primary<SURVEY, true, false, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term,
+                                                              n2u, n3u);
             } else {
-                if (T.weighted) {
-                    RppiPol<SURVEY, false, true> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
-                    traverse(P, n2u, n3u);
-                } else {
-                    RppiPol<SURVEY, false, false> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw);
-                    traverse(P, n2u, n3u);
-                }
+                if (T.weighted)
+                    rppi_primary<SURVEY, false, true, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term,
+                                                              n2u, n3u);
+                else
+                    rppi_primary<SURVEY, false, false, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term,
+                                                               n2u, n3u);
             }
             wave_sync();
         }
         __syncthreads();
     }
+    if (STREAM) ks.close(A, lane);
 }
 
-template __global__ void k_rppi<false>(KArgs);
-template __global__ void k_rppi<true>(KArgs);
+template __global__ void k_rppi<false, false>(KArgs);
+template __global__ void k_rppi<true, false>(KArgs);
+template __global__ void k_rppi<false, true>(KArgs);
+template __global__ void k_rppi<true, true>(KArgs);
+
+// Binning of the rppi key stream: work item blockIdx.x adds its slice of one
+// bucket's keys in an LDS histogram of 2^shift bins (u32 counts or f64 sums),
+// then adds the non-zero bins to the term's dense histogram.  A slice holds
+// fewer than 2^30 keys (each counting at most 2), so u32 counts cannot wrap.
+__global__ __launch_bounds__(1024) void k_rppi_bin(KArgs A) {
+    extern __shared__ __attribute__((aligned(16))) char lds[];
+    const RBItem it = A.rs_items[blockIdx.x];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int nbin = 1 << it.shift;
+    unsigned *h32 = reinterpret_cast<unsigned *>(lds);
+    double *h64 = reinterpret_cast<double *>(lds);
+    for (int i = tid; i < (it.weighted ? 2 * nbin : nbin); i += nthr) h32[i] = 0u;
+    __syncthreads();
+    const unsigned long long b0 = A.rs_base[it.bucket];
+    const unsigned long long capb = A.rs_base[it.bucket + 1] - b0;
+    const unsigned long long cur = A.rs_cursor[it.bucket];
+    const unsigned long long n = cur < capb ? cur : capb;
+    const unsigned long long lo = n * (unsigned long long)it.j / (unsigned long long)it.s;
+    const unsigned long long hi = n * (unsigned long long)(it.j + 1) / (unsigned long long)it.s;
+    const unsigned mask = (unsigned)nbin - 1u;
+    if (it.weighted) {
+        for (unsigned long long i = lo + tid; i < hi; i += nthr) {
+            const unsigned key = A.rs_keys[b0 + i];
+            if (key != kRsPad) unsafeAtomicAdd(&h64[key & mask], A.rs_vals[b0 + i]);
+        }
+    } else {
+        for (unsigned long long i = lo + tid; i < hi; i += nthr) {
+            const unsigned key = A.rs_keys[b0 + i];
+            if (key != kRsPad) atomicAdd(&h32[key & mask], 1u + (key >> 31));
+        }
+    }
+    __syncthreads();
+    unsigned long long *out = A.out + (long long)it.term * A.nbc;
+    const unsigned np2 = (unsigned)(A.npi * A.npi);
+    for (int i = tid; i < nbin; i += nthr) {
+        const bool nz = it.weighted ? h64[i] != 0.0 : h32[i] != 0u;
+        if (!nz) continue;
+        const unsigned c = (unsigned)it.cbase + (unsigned)i;
+        const unsigned r = c / np2;
+        const long long idx = (long long)A.rs_inv[r] * np2 + (c - r * np2);
+        if (it.weighted)
+            unsafeAtomicAdd(reinterpret_cast<double *>(&out[idx]), h64[i]);
+        else
+            atomicAdd(&out[idx], (unsigned long long)h32[i]);
+    }
+}
 
 template __global__ void k_triplets<false, kHistCtab>(KArgs);
 template __global__ void k_triplets<false, kHistT3>(KArgs);
@@ -1522,6 +1835,13 @@ struct nc_context {
     unsigned long long dense_flush = 1ull << 31;
     int sample_stride = 1;  // autotuner: queue only every n-th work item
     std::string dev_name;
+    // rppi key stream: buffers, and the per-bucket capacities last sized for
+    // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
+    bool rppi_stream = true;
+    double rs_scale = 1.0;  // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
+    DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_keys, rs_vals;
+    std::vector<double> rs_key;
+    std::vector<unsigned long long> rs_caps;
 };
 
 namespace {
@@ -1882,9 +2202,57 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const int W = ctx->waves_per_group;
     const int block = W * kWave;
     const int lmode = G.rppi ? (G.survey ? kListRppiSurvey : kListRppiBox) : mixed ? kListMixed : kListPlain;
+    // --- rppi key stream: the reachable dense rp triples (sorted ones and the
+    // next triple, where a pi index equal to npi carries the flat index),
+    // their ranks, and per term a run of buckets of 2^shift compact bins
+    bool stream = G.rppi && ctx->rppi_stream;
+    std::vector<uint16_t> rs_rank;
+    std::vector<int> rs_inv;
+    std::vector<int4> rs_term(nterms);
+    int rs_nbk = 0;
+    bool rs_anyw = false;
+    int rs_maxshift = 0;
+    if (stream) {
+        const int nb = G.nb, dmax = G.npi == 1 ? 2 : 1;
+        const long long n3 = (long long)nb * nb * nb;
+        std::vector<char> reach((size_t)n3, 0);
+        for (int k1 = 0; k1 < nb; ++k1)
+            for (int k2 = k1; k2 < nb; ++k2)
+                for (int k3 = k2; k3 < nb; ++k3) {
+                    const long long T = k3 + (long long)k2 * nb + (long long)k1 * nb * nb;
+                    for (int d = 0; d <= dmax; ++d)
+                        if (T + d < n3) reach[(size_t)(T + d)] = 1;
+                }
+        rs_rank.assign((size_t)n3, 0xffff);
+        for (long long T = 0; T < n3; ++T)
+            if (reach[(size_t)T]) {
+                rs_rank[(size_t)T] = (uint16_t)rs_inv.size();
+                rs_inv.push_back((int)T);
+            }
+        const long long cbins = (long long)rs_inv.size() * G.npi * G.npi;
+        int sh_u = 0, sh_w = 0;
+        while (sh_u < 15 && ((size_t)4 << (sh_u + 1)) <= ctx->lds_hw) ++sh_u;
+        while (sh_w < 14 && ((size_t)8 << (sh_w + 1)) <= ctx->lds_hw) ++sh_w;
+        if (rs_inv.size() >= 65535 || cbins >= (1LL << 31)) stream = false;
+        for (int i = 0; i < nterms && stream; ++i) {
+            const Term &t = terms[i];
+            const bool w = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
+            const int sh = w ? sh_w : sh_u;
+            const long long nbt = (cbins + (1LL << sh) - 1) >> sh;
+            if (nbt > kStreamBuckets) stream = false;
+            rs_term[i] = make_int4(rs_nbk, (int)nbt, sh, 0);
+            rs_nbk += (int)nbt;
+            rs_anyw = rs_anyw || w;
+            rs_maxshift = std::max(rs_maxshift, sh + (w ? 3 : 2));  // log2 of the binning LDS bytes
+        }
+    }
+    size_t rank_bytes = stream ? al16(sizeof(uint16_t) * rs_rank.size()) : 0;
+    int rs_nbm = 8;  // per-wave bucket slots: the largest bucket count of a term, rounded up to 8
+    for (int i = 0; i < nterms && stream; ++i) rs_nbm = std::max(rs_nbm, (rs_term[i].y + 7) & ~7);
+    size_t stage_bytes = stream ? al16(stream_wave_bytes(rs_anyw, rs_nbm)) : 0;
     auto wave_lds = [&](int c) {
         const size_t cp = (size_t)c + 4;
-        if (G.rppi) return al16(cp * sizeof(RElem));
+        if (G.rppi) return al16(cp * sizeof(RElem)) + stage_bytes;
         return mixed ? al16(cp * (sizeof(float4) + sizeof(int) + sizeof(double)) + kQueueCap * sizeof(uint2))
                      : al16(cp * (sizeof(Elem) + sizeof(double)));
     };
@@ -1927,7 +2295,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             o_ctab = o;
             o = al16(o + sizeof(int) * G.ctab.size());
         }
-        return o;
+        return o + (hm == kHistGlobal ? rank_bytes : 0);  // the rank table ends the tables
     };
     size_t off_sq, off_lut, off_ctab, off_lut32, off_t3;
     NC_HIP(ctx, dgrow(ctx->tables, tab_bytes));
@@ -2014,6 +2382,11 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         budget = ctx->lds_hw;
         hm = choose(budget);
     }
+    if (stream && !fits(hm, budget)) {  // no room for the key stream: direct atomics
+        stream = false;
+        rank_bytes = stage_bytes = 0;
+        budget = fits(hm, ctx->lds_max) ? ctx->lds_max : ctx->lds_hw;
+    }
     const size_t hist_bytes = hist_size(hm);
     const size_t lds_tab_bytes = lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3);
 
@@ -2031,7 +2404,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
          (const void *)k_triplets<false, kHistDense>},
         {(const void *)k_triplets<true, kHistCtab>, (const void *)k_triplets<true, kHistT3>,
          (const void *)k_triplets<true, kHistDense>}};
-    const void *kfn = G.rppi ? (G.survey ? (const void *)k_rppi<true> : (const void *)k_rppi<false>)
+    const void *kfn = G.rppi ? (G.survey ? (stream ? (const void *)k_rppi<true, true> : (const void *)k_rppi<true, false>)
+                                         : (stream ? (const void *)k_rppi<false, true> : (const void *)k_rppi<false, false>))
                              : kfns[mixed ? 1 : 0][hm];
     int per_cu = ctx->wgs_per_cu;
     if (per_cu <= 0) {
@@ -2135,19 +2509,128 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.off_wave = (int)lds_tab;
     A.wave_bytes = (int)wave_bytes;
     A.lds_bytes = (int)lds_bytes;
+    NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+
+    // --- rppi key stream: tables, bucket regions, binning work items.  The
+    // regions are sized from the keys per bucket of a dry run over every
+    // S-th work item (kept for the same catalogues and params); keys beyond
+    // a region's capacity still count, through direct atomics.
+    std::vector<unsigned long long> caps, rcaps(std::max(rs_nbk, 1));  // wanted, and allocated, keys per bucket
+    std::vector<double> rs_ckey;
+    if (stream && A.nchunks > 0) {
+        A.off_rank = (int)(hist_bytes + lds_tab_bytes - rank_bytes);
+        A.off_stage = (int)(wave_bytes - stage_bytes);
+        A.rs_lds_off = (int)stream_lds_offset(rs_anyw);
+        A.rs_nbm = rs_nbm;
+        const size_t tb_rank = al16(sizeof(int4) * (size_t)nterms);
+        const size_t tb_inv = al16(tb_rank + sizeof(uint16_t) * rs_rank.size());
+        const size_t tb_bytes = al16(tb_inv + sizeof(int) * rs_inv.size());
+        std::vector<char> tb(tb_bytes);
+        std::memcpy(tb.data(), rs_term.data(), sizeof(int4) * (size_t)nterms);
+        std::memcpy(tb.data() + tb_rank, rs_rank.data(), sizeof(uint16_t) * rs_rank.size());
+        std::memcpy(tb.data() + tb_inv, rs_inv.data(), sizeof(int) * rs_inv.size());
+        NC_HIP(ctx, dgrow(ctx->rs_tab, tb_bytes));
+        NC_HIP(ctx, hipMemcpyAsync(ctx->rs_tab.p, tb.data(), tb_bytes, hipMemcpyHostToDevice, ctx->stream));
+        NC_HIP(ctx, dgrow(ctx->rs_cursor, sizeof(unsigned long long) * (size_t)rs_nbk));
+        NC_HIP(ctx, dgrow(ctx->rs_base, sizeof(unsigned long long) * ((size_t)rs_nbk + 1)));
+        A.rs_term = (const int4 *)ctx->rs_tab.p;
+        A.rs_rank = (const uint16_t *)((char *)ctx->rs_tab.p + tb_rank);
+        A.rs_inv = (const int *)((char *)ctx->rs_tab.p + tb_inv);
+        A.rs_cursor = (unsigned long long *)ctx->rs_cursor.p;
+        A.rs_base = (const unsigned long long *)ctx->rs_base.p;
+        rs_ckey = pkey;
+        for (uint64_t u : uids) rs_ckey.push_back((double)u);
+        if (ctx->sample_stride == 1 && ctx->rs_key == rs_ckey && ctx->rs_caps.size() == (size_t)rs_nbk) {
+            caps = ctx->rs_caps;
+        } else {
+            const size_t S = std::max<size_t>(1, std::min<size_t>(32, chunks.size() / 256));
+            std::vector<Chunk> sample;
+            for (size_t i = 0; i < chunks.size(); i += S) sample.push_back(chunks[i]);
+            NC_HIP(ctx, dgrow(ctx->rs_sample, sizeof(Chunk) * sample.size()));
+            NC_HIP(ctx, hipMemcpyAsync(ctx->rs_sample.p, sample.data(), sizeof(Chunk) * sample.size(),
+                                       hipMemcpyHostToDevice, ctx->stream));
+            KArgs As = A;
+            As.chunks = (const Chunk *)ctx->rs_sample.p;
+            As.nchunks = (int)sample.size();
+            As.rs_dry = 1;
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
+            hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, A.rs_cursor, 0ull, (long long)rs_nbk);
+            void *args[] = {&As};
+            NC_HIP(ctx, hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream));
+            std::vector<unsigned long long> n(rs_nbk);
+            NC_HIP(ctx, hipMemcpyAsync(n.data(), A.rs_cursor, sizeof(unsigned long long) * n.size(),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            caps.resize(rs_nbk);
+            for (int b = 0; b < rs_nbk; ++b)
+                caps[b] = S == 1 ? n[b] + 64 : (unsigned long long)((double)n[b] * (double)S * 1.3) + 4096;
+        }
+        // at most 60% of the free device memory for the regions
+        size_t freeb = 0, totalb = 0;
+        NC_HIP(ctx, hipMemGetInfo(&freeb, &totalb));
+        const double per = rs_anyw ? 12.0 : 4.0;
+        double need = 0.0;
+        for (unsigned long long c : caps) need += (double)c;
+        const double room = 0.6 * ((double)freeb + (double)ctx->rs_keys.bytes + (double)ctx->rs_vals.bytes) / per;
+        const double scale = (need > room ? room / need : 1.0) * ctx->rs_scale;
+        // a wave reserves blk slots at a time and may leave up to blk of
+        // them unused per bucket: each region holds that slack too
+        const double per_wave = need / std::max(1, rs_nbk) / (4.0 * (double)nwaves);
+        int blk = 64;
+        while (blk < 4096 && 2.0 * blk <= per_wave) blk *= 2;
+        A.rs_blk = blk;
+        const unsigned long long slack = (unsigned long long)nwaves * (unsigned long long)blk;
+        std::vector<unsigned long long> base(rs_nbk + 1, 0);
+        for (int b = 0; b < rs_nbk; ++b)
+            base[b + 1] = base[b] + (((unsigned long long)((double)caps[b] * scale) + slack + 63) & ~63ull);
+        NC_HIP(ctx, hipMemcpyAsync(ctx->rs_base.p, base.data(), sizeof(unsigned long long) * base.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        NC_HIP(ctx, dgrow(ctx->rs_keys, sizeof(unsigned) * (size_t)base[rs_nbk]));
+        if (rs_anyw) NC_HIP(ctx, dgrow(ctx->rs_vals, sizeof(double) * (size_t)base[rs_nbk]));
+        A.rs_keys = (unsigned *)ctx->rs_keys.p;
+        A.rs_vals = rs_anyw ? (double *)ctx->rs_vals.p : nullptr;
+        // binning work items: slices in proportion to the region sizes (about
+        // four per CU in all), each under 2^30 keys
+        std::vector<RBItem> items;
+        const double total = std::max(1.0, (double)base[rs_nbk]);
+        for (int i = 0; i < nterms; ++i) {
+            const int4 tr = rs_term[i];
+            for (int bb = 0; bb < tr.y; ++bb) {
+                const int b = tr.x + bb;
+                const unsigned long long cb = base[b + 1] - base[b];
+                long long sl = std::llround(4.0 * ctx->ncu * (double)cb / total);
+                sl = std::max<long long>({1LL, sl, (long long)((cb + (1ull << 30) - 1) >> 30)});
+                for (long long j = 0; j < sl; ++j)
+                    items.push_back({b, i, tr.z, weighted[i] ? 1 : 0, bb << tr.z, (int)j, (int)sl, 0});
+            }
+        }
+        NC_HIP(ctx, dgrow(ctx->rs_items, sizeof(RBItem) * items.size()));
+        NC_HIP(ctx, hipMemcpyAsync(ctx->rs_items.p, items.data(), sizeof(RBItem) * items.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        A.rs_items = (const RBItem *)ctx->rs_items.p;
+        A.rs_nitems = (int)items.size();
+        for (int b = 0; b < rs_nbk; ++b) rcaps[b] = base[b + 1] - base[b];
+    }
+    const size_t bin_lds = (size_t)1 << rs_maxshift;
 
     auto enqueue_count = [&]() -> int {
         hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->out.p, 0ull,
                            (long long)nterms * G.out_words);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
         if (A.nchunks > 0) {
+            if (stream)
+                hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, A.rs_cursor, 0ull,
+                                   (long long)rs_nbk);
             void *args[] = {&A};
             if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
+                return NC_ERR_HIP;
+            if (stream && A.rs_nitems > 0 &&
+                hipLaunchKernel((const void *)k_rppi_bin, dim3(A.rs_nitems), dim3(1024), args, bin_lds,
+                                ctx->stream) != hipSuccess)
                 return NC_ERR_HIP;
         }
         return hipGetLastError() == hipSuccess ? NC_OK : NC_ERR_HIP;
     };
-    NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     if (enqueue_count() != NC_OK) return fail(ctx, NC_ERR_HIP, "triplet kernel launch failed");
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     std::vector<unsigned long long> res((size_t)nterms * G.out_words);
@@ -2161,9 +2644,26 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.ms_build = ms_b;
     ctx->timing.ms_count = ms_c;
     ctx->timing.ms_total = ms_b + ms_c;
-    ctx->timing.n_kernel = A.nchunks > 0 ? 1 : 0;
-    ctx->timing.hist_layout = hm;
+    ctx->timing.n_kernel = A.nchunks > 0 ? (stream ? 2 : 1) : 0;
+    ctx->timing.hist_layout = stream ? kHistStream : hm;
     ctx->timing.lds_bytes = (int32_t)lds_bytes;
+    // key stream: keep the region sizes; after an overflow (counted, but
+    // through direct atomics) size from the keys this run emitted, and
+    // capture no graph so that the next call uses them
+    bool rs_overflow = false;
+    if (stream && A.nchunks > 0) {
+        std::vector<unsigned long long> n(rs_nbk);
+        NC_HIP(ctx, hipMemcpy(n.data(), A.rs_cursor, sizeof(unsigned long long) * n.size(), hipMemcpyDeviceToHost));
+        for (int b = 0; b < rs_nbk; ++b)  // n: slots reserved, at least the keys
+            if (n[b] > rcaps[b]) {
+                rs_overflow = true;
+                caps[b] = std::max(caps[b], n[b] + n[b] / 8 + 64);
+            }
+        if (ctx->sample_stride == 1) {  // not from an autotuning sample
+            ctx->rs_key = rs_ckey;
+            ctx->rs_caps = caps;
+        }
+    }
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
     if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
@@ -2172,7 +2672,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 
     // --- capture the launch sequence for replay (builds, then count); any
     // failure leaves the eager result valid and no plan cached
-    if (use_graph) {
+    if (use_graph && !rs_overflow) {
         BatchPlan p;
         p.uids = uids;
         p.pkey = pkey;
@@ -2181,8 +2681,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.nb = G.nb;
         p.nbc = G.out_words;
         p.rppi = G.rppi;
-        p.nkernel = A.nchunks > 0 ? 1 : 0;
-        p.hist_layout = hm;
+        p.nkernel = A.nchunks > 0 ? (stream ? 2 : 1) : 0;
+        p.hist_layout = stream ? kHistStream : hm;
         p.lds_bytes = (int)lds_bytes;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
                       hipSuccess &&
@@ -2230,6 +2730,11 @@ int nc_create(nc_context **out, int device) {
     ctx->dev_name = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
     ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
     ctx->lds_hw = prop.sharedMemPerBlock - 256;
+    if (const char *e = std::getenv("NC_RPPI_STREAM")) ctx->rppi_stream = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("NC_RPPI_REGION_SCALE")) {
+        const double v = std::strtod(e, nullptr);
+        if (v > 0.0 && v <= 1.0) ctx->rs_scale = v;
+    }
     if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
         const unsigned long long v = std::strtoull(e, nullptr, 10);
         if (v > 0 && v <= (1ull << 31)) ctx->dense_flush = v;
@@ -2249,7 +2754,8 @@ int nc_destroy(nc_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     plan_drop(ctx);
     for (DBuf *b : {&ctx->tables, &ctx->terms, &ctx->chunks, &ctx->out, &ctx->counter, &ctx->errw, &ctx->scratch,
-                    &ctx->bound})
+                    &ctx->bound, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
+                    &ctx->rs_keys, &ctx->rs_vals})
         dfree(*b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

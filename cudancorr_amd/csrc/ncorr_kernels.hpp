@@ -48,6 +48,18 @@ struct Chunk {
     int pad;
 };
 
+// rppi key stream: one work item of the binning kernel, a slice j of s of
+// one bucket's keys.
+struct RBItem {
+    int bucket;    // global bucket id
+    int term;
+    int shift;     // log2 of the bucket's bins
+    int weighted;  // f64 sums (else u32 counts)
+    int cbase;     // first compact bin of the bucket within its term
+    int j, s;
+    int pad;
+};
+
 struct KArgs {
     const TermDesc *terms;
     const Chunk *chunks;
@@ -92,6 +104,23 @@ struct KArgs {
     const uint16_t *t3;           // nb*nb*(nb+2): (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
     // dynamic LDS layout (bytes)
     int off_sq, off_lut, off_ctab, off_lut32, off_t3, off_wave, wave_bytes, lds_bytes;
+    // rppi key stream (k_rppi with STREAM): binned triplets leave as compact
+    // keys c = rank(T)*npi^2 + rem, grouped by bucket c >> shift into
+    // per-bucket regions, then k_rppi_bin adds each bucket in LDS.
+    int rs_dry;                            // count keys per bucket only (region sizing)
+    int rs_blk;                            // keys a wave reserves in a bucket region at a time
+    int rs_lds_off;                        // per-wave run state, from the start of the staging
+    int rs_nbm;                            // per-wave bucket slots (>= buckets of any term)
+    int rs_nitems;                         // k_rppi_bin work items
+    int off_rank, off_stage;               // LDS: rank table, per-wave staging (inside the wave slice)
+    const int4 *rs_term;                   // per term: first bucket, buckets, shift, 0
+    const uint16_t *rs_rank;               // nb^3: rank of a dense rp triple among the reachable ones
+    const int *rs_inv;                     // rank -> dense rp triple
+    unsigned long long *rs_cursor;         // slots reserved per bucket (may pass the capacity); dry: keys
+    const unsigned long long *rs_base;     // region of bucket b: [base[b], base[b+1]) in rs_keys / rs_vals
+    unsigned *rs_keys;                     // c | (count 2 ? 1 << 31 : 0); kRsPad in unused reserved slots
+    double *rs_vals;                       // weighted terms: the summed weight products
+    const RBItem *rs_items;
 };
 
 }  // namespace nck

@@ -86,7 +86,8 @@ typedef struct {
     int32_t hist_layout; /* workgroup histogram of the triplet kernel: 0 sorted slots via an index table,
                             1 sorted slots via a per-bin-triple slot table, 2 dense nb^3 32-bit counts
                             (unit weights only; merged into the sorted slots before they can wrap),
-                            3 the dense global histogram (rppi) */
+                            3 rppi: atomics on the dense global histogram, 4 rppi: binned triplets
+                            streamed as keys into per-bucket regions and added up in LDS */
     int32_t lds_bytes;   /* LDS of one workgroup of the triplet kernel (above half the CU's LDS: one
                             workgroup per CU) */
 } nc_timing;

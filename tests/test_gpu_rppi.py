@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN
+from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, Context
 from cudancorr_amd.gen import neyman_scott_box, prep_cat_survey, survey_cap, uniform_box
 from oracle import oracle
 
@@ -147,3 +147,49 @@ def test_dropin_rppi(dropin, tmp_path, survey):
     nz = want != 0
     assert np.array_equal(got[~nz], want[~nz])
     assert np.max(np.abs(got[nz] - want[nz]) / want[nz], initial=0.0) <= WEIGHT_RTOL
+
+
+def test_key_stream_layout_and_direct_atomics(ctx, monkeypatch):
+    """The key stream (binned per bucket in LDS) and the direct global atomics
+    give the same counts; weighted sums agree to rounding."""
+    _log("rppi stream vs direct")
+    p, edges, npi, box = _box_case(seed=12)
+    rng = np.random.default_rng(3)
+    w = p.copy()
+    w[:, 3] = rng.lognormal(0.0, 0.3, w.shape[0])
+    h_s, c_s = ctx.count(p, p, p, edges, box, npi=npi)
+    assert ctx.timing().hist_layout == 4
+    hw_s, _ = ctx.count(w, w, p, edges, box, npi=npi)
+    monkeypatch.setenv("NC_RPPI_STREAM", "0")
+    with Context(0) as c2:
+        h_d, c_d = c2.count(p, p, p, edges, box, npi=npi)
+        assert c2.timing().hist_layout == 3
+        hw_d, _ = c2.count(w, w, p, edges, box, npi=npi)
+    assert c_s.sum() > 100_000
+    assert np.array_equal(c_s, c_d)
+    _wcheck(hw_s, hw_d)
+
+
+@pytest.mark.parametrize("survey", [False, True])
+def test_key_stream_region_overflow(monkeypatch, survey):
+    """Regions sized at 5% of the sampled need: the keys past them are added
+    directly, on every call, and the results stay exact."""
+    _log(f"rppi region overflow survey={survey}")
+    monkeypatch.setenv("NC_RPPI_REGION_SCALE", "0.05")
+    if survey:
+        d, r = survey_cap(2500, 10, seed=13)
+        d[:, :3] = 1000.0 + (d[:, :3] - 1000.0) * 0.1
+        box, ori = prep_cat_survey(d, r)
+        p, edges, npi = d, np.array([0.0, 2.0, 4.0, 6.0, 9.0]), 12
+    else:
+        p, edges, npi, box = _box_case(seed=5)
+        ori = (0.0, 0.0, 0.0)
+    cnt, ws, _ = oracle.count_rppi(p, p, p, edges, npi, box, survey=survey, ori=ori)
+    with Context(0) as c2:
+        for _ in range(2):
+            hist, got = c2.count(p, p, p, edges, box, survey=survey, npi=npi, ori=ori)
+            assert c2.timing().hist_layout == 4
+            if survey:
+                _wcheck(hist, ws)
+            else:
+                assert np.array_equal(got, cnt)
