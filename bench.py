@@ -79,6 +79,39 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def atomic_rate():
+    """Scattered global atomics per second (torch index_add_: one atomic add
+    per element, random bins), u64 and u32, for histograms of 4096 to 2^21
+    bins: the ceiling of any path that adds each binned triplet to a global
+    histogram."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    n = 1 << 27
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    res = []
+    for bins in (4096, 32768, 262144, 1 << 21):
+        idx = torch.randint(0, bins, (n,), device=dev, generator=g)
+        for dt in (torch.int64, torch.int32):
+            h = torch.zeros(bins, dtype=dt, device=dev)
+            one = torch.ones(n, dtype=dt, device=dev)
+            h.index_add_(0, idx, one)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            best = 1e30
+            for _ in range(3):
+                a.record()
+                h.index_add_(0, idx, one)
+                b.record()
+                b.synchronize()
+                best = min(best, a.elapsed_time(b))
+            if int(h.sum().item()) != 4 * n:
+                raise SystemExit("index_add_ lost updates")
+            res.append({"bins": bins, "dtype": str(dt).replace("torch.", ""), "ms": best, "atomics_per_s": n / (best * 1e-3)})
+    print(json.dumps({"atomic_rate": res}), flush=True)
+
+
 def from_profiles(key):
     """A figure of the committed rocprofv3 summary of this workload's triplet
     kernel: traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy
@@ -281,11 +314,16 @@ def main():
     ap.add_argument("--boxes", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true", help="skip the 64-call plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--atomic-rate", action="store_true",
+                    help="only measure scattered global atomic adds per second and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.atomic_rate:
+        atomic_rate()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
