@@ -272,11 +272,24 @@ struct LdsTabs {
 // rppi box and rppi survey (RElem).
 constexpr int kListPlain = 0, kListMixed = 1, kListRppiBox = 2, kListRppiSurvey = 3;
 
-// One rppi list entry: image position, and to the primary rp^2 and pi^2 (box)
-// or the squared 3D separation and x - ori.x (survey), and the weight.
+// One rppi list entry: image position, to the primary rp^2 (box) or the
+// squared 3D separation (survey) r, then q, and the weight.  Box: q holds the
+// side's FP32 root sqrtf(rp^2), rp bin and pi bin floor(sqrtf(pi^2)) packed
+// (rp_side_pack), so a pair only evaluates its third side; survey: x - ori.x.
 struct RElem {
     double x, y, z, r, q, w;
 };
+__device__ __forceinline__ double rp_side_pack(float f, int k, int p) {
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)(k | (p << 8)) << 32) |
+                                            (unsigned long long)__float_as_uint(f)));
+}
+__device__ __forceinline__ void rp_side_unpack(double q, float &f, int &k, int &p) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(q);
+    f = __uint_as_float((unsigned)u);
+    const unsigned kp = (unsigned)(u >> 32);
+    k = (int)(kp & 0xffu);
+    p = (int)(kp >> 8);
+}
 
 // Bytes of one list entry in a wave's scratch slot.
 __host__ __device__ constexpr size_t list_entry_bytes(int lm) {
@@ -340,7 +353,12 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
             }
             const unsigned long long m = __ballot(pass);
             const int pos = n + lane_rank(m);
-            if (pass & (pos < A.cap)) reinterpret_cast<RElem *>(base)[pos] = RElem{qx, qy, qz, r, q, qw};
+            if (pass & (pos < A.cap)) {
+                if constexpr (LM == kListRppiBox)  // the side's root, rp bin and pi bin (device.cu:214-250)
+                    q = rp_side_pack(__builtin_sqrtf((float)r), bin_of(r, A.lut_scale, lut, sq, A.nb),
+                                     (int)floorf(__builtin_sqrtf((float)q)));
+                reinterpret_cast<RElem *>(base)[pos] = RElem{qx, qy, qz, r, q, qw};
+            }
             n += __popcll(m);
             return;
         }
@@ -1201,6 +1219,41 @@ __device__ __forceinline__ bool rppi_cell(const KArgs &A, const LdsTabs &t, doub
     return true;
 }
 
+// rppi_cell for box sides whose root, rp bin and pi bin are known: the same
+// sort (the same swaps on rp^2, so ties keep their order), triangle test and
+// flat index.
+__device__ __forceinline__ bool rppi_cell_sides(const KArgs &A, double r1, float f1, int k1, int p1, double r2,
+                                                float f2, int k2, int p2, double r3, float f3, int k3, int p3,
+                                                int &T, int &rem) {
+    if (r1 > r2) {
+        const double x = r1; r1 = r2; r2 = x;
+        const float y = f1; f1 = f2; f2 = y;
+        int z = k1; k1 = k2; k2 = z;
+        z = p1; p1 = p2; p2 = z;
+    }
+    if (r1 > r3) {
+        const double x = r1; r1 = r3; r3 = x;
+        const float y = f1; f1 = f3; f3 = y;
+        int z = k1; k1 = k3; k3 = z;
+        z = p1; p1 = p3; p3 = z;
+    }
+    if (r2 > r3) {
+        const float y = f2; f2 = f3; f3 = y;
+        int z = k2; k2 = k3; k3 = z;
+        z = p2; p2 = p3; p3 = z;
+    }
+    if (f1 + f2 <= f3) return false;
+    const int np = A.npi, np2 = np * np, nb = A.nb;
+    T = k3 + k2 * nb + k1 * nb * nb;
+    rem = p2 + p1 * np;
+    if (rem >= np2) {
+        const int c = rem / np2;
+        T += c;
+        rem -= c * np2;
+    }
+    return true;
+}
+
 // Per-wave staging of the rppi key stream: up to kStage keys (and values);
 // a row adds at most 2*64, so the wave flushes once it holds more than
 // kStage - 128.  kStreamBuckets bounds the buckets of one term.  Per bucket
@@ -1497,11 +1550,17 @@ struct RppiPol {
         if constexpr (!SURVEY) {
             const double q3 = dz * dz, r3 = rp2(dx, dy);  // device.cu:357-361
             if (ok && q3 < A.pimax2 && r3 < A.r2max && r3 > A.r2min) {
-                h0 = rppi_cell(A, t, e2.r, e3.r, r3, e2.q, e3.q, q3, T0, rem0);
+                float f2, f3;
+                int k2, p2, k3, p3;
+                rp_side_unpack(e2.q, f2, k2, p2);
+                rp_side_unpack(e3.q, f3, k3, p3);
+                const float f = __builtin_sqrtf((float)r3);
+                const int k = bin_any<false>(A, t, r3), p = (int)floorf(__builtin_sqrtf((float)q3));
+                h0 = rppi_cell_sides(A, e2.r, f2, k2, p2, e3.r, f3, k3, p3, r3, f, k, p, T0, rem0);
                 if (SYM) {
                     const bool tie = e2.r == e3.r || e2.r == r3 || e3.r == r3;
                     if (tie) {
-                        h1 = rppi_cell(A, t, e3.r, e2.r, r3, e3.q, e2.q, q3, T1, rem1);
+                        h1 = rppi_cell_sides(A, e3.r, f3, k3, p3, e2.r, f2, k2, p2, r3, f, k, p, T1, rem1);
                         v0 = (w1 * e2.w) * e3.w;
                         v1 = (w1 * e3.w) * e2.w;
                     } else {
