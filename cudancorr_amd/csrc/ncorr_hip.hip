@@ -3044,9 +3044,13 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
     nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
     int rc = NC_OK;
+    // 16-wave workgroups only where one workgroup per CU is forced anyway
+    // (more than half the CU's LDS); elsewhere two 8-wave ones do better
+    bool big_lds = false;
     {
         // a first run at the current shape sizes the sample to about 100 ms
         rc = run_count(ctx, params, t, path, hist.data(), nullptr, false);
+        big_lds = rc == NC_OK && (size_t)ctx->timing.lds_bytes > ctx->lds_max;
         const double ms = ctx->timing.ms_count;
         if (rc == NC_OK && ms > 200.0)
             ctx->sample_stride = (int)std::min<double>((double)items, ctx->sample_stride * (ms / 100.0));
@@ -3056,6 +3060,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     for (int w : {4, 8, 16})
         for (int c : {16, 64})
             for (int lc : {64, 128, 256}) {
+                if (w == 16 && !big_lds) continue;
                 ctx->waves_per_group = w;
                 ctx->chunk = c;
                 ctx->wgs_per_cu = 0;
