@@ -24,7 +24,10 @@ or four nc_count calls (plain), and one step is all slices.  Binned triplets
 of the weighted terms come from an untimed unit-weight run of the same terms;
 weighted sums of the mixed path are compared with the plain FP64 path.
 
-Prints one JSON line (see the end of main() and run_survey()).
+rppi workload (--workload rppi-box): the reference's rppi binning on the
+1M-point clustered box, the key stream against direct global atomics.
+
+Prints one JSON line (see the end of main(), run_survey() and run_rppi()).
 """
 from __future__ import annotations
 
@@ -302,14 +305,92 @@ def run_survey(args):
     ctx.close()
 
 
+def run_rppi(args):
+    """rppi binning (the reference's binType 1) on the 1M-point clustered box:
+    rp edges 0..30 in 15 bins, 20 unit pi bins, DDD.  The key stream (default)
+    against the reference's direct global atomics (NC_RPPI_STREAM=0, a second
+    context), counts compared bit for bit; the CPU oracle on a sample."""
+    from cudancorr_amd import Context
+    from cudancorr_amd.gen import SEED, edges_linear, neyman_scott_box
+
+    p = neyman_scott_box(10000, 100, 5.0, 1000.0, seed=SEED + 1000)
+    edges, npi, box = edges_linear(0.0, 30.0, 15), 20, np.array([1000.0] * 3)
+    hip = _hip()
+
+    def timed(ctx):
+        c = ctx.catalog(p)
+        for _ in range(max(1, args.warmup)):
+            ctx.count(c, c, c, edges, box, npi=npi)
+        device_sync(hip)
+        ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, cnt = ctx.count(c, c, c, edges, box, npi=npi)
+            ms.append(ctx.timing().ms_count)
+        device_sync(hip)
+        wall = (time.perf_counter() - t0) / args.steps
+        layout = ctx.timing().hist_layout
+        c.close()
+        return cnt, wall, float(np.mean(ms)), layout
+
+    with Context(0) as ctx:
+        cnt_s, wall_s, ms_s, lay_s = timed(ctx)
+    os.environ["NC_RPPI_STREAM"] = "0"
+    try:
+        with Context(0) as ctx:
+            cnt_d, wall_d, ms_d, lay_d = timed(ctx)
+    finally:
+        del os.environ["NC_RPPI_STREAM"]
+    if not np.array_equal(cnt_s, cnt_d):
+        raise SystemExit("key stream and direct atomics differ")
+    trip = int(cnt_s.sum())
+    log(f"rppi key stream (layout {lay_s}): {trip} triplets, count {ms_s:.1f} ms; "
+        f"direct atomics (layout {lay_d}): {ms_d:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+
+        threads = min(16, len(os.sched_getaffinity(0)))
+        rng = np.random.default_rng(1)
+        sample = p[np.sort(rng.choice(p.shape[0], size=131072, replace=False))]
+        t0 = time.perf_counter()
+        c, _, _ = oracle.count_rppi(sample, p, p, edges, npi, box, nthreads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": float(c.sum()) / dt, "unit": "binned triplets/s", "cores": threads, "kind": "port",
+               "sample": f"131072 random primaries against the full box, {int(c.sum())} triplets in {dt:.2f} s"}
+    out = {
+        "metric": "rppi binned triplets/s on 1 MI355X (key stream vs direct atomics)",
+        "value": trip / wall_s,
+        "unit": "binned triplets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Neyman-Scott box)",
+        "config": {"workload": "rppi, 1M-point clustered box (10,000 parents x 100, sigma=5, L=1000), "
+                               "rp edges 0..30 x15, 20 unit pi bins, DDD", "binned_triplets": trip},
+        "error": {"unweighted": "bit-exact (key stream vs direct atomics)"},
+        "direct_atomics": {"wall_ms": wall_d * 1e3, "count_ms": ms_d, "triplets_per_s": trip / wall_d},
+        "count_ms": ms_s,
+        "speedup_vs_direct": wall_d / wall_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["boxes", "survey", "survey-small"], default="boxes",
+    ap.add_argument("--workload", choices=["boxes", "survey", "survey-small", "rppi-box"], default="boxes",
                     help="boxes: 64 clustered periodic boxes (default); survey: 4M data + 20M randoms, "
-                         "four terms; survey-small: 500k data + 2.5M randoms")
+                         "four terms; survey-small: 500k data + 2.5M randoms; rppi-box: rppi binning of "
+                         "the 1M-point box")
     ap.add_argument("--slices", type=int, default=16, help="survey: primaries split into this many launches")
     ap.add_argument("--boxes", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true", help="skip the 64-call plain leg")
@@ -328,6 +409,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "rppi-box":
+        run_rppi(args)
+        return
     if args.workload != "boxes":
         run_survey(args)
         return
