@@ -128,10 +128,22 @@ def from_profiles(key):
     return None
 
 
-def cpu_baseline(cats, edges, box, budget_s=15.0, survey=False):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(cats, edges, box, budget_s=15.0, survey=False, gpu_count=None):
     """Time the CPU oracle (the reference's 27x27-cell loop, FP64, OpenMP over
     primaries) on a bounded sample: random primaries of the largest catalogue
-    against all of it, the sample growing until the run takes a few seconds."""
+    against all of it, the sample growing until the run takes a few seconds.
+    gpu_count(sample, catalogue) -> counts: the same subset counted on the GPU,
+    which must agree bit for bit."""
     from oracle import oracle
 
     big = max(cats, key=lambda c: c.shape[0])
@@ -150,14 +162,21 @@ def cpu_baseline(cats, edges, box, budget_s=15.0, survey=False):
         t0 = time.perf_counter()
         c, _ = oracle.count(sample, big, big, edges, box, survey=survey, nthreads=threads)
         dt = time.perf_counter() - t0
-        best = (int(c.sum()), dt, sample.shape[0])
+        best = (int(c.sum()), dt, sample.shape[0], c, sample)
         if dt > budget_s / 4 or m >= big.shape[0]:
             break
         m *= 4
-    trip, dt, n = best
+    trip, dt, n, c, sample = best
+    checked = None
+    if gpu_count is not None:
+        checked = bool(np.array_equal(gpu_count(sample, big), c))
+        if not checked:
+            raise SystemExit("the GPU count of the CPU baseline's sample differs from the oracle")
     return {"value": trip / dt, "unit": "binned triplets/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
             "sample": f"{n} random primaries of the largest catalogue (N={big.shape[0]}) against all of it, "
-                      f"{trip} binned triplets in {dt:.2f} s"}
+                      f"{trip} binned triplets in {dt:.2f} s",
+            "gpu_counts_bit_exact_on_sample": checked}
 
 
 def survey_workload(name):
@@ -272,7 +291,9 @@ def run_survey(args):
         log(f"plain: wall {wall_p:.2f} s, device {dev_p / 1e3:.2f} s (count kernels {cms_p / 1e3:.2f} s)")
 
     achieved = trip * FLOPS_PER_TRIPLET / (cms_m * 1e-3) / 1e12
-    cpu = None if args.no_cpu else cpu_baseline([r], edges, box, survey=True)
+    cpu = None if args.no_cpu else cpu_baseline(
+        [r], edges, box, survey=True,
+        gpu_count=lambda smp, big: ctx.count(smp, R, R, edges, box, survey=True)[1])
     out = {
         "metric": "binned triplets/s on 1 MI355X, batched mixed vs plain, at stated error",
         "value": trip / wall_m,
@@ -385,7 +406,7 @@ def run_rppi(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["boxes", "survey", "survey-small", "rppi-box"], default="boxes",
                     help="boxes: 64 clustered periodic boxes (default); survey: 4M data + 20M randoms, "
@@ -449,6 +470,7 @@ def main():
     trip = int(cnt_m.sum())
     dev_ms_m = float(np.mean([a + b for a, b in ker_ms]))
     count_ms_m = float(np.mean([b for _, b in ker_ms]))
+    count_ms_med = float(np.median([b for _, b in ker_ms]))
     log(f"mixed batched: {trip} triplets, wall {wall_m * 1e3:.1f} ms, device {dev_ms_m:.1f} ms "
         f"(count kernel {count_ms_m:.1f} ms)")
 
@@ -481,7 +503,9 @@ def main():
 
     value = trip / wall_m
     achieved = trip * FLOPS_PER_TRIPLET / (count_ms_m * 1e-3) / 1e12
-    cpu = None if args.no_cpu else cpu_baseline(cats, edges, box)
+    big_dev = dev[int(np.argmax([c.shape[0] for c in cats]))]
+    cpu = None if args.no_cpu else cpu_baseline(
+        cats, edges, box, gpu_count=lambda smp, big: ctx.count(smp, big_dev, big_dev, edges, box)[1])
     out = {
         "metric": "binned triplets/s on 1 MI355X, batched mixed vs plain, at stated error",
         "value": value,
@@ -505,7 +529,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": from_profiles("traffic_bytes_per_launch"),
                      "valu_busy_pct": from_profiles("valu_busy_pct"),
-                     "kernel": "k_triplets (mixed, batched)", "kernel_ms": count_ms_m},
+                     "kernel": "k_triplets (mixed, batched)", "kernel_ms": count_ms_m,
+                     "kernel_ms_median": count_ms_med},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
