@@ -193,3 +193,32 @@ def test_key_stream_region_overflow(monkeypatch, survey):
                 _wcheck(hist, ws)
             else:
                 assert np.array_equal(got, cnt)
+
+
+def test_key_stream_batched_mixed_weights(ctx):
+    """One launch over weighted and unit-weight terms: the unit-weight terms
+    stage their keys in the weighted layout; every term against the oracle,
+    over replays of the captured graph."""
+    _log("rppi stream batched mixed weights")
+    p, edges, npi, box = _box_case(seed=15)
+    rng = np.random.default_rng(4)
+    w = p.copy()
+    w[:, 3] = rng.lognormal(0.0, 0.3, w.shape[0])
+    q = uniform_box(3000, 96.0, seed=29)
+    q[:, 2] = np.mod(q[:, 2], 60.0)
+    arrs = {"p": p, "w": w, "q": q}
+    cats = {k: ctx.catalog(v) for k, v in arrs.items()}
+    specs = ["ppp", "wwq", "qqq", "wqq", "pqp"]
+    terms = [tuple(cats[ch] for ch in sp) for sp in specs]
+    want = [oracle.count_rppi(*(arrs[ch] for ch in sp), edges, npi, box) for sp in specs]
+    for rep in range(3):
+        hist, cnt = ctx.count_batched(terms, edges, box, path=NC_PATH_PLAIN, npi=npi)
+        assert ctx.timing().hist_layout == 4
+        for i, sp in enumerate(specs):
+            c, ws, _ = want[i]
+            if "w" in sp:
+                _wcheck(hist[i], ws)
+            else:
+                assert np.array_equal(cnt[i], c), f"replay {rep} term {sp}"
+    for c in cats.values():
+        c.close()
