@@ -180,8 +180,8 @@ def cpu_baseline(cats, edges, box, budget_s=15.0, survey=False, gpu_count=None):
 
 
 def survey_workload(name):
-    """(data, randoms, edges, box, description) of a survey workload, shifted
-    by prep_cat_survey (source/survey.cu:60-128)."""
+    """(data, randoms, edges, box, ori, description) of a survey workload,
+    shifted by prep_cat_survey (source/survey.cu:60-128)."""
     from cudancorr_amd.gen import SEED, WORKLOADS, prep_cat_survey, survey_cap
 
     if name == "survey":
@@ -195,15 +195,15 @@ def survey_workload(name):
         edges = WORKLOADS["survey_cap"]["edges"]
         desc = ("survey cap RA 0-60, Dec 0-60 deg, comoving 1000-2000 Mpc/h: 500k uniform data "
                 "(lognormal(0,0.3) f32 weights) + 2.5M randoms (w=1), rmax=40, 20 bins/side, DDD+DDR+DRR+RRR")
-    box, _ = prep_cat_survey(d, r)
-    return d, r, edges, box, desc
+    box, ori = prep_cat_survey(d, r)
+    return d, r, edges, box, ori, desc
 
 
 def run_survey(args):
     from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, Context
 
     t0 = time.perf_counter()
-    d, r, edges, box, desc = survey_workload(args.workload)
+    d, r, edges, box, _, desc = survey_workload(args.workload)
     log(f"generated {d.shape[0]} data + {r.shape[0]} randoms in {time.perf_counter() - t0:.1f}s")
     hip = _hip()
     ctx = Context(0)
@@ -326,6 +326,112 @@ def run_survey(args):
     ctx.close()
 
 
+def run_rppi_survey(args):
+    """rppi binning of the survey workload (500k weighted data + 2.5M randoms,
+    cap/shell), rp edges 0..40 in 20 bins, 20 unit pi bins: DDD, DDR, DRR, RRR
+    in one batched launch with the key stream, against the reference's direct
+    global atomics (NC_RPPI_STREAM=0); binned triplets of the weighted terms
+    from a unit-weight run; the CPU oracle on a sample of RRR primaries, checked
+    against the GPU on the same sample."""
+    from cudancorr_amd import Context
+
+    from cudancorr_amd.gen import edges_linear
+
+    d, r, _, box, ori, _ = survey_workload("survey-small")
+    edges, npi = edges_linear(0.0, 40.0, 20), 20
+    names = ["DDD", "DDR", "DRR", "RRR"]
+    hip = _hip()
+
+    def run(ctx, unit=False, steps=None, warmup=None):
+        """(hist, counts, wall s/step, count ms, tuning); the side legs (unit
+        weights, direct atomics) run one step after one warm-up."""
+        steps = args.steps if steps is None else steps
+        warmup = args.warmup if warmup is None else warmup
+        dd = d.copy() if unit else d
+        if unit:
+            dd[:, 3] = 1.0
+        D, R = ctx.catalog(dd), ctx.catalog(r)
+        terms = [(D, D, D), (D, D, R), (D, R, R), (R, R, R)]
+        tune = ctx.autotune(terms, edges, box, survey=True, npi=npi, ori=ori)
+        for _ in range(warmup):
+            ctx.count_batched(terms, edges, box, survey=True, npi=npi, ori=ori)
+        device_sync(hip)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            hist, cnt = ctx.count_batched(terms, edges, box, survey=True, npi=npi, ori=ori)
+        device_sync(hip)
+        wall = (time.perf_counter() - t0) / steps
+        ms = ctx.timing().ms_count
+        D.close()
+        R.close()
+        return hist, cnt, wall, ms, tune
+
+    with Context(0) as ctx:
+        hist_s, cnt_s, wall_s, ms_s, tune_s = run(ctx)
+        _, cnt_u, _, _, _ = run(ctx, unit=True, steps=1, warmup=0)
+        trip_terms = [int(c.sum()) for c in cnt_u[:3]] + [int(cnt_s[3].sum())]
+        trip = int(sum(trip_terms))
+        log(f"rppi survey key stream: {trip} triplets {dict(zip(names, trip_terms))}, wall {wall_s * 1e3:.1f} ms, "
+            f"count {ms_s:.1f} ms, tuning {tune_s}")
+        cpu = None
+        if not args.no_cpu:
+            from oracle import oracle
+
+            threads = min(16, len(os.sched_getaffinity(0)))
+            rng = np.random.default_rng(1)
+            sample = r[np.sort(rng.choice(r.shape[0], size=2048, replace=False))]
+            t0 = time.perf_counter()
+            c, _, _ = oracle.count_rppi(sample, r, r, edges, npi, box, survey=True, ori=ori, nthreads=threads)
+            dt = time.perf_counter() - t0
+            R = ctx.catalog(r)
+            _, g = ctx.count(sample, R, R, edges, box, survey=True, npi=npi, ori=ori)
+            R.close()
+            if not np.array_equal(g, c):
+                raise SystemExit("the GPU count of the CPU baseline's sample differs from the oracle")
+            cpu = {"value": float(c.sum()) / dt, "unit": "binned triplets/s", "cores": threads, "kind": "port",
+                   "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+                   "sample": f"2048 random RRR primaries against all 2.5M randoms, {int(c.sum())} triplets "
+                             f"in {dt:.2f} s", "gpu_counts_bit_exact_on_sample": True}
+    os.environ["NC_RPPI_STREAM"] = "0"
+    try:
+        with Context(0) as ctx:
+            hist_d, cnt_d, wall_d, ms_d, _ = run(ctx, steps=1, warmup=1)
+    finally:
+        del os.environ["NC_RPPI_STREAM"]
+    if not np.array_equal(cnt_s[3], cnt_d[3]):
+        raise SystemExit("RRR: key stream and direct atomics differ")
+    err = {}
+    for i, n in enumerate(names[:3]):
+        a, b = hist_s[i].ravel(), hist_d[i].ravel()
+        nz = b != 0
+        err[n] = float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz]), initial=0.0))
+    out = {
+        "metric": "rppi binned triplets/s on 1 MI355X (survey, key stream vs direct atomics)",
+        "value": trip / wall_s,
+        "unit": "binned triplets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded survey cap)",
+        "config": {"workload": "rppi survey: 500k weighted data + 2.5M randoms in the cap/shell, rp edges "
+                               "0..40 x20, 20 unit pi bins, DDD+DDR+DRR+RRR in one batched launch",
+                   "binned_triplets": trip, "binned_triplets_per_term": dict(zip(names, trip_terms))},
+        "error": {"unweighted": "RRR bit-exact (key stream vs direct atomics)",
+                  "weighted_rel_max_vs_direct": err},
+        "direct_atomics": {"wall_ms": wall_d * 1e3, "count_ms": ms_d, "triplets_per_s": trip / wall_d},
+        "count_ms": ms_s,
+        "tuning": tune_s,
+        "speedup_vs_direct": wall_d / wall_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_rppi(args):
     """rppi binning (the reference's binType 1) on the 1M-point clustered box:
     rp edges 0..30 in 15 bins, 20 unit pi bins, DDD.  The key stream (default)
@@ -408,10 +514,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["boxes", "survey", "survey-small", "rppi-box"], default="boxes",
+    ap.add_argument("--workload", choices=["boxes", "survey", "survey-small", "rppi-box", "rppi-survey"],
+                    default="boxes",
                     help="boxes: 64 clustered periodic boxes (default); survey: 4M data + 20M randoms, "
                          "four terms; survey-small: 500k data + 2.5M randoms; rppi-box: rppi binning of "
-                         "the 1M-point box")
+                         "the 1M-point box; rppi-survey: rppi binning of the survey-small catalogues")
     ap.add_argument("--slices", type=int, default=16, help="survey: primaries split into this many launches")
     ap.add_argument("--boxes", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true", help="skip the 64-call plain leg")
@@ -432,6 +539,9 @@ def main():
         sys.exit(2)
     if args.workload == "rppi-box":
         run_rppi(args)
+        return
+    if args.workload == "rppi-survey":
+        run_rppi_survey(args)
         return
     if args.workload != "boxes":
         run_survey(args)

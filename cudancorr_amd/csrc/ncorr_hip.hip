@@ -2624,19 +2624,22 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             for (int b = 0; b < rs_nbk; ++b)
                 caps[b] = S == 1 ? n[b] + 64 : (unsigned long long)((double)n[b] * (double)S * 1.3) + 4096;
         }
-        // at most 60% of the free device memory for the regions
+        // at most 60% of the free device memory (counting the context's own
+        // region buffers) for the regions, slack included
         size_t freeb = 0, totalb = 0;
         NC_HIP(ctx, hipMemGetInfo(&freeb, &totalb));
         const double per = rs_anyw ? 12.0 : 4.0;
         double need = 0.0;
         for (unsigned long long c : caps) need += (double)c;
         const double room = 0.6 * ((double)freeb + (double)ctx->rs_keys.bytes + (double)ctx->rs_vals.bytes) / per;
-        const double scale = (need > room ? room / need : 1.0) * ctx->rs_scale;
+        const double want = std::min(need, 0.8 * room);  // the rest of room for the slack
+        const double scale = (need > 0.0 ? want / need : 1.0) * ctx->rs_scale;
         // a wave reserves blk slots at a time and may leave up to blk of
         // them unused per bucket: each region holds that slack too
-        const double per_wave = need / std::max(1, rs_nbk) / (4.0 * (double)nwaves);
+        const double per_wave = want / std::max(1, rs_nbk) / (4.0 * (double)nwaves);
         int blk = 64;
         while (blk < 4096 && 2.0 * blk <= per_wave) blk *= 2;
+        while (blk > 64 && (double)rs_nbk * (double)nwaves * blk > 0.2 * room) blk /= 2;
         A.rs_blk = blk;
         const unsigned long long slack = (unsigned long long)nwaves * (unsigned long long)blk;
         std::vector<unsigned long long> base(rs_nbk + 1, 0);
@@ -2644,8 +2647,16 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             base[b + 1] = base[b] + (((unsigned long long)((double)caps[b] * scale) + slack + 63) & ~63ull);
         NC_HIP(ctx, hipMemcpyAsync(ctx->rs_base.p, base.data(), sizeof(unsigned long long) * base.size(),
                                    hipMemcpyHostToDevice, ctx->stream));
-        NC_HIP(ctx, dgrow(ctx->rs_keys, sizeof(unsigned) * (size_t)base[rs_nbk]));
-        if (rs_anyw) NC_HIP(ctx, dgrow(ctx->rs_vals, sizeof(double) * (size_t)base[rs_nbk]));
+        const size_t kbytes = sizeof(unsigned) * (size_t)base[rs_nbk];
+        const size_t vbytes = rs_anyw ? sizeof(double) * (size_t)base[rs_nbk] : 0;
+        if (kbytes > ctx->rs_keys.bytes || vbytes > ctx->rs_vals.bytes || vbytes == 0) {
+            // release both before growing either: the room above counted them
+            NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (kbytes > ctx->rs_keys.bytes) dfree(ctx->rs_keys);
+            if (vbytes > ctx->rs_vals.bytes || vbytes == 0) dfree(ctx->rs_vals);
+        }
+        NC_HIP(ctx, dgrow(ctx->rs_keys, kbytes));
+        if (rs_anyw) NC_HIP(ctx, dgrow(ctx->rs_vals, vbytes));
         A.rs_keys = (unsigned *)ctx->rs_keys.p;
         A.rs_vals = rs_anyw ? (double *)ctx->rs_vals.p : nullptr;
         // binning work items: slices in proportion to the region sizes (about
@@ -3048,8 +3059,10 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     // (more than half the CU's LDS); elsewhere two 8-wave ones do better
     bool big_lds = false;
     {
-        // a first run at the current shape sizes the sample to about 80 ms,
-        // enough to separate shapes a few percent apart
+        // a first run (8 waves per workgroup, the current chunks) sizes the
+        // sample to about 80 ms, enough to separate shapes a few percent apart
+        ctx->waves_per_group = 8;
+        ctx->wgs_per_cu = 0;
         rc = run_count(ctx, params, t, path, hist.data(), nullptr, false);
         big_lds = rc == NC_OK && (size_t)ctx->timing.lds_bytes > ctx->lds_max;
         const double ms = ctx->timing.ms_count;
