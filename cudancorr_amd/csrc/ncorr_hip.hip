@@ -335,6 +335,27 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                           const double *sq, const uint16_t *lut) {
     constexpr bool MIXED = LM == kListMixed;
     int n = 0;
+    // rppi survey: every triplet's line of sight L = 3a + d2 + d3 (a = p1 - ori,
+    // |d2|, |d3| < smax) is within the angle th = asin(2 smax / 3|a|) of a, so
+    // a side d of the list has, in any triplet, pi >= |ah.d| - th|d| and
+    // rp^2 >= |d|^2 - (|ah.d| + th|d|)^2.  A side failing pi < piMax or
+    // rp^2 < rpMax^2 by these bounds (with 1e-9 relative margins, far above
+    // the FP64 rounding of either evaluation) is rejected in every triplet
+    // (device.cu:470-481), so it is left out of the list.
+    double ahx = 0.0, ahy = 0.0, ahz = 0.0, th = 0.0;
+    bool prune = false;
+    if constexpr (LM == kListRppiSurvey) {
+        const double ax = px - A.ori_x, na = sqrt(dot3(ax, py, pz, ax, py, pz));
+        const double x = 2.0 * sqrt(A.smax2) / (3.0 * na) * (1.0 + 1e-9);
+        prune = na > 0.0 && x < 0.5;
+        if (prune) {
+            ahx = ax / na;
+            ahy = py / na;
+            ahz = pz / na;
+            th = asin(x) * (1.0 + 1e-9) + 1e-15;
+        }
+    }
+    const double pim = sqrt(A.pimax2) * (1.0 + 1e-9), r2m = A.r2max * (1.0 + 1e-9);
     auto keep = [&](int j, bool v, double qx, double qy, double qz, double qw, int s) {
         if constexpr (LM == kListRppiBox || LM == kListRppiSurvey) {
             // rppi: box keeps pi^2 < piMax^2 and rpMin^2 < rp^2 < rpMax^2
@@ -350,6 +371,12 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                 r = sqrsep(px - qx, py - qy, pz - qz);
                 q = qx - A.ori_x;
                 pass = v & (r < A.smax2) & (r > 0.0);
+                if (prune) {
+                    const double u = fabs(dot3(ahx, ahy, ahz, qx - px, qy - py, qz - pz));
+                    const double dl = th * (double)(__builtin_sqrtf((float)r) * 1.000001f);  // >= th |d|
+                    const double hi = u + dl;
+                    pass = pass & !(u - dl > pim) & !(r - hi * hi > r2m + 1e-9 * r);
+                }
             }
             const unsigned long long m = __ballot(pass);
             const int pos = n + lane_rank(m);
@@ -1383,6 +1410,7 @@ struct RppiPol {
     uint16_t *srk;
     StreamLds S;
     int cnt, tbase, nbt, shift;
+    unsigned long long emitted;  // keys flushed by this primary
 
     __device__ RppiPol(const KArgs &A_, const LdsTabs &t_, const char *l2, const char *l3, char *wlds, int lane_,
                        unsigned long long *out_, double px_, double py_, double pz_, double w1_, int4 term)
@@ -1396,6 +1424,7 @@ struct RppiPol {
         nb3 = A.nb * A.nb * A.nb;
         np2 = A.npi * A.npi;
         cnt = 0;
+        emitted = 0ull;
         tbase = term.x;
         nbt = term.y;
         shift = term.z;
@@ -1468,6 +1497,7 @@ struct RppiPol {
     __device__ void flush() {
         if constexpr (STREAM) {
             const int n = cnt;
+            emitted += (unsigned long long)n;
             wave_sync();
             for (int b = lane; b < nbt; b += kWave) S.wcnt[b] = 0u;
             wave_sync();
@@ -1624,13 +1654,16 @@ struct RppiPol {
     }
 };
 
+// Returns the keys the primary emitted (key stream).
 template <bool SURVEY, bool SYM, bool WEIGHTED, bool STREAM>
-__device__ __forceinline__ void rppi_primary(const KArgs &A, const LdsTabs &t, const char *l2, const char *lb,
-                                             char *wlds, int lane, unsigned long long *out, double px, double py,
-                                             double pz, double pw, int4 term, int n2u, int n3u) {
+__device__ __forceinline__ unsigned long long rppi_primary(const KArgs &A, const LdsTabs &t, const char *l2,
+                                                           const char *lb, char *wlds, int lane,
+                                                           unsigned long long *out, double px, double py, double pz,
+                                                           double pw, int4 term, int n2u, int n3u) {
     RppiPol<SURVEY, SYM, WEIGHTED, STREAM> P(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term);
     traverse(P, n2u, n3u);
     if (STREAM && P.cnt > 0) P.flush();
+    return P.emitted;
 }
 
 // Persistent rppi kernel (countTrianglesRppiBox / countTrianglesRppiSurvey,
@@ -1641,6 +1674,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
+    __shared__ unsigned long long s_ckeys;  // dry run: keys of the current work item
     LdsTabs t;
     t.hist = nullptr;
     double *sq = reinterpret_cast<double *>(lds + A.off_sq);
@@ -1651,6 +1685,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     t.lut32 = nullptr;
     t.t3 = nullptr;
     t.rk = nullptr;
+    int prev_ci = -1;  // thread 0: the work item whose key count is pending
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
     for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
@@ -1671,8 +1706,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
     __syncthreads();
     for (;;) {
         if (tid == 0) {
+            if (STREAM && A.rs_dry && prev_ci >= 0) A.rs_chunk_keys[prev_ci] = s_ckeys;
             s_chunk = (int)atomicAdd(A.counter, 1u);
             s_next = 0;
+            s_ckeys = 0ull;
+            prev_ci = s_chunk;
         }
         __syncthreads();
         const int ci = uniform(s_chunk);
@@ -1710,21 +1748,23 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
             if (n2u == 0 || n3u == 0) continue;
             wave_sync();
             const char *lb = T.sym ? l2 : l3;
+            unsigned long long nk;
             if (sym) {
                 if (T.weighted)
-                    rppi_primary<SURVEY, true, true, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term, n2u,
-                                                             n3u);
+                    nk = rppi_primary<SURVEY, true, true, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term,
+                                                                  n2u, n3u);
                 else
-                    rppi_primary<SURVEY, true, false, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term,
-                                                              n2u, n3u);
+                    nk = rppi_primary<SURVEY, true, false, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw,
+                                                                   term, n2u, n3u);
             } else {
                 if (T.weighted)
-                    rppi_primary<SURVEY, false, true, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term,
-                                                              n2u, n3u);
+                    nk = rppi_primary<SURVEY, false, true, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw,
+                                                                   term, n2u, n3u);
                 else
-                    rppi_primary<SURVEY, false, false, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw, term,
-                                                               n2u, n3u);
+                    nk = rppi_primary<SURVEY, false, false, STREAM>(A, t, l2, lb, wlds, lane, out, px, py, pz, pw,
+                                                                    term, n2u, n3u);
             }
+            if (STREAM && A.rs_dry) atomicAdd(&s_ckeys, lane == 0 ? nk : 0ull);  // every lane: no lane-0 branch
             wave_sync();
         }
         __syncthreads();
@@ -1892,15 +1932,18 @@ struct nc_context {
     // dense counts are merged once this many pairs were added (< 2^31 keeps
     // u32 counts from wrapping; NC_DENSE_FLUSH_PAIRS lowers it for tests)
     unsigned long long dense_flush = 1ull << 31;
-    int sample_stride = 1;  // autotuner: queue only every n-th work item
+    int sample_stride = 1;    // autotuner: queue only every n-th work item
+    bool autotuning = false;  // in nc_autotune's timed runs
     std::string dev_name;
     // rppi key stream: buffers, and the per-bucket capacities last sized for
     // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
     bool rppi_stream = true;
-    double rs_scale = 1.0;  // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
-    DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_keys, rs_vals;
+    double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
+    double rs_round_keys = 0.0;  // NC_RPPI_ROUND_KEYS: keys per round (tests of several rounds)
+    DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_chunk_keys, rs_keys, rs_vals;
     std::vector<double> rs_key;
     std::vector<unsigned long long> rs_caps;
+    std::vector<double> rs_cest;  // estimated keys per work item
 };
 
 namespace {
@@ -2442,6 +2485,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         hm = choose(budget);
     }
     if (stream && !fits(hm, budget)) {  // no room for the key stream: direct atomics
+        // (the autotuner skips such shapes rather than time the other path)
+        if (ctx->autotuning) return fail(ctx, NC_ERR_LIMIT, "no LDS room for the rppi key stream");
         stream = false;
         rank_bytes = stage_bytes = 0;
         budget = fits(hm, ctx->lds_max) ? ctx->lds_max : ctx->lds_hw;
@@ -2570,11 +2615,20 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.lds_bytes = (int)lds_bytes;
     NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
 
-    // --- rppi key stream: tables, bucket regions, binning work items.  The
-    // regions are sized from the keys per bucket of a dry run over every
-    // S-th work item (kept for the same catalogues and params); keys beyond
-    // a region's capacity still count, through direct atomics.
-    std::vector<unsigned long long> caps, rcaps(std::max(rs_nbk, 1));  // wanted, and allocated, keys per bucket
+    // --- rppi key stream: tables, rounds, bucket regions, binning work items.
+    // A dry run over every S-th work item counts the keys per bucket and per
+    // work item (kept for the same catalogues and params).  The work queue is
+    // cut into rounds of consecutive work items whose keys fit 60% of the
+    // free device memory; a round's regions get each bucket's keys in
+    // proportion to its term's share of the round, plus the runs' slack.
+    // Keys beyond a region's capacity still count, through direct atomics.
+    struct Round {
+        int first, n;          // work items [first, first + n)
+        int item0, nitems;     // binning work items
+    };
+    std::vector<Round> rounds;
+    std::vector<unsigned long long> caps, rcaps;  // wanted keys per bucket; allocated per (round, bucket)
+    std::vector<double> cest;                     // estimated keys per work item
     std::vector<double> rs_ckey;
     if (stream && A.nchunks > 0) {
         A.off_rank = (int)(hist_bytes + lds_tab_bytes - rank_bytes);
@@ -2590,65 +2644,138 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         std::memcpy(tb.data() + tb_inv, rs_inv.data(), sizeof(int) * rs_inv.size());
         NC_HIP(ctx, dgrow(ctx->rs_tab, tb_bytes));
         NC_HIP(ctx, hipMemcpyAsync(ctx->rs_tab.p, tb.data(), tb_bytes, hipMemcpyHostToDevice, ctx->stream));
-        NC_HIP(ctx, dgrow(ctx->rs_cursor, sizeof(unsigned long long) * (size_t)rs_nbk));
-        NC_HIP(ctx, dgrow(ctx->rs_base, sizeof(unsigned long long) * ((size_t)rs_nbk + 1)));
         A.rs_term = (const int4 *)ctx->rs_tab.p;
         A.rs_rank = (const uint16_t *)((char *)ctx->rs_tab.p + tb_rank);
         A.rs_inv = (const int *)((char *)ctx->rs_tab.p + tb_inv);
-        A.rs_cursor = (unsigned long long *)ctx->rs_cursor.p;
-        A.rs_base = (const unsigned long long *)ctx->rs_base.p;
+        const size_t nch = chunks.size();
         rs_ckey = pkey;
         for (uint64_t u : uids) rs_ckey.push_back((double)u);
-        if (ctx->sample_stride == 1 && ctx->rs_key == rs_ckey && ctx->rs_caps.size() == (size_t)rs_nbk) {
+        rs_ckey.push_back((double)ctx->chunk);  // the work items the estimates refer to
+        if (ctx->sample_stride == 1 && ctx->rs_key == rs_ckey && ctx->rs_caps.size() == (size_t)rs_nbk &&
+            ctx->rs_cest.size() == nch) {
             caps = ctx->rs_caps;
+            cest = ctx->rs_cest;
         } else {
-            const size_t S = std::max<size_t>(1, std::min<size_t>(32, chunks.size() / 256));
+            const size_t S = std::max<size_t>(1, std::min<size_t>(32, nch / 256));
             std::vector<Chunk> sample;
-            for (size_t i = 0; i < chunks.size(); i += S) sample.push_back(chunks[i]);
+            for (size_t i = 0; i < nch; i += S) sample.push_back(chunks[i]);
             NC_HIP(ctx, dgrow(ctx->rs_sample, sizeof(Chunk) * sample.size()));
+            NC_HIP(ctx, dgrow(ctx->rs_cursor, sizeof(unsigned long long) * (size_t)rs_nbk));
+            NC_HIP(ctx, dgrow(ctx->rs_chunk_keys, sizeof(unsigned long long) * sample.size()));
             NC_HIP(ctx, hipMemcpyAsync(ctx->rs_sample.p, sample.data(), sizeof(Chunk) * sample.size(),
                                        hipMemcpyHostToDevice, ctx->stream));
             KArgs As = A;
             As.chunks = (const Chunk *)ctx->rs_sample.p;
             As.nchunks = (int)sample.size();
             As.rs_dry = 1;
+            As.rs_cursor = (unsigned long long *)ctx->rs_cursor.p;
+            As.rs_chunk_keys = (unsigned long long *)ctx->rs_chunk_keys.p;
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
-            hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, A.rs_cursor, 0ull, (long long)rs_nbk);
+            hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, As.rs_cursor, 0ull, (long long)rs_nbk);
+            hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, As.rs_chunk_keys, 0ull,
+                               (long long)sample.size());
             void *args[] = {&As};
             NC_HIP(ctx, hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream));
-            std::vector<unsigned long long> n(rs_nbk);
-            NC_HIP(ctx, hipMemcpyAsync(n.data(), A.rs_cursor, sizeof(unsigned long long) * n.size(),
+            std::vector<unsigned long long> n(rs_nbk), ck(sample.size());
+            NC_HIP(ctx, hipMemcpyAsync(n.data(), As.rs_cursor, sizeof(unsigned long long) * n.size(),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            NC_HIP(ctx, hipMemcpyAsync(ck.data(), As.rs_chunk_keys, sizeof(unsigned long long) * ck.size(),
                                        hipMemcpyDeviceToHost, ctx->stream));
             NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
             caps.resize(rs_nbk);
             for (int b = 0; b < rs_nbk; ++b)
                 caps[b] = S == 1 ? n[b] + 64 : (unsigned long long)((double)n[b] * (double)S * 1.3) + 4096;
+            cest.resize(nch);
+            for (size_t i = 0; i < nch; ++i) cest[i] = (double)ck[i / S] * (S == 1 ? 1.0 : 1.3);
         }
-        // at most 60% of the free device memory (counting the context's own
-        // region buffers) for the regions, slack included
+        // memory: 60% of the free device memory, counting the context's own
+        // region buffers; the runs' slack (every bucket, every wave) first
         size_t freeb = 0, totalb = 0;
         NC_HIP(ctx, hipMemGetInfo(&freeb, &totalb));
         const double per = rs_anyw ? 12.0 : 4.0;
+        const double room = 0.6 * ((double)freeb + (double)ctx->rs_keys.bytes + (double)ctx->rs_vals.bytes) / per;
         double need = 0.0;
         for (unsigned long long c : caps) need += (double)c;
-        const double room = 0.6 * ((double)freeb + (double)ctx->rs_keys.bytes + (double)ctx->rs_vals.bytes) / per;
-        const double want = std::min(need, 0.8 * room);  // the rest of room for the slack
-        const double scale = (need > 0.0 ? want / need : 1.0) * ctx->rs_scale;
-        // a wave reserves blk slots at a time and may leave up to blk of
-        // them unused per bucket: each region holds that slack too
-        const double per_wave = want / std::max(1, rs_nbk) / (4.0 * (double)nwaves);
-        int blk = 64;
-        while (blk < 4096 && 2.0 * blk <= per_wave) blk *= 2;
-        while (blk > 64 && (double)rs_nbk * (double)nwaves * blk > 0.2 * room) blk /= 2;
+        int blk = 64;  // a wave reserves blk slots of a region at a time
+        while (blk < 4096 && 2.0 * blk <= need / std::max(1, rs_nbk) / (4.0 * (double)nwaves)) blk *= 2;
+        while (blk > 64 && (double)rs_nbk * (double)nwaves * blk > 0.25 * room) blk /= 2;
         A.rs_blk = blk;
         const unsigned long long slack = (unsigned long long)nwaves * (unsigned long long)blk;
-        std::vector<unsigned long long> base(rs_nbk + 1, 0);
-        for (int b = 0; b < rs_nbk; ++b)
-            base[b + 1] = base[b] + (((unsigned long long)((double)caps[b] * scale) + slack + 63) & ~63ull);
+        double K = std::max(0.0, room - (double)rs_nbk * (double)slack);  // keys per round
+        double total_est = 0.0;
+        for (double e : cest) total_est += e;
+        K = std::max({K, 1e7, total_est / 256.0});  // at most ~256 rounds (past the room: allocation fails)
+        if (ctx->rs_round_keys > 0.0) K = std::max(ctx->rs_round_keys, total_est / 256.0);
+        // per term: estimated keys, and work items (for terms the sample saw no key of)
+        std::vector<double> test(nterms, 0.0), tcnt(nterms, 0.0);
+        for (size_t i = 0; i < nch; ++i) {
+            test[chunks[i].term] += cest[i];
+            tcnt[chunks[i].term] += 1.0;
+        }
+        for (size_t i = 0; i < nch;) {
+            double acc = 0.0;
+            size_t j = i;
+            while (j < nch && (j == i || acc + cest[j] <= K)) acc += cest[j++];
+            rounds.push_back({(int)i, (int)(j - i), 0, 0});
+            i = j;
+        }
+        const int R = (int)rounds.size();
+        rcaps.assign((size_t)R * rs_nbk, 0ull);
+        std::vector<unsigned long long> base((size_t)R * (rs_nbk + 1), 0ull);
+        unsigned long long kmax = 0;
+        std::vector<RBItem> items;
+        for (int r = 0; r < R; ++r) {
+            std::vector<double> fe(nterms, 0.0), fc(nterms, 0.0);
+            for (int i = rounds[r].first; i < rounds[r].first + rounds[r].n; ++i) {
+                fe[chunks[i].term] += cest[i];
+                fc[chunks[i].term] += 1.0;
+            }
+            double want = 0.0;
+            std::vector<double> w(rs_nbk, 0.0);
+            std::vector<char> live(rs_nbk, 0);  // buckets of the terms with work in this round
+            for (int t = 0; t < nterms; ++t) {
+                if (fc[t] == 0.0) continue;
+                const double f = test[t] > 0.0 ? fe[t] / test[t] : fc[t] / tcnt[t];
+                for (int bb = 0; bb < rs_term[t].y; ++bb) {
+                    w[rs_term[t].x + bb] = (double)caps[rs_term[t].x + bb] * f * ctx->rs_scale;
+                    live[rs_term[t].x + bb] = 1;
+                    want += w[rs_term[t].x + bb];
+                }
+            }
+            const double sc = want > K ? K / want : 1.0;  // one work item past the room: overflow
+            unsigned long long *bs = base.data() + (size_t)r * (rs_nbk + 1);
+            for (int b = 0; b < rs_nbk; ++b) {
+                const unsigned long long c =
+                    live[b] ? (((unsigned long long)(w[b] * sc) + slack + 63) & ~63ull) : 0ull;
+                rcaps[(size_t)r * rs_nbk + b] = c;
+                bs[b + 1] = bs[b] + c;
+            }
+            kmax = std::max(kmax, bs[rs_nbk]);
+            // binning work items: slices in proportion to the region sizes
+            // (about four per CU in all), each under 2^30 keys
+            rounds[r].item0 = (int)items.size();
+            const double total = std::max(1.0, (double)bs[rs_nbk]);
+            for (int t = 0; t < nterms; ++t) {
+                if (fc[t] == 0.0) continue;
+                const int4 tr = rs_term[t];
+                for (int bb = 0; bb < tr.y; ++bb) {
+                    const int b = tr.x + bb;
+                    const unsigned long long cb = bs[b + 1] - bs[b];
+                    if (cb == 0) continue;
+                    long long sl = std::llround(4.0 * ctx->ncu * (double)cb / total);
+                    sl = std::max<long long>({1LL, sl, (long long)((cb + (1ull << 30) - 1) >> 30)});
+                    for (long long j = 0; j < sl; ++j)
+                        items.push_back({b, t, tr.z, weighted[t] ? 1 : 0, bb << tr.z, (int)j, (int)sl, 0});
+                }
+            }
+            rounds[r].nitems = (int)items.size() - rounds[r].item0;
+        }
+        NC_HIP(ctx, dgrow(ctx->rs_base, sizeof(unsigned long long) * base.size()));
+        NC_HIP(ctx, dgrow(ctx->rs_cursor, sizeof(unsigned long long) * (size_t)R * rs_nbk));
         NC_HIP(ctx, hipMemcpyAsync(ctx->rs_base.p, base.data(), sizeof(unsigned long long) * base.size(),
                                    hipMemcpyHostToDevice, ctx->stream));
-        const size_t kbytes = sizeof(unsigned) * (size_t)base[rs_nbk];
-        const size_t vbytes = rs_anyw ? sizeof(double) * (size_t)base[rs_nbk] : 0;
+        const size_t kbytes = sizeof(unsigned) * (size_t)kmax;
+        const size_t vbytes = rs_anyw ? sizeof(double) * (size_t)kmax : 0;
         if (kbytes > ctx->rs_keys.bytes || vbytes > ctx->rs_vals.bytes || vbytes == 0) {
             // release both before growing either: the room above counted them
             NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2659,45 +2786,49 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         if (rs_anyw) NC_HIP(ctx, dgrow(ctx->rs_vals, vbytes));
         A.rs_keys = (unsigned *)ctx->rs_keys.p;
         A.rs_vals = rs_anyw ? (double *)ctx->rs_vals.p : nullptr;
-        // binning work items: slices in proportion to the region sizes (about
-        // four per CU in all), each under 2^30 keys
-        std::vector<RBItem> items;
-        const double total = std::max(1.0, (double)base[rs_nbk]);
-        for (int i = 0; i < nterms; ++i) {
-            const int4 tr = rs_term[i];
-            for (int bb = 0; bb < tr.y; ++bb) {
-                const int b = tr.x + bb;
-                const unsigned long long cb = base[b + 1] - base[b];
-                long long sl = std::llround(4.0 * ctx->ncu * (double)cb / total);
-                sl = std::max<long long>({1LL, sl, (long long)((cb + (1ull << 30) - 1) >> 30)});
-                for (long long j = 0; j < sl; ++j)
-                    items.push_back({b, i, tr.z, weighted[i] ? 1 : 0, bb << tr.z, (int)j, (int)sl, 0});
-            }
-        }
-        NC_HIP(ctx, dgrow(ctx->rs_items, sizeof(RBItem) * items.size()));
-        NC_HIP(ctx, hipMemcpyAsync(ctx->rs_items.p, items.data(), sizeof(RBItem) * items.size(),
-                                   hipMemcpyHostToDevice, ctx->stream));
-        A.rs_items = (const RBItem *)ctx->rs_items.p;
-        A.rs_nitems = (int)items.size();
-        for (int b = 0; b < rs_nbk; ++b) rcaps[b] = base[b + 1] - base[b];
+        NC_HIP(ctx, dgrow(ctx->rs_items, sizeof(RBItem) * std::max<size_t>(1, items.size())));
+        if (!items.empty())
+            NC_HIP(ctx, hipMemcpyAsync(ctx->rs_items.p, items.data(), sizeof(RBItem) * items.size(),
+                                       hipMemcpyHostToDevice, ctx->stream));
     }
+    // the launch arguments of round r
+    auto round_args = [&](int r) {
+        KArgs Ar = A;
+        const Round &rd = rounds[r];
+        Ar.chunks = (const Chunk *)ctx->chunks.p + rd.first;
+        Ar.nchunks = rd.n;
+        Ar.rs_base = (const unsigned long long *)ctx->rs_base.p + (size_t)r * (rs_nbk + 1);
+        Ar.rs_cursor = (unsigned long long *)ctx->rs_cursor.p + (size_t)r * rs_nbk;
+        Ar.rs_items = (const RBItem *)ctx->rs_items.p + rd.item0;
+        Ar.rs_nitems = rd.nitems;
+        return Ar;
+    };
     const size_t bin_lds = (size_t)1 << rs_maxshift;
 
     auto enqueue_count = [&]() -> int {
         hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->out.p, 0ull,
                            (long long)nterms * G.out_words);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
-        if (A.nchunks > 0) {
-            if (stream)
-                hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, A.rs_cursor, 0ull,
-                                   (long long)rs_nbk);
+        if (A.nchunks > 0 && !stream) {
             void *args[] = {&A};
             if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
                 return NC_ERR_HIP;
-            if (stream && A.rs_nitems > 0 &&
-                hipLaunchKernel((const void *)k_rppi_bin, dim3(A.rs_nitems), dim3(1024), args, bin_lds,
-                                ctx->stream) != hipSuccess)
-                return NC_ERR_HIP;
+        }
+        if (A.nchunks > 0 && stream) {
+            hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->rs_cursor.p,
+                               0ull, (long long)rounds.size() * rs_nbk);
+            for (int r = 0; r < (int)rounds.size(); ++r) {
+                KArgs Ar = round_args(r);
+                void *args[] = {&Ar};
+                if (r > 0)
+                    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u,
+                                       4LL);
+                if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
+                    return NC_ERR_HIP;
+                if (Ar.rs_nitems > 0 && hipLaunchKernel((const void *)k_rppi_bin, dim3(Ar.rs_nitems), dim3(1024),
+                                                        args, bin_lds, ctx->stream) != hipSuccess)
+                    return NC_ERR_HIP;
+            }
         }
         return hipGetLastError() == hipSuccess ? NC_OK : NC_ERR_HIP;
     };
@@ -2714,24 +2845,33 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.ms_build = ms_b;
     ctx->timing.ms_count = ms_c;
     ctx->timing.ms_total = ms_b + ms_c;
-    ctx->timing.n_kernel = A.nchunks > 0 ? (stream ? 2 : 1) : 0;
+    const int nrounds = stream && A.nchunks > 0 ? (int)rounds.size() : 0;
+    ctx->timing.n_kernel = A.nchunks > 0 ? (stream ? 2 * nrounds : 1) : 0;
     ctx->timing.hist_layout = stream ? kHistStream : hm;
     ctx->timing.lds_bytes = (int32_t)lds_bytes;
     // key stream: keep the region sizes; after an overflow (counted, but
-    // through direct atomics) size from the keys this run emitted, and
-    // capture no graph so that the next call uses them
+    // through direct atomics) scale the buckets that overflowed by what this
+    // run reserved, and capture no graph so that the next call uses that
     bool rs_overflow = false;
-    if (stream && A.nchunks > 0) {
-        std::vector<unsigned long long> n(rs_nbk);
-        NC_HIP(ctx, hipMemcpy(n.data(), A.rs_cursor, sizeof(unsigned long long) * n.size(), hipMemcpyDeviceToHost));
-        for (int b = 0; b < rs_nbk; ++b)  // n: slots reserved, at least the keys
-            if (n[b] > rcaps[b]) {
-                rs_overflow = true;
-                caps[b] = std::max(caps[b], n[b] + n[b] / 8 + 64);
+    if (nrounds > 0) {
+        std::vector<unsigned long long> n((size_t)nrounds * rs_nbk);
+        NC_HIP(ctx, hipMemcpy(n.data(), ctx->rs_cursor.p, sizeof(unsigned long long) * n.size(),
+                              hipMemcpyDeviceToHost));
+        for (int b = 0; b < rs_nbk; ++b) {
+            double grow = 1.0;
+            for (int r = 0; r < nrounds; ++r) {  // n: slots reserved, at least the keys
+                const unsigned long long got = n[(size_t)r * rs_nbk + b], had = rcaps[(size_t)r * rs_nbk + b];
+                if (got > had) grow = std::max(grow, (double)got / (double)std::max(had, 1ull));
             }
+            if (grow > 1.0) {
+                rs_overflow = true;
+                caps[b] = (unsigned long long)((double)caps[b] * grow * 1.125) + 64;
+            }
+        }
         if (ctx->sample_stride == 1) {  // not from an autotuning sample
             ctx->rs_key = rs_ckey;
             ctx->rs_caps = caps;
+            ctx->rs_cest = cest;
         }
     }
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
@@ -2751,7 +2891,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.nb = G.nb;
         p.nbc = G.out_words;
         p.rppi = G.rppi;
-        p.nkernel = A.nchunks > 0 ? (stream ? 2 : 1) : 0;
+        p.nkernel = A.nchunks > 0 ? (stream ? 2 * nrounds : 1) : 0;
         p.hist_layout = stream ? kHistStream : hm;
         p.lds_bytes = (int)lds_bytes;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
@@ -2801,6 +2941,10 @@ int nc_create(nc_context **out, int device) {
     ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
     ctx->lds_hw = prop.sharedMemPerBlock - 256;
     if (const char *e = std::getenv("NC_RPPI_STREAM")) ctx->rppi_stream = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("NC_RPPI_ROUND_KEYS")) {
+        const double v = std::strtod(e, nullptr);
+        if (v >= 1.0) ctx->rs_round_keys = v;
+    }
     if (const char *e = std::getenv("NC_RPPI_REGION_SCALE")) {
         const double v = std::strtod(e, nullptr);
         if (v > 0.0 && v <= 1.0) ctx->rs_scale = v;
@@ -2825,7 +2969,7 @@ int nc_destroy(nc_context *ctx) {
     plan_drop(ctx);
     for (DBuf *b : {&ctx->tables, &ctx->terms, &ctx->chunks, &ctx->out, &ctx->counter, &ctx->errw, &ctx->scratch,
                     &ctx->bound, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
-                    &ctx->rs_keys, &ctx->rs_vals})
+                    &ctx->rs_chunk_keys, &ctx->rs_keys, &ctx->rs_vals})
         dfree(*b);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -3053,6 +3197,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     // sample: about 64 work items per workgroup slot of the chip at the largest chunk
     const long long items = std::max<long long>(1, nprim / 16);
     ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
+    ctx->autotuning = true;
     nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
     int rc = NC_OK;
     // 16-wave workgroups only where one workgroup per CU is forced anyway
@@ -3099,6 +3244,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     if (b.ms < 1e30f) rc = NC_OK;  // at least one shape ran
 done:
     ctx->sample_stride = 1;
+    ctx->autotuning = false;
     plan_drop(ctx);
     if (rc) {
         ctx->waves_per_group = saved[0];

@@ -121,6 +121,7 @@ struct KArgs {
     unsigned *rs_keys;                     // c | (count 2 ? 1 << 31 : 0); kRsPad in unused reserved slots
     double *rs_vals;                       // weighted terms: the summed weight products
     const RBItem *rs_items;
+    unsigned long long *rs_chunk_keys;     // dry: keys of each work item (region sizing per round)
 };
 
 }  // namespace nck

@@ -222,3 +222,43 @@ def test_key_stream_batched_mixed_weights(ctx):
                 assert np.array_equal(cnt[i], c), f"replay {rep} term {sp}"
     for c in cats.values():
         c.close()
+
+
+@pytest.mark.parametrize("survey", [False, True])
+def test_key_stream_rounds(monkeypatch, survey):
+    """The work queue cut into many rounds (NC_RPPI_ROUND_KEYS), each with its
+    own regions: the same results, eagerly and over graph replays."""
+    _log(f"rppi rounds survey={survey}")
+    monkeypatch.setenv("NC_RPPI_ROUND_KEYS", "5000")
+    if survey:
+        d, r = survey_cap(2500, 2000, seed=19)
+        for c in (d, r):
+            c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.1
+        box, ori = prep_cat_survey(d, r)
+        edges, npi = np.array([0.0, 2.0, 4.0, 6.0, 9.0]), 12
+        specs = {"D": d, "R": r}
+        terms = ["DDD", "DRR", "RRR"]
+    else:
+        p, edges, npi, box = _box_case(seed=21)
+        q = uniform_box(3000, 96.0, seed=31)
+        q[:, 2] = np.mod(q[:, 2], 60.0)
+        ori = (0.0, 0.0, 0.0)
+        specs = {"D": p, "R": q}
+        terms = ["DDD", "DDR", "RRR"]
+    want = [oracle.count_rppi(*(specs[ch] for ch in t), edges, npi, box, survey=survey, ori=ori) for t in terms]
+    with Context(0) as c2:
+        cats = {k: c2.catalog(v) for k, v in specs.items()}
+        tl = [tuple(cats[ch] for ch in t) for t in terms]
+        _, got = c2.count(*tl[-1], edges, box, survey=survey, npi=npi, ori=ori)
+        assert c2.timing().n_kernel >= 6  # at least three rounds of two kernels
+        assert np.array_equal(got, want[-1][0])
+        for rep in range(3):
+            hist, cnt = c2.count_batched(tl, edges, box, survey=survey, npi=npi, ori=ori)
+            for i, t in enumerate(terms):
+                c, ws, _ = want[i]
+                if "D" in t and survey:
+                    _wcheck(hist[i], ws)
+                else:
+                    assert np.array_equal(cnt[i], c), f"replay {rep} term {t}"
+        for c in cats.values():
+            c.close()
