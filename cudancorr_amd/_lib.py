@@ -27,6 +27,7 @@ EXPORTS = (
     "nc_create", "nc_destroy", "nc_last_error", "nc_catalog_create", "nc_catalog_wrap_device",
     "nc_catalog_destroy", "nc_catalog_size", "nc_hist_size", "nc_compact_size", "nc_hist_len", "nc_count",
     "nc_count_host", "nc_count_batched", "nc_last_timing", "nc_set_tuning", "nc_get_tuning", "nc_autotune",
+    "nc_mixed_classify",
 )
 
 
@@ -40,7 +41,7 @@ class NcTiming(ctypes.Structure):
     _fields_ = [("ms_build", ctypes.c_float), ("ms_count", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("n_kernel", ctypes.c_int64),
                 ("hist_layout", ctypes.c_int32),
-                ("lds_bytes", ctypes.c_int32)]
+                ("lds_bytes", ctypes.c_int32), ("path", ctypes.c_int32)]
 
 
 class NcTuning(ctypes.Structure):
@@ -94,8 +95,26 @@ def lib():
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.nc_autotune.argtypes = [vp, ctypes.POINTER(NcParams), ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp),
                               ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(NcTuning)]
+    L.nc_mixed_classify.argtypes = [dp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), i64,
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     _LIB = L
     return L
+
+
+def mixed_classify(edges, d):
+    """The mixed path's FP32 bin lookup evaluated on the host (no GPU): bin of
+    each FP32 squared separation in d, -1 out of range, -2 in an edge's guard
+    band (FP64 recheck).  Returns (bins, table entries); NcError when the mixed
+    path is disabled for these edges."""
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    x = np.ascontiguousarray(d, dtype=np.float32)
+    out = np.empty(x.size, dtype=np.int32)
+    n = ctypes.c_int32(0)
+    rc = lib().nc_mixed_classify(_dp(e), e.size, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), x.size,
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(n))
+    if rc:
+        raise NcError(rc, "nc_mixed_classify")
+    return out, n.value
 
 
 def _dp(a):

@@ -854,7 +854,9 @@ struct MixedPol {
     double *lw;
     uint2 *q;
     int qn, C, row_first;
-    float jmax, g32, c1, c2t;
+    const uint2 *lb;  // lookup table shifted down by its first cell: read at cell indices >= lut32_c0 only
+    float lo32, hi32, g32, c1, c2t;
+    int sh32;
     unsigned nb;
     float4 rb;
     int kb, bidx;
@@ -874,7 +876,10 @@ struct MixedPol {
         q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
         qn = 0;
         row_first = 0;
-        jmax = (float)(A.lut32_m + 1);
+        lb = t.lut32 - A.lut32_c0;
+        lo32 = A.lut32_lo;
+        hi32 = A.lut32_hi;
+        sh32 = A.lut32_shift;
         g32 = A.g32;
         c1 = A.tri_c1;
         c2t = A.tri_c2;
@@ -934,7 +939,7 @@ struct MixedPol {
             dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
             ss[u] = ra[u].w + rb.w;
             d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
-            e[u] = t.lut32[(unsigned)__builtin_amdgcn_fmed3f(d[u], 0.0f, jmax)];
+            e[u] = lb[__float_as_uint(__builtin_amdgcn_fmed3f(d[u], lo32, hi32)) >> sh32];
         }
         bool acc[U], amb[U];
         int slot[U];
@@ -1100,7 +1105,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         // FP64 tables are only read by the rare exact rechecks: global memory
         t.sq = A.sq;
         t.lut = A.lut;
-        for (int i = tid; i <= A.lut32_m + 1; i += nthr) lut32[i] = A.lut32[i];
+        for (int i = tid; i < A.lut32_n; i += nthr) lut32[i] = A.lut32[i];
     } else {
         for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
         for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
@@ -1878,8 +1883,8 @@ struct Geometry {
     bool mixed_ok = false;
     std::vector<uint2> lut32;
     double sigma32 = 0.0;
-    float lut_scale32 = 0.f;
-    int lut32_m = 0;
+    float lut32_lo = 0.f, lut32_hi = 0.f;
+    int lut32_shift = 0, lut32_c0 = 0;
     float g32 = 0.f, tri_c1 = 0.f, tri_c2 = 0.f;
     std::vector<uint16_t> t3;  // (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
 };
@@ -1908,7 +1913,7 @@ struct BatchPlan {
     unsigned long long *h_out = nullptr;
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
-    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0;
+    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0;
     long long nbc = 0;  // output words per term
 };
 
@@ -1991,12 +1996,18 @@ int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
 // thresholds g32, c1, c2 and the edges are stored scaled (by s32, s32, s32^2,
 // s32).
 //
-// Lookup: j = min(d32', M+1) with d32' = s32*d32 over [0, 1.25*rmax^2*s32);
-// any d32 mapped to j lies in [(j-1)/s32, (j+2)/s32].  Entry j = {bin below
-// the one band that meets that interval (or below all of it), the band's edge
-// s32*E as float bits, +inf if no band meets it}.  Tables where an interval
-// meets two bands are rejected (s32 is multiplied by 4); bins narrower than
-// the bands disable the path.
+// Lookup.  For a positive float x, bits(x) >> sh is monotone in x: cell c
+// holds the floats [val(c << sh), val((c+1) << sh)), 2^(23-sh) log-spaced
+// cells per octave, fine near zero where squared edges crowd and coarse where
+// they are sparse.  The kernel clamps d32' = s32*d32 to [val(c0 << sh),
+// val(c1 << sh)] (cell c0 thereby takes every smaller value, negative ones
+// included, and cell c1 every larger one) and reads entry c - c0 = {bin below
+// the one band that meets the cell (or below all of it), the band's edge
+// s32*E as float bits, +inf if no band meets it}.  c1 is the cell of the
+// scaled rmax^2, c0 at most 2^w - 1 cells below it and at least the cell of
+// the smallest normal float.  The smallest table (w = 8..13) for which some
+// cell density (8..4096 per octave) leaves no cell meeting two bands is kept;
+// bins narrower than the bands, or no such table, disable the path.
 static void make_mixed_tables(Geometry &G) {
     G.mixed_ok = false;
     const double u = std::ldexp(1.0, -24);
@@ -2007,44 +2018,58 @@ static void make_mixed_tables(Geometry &G) {
     for (int k = 0; k < ne; ++k) E[k] = (float)G.sq[k];
     for (int k = 0; k + 1 < ne; ++k)
         if (!((double)E[k] + g < (double)E[k + 1] - g)) return;  // bins narrower than the guard band
-    const double r2max = G.sq[G.nb];
-    // s32 = 4^kk from the smallest power of 4 with M >= 512 on, x4 per retry
-    const int kk0 = (int)std::ceil(std::log2(512.0 / (1.25 * r2max)) / 2.0);
-    for (int kk = kk0; kk <= kk0 + 3; ++kk) {
-        const double s = std::ldexp(1.0, 2 * kk);
-        const int M = (int)std::ceil(1.25 * r2max * s);
-        if (M > 16384) break;
-        std::vector<uint2> lut(M + 2);
-        bool ok = true;
-        for (int j = 0; j <= M + 1 && ok; ++j) {
-            const double a = j == 0 ? -INFINITY : (double)(j - 1) / s * (1.0 - 1e-6);
-            const double b = j == M + 1 ? INFINITY : (double)(j + 2) / s * (1.0 + 1e-6);
-            int below = 0, meet = 0, which = -1;
-            for (int k = 0; k < ne; ++k) {
-                if ((double)E[k] + g < a)
-                    ++below;
-                else if ((double)E[k] - g <= b) {
-                    ++meet;
-                    which = k;
-                }
+    // s32 = 4^kk with s32 * rmax^2 in [1, 4)
+    const int kk = -(int)std::floor(std::log2(G.sq[G.nb]) / 2.0);
+    const double s = std::ldexp(1.0, 2 * kk);
+    std::vector<double> Es(ne);
+    for (int k = 0; k < ne; ++k) Es[k] = (double)E[k] * s;  // exact: s is a power of 4
+    const double gs = g * s;
+    auto fbits = [](float x) {
+        uint32_t b;
+        std::memcpy(&b, &x, 4);
+        return b;
+    };
+    auto fval = [](uint32_t b) {
+        float x;
+        std::memcpy(&x, &b, 4);
+        return (double)x;
+    };
+    const uint32_t top = fbits((float)Es[G.nb]);
+    for (int w = 8; w <= 13; ++w)
+        for (int p = 3; p <= 12; ++p) {
+            const int sh = 23 - p;
+            const int64_t c1 = top >> sh;
+            const int64_t c0 = std::max<int64_t>(c1 - ((int64_t)1 << w) + 1, (int64_t)(0x00800000u >> sh) + 1);
+            const int n = (int)(c1 - c0 + 1);
+            std::vector<uint2> lut(n);
+            bool ok = n >= 2;
+            // cells and bands both ascend: edges [0, below) lie under the
+            // cell, [below, over) meet it
+            int below = 0, over = 0;
+            for (int j = 0; j < n && ok; ++j) {
+                const uint32_t c = (uint32_t)(c0 + j);
+                const double a = j == 0 ? -INFINITY : fval(c << sh) * (1.0 - 1e-6);
+                const double b = j == n - 1 ? INFINITY : fval((c + 1) << sh) * (1.0 + 1e-6);
+                while (below < ne && Es[below] + gs < a) ++below;
+                over = std::max(over, below);
+                while (over < ne && Es[over] - gs <= b) ++over;
+                if (over - below > 1) ok = false;
+                const float edge = over > below ? (float)Es[below] : INFINITY;
+                lut[j] = make_uint2((uint32_t)(below - 1), fbits(edge));
             }
-            if (meet > 1 || (meet == 1 && which != below)) ok = false;
-            const float edge = meet ? (float)((double)E[which] * s) : INFINITY;  // exact: s is a power of 4
-            uint32_t eb;
-            std::memcpy(&eb, &edge, 4);
-            lut[j] = make_uint2((uint32_t)(below - 1), eb);
+            if (!ok) continue;
+            G.lut32 = std::move(lut);
+            G.lut32_shift = sh;
+            G.lut32_c0 = (int)c0;
+            G.lut32_lo = (float)fval((uint32_t)c0 << sh);
+            G.lut32_hi = (float)fval((uint32_t)c1 << sh);
+            G.sigma32 = std::ldexp(1.0, kk);
+            G.g32 = (float)gs;
+            G.tri_c1 = (float)(80.0 * u * R * R * s);
+            G.tri_c2 = (float)(64.0 * u * R * R * R * R * s * s);
+            G.mixed_ok = true;
+            return;
         }
-        if (!ok) continue;
-        G.lut32 = std::move(lut);
-        G.lut32_m = M;
-        G.sigma32 = std::ldexp(1.0, kk);
-        G.lut_scale32 = (float)s;
-        G.g32 = (float)(g * s);
-        G.tri_c1 = (float)(80.0 * u * R * R * s);
-        G.tri_c2 = (float)(64.0 * u * R * R * R * R * s * s);
-        G.mixed_ok = true;
-        return;
-    }
 }
 
 static void make_slot_table(Geometry &G) {
@@ -2259,6 +2284,7 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     ctx->timing.n_kernel = p.nkernel;
     ctx->timing.hist_layout = p.hist_layout;
     ctx->timing.lds_bytes = p.lds_bytes;
+    ctx->timing.path = p.path;
     if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (*p.h_err & kErrOverflow) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
         plan_drop(ctx);
@@ -2299,9 +2325,15 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     }
     plan_drop(ctx);
     const int nterms = (int)terms.size();
-    const bool mixed = path == NC_PATH_MIXED && G.mixed_ok;
     auto al16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
     const int W = ctx->waves_per_group;
+    // the mixed lookup table must leave LDS room for the smallest layout
+    // (ctab) and 64-entry chunks, else the count runs the plain path
+    const size_t mixed_min_lds = al16(sizeof(uint2) * G.lut32.size()) + al16((size_t)G.nbc * 8) +
+                                 al16(sizeof(int) * G.ctab.size()) +
+                                 (size_t)W * al16(68 * (sizeof(float4) + sizeof(int) + sizeof(double)) +
+                                                  kQueueCap * sizeof(uint2));
+    const bool mixed = path == NC_PATH_MIXED && G.mixed_ok && mixed_min_lds <= ctx->lds_hw;
     const int block = W * kWave;
     const int lmode = G.rppi ? (G.survey ? kListRppiSurvey : kListRppiBox) : mixed ? kListMixed : kListPlain;
     // --- rppi key stream: the reachable dense rp triples (sorted ones and the
@@ -2598,8 +2630,11 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.lut32 = (const uint2 *)((char *)ctx->tables.p + g_lut32);
     A.t3 = (const uint16_t *)((char *)ctx->tables.p + g_t3);
     A.sigma32 = G.sigma32;
-    A.lut_scale32 = G.lut_scale32;
-    A.lut32_m = G.lut32_m;
+    A.lut32_lo = G.lut32_lo;
+    A.lut32_hi = G.lut32_hi;
+    A.lut32_shift = G.lut32_shift;
+    A.lut32_c0 = G.lut32_c0;
+    A.lut32_n = (int)G.lut32.size();
     A.g32 = G.g32;
     A.tri_c1 = G.tri_c1;
     A.tri_c2 = G.tri_c2;
@@ -2849,6 +2884,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.n_kernel = A.nchunks > 0 ? (stream ? 2 * nrounds : 1) : 0;
     ctx->timing.hist_layout = stream ? kHistStream : hm;
     ctx->timing.lds_bytes = (int32_t)lds_bytes;
+    ctx->timing.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
     // key stream: keep the region sizes; after an overflow (counted, but
     // through direct atomics) scale the buckets that overflowed by what this
     // run reserved, and capture no graph so that the next call uses that
@@ -2894,6 +2930,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.nkernel = A.nchunks > 0 ? (stream ? 2 * nrounds : 1) : 0;
         p.hist_layout = stream ? kHistStream : hm;
         p.lds_bytes = (int)lds_bytes;
+        p.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
                       hipSuccess &&
                   hipHostMalloc((void **)&p.h_err, 16) == hipSuccess;
@@ -3124,6 +3161,37 @@ int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_ca
 int nc_last_timing(const nc_context *ctx, nc_timing *out) {
     if (!ctx || !out) return NC_ERR_ARG;
     *out = ctx->timing;
+    return NC_OK;
+}
+
+// The lookup of MixedPol::group on the host: same clamp, cell, entry and
+// comparisons, in FP32 on scaled values (the scaling is exact).
+int nc_mixed_classify(const double *edges, int nedges, const float *d, int64_t n, int32_t *out, int32_t *entries) {
+    if (!edges || nedges < 2 || n < 0 || (n > 0 && (!d || !out))) return NC_ERR_ARG;
+    if (nedges > 41) return NC_ERR_LIMIT;
+    Geometry G;
+    G.edges.assign(edges, edges + nedges);
+    if (!(G.edges[0] >= 0.0)) return NC_ERR_BINS;
+    for (int i = 1; i < nedges; ++i)
+        if (!(G.edges[i] > G.edges[i - 1])) return NC_ERR_BINS;
+    G.nb = nedges - 1;
+    G.sq.resize(nedges);
+    for (int i = 0; i < nedges; ++i) G.sq[i] = G.edges[i] * G.edges[i];
+    make_mixed_tables(G);
+    if (!G.mixed_ok) return NC_ERR_LIMIT;
+    if (entries) *entries = (int32_t)G.lut32.size();
+    const float s = (float)(G.sigma32 * G.sigma32);
+    for (int64_t i = 0; i < n; ++i) {
+        const float x = d[i] * s;
+        const float c = std::min(std::max(x, G.lut32_lo), G.lut32_hi);
+        uint32_t bits;
+        std::memcpy(&bits, &c, 4);
+        const uint2 e = G.lut32[(bits >> G.lut32_shift) - (uint32_t)G.lut32_c0];
+        float E;
+        std::memcpy(&E, &e.y, 4);
+        const int k = (int)e.x + (x > E ? 1 : 0);
+        out[i] = std::fabs(x - E) <= G.g32 ? -2 : (unsigned)k < (unsigned)G.nb ? k : -1;
+    }
     return NC_OK;
 }
 

@@ -88,19 +88,23 @@ struct KArgs {
     double lut_scale;
     int lut_m;
     // mixed path
-    // Quantities in units scaled by sigma^2 = lut_scale32, a power of 4: list
-    // coordinates are stored as sigma*(q - p1), so r23^2 arrives pre-scaled
-    // and its lookup index is the value itself.
-    double sigma32;     // power of 2
-    float lut_scale32;  // sigma32^2
-    int lut32_m;
+    // Quantities in units scaled by sigma^2, a power of 4 that brings rmax^2
+    // into [1, 4): list coordinates are stored as sigma*(q - p1), so r23^2
+    // arrives pre-scaled.  Lookup cell of a scaled r23^2 d: the float bits of
+    // med3(d, lut32_lo, lut32_hi) shifted right by lut32_shift, minus lut32_c0
+    // (2^(23-lut32_shift) log-spaced cells per octave).
+    double sigma32;  // power of 2
+    float lut32_lo, lut32_hi;
+    int lut32_shift;
+    int lut32_c0;
+    int lut32_n;  // entries
     float g32;     // guard band: |fp32 r23^2 - fp64 r23^2| bound (incl. edge rounding), scaled
     float tri_c1;  // collinearity threshold: s_a*s_b - dot^2 <= tri_c1*(s_a+s_b) + tri_c2 => recheck, scaled
     float tri_c2;
     const double *sq;             // nb+1 squared edges
     const uint16_t *lut;          // lut_m+1 lower-bound bins (FP64 path)
     const int *ctab;              // nb*nb compact base index
-    const uint2 *lut32;           // lut32_m+2 entries (mixed path): {bin below the edge, scaled edge as float bits}
+    const uint2 *lut32;           // lut32_n entries (mixed path): {bin below the edge, scaled edge as float bits}
     const uint16_t *t3;           // nb*nb*(nb+2): (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
     // dynamic LDS layout (bytes)
     int off_sq, off_lut, off_ctab, off_lut32, off_t3, off_wave, wave_bytes, lds_bytes;

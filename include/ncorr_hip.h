@@ -90,6 +90,8 @@ typedef struct {
                             streamed as keys into per-bucket regions and added up in LDS */
     int32_t lds_bytes;   /* LDS of one workgroup of the triplet kernel (above half the CU's LDS: one
                             workgroup per CU) */
+    int32_t path;        /* nc_path that ran: NC_PATH_MIXED was asked for but NC_PATH_PLAIN ran when the
+                            bins are too narrow for its guard bands; rppi counts always run NC_PATH_PLAIN */
 } nc_timing;
 
 /* Create a context on HIP device `device` (replaces the implicit CUDA context
@@ -174,6 +176,15 @@ typedef struct {
 } nc_tuning;
 int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1, nc_catalog *const *c2,
                 nc_catalog *const *c3, int path, nc_tuning *best);
+
+/* Host evaluation of the mixed path's FP32 bin lookup (no counterpart in the
+ * reference; a check of the lookup tables, no GPU used): for each FP32 squared
+ * separation d[i] (unscaled), out[i] = the bin the lookup assigns, -1 outside
+ * (edges[0]^2, edges[nb]^2), or -2 when d[i] lies in an edge's guard band and
+ * the pair would be re-evaluated in FP64.  *entries (optional) receives the
+ * table length.  NC_ERR_LIMIT when the mixed path is disabled for these edges
+ * (bins narrower than the guard bands). */
+int nc_mixed_classify(const double *edges, int nedges, const float *d, int64_t n, int32_t *out, int32_t *entries);
 
 #ifdef __cplusplus
 }

@@ -87,15 +87,15 @@ HALF_CU_LDS = 81920 - 256
 
 @pytest.mark.parametrize("path", PATHS)
 def test_wide_binning_uses_the_whole_cu_lds(ctx, path):
-    """25 bins over rmax = 50 (the large survey's binning): the mixed path's
-    lookup and slot tables with the weighted histogram need more than half
-    the CU's LDS, so the launch runs one workgroup per CU."""
+    """36 bins over rmax = 50: the weighted sorted-bin histogram alone
+    (C(38,3) slots, 67.5 KB) leaves too little of half the CU's LDS for the
+    waves' staging, so the launch runs one workgroup per CU."""
     _log(f"layouts wide binning path={path}")
     d, r = survey_cap(1500, 2500, seed=21)
     for c in (d, r):
         c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.1
     box, _ = prep_cat_survey(d, r)
-    edges = edges_linear(0.0, 50.0, 25)
+    edges = edges_linear(0.0, 50.0, 36)
     cats = {"D": ctx.catalog(d), "R": ctx.catalog(r)}
     want = {}
     for term in ("DDR", "RRR"):

@@ -5,7 +5,8 @@ within 1e-9 relative."""
 import numpy as np
 import pytest
 
-from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN
+from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, NcError
+from cudancorr_amd._lib import mixed_classify
 from cudancorr_amd.gen import edges_linear, logspace, neyman_scott_box, prep_cat_survey, survey_cap, uniform_box
 from oracle import oracle
 
@@ -25,6 +26,15 @@ def _check(hist, cnt, want_c, want_w, weighted):
         assert np.max(np.abs(hist[nz] - want_w[nz]) / np.abs(want_w[nz]), initial=0.0) <= WEIGHT_RTOL
     else:
         assert np.array_equal(cnt, want_c)
+
+
+def _mixed_runs(edges):
+    """Whether the mixed path can take these edges (its lookup tables exist)."""
+    try:
+        mixed_classify(edges, np.zeros(1, np.float32))
+        return True
+    except NcError:
+        return False
 
 
 def _case(seed):
@@ -68,6 +78,7 @@ def test_random_threed(ctx, seed):
     assert want_c.sum() > 0
     for path in (NC_PATH_PLAIN, NC_PATH_MIXED):
         hist, cnt = ctx.count(*cats, edges, box, survey=survey, path=path)
+        assert ctx.timing().path == (NC_PATH_MIXED if path == NC_PATH_MIXED and _mixed_runs(edges) else NC_PATH_PLAIN)
         _check(hist, cnt, want_c, want_w, weighted)
 
 
@@ -95,4 +106,5 @@ def test_reference_driver_bins(ctx):
     assert want.sum() > 100_000
     for path in (NC_PATH_PLAIN, NC_PATH_MIXED):
         _, got = ctx.count(p, p, p, edges, box, path=path)
+        assert ctx.timing().path == path  # the log-spaced lookup takes these bins
         assert np.array_equal(got, want)

@@ -31,9 +31,9 @@ def fma32(a, b, c):
 
 
 def kernel_scale_exponent(edges):
-    """kk of the kernel's first table attempt: the smallest power of 4, s32 =
-    4^kk, with 1.25*rmax^2*s32 >= 512."""
-    return int(np.ceil(np.log2(512.0 / (1.25 * float(edges[-1]) ** 2)) / 2.0))
+    """kk of the kernel's scale: the power of 4, s32 = 4^kk, with
+    rmax^2*s32 in [1, 4)."""
+    return -int(np.floor(np.log2(float(edges[-1]) ** 2) / 2.0))
 
 
 def mixed_decisions(P, A, B, edges, kk=0, guard=80.0):
