@@ -855,7 +855,7 @@ struct MixedPol {
     uint2 *q;
     int qn, C, row_first;
     const uint2 *lb;  // lookup table shifted down by its first cell: read at cell indices >= lut32_c0 only
-    float lo32, hi32, g32, c1, c2t;
+    float lo32, g32, tri_t;
     int sh32;
     unsigned nb;
     float4 rb;
@@ -878,11 +878,9 @@ struct MixedPol {
         row_first = 0;
         lb = t.lut32 - A.lut32_c0;
         lo32 = A.lut32_lo;
-        hi32 = A.lut32_hi;
         sh32 = A.lut32_shift;
         g32 = A.g32;
-        c1 = A.tri_c1;
-        c2t = A.tri_c2;
+        tri_t = A.tri_t;
         nb = (unsigned)A.nb;
     }
 
@@ -939,7 +937,7 @@ struct MixedPol {
             dot[u] = __builtin_fmaf(ra[u].z, rb.z, __builtin_fmaf(ra[u].y, rb.y, ra[u].x * rb.x));
             ss[u] = ra[u].w + rb.w;
             d[u] = __builtin_fmaf(-2.0f, dot[u], ss[u]);
-            e[u] = lb[__float_as_uint(__builtin_amdgcn_fmed3f(d[u], lo32, hi32)) >> sh32];
+            e[u] = lb[__float_as_uint(__builtin_amdgcn_fmed3f(d[u], lo32, 4.0f)) >> sh32];
         }
         bool acc[U], amb[U];
         int slot[U];
@@ -950,7 +948,7 @@ struct MixedPol {
             const bool amb_b = fabsf(d[u] - E) <= g32;
             const bool valid = (unsigned)k < nb;
             const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
-            const bool amb_t = delta <= __builtin_fmaf(c1, ss[u], c2t);
+            const bool amb_t = delta <= tri_t;
             // bitwise, so the predicates stay lane masks in scalar registers
             amb[u] = ok[u] & (amb_b | (valid & amb_t));
             acc[u] = ok[u] & valid & !amb_b & !amb_t;
@@ -959,7 +957,7 @@ struct MixedPol {
         bool any = false;
 #pragma unroll
         for (int u = 0; u < U; ++u) any = any | amb[u];
-        if (__ballot(any)) {  // rare
+        if (__builtin_amdgcn_ballot_w64(any)) {  // rare
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const unsigned long long m = __ballot(amb[u]);
@@ -1883,9 +1881,9 @@ struct Geometry {
     bool mixed_ok = false;
     std::vector<uint2> lut32;
     double sigma32 = 0.0;
-    float lut32_lo = 0.f, lut32_hi = 0.f;
+    float lut32_lo = 0.f;
     int lut32_shift = 0, lut32_c0 = 0;
-    float g32 = 0.f, tri_c1 = 0.f, tri_c2 = 0.f;
+    float g32 = 0.f, tri_t = 0.f;
     std::vector<uint16_t> t3;  // (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
 };
 
@@ -1986,26 +1984,29 @@ int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
 // it accepts whenever min(a + b - c, c - |a - b|) > 20uR.  Both slacks are at
 // least ab*sin^2(t)/(2(a+b)) = (S_a S_b - D^2)/(2ab(a+b)) and ab(a+b) <=
 // R(S_a + S_b), so  S_a S_b - D^2 > 40uR^2 (S_a + S_b)  is sufficient.  The
-// FP32 value s_a*s_b - dot^2 is within 28uR^4 of the exact one, hence a pair
-// is accepted when  s_a*s_b - dot^2 > 80uR^2 (s_a + s_b) + 64uR^4  (both
-// terms doubled) and otherwise re-evaluated exactly.
+// FP32 value s_a*s_b - dot^2 is within 28uR^4 of the exact one, so
+// s_a*s_b - dot^2 > 80uR^2 (s_a + s_b) + 64uR^4  (both terms doubled)
+// suffices; with s_a, s_b <= R^2 (1 + 8u) (list entries lie within R) the
+// one constant  tri_t = 80uR^2 * 2R^2 (1 + 1e-5) + 64uR^4  bounds the right
+// side for every pair.  A pair with s_a*s_b - dot^2 > tri_t is accepted,
+// any other re-evaluated exactly.
 //
 // Scaling: s32 = sigma^2 is a power of 4 and the lists hold sigma*(A - P)
 // (rounded once, from the exact FP64 product), so every FP32 quantity above
 // is the unscaled one times a power of 2, with the same rounding; the
-// thresholds g32, c1, c2 and the edges are stored scaled (by s32, s32, s32^2,
+// thresholds g32, tri_t and the edges are stored scaled (by s32, s32^2,
 // s32).
 //
 // Lookup.  For a positive float x, bits(x) >> sh is monotone in x: cell c
 // holds the floats [val(c << sh), val((c+1) << sh)), 2^(23-sh) log-spaced
 // cells per octave, fine near zero where squared edges crowd and coarse where
-// they are sparse.  The kernel clamps d32' = s32*d32 to [val(c0 << sh),
-// val(c1 << sh)] (cell c0 thereby takes every smaller value, negative ones
-// included, and cell c1 every larger one) and reads entry c - c0 = {bin below
-// the one band that meets the cell (or below all of it), the band's edge
-// s32*E as float bits, +inf if no band meets it}.  c1 is the cell of the
-// scaled rmax^2, c0 at most 2^w - 1 cells below it and at least the cell of
-// the smallest normal float.  The smallest table (w = 8..13) for which some
+// they are sparse.  The kernel clamps d32' = s32*d32 to [val(c0 << sh), 4]
+// (cell c0 thereby takes every smaller value, negative ones included, and
+// the cell c1 of 4.0, above the scaled rmax^2 < 4, every larger one) and
+// reads entry c - c0 = {bin below the one band that meets the cell (or below
+// all of it), the band's edge s32*E as float bits, +inf if no band meets
+// it}.  c0 is at most 2^w - 1 cells below c1 and at least the cell of the
+// smallest normal float.  The smallest table (w = 8..13) for which some
 // cell density (8..4096 per octave) leaves no cell meeting two bands is kept;
 // bins narrower than the bands, or no such table, disable the path.
 static void make_mixed_tables(Geometry &G) {
@@ -2034,7 +2035,7 @@ static void make_mixed_tables(Geometry &G) {
         std::memcpy(&x, &b, 4);
         return (double)x;
     };
-    const uint32_t top = fbits((float)Es[G.nb]);
+    const uint32_t top = fbits(4.0f);  // the kernel's upper clamp, above s32 * rmax^2
     for (int w = 8; w <= 13; ++w)
         for (int p = 3; p <= 12; ++p) {
             const int sh = 23 - p;
@@ -2062,11 +2063,11 @@ static void make_mixed_tables(Geometry &G) {
             G.lut32_shift = sh;
             G.lut32_c0 = (int)c0;
             G.lut32_lo = (float)fval((uint32_t)c0 << sh);
-            G.lut32_hi = (float)fval((uint32_t)c1 << sh);
             G.sigma32 = std::ldexp(1.0, kk);
             G.g32 = (float)gs;
-            G.tri_c1 = (float)(80.0 * u * R * R * s);
-            G.tri_c2 = (float)(64.0 * u * R * R * R * R * s * s);
+            const double R2s = R * R * s;
+            G.tri_t = std::nextafter((float)(80.0 * u * R2s * 2.0 * R2s * (1.0 + 1e-5) + 64.0 * u * R2s * R2s),
+                                     INFINITY);
             G.mixed_ok = true;
             return;
         }
@@ -2631,13 +2632,11 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.t3 = (const uint16_t *)((char *)ctx->tables.p + g_t3);
     A.sigma32 = G.sigma32;
     A.lut32_lo = G.lut32_lo;
-    A.lut32_hi = G.lut32_hi;
     A.lut32_shift = G.lut32_shift;
     A.lut32_c0 = G.lut32_c0;
     A.lut32_n = (int)G.lut32.size();
     A.g32 = G.g32;
-    A.tri_c1 = G.tri_c1;
-    A.tri_c2 = G.tri_c2;
+    A.tri_t = G.tri_t;
     A.nslots = hm == kHistDense ? (int)((nb3 + 1) / 2) : hm == kHistGlobal ? 0 : G.nbc;
     A.flush_pairs = ctx->dense_flush;
     A.off_sq = (int)hist_bytes;
@@ -3183,7 +3182,7 @@ int nc_mixed_classify(const double *edges, int nedges, const float *d, int64_t n
     const float s = (float)(G.sigma32 * G.sigma32);
     for (int64_t i = 0; i < n; ++i) {
         const float x = d[i] * s;
-        const float c = std::min(std::max(x, G.lut32_lo), G.lut32_hi);
+        const float c = std::min(std::max(x, G.lut32_lo), 4.0f);
         uint32_t bits;
         std::memcpy(&bits, &c, 4);
         const uint2 e = G.lut32[(bits >> G.lut32_shift) - (uint32_t)G.lut32_c0];

@@ -91,16 +91,15 @@ struct KArgs {
     // Quantities in units scaled by sigma^2, a power of 4 that brings rmax^2
     // into [1, 4): list coordinates are stored as sigma*(q - p1), so r23^2
     // arrives pre-scaled.  Lookup cell of a scaled r23^2 d: the float bits of
-    // med3(d, lut32_lo, lut32_hi) shifted right by lut32_shift, minus lut32_c0
+    // med3(d, lut32_lo, 4.0) shifted right by lut32_shift, minus lut32_c0
     // (2^(23-lut32_shift) log-spaced cells per octave).
     double sigma32;  // power of 2
-    float lut32_lo, lut32_hi;
+    float lut32_lo;
     int lut32_shift;
     int lut32_c0;
     int lut32_n;  // entries
     float g32;     // guard band: |fp32 r23^2 - fp64 r23^2| bound (incl. edge rounding), scaled
-    float tri_c1;  // collinearity threshold: s_a*s_b - dot^2 <= tri_c1*(s_a+s_b) + tri_c2 => recheck, scaled
-    float tri_c2;
+    float tri_t;   // collinearity threshold: s_a*s_b - dot^2 <= tri_t => recheck, scaled
     const double *sq;             // nb+1 squared edges
     const uint16_t *lut;          // lut_m+1 lower-bound bins (FP64 path)
     const int *ctab;              // nb*nb compact base index

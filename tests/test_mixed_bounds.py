@@ -3,7 +3,8 @@
 The mixed kernel (cudancorr_amd/csrc/ncorr_hip.hip, make_mixed_tables and
 pair_loop_mixed) decides a pair without FP64 arithmetic when
   * its FP32 r23^2 is farther than g32 = 80uR^2 from every squared edge, and
-  * s_a*s_b - dot^2 > 80uR^2 (s_a + s_b) + 64uR^4  (triangle far from degenerate),
+  * s_a*s_b - dot^2 > tri_t = 80uR^2 * 2R^2 (1 + 1e-5) + 64uR^4, the largest value of
+    80uR^2 (s_a + s_b) + 64uR^4 over list entries (triangle far from degenerate),
 with u = 2^-24, R = rmax, r = fl32(q - p1), s = fl32(|r|^2),
 dot = fl32(r_a . r_b), d32 = fl32(s_a + s_b - 2 dot).  Every other pair is
 re-evaluated in FP64.  The kernel works in units scaled by a power of 4
@@ -43,8 +44,9 @@ def mixed_decisions(P, A, B, edges, kk=0, guard=80.0):
     sq = edges * edges
     E32 = f32(f32(sq) * s32)
     g = guard * U * R * R * s32
-    c1 = np.float32(80.0 * U * R * R * s32)
-    c2 = np.float32(64.0 * U * R ** 4 * s32 * s32)
+    R2s = R * R * s32
+    tri_t = np.nextafter(np.float32(80.0 * U * R2s * 2.0 * R2s * (1.0 + 1e-5) + 64.0 * U * R2s * R2s),
+                         np.float32(np.inf))
     ra = f32((A - P) * sg)
     rb = f32((B - P) * sg)
     sa = fma32(ra[:, 2], ra[:, 2], fma32(ra[:, 1], ra[:, 1], f32(ra[:, 0] * ra[:, 0])))
@@ -56,7 +58,7 @@ def mixed_decisions(P, A, B, edges, kk=0, guard=80.0):
     k32 = np.searchsorted(E32, d32, side="right") - 1
     valid32 = (k32 >= 0) & (k32 < edges.size - 1)
     delta = fma32(-dot, dot, f32(sa * sb))
-    amb_t = delta <= fma32(c1, ss, c2)
+    amb_t = delta <= tri_t
     decided = ~near_edge & ~(valid32 & amb_t)
     accepted = decided & valid32
     return decided, accepted, k32
