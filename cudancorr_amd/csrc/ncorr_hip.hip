@@ -515,6 +515,28 @@ __device__ __forceinline__ int lane_bin(const KArgs &A, int k) {
     if constexpr (HM == kHistDense) return 4 * __mul24(k, A.nb);
     return k;
 }
+// (a << SH) + b as one v_lshl_add_u32.  For a in {0, -1} the compiler would
+// otherwise rewrite the shift as a mask and spend a third instruction.
+template <int SH>
+__device__ __forceinline__ int shl_add(int a, int b) {
+    if constexpr (SH == 0) {
+        return a + b;
+    } else {
+        int r;
+        asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "i"(SH), "v"(b));
+        return r;
+    }
+}
+
+// The same with the third bin in layout units: 4k (kHistDense), 2k (kHistT3)
+// or k (kHistCtab).
+template <int HM>
+__device__ __forceinline__ int pair_slot_x(const KArgs &A, const LdsTabs &t, int kr, int kl, int kx) {
+    if constexpr (HM == kHistT3)
+        return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(t.t3) + kr + kl + kx);
+    if constexpr (HM == kHistDense) return kr + kl + kx;
+    return sorted_slot<kHistCtab>(A, t, kr, kl, kx);
+}
 template <int HM>
 __device__ __forceinline__ int pair_slot(const KArgs &A, const LdsTabs &t, int kr, int kl, int k) {
     if constexpr (HM == kHistT3)
@@ -841,6 +863,8 @@ template <bool SYM, bool WEIGHTED, int HM>
 struct MixedPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 4;
+    static constexpr int kShift = HM == kHistDense ? 2 : HM == kHistT3 ? 1 : 0;  // bin units of the layout
+    static __device__ __forceinline__ float E_(uint2 e) { return __uint_as_float(e.y); }
     const KArgs &A;
     const LdsTabs &t;
     MList m2, m3;
@@ -857,7 +881,7 @@ struct MixedPol {
     const uint2 *lb;  // lookup table shifted down by its first cell: read at cell indices >= lut32_c0 only
     float lo32, g32, tri_t;
     int sh32;
-    unsigned nb;
+    unsigned nb, nbx;
     float4 rb;
     int kb, bidx;
     double wb, w1b;
@@ -882,6 +906,7 @@ struct MixedPol {
         g32 = A.g32;
         tri_t = A.tri_t;
         nb = (unsigned)A.nb;
+        nbx = nb << kShift;
     }
 
     __device__ void stage(bool from3, int first, int last) {
@@ -943,17 +968,28 @@ struct MixedPol {
         int slot[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const float E = __uint_as_float(e[u].y);
-            const int k = (int)e[u].x + (d[u] > E ? 1 : 0);
-            const bool amb_b = fabsf(d[u] - E) <= g32;
-            const bool valid = (unsigned)k < nb;
+            // bin in layout units (the table's .x is pre-scaled): edges below
+            // the cell, minus one when d < E (sign of d - E; d == E lies in
+            // the guard band and is rechecked)
+            const float dE = d[u] - E_(e[u]);
+            const int kx = shl_add<kShift>(__float_as_int(dE) >> 31, (int)e[u].x);
+            const bool amb_b = fabsf(dE) <= g32;
+            const bool valid = (unsigned)kx < nbx;
             const float delta = __builtin_fmaf(-dot[u], dot[u], ra[u].w * rb.w);
             const bool amb_t = delta <= tri_t;
             // bitwise, so the predicates stay lane masks in scalar registers
             amb[u] = ok[u] & (amb_b | (valid & amb_t));
             acc[u] = ok[u] & valid & !amb_b & !amb_t;
-            slot[u] = pair_slot<HM>(A, t, ka[u], kb, k);
+            slot[u] = pair_slot_x<HM>(A, t, ka[u], kb, kx);
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (acc[u]) {
+                if (SWAP)
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1b, wb, w1 * wa[u], wa[u]);
+                else
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
+            }
         bool any = false;
 #pragma unroll
         for (int u = 0; u < U; ++u) any = any | amb[u];
@@ -969,14 +1005,6 @@ struct MixedPol {
             // one group adds at most U*64 entries: drain once the queue could overflow
             if (qn > kQueueCap - U * kWave) drain();
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (acc[u]) {
-                if (SWAP)
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1b, wb, w1 * wa[u], wa[u]);
-                else
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
-            }
     }
 
     template <bool SWAP>
@@ -1103,7 +1131,14 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void 
         // FP64 tables are only read by the rare exact rechecks: global memory
         t.sq = A.sq;
         t.lut = A.lut;
-        for (int i = tid; i < A.lut32_n; i += nthr) lut32[i] = A.lut32[i];
+        // entry .x (edges below the cell) in the layout's bin units: bytes
+        // of a dense count (4), of a t3 entry (2), or bins (ctab)
+        constexpr int kshift = HM == kHistDense ? 2 : HM == kHistT3 ? 1 : 0;
+        for (int i = tid; i < A.lut32_n; i += nthr) {
+            uint2 v = A.lut32[i];
+            v.x <<= kshift;
+            lut32[i] = v;
+        }
     } else {
         for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
         for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
@@ -2003,9 +2038,9 @@ int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
 // they are sparse.  The kernel clamps d32' = s32*d32 to [val(c0 << sh), 4]
 // (cell c0 thereby takes every smaller value, negative ones included, and
 // the cell c1 of 4.0, above the scaled rmax^2 < 4, every larger one) and
-// reads entry c - c0 = {bin below the one band that meets the cell (or below
-// all of it), the band's edge s32*E as float bits, +inf if no band meets
-// it}.  c0 is at most 2^w - 1 cells below c1 and at least the cell of the
+// reads entry c - c0 = {number of edges whose bands lie wholly below the
+// cell, the edge s32*E of the one band that meets the cell as float bits,
+// +inf if none}: the bin is that number, minus one when d32' < E.  c0 is at most 2^w - 1 cells below c1 and at least the cell of the
 // smallest normal float.  The smallest table (w = 8..13) for which some
 // cell density (8..4096 per octave) leaves no cell meeting two bands is kept;
 // bins narrower than the bands, or no such table, disable the path.
@@ -2056,7 +2091,7 @@ static void make_mixed_tables(Geometry &G) {
                 while (over < ne && Es[over] - gs <= b) ++over;
                 if (over - below > 1) ok = false;
                 const float edge = over > below ? (float)Es[below] : INFINITY;
-                lut[j] = make_uint2((uint32_t)(below - 1), fbits(edge));
+                lut[j] = make_uint2((uint32_t)below, fbits(edge));
             }
             if (!ok) continue;
             G.lut32 = std::move(lut);
@@ -3188,8 +3223,9 @@ int nc_mixed_classify(const double *edges, int nedges, const float *d, int64_t n
         const uint2 e = G.lut32[(bits >> G.lut32_shift) - (uint32_t)G.lut32_c0];
         float E;
         std::memcpy(&E, &e.y, 4);
-        const int k = (int)e.x + (x > E ? 1 : 0);
-        out[i] = std::fabs(x - E) <= G.g32 ? -2 : (unsigned)k < (unsigned)G.nb ? k : -1;
+        const float dE = x - E;
+        const int k = (int)e.x - (std::signbit(dE) ? 1 : 0);
+        out[i] = std::fabs(dE) <= G.g32 ? -2 : (unsigned)k < (unsigned)G.nb ? k : -1;
     }
     return NC_OK;
 }
