@@ -897,7 +897,9 @@ struct MixedPol {
         la = reinterpret_cast<float4 *>(wlds);
         lk = reinterpret_cast<int *>(wlds + (size_t)Cp * sizeof(float4));
         lw = reinterpret_cast<double *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int)));
-        q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) + sizeof(double)));
+        // unit-weight layout (kHistDense): no weights staged, the queue follows the bins
+        q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) +
+                                                           (HM == kHistDense ? 0 : sizeof(double))));
         qn = 0;
         row_first = 0;
         lb = t.lut32 - A.lut32_c0;
@@ -1106,8 +1108,12 @@ __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, doub
 // moves to another term and at exit.  Dense u32 counts are also merged once
 // the pairs added since the last merge reach A.flush_pairs <= 2^31: one chunk
 // adds at most chunk * cap^2 < 2^31 (checked on the host), so no count wraps.
+// The unit-weight mixed variant is held to 96 VGPRs (five waves per SIMD):
+// its pair loop is latency bound, and with four-wave workgroups of about
+// 31 KB five of them share a CU (20 waves instead of 16).
 template <bool MIXED, int HM>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_triplets(KArgs A) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MIXED && HM == kHistDense ? 5 : 4))) void
+k_triplets(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
@@ -2420,10 +2426,13 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     int rs_nbm = 8;  // per-wave bucket slots: the largest bucket count of a term, rounded up to 8
     for (int i = 0; i < nterms && stream; ++i) rs_nbm = std::max(rs_nbm, (rs_term[i].y + 7) & ~7);
     size_t stage_bytes = stream ? al16(stream_wave_bytes(rs_anyw, rs_nbm)) : 0;
-    auto wave_lds = [&](int c) {
+    // per-wave LDS: staged list chunk (mixed: float4 + bin, + weight unless
+    // the layout is the unit-weight kHistDense) and the recheck queue
+    auto wave_lds = [&](int c, int hm) {
         const size_t cp = (size_t)c + 4;
         if (G.rppi) return al16(cp * sizeof(RElem)) + stage_bytes;
-        return mixed ? al16(cp * (sizeof(float4) + sizeof(int) + sizeof(double)) + kQueueCap * sizeof(uint2))
+        return mixed ? al16(cp * (sizeof(float4) + sizeof(int) + (hm == kHistDense ? 0 : sizeof(double))) +
+                            kQueueCap * sizeof(uint2))
                      : al16(cp * (sizeof(Elem) + sizeof(double)));
     };
     // --- tables.  Global copies of all; the kernel copies into LDS what its
@@ -2535,7 +2544,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // The budget is first half the CU's LDS (two workgroups per CU); a layout
     // that only fits the whole of it runs one workgroup per CU.
     auto fits = [&](int hm, size_t budget) {
-        return hist_size(hm) + lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64) <=
+        return hist_size(hm) + lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64, hm) <=
                budget;
     };
     auto choose = [&](size_t budget) {
@@ -2567,8 +2576,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const size_t lds_tab = al16(hist_bytes + lds_tab_bytes);
     // chunks are whole 64-entry blocks (the circulant diagonal blocks need it)
     C = std::max(64, C - C % 64);
-    while (C > 64 && lds_tab + W * wave_lds(C) > budget) C -= 64;
-    const size_t wave_bytes = wave_lds(C);
+    while (C > 64 && lds_tab + W * wave_lds(C, hm) > budget) C -= 64;
+    const size_t wave_bytes = wave_lds(C, hm);
     const size_t lds_bytes = lds_tab + W * wave_bytes;
     if (lds_bytes > budget) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
     const void *kfns[2][3] = {
