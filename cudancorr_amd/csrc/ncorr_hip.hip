@@ -826,17 +826,33 @@ struct PlainPol {
     }
 };
 
+// Recheck queue entries: (S2 index, S3 index).  The unit-weight layout packs
+// them into 16 bits each (its lists hold fewer than 46341 entries: the host
+// uses kHistDense only when chunk * cap^2 < 2^31); the others keep two words.
+template <int HM>
+struct QueueEntry {
+    using T = uint2;
+    static __device__ __forceinline__ T pack(unsigned a, unsigned b) { return make_uint2(a, b); }
+    static __device__ __forceinline__ uint2 unpack(T e) { return e; }
+};
+template <>
+struct QueueEntry<kHistDense> {
+    using T = uint32_t;
+    static __device__ __forceinline__ T pack(unsigned a, unsigned b) { return a | (b << 16); }
+    static __device__ __forceinline__ uint2 unpack(T e) { return make_uint2(e & 0xffffu, e >> 16); }
+};
+
 // Mixed path, deferred exact rechecks: every queued (S2 entry, S3 entry) pair
 // is evaluated with the FP64 image positions (catalogue point + periodic
 // shift of its neighbour entry, rebuilt as in build_list), one pair per lane.
 template <bool SYM, bool WEIGHTED, int HM>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
-                            const CatRef &c2, const CatRef &c3, int cx, int cy, int cz, const uint2 *q, int qn,
-                            int lane) {
+                            const CatRef &c2, const CatRef &c3, int cx, int cy, int cz,
+                            const typename QueueEntry<HM>::T *q, int qn, int lane) {
     const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
     wave_sync();
     for (int i = lane; i < qn; i += kWave) {
-        const uint2 e = q[i];
+        const uint2 e = QueueEntry<HM>::unpack(q[i]);
         const int2 ka = m2.KF[e.x], kb = m3.KF[e.y];
         const int ja = m2.I[e.x], jb = m3.I[e.y];
         double sax, say, saz, sbx, sby, sbz;
@@ -876,7 +892,7 @@ struct MixedPol {
     float4 *la;
     int *lk;
     double *lw;
-    uint2 *q;
+    typename QueueEntry<HM>::T *q;
     int qn, C, row_first;
     const uint2 *lb;  // lookup table shifted down by its first cell: read at cell indices >= lut32_c0 only
     float lo32, g32, tri_t;
@@ -898,7 +914,7 @@ struct MixedPol {
         lk = reinterpret_cast<int *>(wlds + (size_t)Cp * sizeof(float4));
         lw = reinterpret_cast<double *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int)));
         // unit-weight layout (kHistDense): no weights staged, the queue follows the bins
-        q = reinterpret_cast<uint2 *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) +
+        q = reinterpret_cast<typename QueueEntry<HM>::T *>(wlds + (size_t)Cp * (sizeof(float4) + sizeof(int) +
                                                            (HM == kHistDense ? 0 : sizeof(double))));
         qn = 0;
         row_first = 0;
@@ -1000,8 +1016,8 @@ struct MixedPol {
             for (int u = 0; u < U; ++u) {
                 const unsigned long long m = __ballot(amb[u]);
                 if (amb[u])
-                    q[qn + lane_rank(m)] = SWAP ? make_uint2((unsigned)bidx, (unsigned)ai[u])
-                                                : make_uint2((unsigned)ai[u], (unsigned)bidx);
+                    q[qn + lane_rank(m)] = SWAP ? QueueEntry<HM>::pack((unsigned)bidx, (unsigned)ai[u])
+                                                : QueueEntry<HM>::pack((unsigned)ai[u], (unsigned)bidx);
                 qn += __popcll(m);
             }
             // one group adds at most U*64 entries: drain once the queue could overflow
@@ -2427,12 +2443,13 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     for (int i = 0; i < nterms && stream; ++i) rs_nbm = std::max(rs_nbm, (rs_term[i].y + 7) & ~7);
     size_t stage_bytes = stream ? al16(stream_wave_bytes(rs_anyw, rs_nbm)) : 0;
     // per-wave LDS: staged list chunk (mixed: float4 + bin, + weight unless
-    // the layout is the unit-weight kHistDense) and the recheck queue
+    // the layout is the unit-weight kHistDense) and the recheck queue (4- or
+    // 8-byte entries, as QueueEntry)
     auto wave_lds = [&](int c, int hm) {
         const size_t cp = (size_t)c + 4;
         if (G.rppi) return al16(cp * sizeof(RElem)) + stage_bytes;
         return mixed ? al16(cp * (sizeof(float4) + sizeof(int) + (hm == kHistDense ? 0 : sizeof(double))) +
-                            kQueueCap * sizeof(uint2))
+                            kQueueCap * (hm == kHistDense ? sizeof(uint32_t) : sizeof(uint2)))
                      : al16(cp * (sizeof(Elem) + sizeof(double)));
     };
     // --- tables.  Global copies of all; the kernel copies into LDS what its
