@@ -1729,8 +1729,12 @@ __device__ __forceinline__ unsigned long long rppi_primary(const KArgs &A, const
 // Persistent rppi kernel (countTrianglesRppiBox / countTrianglesRppiSurvey,
 // device.cu:317-379, 434-505): the work queue, list build and traversal of
 // k_triplets with the rppi filters and pair arithmetic.
+// The survey key-stream variant is built for workgroups of at most 8 waves
+// and 168 VGPRs (three waves per SIMD): its per-wave key staging already
+// holds a CU to about three waves per SIMD, and 128 VGPRs spilled 39.
 template <bool SURVEY, bool STREAM>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_rppi(KArgs A) {
+__global__ __launch_bounds__(SURVEY && STREAM ? 512 : 1024)
+    __attribute__((amdgpu_waves_per_eu(SURVEY && STREAM ? 3 : 4))) void k_rppi(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
@@ -2605,6 +2609,15 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const void *kfn = G.rppi ? (G.survey ? (stream ? (const void *)k_rppi<true, true> : (const void *)k_rppi<true, false>)
                                          : (stream ? (const void *)k_rppi<false, true> : (const void *)k_rppi<false, false>))
                              : kfns[mixed ? 1 : 0][hm];
+    {
+        // some variants are compiled for smaller workgroups (more VGPRs per lane)
+        hipFuncAttributes fa;
+        NC_HIP(ctx, hipFuncGetAttributes(&fa, kfn));
+        if (block > fa.maxThreadsPerBlock)
+            return fail(ctx, NC_ERR_LIMIT, "waves_per_group " + std::to_string(W) + " exceeds the " +
+                                               std::to_string(fa.maxThreadsPerBlock / kWave) +
+                                               " waves this kernel variant is built for");
+    }
     int per_cu = ctx->wgs_per_cu;
     if (per_cu <= 0) {
         int occ = 0;

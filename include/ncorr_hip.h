@@ -154,7 +154,8 @@ int nc_last_timing(const nc_context *ctx, nc_timing *out);
 
 /* Tuning knobs of the triplet kernel.  Values <= 0 keep the current setting
  * (wgs_per_cu: 0 = as many as fit, < 0 keeps).
- *   waves_per_group  in {1, 2, 4, 8, 16}
+ *   waves_per_group  in {1, 2, 4, 8, 16}; a count whose kernel variant is built for smaller
+ *                    workgroups (the rppi survey key stream: at most 8) fails with NC_ERR_LIMIT
  *   chunk            primaries per work item of the device queue
  *   wgs_per_cu       persistent workgroups per compute unit
  *   list_chunk       neighbour-list entries staged in LDS per wave (multiple of 64) */

@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, Context
+from cudancorr_amd import NC_PATH_MIXED, NC_PATH_PLAIN, Context, NcError
 from cudancorr_amd.gen import neyman_scott_box, prep_cat_survey, survey_cap, uniform_box
 from oracle import oracle
 
@@ -262,3 +262,31 @@ def test_key_stream_rounds(monkeypatch, survey):
                     assert np.array_equal(cnt[i], c), f"replay {rep} term {t}"
         for c in cats.values():
             c.close()
+
+
+def test_survey_stream_workgroup_limit():
+    """The survey key-stream kernel is built for at most 8 waves per workgroup:
+    16 is refused with NC_ERR_LIMIT before any launch, 8 counts as usual, and
+    the autotuner never picks 16 there."""
+    _log("rppi survey stream workgroup limit")
+    d, r = survey_cap(1500, 2000, seed=31)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.1
+    box, ori = prep_cat_survey(d, r)
+    edges = np.array([0.0, 2.0, 4.0, 6.0, 9.0])
+    npi = 12
+    cnt, _, _ = oracle.count_rppi(r, r, r, edges, npi, box, survey=True, ori=ori)
+    with Context(0) as c2:
+        c2.set_tuning(waves_per_group=16)
+        with pytest.raises(NcError) as e:
+            c2.count(r, r, r, edges, box, survey=True, npi=npi, ori=ori)
+        assert e.value.code == 6
+        c2.set_tuning(waves_per_group=8)
+        _, got = c2.count(r, r, r, edges, box, survey=True, npi=npi, ori=ori)
+        assert np.array_equal(got, cnt)
+        R = c2.catalog(r)
+        best = c2.autotune([(R, R, R)], edges, box, survey=True, npi=npi, ori=ori)
+        assert best["waves_per_group"] <= 8
+        _, got = c2.count_batched([(R, R, R)], edges, box, survey=True, npi=npi, ori=ori)
+        assert np.array_equal(got[0], cnt)
+        R.close()
