@@ -40,6 +40,31 @@ def test_library_exports_every_symbol():
         assert hasattr(lib, name), name
 
 
+def test_struct_layouts_match_binding(tmp_path):
+    """Every field offset and size of nc_params, nc_timing and nc_tuning, as a
+    C compiler lays them out from include/ncorr_hip.h, equals the ctypes
+    binding's."""
+    structs = {"nc_params": _lib.NcParams, "nc_timing": _lib.NcTiming, "nc_tuning": _lib.NcTuning}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "ncorr_hip.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'    printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'    printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = str(tmp_path / "layout")
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", exe], check=True)
+    got = {}
+    for line in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, f, v = line.split()
+        got[(cname, f)] = int(v)
+    for cname, cls in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
+
+
 def test_size_helpers():
     L = _lib.lib()
     for ne, nh, nc in ((11, 1000, 220), (16, 3375, 680), (21, 8000, 1540), (26, 15625, 2925), (1, 0, 0)):
