@@ -474,6 +474,34 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
     return n;
 }
 
+// Longest neighbour list (S2 or S3) over the primaries of one term: the exact
+// lengths build_list returns, which stores nothing when A.cap is 0.  The
+// dense layout's no-wrap rule needs chunk * (list length)^2 < 2^31; the
+// 27-cell candidate bound is several times the list length, too loose for
+// dense catalogues.  One wave per primary, grid-stride.
+__global__ __launch_bounds__(256) void k_list_bound(KArgs A, TermDesc T, unsigned *__restrict__ out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    const int nwaves = (int)(gridDim.x * blockDim.x / kWave);
+    int best = 0;
+    for (int p = wave; p < T.n1; p += nwaves) {
+        const double px = T.x1[p], py = T.y1[p], pz = T.z1[p];
+        bool bad = false;
+        const bool clamp = !A.survey;
+        const int cx = axis_cell(px, A.cs[0], A.nc[0], clamp, bad);
+        const int cy = axis_cell(py, A.cs[1], A.nc[1], clamp, bad);
+        const int cz = axis_cell(pz, A.cs[2], A.nc[2], clamp, bad);
+        if (bad) continue;  // the triplet kernel reports it
+        const int n2 = build_list<kListPlain>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, nullptr, lane,
+                                              nullptr, nullptr);
+        const int n3 = T.sym ? n2
+                             : build_list<kListPlain>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz,
+                                                      nullptr, lane, nullptr, nullptr);
+        best = max(best, max(n2, n3));
+    }
+    if (lane == 0) atomicMax(out, (unsigned)best);
+}
+
 // LDS histogram layouts of the triplet kernel (HM):
 //   kHistCtab   compact C(nb+2,3) sorted slots (u64 counts / f64 sums), slot
 //               index through ctab; the plain path's FP64 bin lookup may take
@@ -1221,7 +1249,7 @@ k_triplets(KArgs A) {
                                                      lane, t.sq, t.lut);
             // wave-uniform lengths keep the pair loops' control in scalar registers
             const int n2u = uniform(n2), n3u = uniform(n3);
-            if (n2u > A.cap || n3u > A.cap) {
+            if (n2u > A.nmax || n3u > A.nmax) {
                 if (lane == 0) atomicOr(A.err, kErrOverflow);
                 continue;
             }
@@ -2270,6 +2298,30 @@ struct Term {
     nc_catalog *c1, *c2, *c3;
 };
 
+// Device view of a term: the cell-sorted arrays of its three catalogues.
+TermDesc term_desc(const Term &t) {
+    TermDesc d;
+    d.x1 = (const double *)t.c1->x.p;
+    d.y1 = (const double *)t.c1->y.p;
+    d.z1 = (const double *)t.c1->z.p;
+    d.w1 = (const double *)t.c1->w.p;
+    d.x2 = (const double *)t.c2->x.p;
+    d.y2 = (const double *)t.c2->y.p;
+    d.z2 = (const double *)t.c2->z.p;
+    d.w2 = (const double *)t.c2->w.p;
+    d.st2 = (const uint32_t *)t.c2->start.p;
+    d.x3 = (const double *)t.c3->x.p;
+    d.y3 = (const double *)t.c3->y.p;
+    d.z3 = (const double *)t.c3->z.p;
+    d.w3 = (const double *)t.c3->w.p;
+    d.st3 = (const uint32_t *)t.c3->start.p;
+    d.n1 = (int)t.c1->n;
+    d.sym = t.c2 == t.c3;
+    d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
+    d.pad = 0;
+    return d;
+}
+
 void plan_drop(nc_context *ctx) {
     BatchPlan &p = ctx->plan;
     if (p.build_exec) (void)hipGraphExecDestroy(p.build_exec);
@@ -2568,17 +2620,52 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         return hist_size(hm) + lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64, hm) <=
                budget;
     };
+    // Longest list the launch may meet: the 27-cell candidate bound, or, when
+    // only the exact lengths can admit the dense layout, the longest list
+    // k_list_bound finds (one extra pass over the primaries).
+    const double kWrap = 2147483648.0;
+    const bool dense_cand = !G.rppi && (mixed || G.lut_single_step) && all_unit;
+    int nmax = cap;
+    if (dense_cand && (double)ctx->chunk * (double)cap * (double)cap >= kWrap && fits(kHistDense, ctx->lds_hw)) {
+        KArgs B;
+        std::memset(&B, 0, sizeof(B));
+        for (int a = 0; a < 3; ++a) {
+            B.box[a] = G.box[a];
+            B.cs[a] = G.cs[a];
+            B.nc[a] = G.nc[a];
+        }
+        B.survey = G.survey;
+        B.r2min = G.sq[0];
+        B.r2max = G.sq[G.nb];
+        B.cap = 0;  // count only
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->bound.p, 0u, 1LL);
+        for (const Term &t : terms) {
+            const TermDesc d = term_desc(t);
+            if (d.n1 == 0 || t.c2->n == 0 || t.c3->n == 0) continue;
+            const int grid = (int)std::min<long long>((d.n1 + 3) / 4, (long long)ctx->ncu * 32);
+            hipLaunchKernelGGL(k_list_bound, dim3(grid), dim3(256), 0, ctx->stream, B, d, (unsigned *)ctx->bound.p);
+        }
+        NC_HIP(ctx, hipGetLastError());
+        unsigned lb = 0;
+        NC_HIP(ctx, hipMemcpyAsync(&lb, ctx->bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        nmax = std::min(cap, (int)std::max(1u, lb));
+    }
+    const bool dense_ok = dense_cand && (double)ctx->chunk * (double)nmax * (double)nmax < kWrap;
     auto choose = [&](size_t budget) {
         if (G.rppi) return kHistGlobal;
-        if ((mixed || G.lut_single_step) && all_unit &&
-            (double)ctx->chunk * (double)cap * (double)cap < 2147483648.0 && fits(kHistDense, budget))
-            return kHistDense;
+        if (dense_ok && fits(kHistDense, budget)) return kHistDense;
         if ((mixed || G.lut_single_step) && !G.t3.empty() && fits(kHistT3, budget)) return kHistT3;
         return kHistCtab;
     };
     size_t budget = ctx->lds_max;
     int hm = choose(budget);
-    if (!fits(hm, budget) && ctx->lds_hw > budget) {
+    if (mixed && dense_ok && hm != kHistDense && fits(kHistDense, ctx->lds_hw)) {
+        // mixed path: dense counts over a whole CU beat the sorted-slot
+        // layouts at half of it (25 bins, survey randoms: 4405 against 5428 ms)
+        budget = ctx->lds_hw;
+        hm = kHistDense;
+    } else if (!fits(hm, budget) && ctx->lds_hw > budget) {
         budget = ctx->lds_hw;
         hm = choose(budget);
     }
@@ -2639,24 +2726,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     for (int i = 0; i < nterms; ++i) {
         const Term &t = terms[i];
         TermDesc &d = td[i];
-        d.x1 = (const double *)t.c1->x.p;
-        d.y1 = (const double *)t.c1->y.p;
-        d.z1 = (const double *)t.c1->z.p;
-        d.w1 = (const double *)t.c1->w.p;
-        d.x2 = (const double *)t.c2->x.p;
-        d.y2 = (const double *)t.c2->y.p;
-        d.z2 = (const double *)t.c2->z.p;
-        d.w2 = (const double *)t.c2->w.p;
-        d.st2 = (const uint32_t *)t.c2->start.p;
-        d.x3 = (const double *)t.c3->x.p;
-        d.y3 = (const double *)t.c3->y.p;
-        d.z3 = (const double *)t.c3->z.p;
-        d.w3 = (const double *)t.c3->w.p;
-        d.st3 = (const uint32_t *)t.c3->start.p;
-        d.n1 = (int)t.c1->n;
-        d.sym = t.c2 == t.c3;
-        d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
-        d.pad = 0;
+        d = term_desc(t);
         weighted[i] = d.weighted;
         if (t.c2->n == 0 || t.c3->n == 0) continue;
         for (int b = 0; b < d.n1; b += ctx->chunk, ++nitems)
@@ -2682,6 +2752,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.scratch = (char *)ctx->scratch.p;
     A.slot_bytes = slot_bytes;
     A.cap = cap;
+    A.nmax = hm == kHistDense ? nmax : cap;
     A.chunk_elems = C;
     for (int a = 0; a < 3; ++a) {
         A.box[a] = G.box[a];
@@ -3030,6 +3101,57 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 // ---------------------------------------------------------------------------
 // C-ABI
 
+// A count over terms, as nc_count_batched runs it.  Unit-weight terms beside
+// weighted ones (the survey's RRR beside DDD, DDR, DRR): on the mixed path the
+// unit-weight ones may take the dense u32 layout, which the weighted ones
+// cannot, so they run as a launch of their own whenever a dense histogram
+// could fit in a CU's LDS.  Those two launches are not captured in a graph.
+int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
+                uint64_t *counts, bool use_graph) {
+    const int nterms = (int)t.size();
+    if (path == NC_PATH_MIXED && params && params->bin_type == NC_BIN_THREED && params->nedges >= 2 &&
+        params->nedges <= 41) {
+        std::vector<int> iu, iw;
+        for (int i = 0; i < nterms; ++i) {
+            if (!t[i].c1 || !t[i].c2 || !t[i].c3) return fail(ctx, NC_ERR_ARG, "null catalogue");
+            (t[i].c1->unit_w && t[i].c2->unit_w && t[i].c3->unit_w ? iu : iw).push_back(i);
+        }
+        const size_t nb = (size_t)params->nedges - 1;
+        if (!iu.empty() && !iw.empty() && nb * nb * nb * 4 + 48 * 1024 <= ctx->lds_hw) {
+            if (!hist) return fail(ctx, NC_ERR_ARG, "null params or output");
+            const size_t len = (size_t)nc_hist_len(params);
+            std::vector<double> h[2];
+            std::vector<uint64_t> c[2];
+            nc_timing tm[2];
+            for (int g = 0; g < 2; ++g) {
+                const std::vector<int> &idx = g ? iw : iu;
+                std::vector<Term> sub;
+                for (int i : idx) sub.push_back(t[i]);
+                h[g].resize(idx.size() * len);
+                if (counts) c[g].resize(idx.size() * len);
+                const int rc = run_count(ctx, params, sub, path, h[g].data(), counts ? c[g].data() : nullptr, false);
+                if (rc) return rc;
+                tm[g] = ctx->timing;
+                for (size_t k = 0; k < idx.size(); ++k) {
+                    std::memcpy(hist + (size_t)idx[k] * len, h[g].data() + k * len, sizeof(double) * len);
+                    if (counts)
+                        std::memcpy(counts + (size_t)idx[k] * len, c[g].data() + k * len, sizeof(uint64_t) * len);
+                }
+            }
+            // the sum of both launches; layout and LDS of the unit-weight one
+            ctx->timing.ms_build = tm[0].ms_build + tm[1].ms_build;
+            ctx->timing.ms_count = tm[0].ms_count + tm[1].ms_count;
+            ctx->timing.ms_total = tm[0].ms_total + tm[1].ms_total;
+            ctx->timing.n_kernel = tm[0].n_kernel + tm[1].n_kernel;
+            ctx->timing.hist_layout = tm[0].hist_layout;
+            ctx->timing.lds_bytes = tm[0].lds_bytes;
+            ctx->timing.path = tm[0].path;
+            return NC_OK;
+        }
+    }
+    return run_count(ctx, params, t, path, hist, counts, use_graph);
+}
+
 extern "C" {
 
 int nc_create(nc_context **out, int device) {
@@ -3228,7 +3350,7 @@ int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_ca
     if (nterms < 1 || !c1 || !c2 || !c3) return fail(ctx, NC_ERR_ARG, "bad term list");
     std::vector<Term> t(nterms);
     for (int i = 0; i < nterms; ++i) t[i] = {c1[i], c2[i], c3[i]};
-    return run_count(ctx, params, t, path, hist, counts, true);
+    return count_terms(ctx, params, t, path, hist, counts, true);
 }
 
 int nc_last_timing(const nc_context *ctx, nc_timing *out) {
@@ -3350,7 +3472,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
         // sample to about 80 ms, enough to separate shapes a few percent apart
         ctx->waves_per_group = 8;
         ctx->wgs_per_cu = 0;
-        rc = run_count(ctx, params, t, path, hist.data(), nullptr, false);
+        rc = count_terms(ctx, params, t, path, hist.data(), nullptr, false);
         big_lds = rc == NC_OK && (size_t)ctx->timing.lds_bytes > ctx->lds_max;
         const double ms = ctx->timing.ms_count;
         if (rc == NC_OK && ms > 0.0)
@@ -3370,7 +3492,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
                 float ms = 1e30f;
                 int r = NC_OK;
                 for (int rep = 0; rep < 2 && r == NC_OK; ++rep) {  // the better of two runs
-                    r = run_count(ctx, params, t, path, hist.data(), nullptr, false);
+                    r = count_terms(ctx, params, t, path, hist.data(), nullptr, false);
                     if (r == NC_OK) ms = std::min(ms, ctx->timing.ms_count);
                 }
                 if (r == NC_ERR_LIMIT) {  // this shape does not fit: try the next

@@ -70,6 +70,7 @@ struct KArgs {
     char *scratch;
     long long slot_bytes;
     int cap;
+    int nmax;         // longest list the launch was planned for (<= cap); longer ones flag an overflow
     int chunk_elems;  // a-entries staged in LDS per wave
     double box[3];
     double cs[3];

@@ -144,8 +144,13 @@ int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, in
  * terms (c1[t], c2[t], c3[t]) with shared params run in one persistent launch
  * that pulls (term, primary-tile) items from a device work queue.  The launch
  * sequence is captured in a HIP graph and replayed while the same terms are
- * passed again.  hist: nterms*nb^3 doubles; counts: nterms*nb^3 uint64 or
- * NULL (a term's counts are filled when all its weights are 1). */
+ * passed again.  On NC_PATH_MIXED (threeD), unit-weight terms beside weighted
+ * ones run as a second launch, in the dense 32-bit layout that weighted terms
+ * cannot use, whenever a dense nb^3 histogram could fit in a CU's LDS; those
+ * two launches are not captured, and nc_last_timing then sums their times and
+ * reports the unit-weight launch's layout.  hist: nterms*nb^3 doubles; counts:
+ * nterms*nb^3 uint64 or NULL (a term's counts are filled when all its weights
+ * are 1). */
 int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1,
                      nc_catalog *const *c2, nc_catalog *const *c3, int path, double *hist, uint64_t *counts);
 

@@ -135,3 +135,51 @@ def test_workgroup_sizes(ctx, path, waves):
         cat.close()
     finally:
         ctx.set_tuning(*before)
+
+
+def test_dense_admitted_by_exact_list_bound(ctx):
+    """chunk * (27-cell candidate bound)^2 >= 2^31 but chunk * (longest list)^2
+    < 2^31: the exact-length pass (k_list_bound) admits the dense layout, and
+    the counts stay bit-exact.  20,000 uniform points in 9^3 cells: about 740
+    candidates but about 115 list entries per primary."""
+    _log("layouts exact list bound")
+    from cudancorr_amd.gen import uniform_box
+    L = 216.0
+    p = uniform_box(20000, L, seed=5)
+    edges = edges_linear(0.0, 24.0, 12)
+    box = np.array([L] * 3)
+    want, _ = oracle.count(p, p, p, edges, box)
+    before = ctx.get_tuning()
+    try:
+        ctx.set_tuning(-1, 16384, -1, 0)
+        for path in PATHS:
+            _, got = ctx.count(p, p, p, edges, box, path=path)
+            assert ctx.timing().hist_layout == 2
+            assert np.array_equal(got, want)
+    finally:
+        ctx.set_tuning(*before)
+
+
+def test_batched_unit_terms_split_to_dense(ctx):
+    """A weighted term beside a unit-weight one at 25 bins: the unit-weight
+    term runs as its own launch in the dense layout (whole-CU LDS), the
+    weighted one in the sorted slots; both match the oracle."""
+    _log("layouts batched split")
+    d, r = survey_cap(1200, 2000, seed=17)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 50.0, 25)
+    _, wsw = oracle.count(d, d, r, edges, box, survey=True)
+    cr, _ = oracle.count(r, r, r, edges, box, survey=True)
+    D, R = ctx.catalog(d), ctx.catalog(r)
+    hist, cnt = ctx.count_batched([(D, D, R), (R, R, R)], edges, box, survey=True, path=NC_PATH_MIXED)
+    t = ctx.timing()
+    assert t.hist_layout == 2 and t.path == NC_PATH_MIXED
+    assert cr.sum() > 10_000
+    assert np.array_equal(cnt[1], cr)
+    nz = wsw != 0
+    assert np.array_equal(hist[0][~nz], wsw[~nz])
+    assert np.max(np.abs(hist[0][nz] - wsw[nz]) / wsw[nz]) <= 1e-9
+    D.close()
+    R.close()
