@@ -897,11 +897,13 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
 // Mixed path.  With r = q - p1 re-centred in FP32 and s = |r|^2:
 // r23^2 = s_a + s_b - 2 r_a.r_b, and the triangle (p1, a, b) is degenerate
 // only when r_a and r_b are nearly collinear, i.e. when s_a*s_b - (r_a.r_b)^2
-// is small.  Per pair: one dot product, one lookup over r23^2*scale32 that
-// returns the bin below the (at most one) edge near that value and the edge
-// itself, and the collinearity bound.  A pair whose r23^2 lies within the
-// guard band g32 of that edge, or whose collinearity measure is within the
-// error bound (make_mixed_tables), is queued for the exact FP64 evaluation.
+// is small.  Per pair: one dot product, one lookup of the log-spaced cell of
+// r23^2 that returns the number of squared edges below the cell and the (at
+// most one) edge inside its guard-band reach, the bin from the sign of
+// r23^2 - edge, and the collinearity bound.  A pair whose r23^2 lies within
+// the guard band g32 of that edge, or whose collinearity measure is within
+// the error bound (make_mixed_tables), is queued for the exact FP64
+// evaluation.
 // Rows: float4 {r, s} + bin + weight in LDS; lanes in registers.
 template <bool SYM, bool WEIGHTED, int HM>
 struct MixedPol {
@@ -925,7 +927,7 @@ struct MixedPol {
     const uint2 *lb;  // lookup table shifted down by its first cell: read at cell indices >= lut32_c0 only
     float lo32, g32, tri_t;
     int sh32;
-    unsigned nb, nbx;
+    unsigned nbx;  // nb in the layout's bin units
     float4 rb;
     int kb, bidx;
     double wb, w1b;
@@ -951,8 +953,7 @@ struct MixedPol {
         sh32 = A.lut32_shift;
         g32 = A.g32;
         tri_t = A.tri_t;
-        nb = (unsigned)A.nb;
-        nbx = nb << kShift;
+        nbx = (unsigned)A.nb << kShift;
     }
 
     __device__ void stage(bool from3, int first, int last) {
