@@ -192,3 +192,29 @@ def test_repeat_bit_identical(ctx):
     _, a = ctx.count(p, p, p, edges, box)
     _, b = ctx.count(p, p, p, edges, box)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_lattice_edges_and_collinear_triples(ctx, path):
+    """An integer lattice with edges sqrt(k): nearly every squared separation
+    lies exactly on a squared edge and many triples are exactly collinear, so
+    on the mixed path most pairs go through the FP64 recheck queue, which fills
+    and drains over and over.  Unit weights (dense layout, 16-bit queue
+    entries) and weighted (sorted slots, two-word entries): the counts
+    bit-exact and the sums within tolerance."""
+    _log(f"lattice path={path}")
+    n = 12
+    g = np.arange(n, dtype=np.float64)
+    xyz = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    p = np.c_[xyz, np.ones(xyz.shape[0])]
+    edges = np.sqrt(np.arange(10, dtype=np.float64))  # 0, 1, sqrt2, ..., 3
+    box = np.array([float(n)] * 3)
+    want, _ = oracle.count(p, p, p, edges, box)
+    assert want.sum() > 1_000_000
+    _, got = ctx.count(p, p, p, edges, box, path=path)
+    assert np.array_equal(got, want)
+    w = p.copy()
+    w[:, 3] = 1.0 + (np.arange(w.shape[0]) % 7) / 8.0
+    _, ws = oracle.count(w, w, w, edges, box)
+    hw, _ = ctx.count(w, w, w, edges, box, path=path)
+    _wcheck(hw, ws)
