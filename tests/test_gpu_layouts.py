@@ -183,3 +183,30 @@ def test_batched_unit_terms_split_to_dense(ctx):
     assert np.max(np.abs(hist[0][nz] - wsw[nz]) / wsw[nz]) <= 1e-9
     D.close()
     R.close()
+
+
+def test_autotune_split_batch(ctx):
+    """The autotuner times a mixed-weight batch as it runs (two launches) and
+    the tuned shape counts it bit-exact / within tolerance."""
+    _log("layouts autotune split batch")
+    d, r = survey_cap(1200, 2000, seed=23)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 50.0, 25)
+    _, wsw = oracle.count(d, d, r, edges, box, survey=True)
+    cr, _ = oracle.count(r, r, r, edges, box, survey=True)
+    before = ctx.get_tuning()
+    D, R = ctx.catalog(d), ctx.catalog(r)
+    try:
+        best = ctx.autotune([(D, D, R), (R, R, R)], edges, box, survey=True, path=NC_PATH_MIXED)
+        assert best["waves_per_group"] in (4, 8, 16) and best["ms"] > 0
+        hist, cnt = ctx.count_batched([(D, D, R), (R, R, R)], edges, box, survey=True, path=NC_PATH_MIXED)
+        assert np.array_equal(cnt[1], cr)
+        nz = wsw != 0
+        assert np.array_equal(hist[0][~nz], wsw[~nz])
+        assert np.max(np.abs(hist[0][nz] - wsw[nz]) / wsw[nz]) <= 1e-9
+    finally:
+        D.close()
+        R.close()
+        ctx.set_tuning(*before)
