@@ -95,7 +95,12 @@ typedef struct {
 } nc_timing;
 
 /* Create a context on HIP device `device` (replaces the implicit CUDA context
- * and the __constant__ parameter block of the reference, device.cu:9-18). */
+ * and the __constant__ parameter block of the reference, device.cu:9-18).
+ * Environment read here (test and tuning knobs; unset means the default):
+ *   NC_RPPI_STREAM=0          rppi through direct global atomics, not the key stream
+ *   NC_RPPI_ROUND_KEYS=<n>    key-stream keys per round (default: 60% of free device memory)
+ *   NC_RPPI_REGION_SCALE=<f>  key-stream regions at f (0 < f <= 1) of the sampled need
+ *   NC_DENSE_FLUSH_PAIRS=<n>  dense counts merged once n pairs were added (default and max 2^31) */
 int nc_create(nc_context **ctx, int device);
 /* Catalogues of a destroyed context may still be passed to nc_catalog_destroy
  * (in any order); nothing else. */
