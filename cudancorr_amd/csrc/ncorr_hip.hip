@@ -2627,7 +2627,10 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const double kWrap = 2147483648.0;
     const bool dense_cand = !G.rppi && (mixed || G.lut_single_step) && all_unit;
     int nmax = cap;
-    if (dense_cand && (double)ctx->chunk * (double)cap * (double)cap >= kWrap && fits(kHistDense, ctx->lds_hw)) {
+    // (the mixed path takes the dense layout over a whole CU, the plain path
+    // only within half of one)
+    if (dense_cand && (double)ctx->chunk * (double)cap * (double)cap >= kWrap &&
+        fits(kHistDense, mixed ? ctx->lds_hw : ctx->lds_max)) {
         KArgs B;
         std::memset(&B, 0, sizeof(B));
         for (int a = 0; a < 3; ++a) {
