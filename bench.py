@@ -367,6 +367,7 @@ def run_rppi_survey(args):
         R.close()
         return hist, cnt, wall, ms, tune
 
+    os.environ["NC_RPPI_STREAM"] = "1"  # surveys default to direct atomics: force the key stream here
     with Context(0) as ctx:
         hist_s, cnt_s, wall_s, ms_s, tune_s = run(ctx)
         _, cnt_u, _, _, _ = run(ctx, unit=True, steps=1, warmup=0)
@@ -424,6 +425,7 @@ def run_rppi_survey(args):
                    "binned_triplets": trip, "binned_triplets_per_term": dict(zip(names, trip_terms))},
         "error": {"unweighted": "RRR bit-exact (key stream vs direct atomics)",
                   "weighted_rel_max_vs_direct": err},
+        "default_output": "direct atomics (surveys; NC_RPPI_STREAM=1 selects the key stream)",
         "direct_atomics": {"wall_ms": wall_d * 1e3, "count_ms": ms_d, "triplets_per_s": trip / wall_d},
         "count_ms": ms_s,
         "tuning": tune_s,

@@ -1470,9 +1470,10 @@ struct KeyStream {
 // false): rp^2 and pi^2 of the third side from the image positions; with
 // p2 and p3 the same catalogue each unordered pair is evaluated once and
 // counted for both orders (evaluated twice when two rp^2 tie, since the sort
-// then keeps the input order of the tied sides).  Survey: every ordered pair,
-// as the line of sight (p1 + p2 + p3 - 3 ori) depends on the order of the
-// FP64 sum.
+// then keeps the input order of the tied sides).  Survey: the line of sight
+// (p1 + p2 + p3 - 3 ori) depends on the order of the FP64 sum, so with p2 and
+// p3 the same catalogue each unordered pair evaluates both orders and leaves
+// one key of count 2 when they fall in the same cell, else one key each.
 //
 // Binned triplets leave either as atomics on the term's dense global
 // histogram (STREAM false) or as keys staged in LDS and flushed, grouped by
@@ -1712,10 +1713,37 @@ struct RppiPol {
                         v0 = (w1 * e2.w) * e3.w;
                     }
                 }
+                if (SYM) {
+                    // the other order (p2, p3) = (e3, e2), with its own line-of-sight
+                    // sum; one key of count 2 when both orders land in the same cell
+                    const double mx = (ppx + e3.q) + e2.q, my = (py + e3.y) + e2.y, mz = (pz + e3.z) + e2.z;
+                    const double sm = dot3(mx, my, mz, mx, my, mz);
+                    const double limm = A.pimax2 * sm;
+                    double u1 = dot3(mx, my, mz, s3x, s3y, s3z);
+                    u1 = u1 * u1;
+                    double u2 = dot3(mx, my, mz, s2x, s2y, s2z);
+                    u2 = u2 * u2;
+                    double u3 = dot3(mx, my, mz, -dx, -dy, -dz);
+                    u3 = u3 * u3;
+                    if (!(limm < u1 || limm < u2 || limm < u3)) {
+                        const double q1 = u1 / sm, q2 = u2 / sm, q3 = u3 / sm;
+                        const double r1 = e3.r - q1, r2 = e2.r - q2, r3 = s23 - q3;
+                        if (r1 < A.r2max && r2 < A.r2max && r3 < A.r2max && r1 > A.r2min && r2 > A.r2min &&
+                            r3 > A.r2min && q1 < A.pimax2 && q2 < A.pimax2) {
+                            h1 = rppi_cell(A, t, r1, r2, r3, q1, q2, q3, T1, rem1);
+                            v1 = (w1 * e3.w) * e2.w;
+                        }
+                    }
+                    if (h0 && h1 && T0 == T1 && rem0 == rem1) {
+                        n0 = 2;
+                        v0 += v1;
+                        h1 = false;
+                    }
+                }
             }
         }
         emit(h0, T0, rem0, n0, v0);
-        if (SYM && !SURVEY) emit(h1, T1, rem1, 1, v1);
+        if (SYM) emit(h1, T1, rem1, 1, v1);
         if (STREAM && cnt > kStage - 2 * kWave) flush();
     }
 
@@ -1826,8 +1854,9 @@ __global__ __launch_bounds__(SURVEY && STREAM ? 512 : 1024)
                 if (lane == 0) atomicOr(A.err, kErrDomain);
                 continue;
             }
-            // survey: every ordered pair (the LOS sum is order dependent)
-            const bool sym = !SURVEY && T.sym;
+            // p2 and p3 from one catalogue: each unordered pair once (survey: both
+            // orders evaluated in it, the line-of-sight sum being order dependent)
+            const bool sym = T.sym;
             const int n2 = build_list<LM>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
                                           t.lut);
             const int n3 = T.sym ? n2
@@ -2030,7 +2059,7 @@ struct nc_context {
     std::string dev_name;
     // rppi key stream: buffers, and the per-bucket capacities last sized for
     // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
-    bool rppi_stream = true;
+    int rppi_stream = -1;  // rppi key stream: 1 on, 0 off (direct atomics), -1 box on, survey off
     double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
     double rs_round_keys = 0.0;  // NC_RPPI_ROUND_KEYS: keys per round (tests of several rounds)
     DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_chunk_keys, rs_keys, rs_vals;
@@ -2454,7 +2483,11 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // --- rppi key stream: the reachable dense rp triples (sorted ones and the
     // next triple, where a pi index equal to npi carries the flat index),
     // their ranks, and per term a run of buckets of 2^shift compact bins
-    bool stream = G.rppi && ctx->rppi_stream;
+    // rppi output: the key stream by default for boxes (6x the direct atomics
+    // on the 1M-point box); direct atomics for surveys, whose pair arithmetic
+    // bounds both paths (2.0 s direct against 2.4 s streamed on 500k data +
+    // 2.5M randoms), unless NC_RPPI_STREAM forces one
+    bool stream = G.rppi && (ctx->rppi_stream > 0 || (ctx->rppi_stream < 0 && !G.survey));
     std::vector<uint16_t> rs_rank;
     std::vector<int> rs_inv;
     std::vector<int4> rs_term(nterms);
@@ -3176,7 +3209,7 @@ int nc_create(nc_context **out, int device) {
     ctx->dev_name = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
     ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
     ctx->lds_hw = prop.sharedMemPerBlock - 256;
-    if (const char *e = std::getenv("NC_RPPI_STREAM")) ctx->rppi_stream = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("NC_RPPI_STREAM")) ctx->rppi_stream = std::strcmp(e, "0") != 0 ? 1 : 0;
     if (const char *e = std::getenv("NC_RPPI_ROUND_KEYS")) {
         const double v = std::strtod(e, nullptr);
         if (v >= 1.0) ctx->rs_round_keys = v;

@@ -97,7 +97,8 @@ typedef struct {
 /* Create a context on HIP device `device` (replaces the implicit CUDA context
  * and the __constant__ parameter block of the reference, device.cu:9-18).
  * Environment read here (test and tuning knobs; unset means the default):
- *   NC_RPPI_STREAM=0          rppi through direct global atomics, not the key stream
+ *   NC_RPPI_STREAM=0|1        rppi through direct global atomics (0) or the key stream (1);
+ *                             unset: the key stream for boxes, direct atomics for surveys
  *   NC_RPPI_ROUND_KEYS=<n>    key-stream keys per round (default: 60% of free device memory)
  *   NC_RPPI_REGION_SCALE=<f>  key-stream regions at f (0 < f <= 1) of the sampled need
  *   NC_DENSE_FLUSH_PAIRS=<n>  dense counts merged once n pairs were added (default and max 2^31) */

@@ -176,6 +176,7 @@ def test_key_stream_region_overflow(monkeypatch, survey):
     directly, on every call, and the results stay exact."""
     _log(f"rppi region overflow survey={survey}")
     monkeypatch.setenv("NC_RPPI_REGION_SCALE", "0.05")
+    monkeypatch.setenv("NC_RPPI_STREAM", "1")  # surveys default to direct atomics
     if survey:
         d, r = survey_cap(2500, 10, seed=13)
         d[:, :3] = 1000.0 + (d[:, :3] - 1000.0) * 0.1
@@ -230,6 +231,7 @@ def test_key_stream_rounds(monkeypatch, survey):
     own regions: the same results, eagerly and over graph replays."""
     _log(f"rppi rounds survey={survey}")
     monkeypatch.setenv("NC_RPPI_ROUND_KEYS", "5000")
+    monkeypatch.setenv("NC_RPPI_STREAM", "1")  # surveys default to direct atomics
     if survey:
         d, r = survey_cap(2500, 2000, seed=19)
         for c in (d, r):
@@ -264,11 +266,12 @@ def test_key_stream_rounds(monkeypatch, survey):
             c.close()
 
 
-def test_survey_stream_workgroup_limit():
+def test_survey_stream_workgroup_limit(monkeypatch):
     """The survey key-stream kernel is built for at most 8 waves per workgroup:
     16 is refused with NC_ERR_LIMIT before any launch, 8 counts as usual, and
     the autotuner never picks 16 there."""
     _log("rppi survey stream workgroup limit")
+    monkeypatch.setenv("NC_RPPI_STREAM", "1")  # surveys default to direct atomics
     d, r = survey_cap(1500, 2000, seed=31)
     for c in (d, r):
         c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.1
@@ -290,3 +293,27 @@ def test_survey_stream_workgroup_limit():
         _, got = c2.count_batched([(R, R, R)], edges, box, survey=True, npi=npi, ori=ori)
         assert np.array_equal(got[0], cnt)
         R.close()
+
+
+def test_default_output_path(monkeypatch):
+    """Without NC_RPPI_STREAM, boxes take the key stream (layout 4) and surveys
+    the direct atomics (layout 3); NC_RPPI_STREAM=1 streams a survey too."""
+    _log("rppi default output path")
+    p, edges, npi, box = _box_case(seed=7)
+    cnt_b, _, _ = oracle.count_rppi(p, p, p, edges, npi, box)
+    d, r = survey_cap(1500, 10, seed=11)
+    d[:, :3] = 1000.0 + (d[:, :3] - 1000.0) * 0.1
+    sbox, ori = prep_cat_survey(d, r)
+    sedges, snpi = np.array([0.0, 2.0, 4.0, 6.0, 9.0]), 12
+    d[:, 3] = 1.0
+    cnt_s, _, _ = oracle.count_rppi(d, d, d, sedges, snpi, sbox, survey=True, ori=ori)
+    monkeypatch.delenv("NC_RPPI_STREAM", raising=False)
+    with Context(0) as c2:
+        _, got = c2.count(p, p, p, edges, box, npi=npi)
+        assert c2.timing().hist_layout == 4 and np.array_equal(got, cnt_b)
+        _, got = c2.count(d, d, d, sedges, sbox, survey=True, npi=snpi, ori=ori)
+        assert c2.timing().hist_layout == 3 and np.array_equal(got, cnt_s)
+    monkeypatch.setenv("NC_RPPI_STREAM", "1")
+    with Context(0) as c2:
+        _, got = c2.count(d, d, d, sedges, sbox, survey=True, npi=snpi, ori=ori)
+        assert c2.timing().hist_layout == 4 and np.array_equal(got, cnt_s)
