@@ -3415,6 +3415,10 @@ int nc_mixed_classify(const double *edges, int nedges, const float *d, int64_t n
     const float s = (float)(G.sigma32 * G.sigma32);
     for (int64_t i = 0; i < n; ++i) {
         const float x = d[i] * s;
+        if (std::isnan(x)) {  // no separation: outside every bin
+            out[i] = -1;
+            continue;
+        }
         const float c = std::min(std::max(x, G.lut32_lo), 4.0f);
         uint32_t bits;
         std::memcpy(&bits, &c, 4);

@@ -192,8 +192,8 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
 /* Host evaluation of the mixed path's FP32 bin lookup (no counterpart in the
  * reference; a check of the lookup tables, no GPU used): for each FP32 squared
  * separation d[i] (unscaled), out[i] = the bin the lookup assigns, -1 outside
- * (edges[0]^2, edges[nb]^2), or -2 when d[i] lies in an edge's guard band and
- * the pair would be re-evaluated in FP64.  *entries (optional) receives the
+ * (edges[0]^2, edges[nb]^2) or NaN, or -2 when d[i] lies in an edge's guard
+ * band and the pair would be re-evaluated in FP64.  *entries (optional) receives the
  * table length.  NC_ERR_LIMIT when the mixed path is disabled for these edges
  * (bins narrower than the guard bands). */
 int nc_mixed_classify(const double *edges, int nedges, const float *d, int64_t n, int32_t *out, int32_t *entries);

@@ -105,3 +105,10 @@ def test_narrow_bins_disable_the_path():
 def test_driver_bins_table_is_small():
     """The log-spaced cells keep the reference driver's bins on the mixed path in a few KB of LDS."""
     assert _check(logspace(0.1, 20.0, 13)) <= 512
+
+
+def test_non_finite_inputs():
+    """NaN is outside every bin; +-inf clamp to the ends of the table."""
+    edges = edges_linear(0.0, 10.0, 20)
+    got, _ = mixed_classify(edges, np.array([np.nan, np.inf, -np.inf, -np.nan], dtype=np.float32))
+    np.testing.assert_array_equal(got, [-1, -1, -1, -1])
