@@ -129,12 +129,18 @@ def _points(p):
 class Catalog:
     """Device-resident catalogue of double4 points (nc_catalog)."""
 
-    def __init__(self, ctx: "Context", points):
+    def __init__(self, ctx: "Context", points=None, device_ptr=None, n=0):
+        """Upload host points, or wrap n double4 points at device address
+        device_ptr (not copied; the memory must outlive the catalogue)."""
         self.ctx = ctx
-        a = _points(points)
-        self.n = a.shape[0]
         h = ctypes.c_void_p()
-        ctx._check(lib().nc_catalog_create(ctx.h, _dp(a), self.n, ctypes.byref(h)))
+        if device_ptr is not None:
+            self.n = int(n)
+            ctx._check(lib().nc_catalog_wrap_device(ctx.h, ctypes.c_void_p(device_ptr), self.n, ctypes.byref(h)))
+        else:
+            a = _points(points)
+            self.n = a.shape[0]
+            ctx._check(lib().nc_catalog_create(ctx.h, _dp(a), self.n, ctypes.byref(h)))
         self.h = h
 
     def close(self):
@@ -202,6 +208,10 @@ class Context:
 
     def catalog(self, points) -> Catalog:
         return Catalog(self, points)
+
+    def catalog_device(self, device_ptr, n) -> Catalog:
+        """nc_catalog_wrap_device: n double4 points already in device memory."""
+        return Catalog(self, device_ptr=device_ptr, n=n)
 
     def count(self, c1, c2, c3, edges, box, survey=False, path=NC_PATH_PLAIN, npi=0, ori=(0.0, 0.0, 0.0)):
         """nc_count on catalogues (Catalog or point arrays).  Returns

@@ -117,3 +117,48 @@ def test_catalogue_outlives_its_context():
         _, cnt = c2.count(q, q, q, edges_linear(0.0, 10.0, 5), [100.0] * 3)
         assert cnt.sum() > 0
         q.close()
+
+
+def test_wrapped_device_catalogue_changes_under_replay(ctx):
+    """A catalogue wrapping device memory (nc_catalog_wrap_device): the first
+    batched count plans the dense layout from the exact longest list (chunk
+    16384, so the list-bound pass runs) and captures a graph; the points are
+    then overwritten in place with clustered ones whose lists are far longer.
+    The replay must notice the overflow and recount eagerly: both counts equal
+    the oracle's."""
+    from cudancorr_amd.gen import uniform_box
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    H2D = 1  # hipMemcpyHostToDevice
+
+    print("batched wrapped device catalogue", flush=True)
+    L = 216.0
+    edges = edges_linear(0.0, 24.0, 12)
+    box = np.array([L] * 3)
+    a = uniform_box(20000, L, seed=41)
+    b = neyman_scott_box(200, 100, 2.0, L, seed=43)
+    want_a, _ = oracle.count(a, a, a, edges, box)
+    want_b, _ = oracle.count(b, b, b, edges, box)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    assert a.shape == b.shape
+    dev = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dev), a.nbytes) == 0
+    before = ctx.get_tuning()
+    try:
+        assert hip.hipMemcpy(dev, a.ctypes.data, a.nbytes, H2D) == 0
+        ctx.set_tuning(-1, 16384, -1, 0)
+        C = ctx.catalog_device(dev.value, a.shape[0])
+        _, got = ctx.count_batched([(C, C, C)], edges, box, path=NC_PATH_MIXED)
+        assert ctx.timing().hist_layout == 2
+        assert np.array_equal(got[0], want_a)
+        assert hip.hipMemcpy(dev, b.ctypes.data, b.nbytes, H2D) == 0
+        _, got = ctx.count_batched([(C, C, C)], edges, box, path=NC_PATH_MIXED)
+        assert np.array_equal(got[0], want_b)
+        C.close()
+    finally:
+        ctx.set_tuning(*before)
+        hip.hipFree(dev)
