@@ -2060,6 +2060,7 @@ struct nc_context {
     // rppi key stream: buffers, and the per-bucket capacities last sized for
     // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
     int rppi_stream = -1;  // rppi key stream: 1 on, 0 off (direct atomics), -1 box on, survey off
+    bool batch_split = true;  // unit-weight terms of a mixed-weight batch as their own launch
     double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
     double rs_round_keys = 0.0;  // NC_RPPI_ROUND_KEYS: keys per round (tests of several rounds)
     DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_chunk_keys, rs_keys, rs_vals;
@@ -3146,8 +3147,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
                 uint64_t *counts, bool use_graph) {
     const int nterms = (int)t.size();
-    if (path == NC_PATH_MIXED && params && params->bin_type == NC_BIN_THREED && params->nedges >= 2 &&
-        params->nedges <= 41) {
+    if (ctx->batch_split && path == NC_PATH_MIXED && params && params->bin_type == NC_BIN_THREED &&
+        params->nedges >= 2 && params->nedges <= 41) {
         std::vector<int> iu, iw;
         for (int i = 0; i < nterms; ++i) {
             if (!t[i].c1 || !t[i].c2 || !t[i].c3) return fail(ctx, NC_ERR_ARG, "null catalogue");
@@ -3218,6 +3219,7 @@ int nc_create(nc_context **out, int device) {
         const double v = std::strtod(e, nullptr);
         if (v > 0.0 && v <= 1.0) ctx->rs_scale = v;
     }
+    if (const char *e = std::getenv("NC_BATCH_SPLIT")) ctx->batch_split = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
         const unsigned long long v = std::strtoull(e, nullptr, 10);
         if (v > 0 && v <= (1ull << 31)) ctx->dense_flush = v;

@@ -101,7 +101,8 @@ typedef struct {
  *                             unset: the key stream for boxes, direct atomics for surveys
  *   NC_RPPI_ROUND_KEYS=<n>    key-stream keys per round (default: 60% of free device memory)
  *   NC_RPPI_REGION_SCALE=<f>  key-stream regions at f (0 < f <= 1) of the sampled need
- *   NC_DENSE_FLUSH_PAIRS=<n>  dense counts merged once n pairs were added (default and max 2^31) */
+ *   NC_DENSE_FLUSH_PAIRS=<n>  dense counts merged once n pairs were added (default and max 2^31)
+ *   NC_BATCH_SPLIT=0          nc_count_batched keeps mixed-weight batches in one launch */
 int nc_create(nc_context **ctx, int device);
 /* Catalogues of a destroyed context may still be passed to nc_catalog_destroy
  * (in any order); nothing else. */
