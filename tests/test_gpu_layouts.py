@@ -210,3 +210,27 @@ def test_autotune_split_batch(ctx):
         D.close()
         R.close()
         ctx.set_tuning(*before)
+
+
+def test_batch_split_switched_off(monkeypatch):
+    """NC_BATCH_SPLIT=0: the mixed-weight batch stays one launch in a sorted-slot
+    layout, with the same results."""
+    _log("layouts batch split off")
+    d, r = survey_cap(1200, 2000, seed=17)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 50.0, 25)
+    _, wsw = oracle.count(d, d, r, edges, box, survey=True)
+    cr, _ = oracle.count(r, r, r, edges, box, survey=True)
+    monkeypatch.setenv("NC_BATCH_SPLIT", "0")
+    with Context(0) as c2:
+        D, R = c2.catalog(d), c2.catalog(r)
+        hist, cnt = c2.count_batched([(D, D, R), (R, R, R)], edges, box, survey=True, path=NC_PATH_MIXED)
+        assert c2.timing().hist_layout in (0, 1)
+        assert np.array_equal(cnt[1], cr)
+        nz = wsw != 0
+        assert np.array_equal(hist[0][~nz], wsw[~nz])
+        assert np.max(np.abs(hist[0][nz] - wsw[nz]) / wsw[nz]) <= 1e-9
+        D.close()
+        R.close()
