@@ -73,9 +73,10 @@ ncorr::ncorr(double3 box, int bin_type, std::vector<double> &bins, int numMinorB
     binsMid_ = getBinsMid(bins_);
     rpMax_ = bins_.back();
     rpMin_ = bins_[0];
-    for (double i = 0; i <= static_cast<double>(numMinorBins); i = i + 1.0) piBins_.push_back(i);
-    numPiBins_ = (int)piBins_.size() - 1;
-    if (numPiBins_ != numMinorBins_) throw std::runtime_error("number of piBins do not match input, could be a bug");
+    // unit-width pi bin edges 0, 1, ..., numMinorBins (cudaNcorr.cu:173-175)
+    piBins_.resize((size_t)numMinorBins + 1);
+    for (int i = 0; i <= numMinorBins; ++i) piBins_[(size_t)i] = (double)i;
+    numPiBins_ = numMinorBins;
     piMax_ = piBins_.back();
     const double sqrRMax = rpMax_ * rpMax_ + piMax_ * piMax_;
     rMax_ = sqrtf((float)sqrRMax);  // FP32, as cudaNcorr.cu:187

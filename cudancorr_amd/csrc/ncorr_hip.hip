@@ -2024,6 +2024,7 @@ struct nc_catalog {
 struct BatchPlan {
     bool valid = false;
     std::vector<uint64_t> uids;
+    std::vector<int> wkey;  // per term: unit-weight flags of c1, c2, c3 (bits 0-2)
     std::vector<double> pkey;
     std::vector<int> tkey;
     hipGraphExec_t build_exec = nullptr, count_exec = nullptr;
@@ -2044,7 +2045,7 @@ struct nc_context {
     int ncu = 0;
     size_t lds_max = 0;  // LDS budget of a workgroup with two per CU
     size_t lds_hw = 0;   // the per-workgroup maximum
-    DBuf tables, terms, chunks, out, counter, errw, scratch, bound;
+    DBuf tables, terms, chunks, out, counter, errw, scratch, bound, wflags;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     nc_timing timing{};
     int waves_per_group = 8;
@@ -2456,6 +2457,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // --- graph plan: key = catalogues, params, path, tuning
     std::vector<uint64_t> uids;
     for (const Term &t : terms) uids.insert(uids.end(), {t.c1->uid, t.c2->uid, t.c3->uid});
+    // weight status of each catalogue (wrapped ones may change it between calls)
+    std::vector<int> wkey;
+    for (const Term &t : terms) wkey.push_back((t.c1->unit_w ? 1 : 0) | (t.c2->unit_w ? 2 : 0) | (t.c3->unit_w ? 4 : 0));
     std::vector<double> pkey(P->box, P->box + 3);
     pkey.push_back((double)G.survey);
     pkey.push_back((double)G.rppi);
@@ -2463,7 +2467,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     pkey.push_back(G.ori_x);
     pkey.insert(pkey.end(), G.edges.begin(), G.edges.end());
     const std::vector<int> tkey = {path, ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
-    if (use_graph && ctx->plan.valid && ctx->plan.uids == uids && ctx->plan.pkey == pkey && ctx->plan.tkey == tkey) {
+    if (use_graph && ctx->plan.valid && ctx->plan.uids == uids && ctx->plan.wkey == wkey && ctx->plan.pkey == pkey &&
+        ctx->plan.tkey == tkey) {
         bool retry = false;
         rc = plan_replay(ctx, hist, counts, retry);
         if (rc || !retry) return rc;
@@ -2867,6 +2872,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         const size_t nch = chunks.size();
         rs_ckey = pkey;
         for (uint64_t u : uids) rs_ckey.push_back((double)u);
+        for (int u : wkey) rs_ckey.push_back((double)u);
         rs_ckey.push_back((double)ctx->chunk);  // the work items the estimates refer to
         if (ctx->sample_stride == 1 && ctx->rs_key == rs_ckey && ctx->rs_caps.size() == (size_t)rs_nbk &&
             ctx->rs_cest.size() == nch) {
@@ -3103,6 +3109,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (use_graph && !rs_overflow) {
         BatchPlan p;
         p.uids = uids;
+        p.wkey = wkey;
         p.pkey = pkey;
         p.tkey = tkey;
         p.weighted = weighted;
@@ -3139,6 +3146,37 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 // ---------------------------------------------------------------------------
 // C-ABI
 
+// The points of a catalogue that wraps caller-owned device memory
+// (nc_catalog_wrap_device) may be overwritten between counts, weights
+// included.  Whether all weights are 1 decides the histogram layout, the
+// batch split and whether integer counts are returned, so it is derived again
+// from the current contents before every count over such catalogues (one
+// read of their weights and one synchronisation).  A captured plan keys on
+// these flags, so a changed weight status is never replayed.
+int refresh_weights(nc_context *ctx, const std::vector<Term> &terms) {
+    std::vector<nc_catalog *> w;
+    for (const Term &t : terms)
+        for (nc_catalog *c : {t.c1, t.c2, t.c3})
+            if (c && !c->owns && c->n > 0 && std::find(w.begin(), w.end(), c) == w.end()) w.push_back(c);
+    if (w.empty()) return NC_OK;
+    NC_HIP(ctx, hipSetDevice(ctx->device));
+    NC_HIP(ctx, dgrow(ctx->wflags, sizeof(unsigned) * w.size()));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, (uint32_t *)ctx->wflags.p, 0u,
+                       (long long)w.size());
+    for (size_t i = 0; i < w.size(); ++i) {
+        const int grid = (int)std::min<long long>((w[i]->n + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_unit_weight, dim3(grid), dim3(256), 0, ctx->stream, (const double4 *)w[i]->d_raw,
+                           (long long)w[i]->n, (unsigned *)ctx->wflags.p + i);
+    }
+    NC_HIP(ctx, hipGetLastError());
+    std::vector<unsigned> f(w.size());
+    NC_HIP(ctx, hipMemcpyAsync(f.data(), ctx->wflags.p, sizeof(unsigned) * f.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < w.size(); ++i) w[i]->unit_w = f[i] == 0u;
+    return NC_OK;
+}
+
 // A count over terms, as nc_count_batched runs it.  Unit-weight terms beside
 // weighted ones (the survey's RRR beside DDD, DDR, DRR): on the mixed path the
 // unit-weight ones may take the dense u32 layout, which the weighted ones
@@ -3147,6 +3185,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
                 uint64_t *counts, bool use_graph) {
     const int nterms = (int)t.size();
+    if (const int rc = refresh_weights(ctx, t)) return rc;
     if (ctx->batch_split && path == NC_PATH_MIXED && params && params->bin_type == NC_BIN_THREED &&
         params->nedges >= 2 && params->nedges <= 41) {
         std::vector<int> iu, iw;
@@ -3239,7 +3278,7 @@ int nc_destroy(nc_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     plan_drop(ctx);
     for (DBuf *b : {&ctx->tables, &ctx->terms, &ctx->chunks, &ctx->out, &ctx->counter, &ctx->errw, &ctx->scratch,
-                    &ctx->bound, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
+                    &ctx->bound, &ctx->wflags, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
                     &ctx->rs_chunk_keys, &ctx->rs_keys, &ctx->rs_vals})
         dfree(*b);
     for (auto &e : ctx->ev)
@@ -3351,7 +3390,9 @@ int64_t nc_catalog_size(const nc_catalog *c) { return c ? c->n : -1; }
 
 int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalog *c2, nc_catalog *c3, int path,
              double *hist, uint64_t *counts) {
+    if (!ctx) return NC_ERR_ARG;
     std::vector<Term> t{{c1, c2, c3}};
+    if (const int rc = refresh_weights(ctx, t)) return rc;
     return run_count(ctx, params, t, path, hist, counts, false);
 }
 

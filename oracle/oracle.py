@@ -247,6 +247,92 @@ def brute_force(p1, p2, p3, edges, box=None, exact=False):
     return cnt, ws
 
 
+def wrap_shifts(c1, cq, n, L):
+    """Image shifts under which a point of cell cq is a neighbour of cell c1
+    along one axis of n cells: the reference's one-wrap rule
+    (device.cu:89-122, cudaNcorr.cu:37-48) restated per axis.  Entry c1 + d
+    (d = -1, 0, 1) is cell c1 + d, except that n wraps to 0 with shift +L and
+    -1 to n - 1 with shift -L.  With n = 1 or 2 a cell is reached more than
+    once (three or two images)."""
+    out = []
+    for d in (-1, 0, 1):
+        e, s = c1 + d, 0.0
+        if e == n:
+            e, s = 0, L
+        if e == -1:
+            e, s = n - 1, -L
+        if e == cq:
+            out.append(s)
+    return out
+
+
+def _axis_cells(v, cs, n):
+    c = np.trunc(v / cs).astype(np.int64)  # cudaNcorr.cu:50-58: int(x / cellSize)
+    c[c == n] = n - 1
+    return c
+
+
+def brute_force_wrap(p1, p2, p3, edges, box):
+    """Independent restatement of the box count for ANY box, commensurate or
+    not, with no 27-entry loop: for each primary, every image q + s of every
+    secondary whose per-axis shift s is one of wrap_shifts(cell(p1), cell(q)),
+    filtered on r12 / r13, then all (p2 image, p3 image) pairs filtered on
+    r23 (from the image positions), the FP32 triangle test and half-open bins
+    on the sorted squared sides.  Cells are ceil(L / rmax) per axis of size
+    rmax (cudaNcorr.cu:137-140).  Squared separations in the uncontracted
+    order (the oracle's contract=1).  Returns (counts, wsum) as ``count``."""
+    p1 = np.asarray(p1, dtype=np.float64).reshape(-1, 4)
+    p2 = np.asarray(p2, dtype=np.float64).reshape(-1, 4)
+    p3 = np.asarray(p3, dtype=np.float64).reshape(-1, 4)
+    e = np.asarray(edges, dtype=np.float64)
+    box = np.asarray(box, dtype=np.float64)
+    sq = e * e
+    nb = e.size - 1
+    cs = e[-1]
+    n = [int(np.ceil(box[a] / cs)) for a in range(3)]
+    cells = [np.stack([_axis_cells(p[:, a], cs, n[a]) for a in range(3)], 1) for p in (p1, p2, p3)]
+    lo, hi = sq[0], sq[-1]
+    cnt = np.zeros((nb, nb, nb), dtype=np.uint64)
+    ws = np.zeros((nb, nb, nb), dtype=np.float64)
+
+    def images(a, ca, q, cq):
+        """(index, image position) of every neighbour image of q around a, r in range."""
+        idx, pos = [], []
+        for j in range(q.shape[0]):
+            per_axis = [wrap_shifts(int(ca[ax]), int(cq[j, ax]), n[ax], box[ax]) for ax in range(3)]
+            for sx in per_axis[0]:
+                for sy in per_axis[1]:
+                    for sz in per_axis[2]:
+                        idx.append(j)
+                        pos.append((q[j, 0] + sx, q[j, 1] + sy, q[j, 2] + sz))
+        if not idx:
+            return np.zeros(0, dtype=np.int64), np.zeros((0, 3)), np.zeros(0)
+        pos = np.array(pos)
+        d = a[None, :3] - pos
+        r = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        keep = (r > lo) & (r < hi)
+        return np.array(idx)[keep], pos[keep], r[keep]
+
+    for i, a in enumerate(p1):
+        j2, q2, r12 = images(a, cells[0][i], p2, cells[1])
+        j3, q3, r13 = images(a, cells[0][i], p3, cells[2])
+        if j2.size == 0 or j3.size == 0:
+            continue
+        d = q2[:, None, :] - q3[None, :, :]
+        r23 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        s = np.sort(np.stack([np.broadcast_to(r12[:, None], r23.shape), np.broadcast_to(r13[None, :], r23.shape),
+                              r23], -1), axis=-1)
+        f = np.sqrt(s.astype(np.float32))
+        ok = (r23 > lo) & (r23 < hi) & ((f[..., 0] + f[..., 1]) > f[..., 2])
+        bins = np.searchsorted(sq, s, side="right") - 1
+        w = (a[3] * p2[j2, 3])[:, None] * p3[j3, 3][None, :]
+        for jj, kk in zip(*np.nonzero(ok)):
+            b1, b2, b3 = bins[jj, kk]
+            cnt[b1, b2, b3] += 1
+            ws[b1, b2, b3] += w[jj, kk]
+    return cnt, ws
+
+
 def output_bins(edges):
     """Bin mids of the compact output (cudaNcorr.cu:145-154), as an (m,3) array."""
     e = np.asarray(edges, dtype=np.float64)

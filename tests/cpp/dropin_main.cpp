@@ -7,6 +7,8 @@
 //   dropin_main count <cat1> <cat2> <cat3> <survey> <precision> <bx> <by> <bz> <edge0> <edge1> ...
 //       catN: binary files of n x double4; the same path passed twice means the
 //       same std::vector object.  Prints getTriCountsOut(), one value per line.
+//   dropin_main countbins ...            as count, each line "count mid1 mid2 mid3" from
+//       getTriCountsOut() and getTriBinsOut() at the same index.
 //   dropin_main rppi <cat1> <cat2> <cat3> <survey> <npi> <ori_x> <bx> <by> <bz> <edge0> <edge1> ...
 //       the same with rppi binning (rp edges, npi pi bins).
 #include <cstdio>
@@ -117,7 +119,7 @@ static std::vector<double4> load(const std::string &path) {
     return v;
 }
 
-static int mode_count(int argc, char **argv) {
+static int mode_count(int argc, char **argv, bool with_bins) {
     std::map<std::string, std::vector<double4>> cats;
     std::string p[3] = {argv[2], argv[3], argv[4]};
     for (auto &s : p)
@@ -130,7 +132,15 @@ static int mode_count(int argc, char **argv) {
     ncorr corr(box, 0, edges, surveyMode, {0.0, 0.0, 0.0});
     corr.setPrecision(prec ? ncorr::mixedFP32 : ncorr::plainFP64);
     corr.calculateD1D2D3(cats[p[0]], cats[p[1]], cats[p[2]]);
-    for (double v : corr.getTriCountsOut()) std::printf("%.17g\n", v);
+    const std::vector<double> c = corr.getTriCountsOut();
+    const std::vector<double3> b = corr.getTriBinsOut();
+    if (b.size() != c.size()) return 4;
+    for (size_t i = 0; i < c.size(); ++i) {
+        if (with_bins)
+            std::printf("%.17g %.17g %.17g %.17g\n", c[i], b[i].x, b[i].y, b[i].z);
+        else
+            std::printf("%.17g\n", c[i]);
+    }
     return 0;
 }
 
@@ -157,7 +167,8 @@ int main(int argc, char **argv) {
     try {
         if (m == "ctor") return mode_ctor();
         if (m == "survey") return mode_survey();
-        if (m == "count" && argc >= 12) return mode_count(argc, argv);
+        if (m == "count" && argc >= 12) return mode_count(argc, argv, false);
+        if (m == "countbins" && argc >= 12) return mode_count(argc, argv, true);
         if (m == "rppi" && argc >= 13) return mode_rppi(argc, argv);
     } catch (const std::exception &e) {
         std::printf("EXCEPTION %s\n", e.what());

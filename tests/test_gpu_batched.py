@@ -162,3 +162,54 @@ def test_wrapped_device_catalogue_changes_under_replay(ctx):
     finally:
         ctx.set_tuning(*before)
         hip.hipFree(dev)
+
+
+def test_wrapped_device_catalogue_weights_change(ctx):
+    """The weights of a wrapped device catalogue change in place between
+    counts: unit -> lognormal -> unit.  The unit-weight status decides the
+    histogram layout (dense integer counts), the batch split and whether
+    counts are returned, so every call must follow the current contents:
+    weighted sums within 1e-9 of the oracle's compensated FP64 sums while
+    the weights are not 1, exact counts when they are, for nc_count and for
+    nc_count_batched (whose captured plan must not be replayed across the
+    change)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    H2D = 1
+
+    print("wrapped device catalogue, weights changed in place", flush=True)
+    L = 120.0
+    edges = edges_linear(0.0, 12.0, 6)
+    box = np.array([L] * 3)
+    unit = np.ascontiguousarray(neyman_scott_box(60, 50, 2.0, L, seed=51), dtype=np.float64)
+    wtd = unit.copy()
+    rng = np.random.default_rng(52)
+    wtd[:, 3] = rng.lognormal(0.0, 0.3, unit.shape[0]).astype(np.float32)
+    want_c, _ = oracle.count(unit, unit, unit, edges, box)
+    _, want_w = oracle.count(wtd, wtd, wtd, edges, box)
+    dev = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(dev), unit.nbytes) == 0
+    try:
+        C = ctx.catalog_device(dev.value, unit.shape[0])
+        for step, pts in enumerate((unit, wtd, unit, wtd)):
+            assert hip.hipMemcpy(dev, pts.ctypes.data, pts.nbytes, H2D) == 0
+            weighted = pts is wtd
+            for path in (NC_PATH_PLAIN, NC_PATH_MIXED):
+                print("step", step, "weighted", weighted, "path", path, flush=True)
+                for rep in range(2):  # the second batched call may replay a plan
+                    h, c = ctx.count_batched([(C, C, C)], edges, box, path=path)
+                    h1, c1 = ctx.count(C, C, C, edges, box, path=path)
+                    for hh, cc in ((h[0], c[0]), (h1, c1)):
+                        if weighted:
+                            nz = want_w != 0
+                            assert np.array_equal(hh[~nz], want_w[~nz])
+                            assert np.max(np.abs(hh[nz] / want_w[nz] - 1.0)) <= 1e-9
+                            assert not cc.any()  # no integer counts for weighted terms
+                        else:
+                            assert np.array_equal(cc, want_c)
+                            assert np.array_equal(hh, want_c.astype(np.float64))
+        C.close()
+    finally:
+        hip.hipFree(dev)

@@ -65,3 +65,28 @@ def test_dropin_survey_terms(dropin, tmp_path, term):
     nz = want != 0
     assert np.array_equal(got[~nz], want[~nz])
     assert np.max(np.abs(got[nz] - want[nz]) / want[nz]) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dropin_non_diagonal_bin_position(dropin, tmp_path, precision):
+    """Known answer for the output order (cudaNcorr.cu:72-86, 145-154): one
+    triangle with sides 1.5, 2, 2.5 (a 3-4-5 triangle halved) and edges
+    1, 2, 3, 4 lies in the sorted bins (0, 1, 1) -- 2 is an edge and goes to
+    the upper bin.  Its 6 ordered triplets must appear at the one
+    getTriCountsOut() index whose getTriBinsOut() entry holds the mids
+    (1.5, 2.5, 2.5), and nowhere else."""
+    tri = np.array([[5.0, 5.0, 5.0, 1.0], [6.5, 5.0, 5.0, 1.0], [5.0, 7.0, 5.0, 1.0]])
+    edges = [1.0, 2.0, 3.0, 4.0]
+    p = str(tmp_path / "T.bin")
+    tri.tofile(p)
+    args = [dropin, "countbins", p, p, p, "1", str(precision), "20.0", "20.0", "20.0"] + [repr(e) for e in edges]
+    print("dropin countbins precision", precision, flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    rows = np.array([[float(x) for x in line.split()] for line in out.stdout.split("\n") if line.strip()])
+    # mids 1.5, 2.5, 3.5: the 9 sorted triples passing mid_k <= mid_i + mid_j
+    # (only bins (0, 0, 2), mids 1.5, 1.5, 3.5, fail), in (i, j, k) order
+    assert rows[:, 1:].tolist() == [[1.5, 1.5, 1.5], [1.5, 1.5, 2.5], [1.5, 2.5, 2.5], [1.5, 2.5, 3.5],
+                                    [1.5, 3.5, 3.5], [2.5, 2.5, 2.5], [2.5, 2.5, 3.5], [2.5, 3.5, 3.5],
+                                    [3.5, 3.5, 3.5]]
+    assert rows[:, 0].tolist() == [0, 0, 6, 0, 0, 0, 0, 0, 0]

@@ -236,3 +236,119 @@ def test_exact_fma_helper():
     want = float(Fraction(a) * Fraction(b) + Fraction(c))
     assert oracle._fma_exact(a, b, c) == want == -(2.0 ** -60)
     assert a * b + c == 0.0
+
+
+# ---------------------------------------------------------------------------
+# Known answers for the output compaction (cudaNcorr.cu:72-86) and the
+# one-wrap image rule of non-commensurate boxes (device.cu:89-122).
+
+
+def _compact_transposed(edges, dense):
+    """The compaction with the dense index transposed (i_largest major): a
+    planted mistake that the known answer must reject."""
+    e = np.asarray(edges, dtype=np.float64)
+    mid = (e[1:] + e[:-1]) / 2.0
+    nb = mid.size
+    d = np.asarray(dense, dtype=np.float64).reshape(-1)
+    return np.array([d[i + j * nb + k * nb * nb] for i in range(nb) for j in range(i, nb) for k in range(j, nb)
+                     if mid[k] <= mid[i] + mid[j]])
+
+
+def test_compaction_known_answer():
+    """edges 0,1,2,3 (mids 0.5, 1.5, 2.5): the sorted bin triples with
+    mid_k <= mid_i + mid_j, worked out by hand, are (0,0,0) (0,1,1) (0,2,2)
+    (1,1,1) (1,1,2) (1,2,2) (2,2,2); their dense indices k + 3j + 9i are
+    0, 4, 8, 13, 14, 17, 26."""
+    edges = np.array([0.0, 1.0, 2.0, 3.0])
+    dense = np.arange(27, dtype=np.float64)
+    want = [0, 4, 8, 13, 14, 17, 26]
+    assert oracle.compact(edges, dense).tolist() == want
+    mids = oracle.output_bins(edges).tolist()
+    assert mids == [[0.5, 0.5, 0.5], [0.5, 1.5, 1.5], [0.5, 2.5, 2.5], [1.5, 1.5, 1.5], [1.5, 1.5, 2.5],
+                    [1.5, 2.5, 2.5], [2.5, 2.5, 2.5]]
+    # the planted transposition passes the size check but not the known answer
+    bad = _compact_transposed(edges, dense)
+    assert bad.size == len(want) and bad.tolist() != want
+
+
+def test_compaction_known_answer_four_bins():
+    """edges 0..4 (mids 0.5 .. 3.5): 13 of the 20 sorted triples pass the
+    filter; spot values by hand: (0,0,0) -> 0, (0,3,3) -> 15, (1,2,3) -> 27,
+    (3,3,3) -> 63, and (0,1,2) (2.5 > 2.0) is dropped."""
+    edges = np.arange(5, dtype=np.float64)
+    dense = np.arange(64, dtype=np.float64)
+    got = oracle.compact(edges, dense).tolist()
+    assert got == [0, 5, 10, 15, 21, 22, 26, 27, 31, 42, 43, 47, 63]
+    assert 6 not in got  # (0,1,2)
+
+
+def _grid_points(rng, n, L, span):
+    """Points within `span` of the corner of the periodic box, on a 1/64 grid
+    (exact differences and squares: every sqrSep order agrees)."""
+    x = rng.uniform(-span, span, (n, 3))
+    x = np.round(x * 64.0) / 64.0
+    x = np.where(x < 0.0, x + L, x)
+    p = np.ones((n, 4))
+    p[:, :3] = x
+    p[:, 3] = np.round(rng.uniform(0.5, 2.0, n) * 256.0) / 256.0
+    return p
+
+
+@pytest.mark.parametrize("ratio", [0.8, 1.5, 2.5, 3.5, 12.5])
+def test_non_commensurate_box_matches_wrap_brute_force(ratio):
+    """Boxes of L = ratio * rmax: 1, 2, 3, 4 and 13 cells, the last cell
+    partial.  The oracle's 27-entry search equals the per-axis restatement of
+    the one-wrap rule bin for bin, including the pairs the minimal image
+    would count but the reference misses, and the repeated images of grids
+    of one and two cells."""
+    rng = np.random.default_rng(int(ratio * 10))
+    rmax = 2.0
+    L = ratio * rmax
+    span = min(2.6, L / 2.0)
+    # one or two cells hold every point several times over: fewer points
+    n = {0.8: 14, 1.5: 40}.get(ratio, 90)
+    p = _grid_points(rng, n, L, span)
+    q = _grid_points(rng, n - n // 5, L, span)
+    edges = np.array([0.25, 0.75, 1.25, 1.6, 2.0])
+    box = [L, L, L]
+    for a, b, c in ((p, p, p), (p, q, q), (q, p, q)):
+        got, gw = oracle.count(a, b, c, edges, box, contract=1)
+        want, ww = oracle.brute_force_wrap(a, b, c, edges, box)
+        assert want.sum() > 50
+        assert np.array_equal(got, want)
+        np.testing.assert_allclose(gw, ww, rtol=1e-12, atol=0)
+
+
+def test_wrap_brute_force_detects_planted_shift_errors():
+    """A planted wrong shift makes the independent restatement disagree with
+    the oracle: (a) the minimal image (what a correct periodic count would
+    give) on a 13-cell box with a partial last cell, (b) the wrap shift with
+    the wrong sign."""
+    rng = np.random.default_rng(7)
+    rmax, L = 2.0, 25.0
+    p = _grid_points(rng, 120, L, 2.6)
+    edges = np.array([0.25, 0.75, 1.25, 1.6, 2.0])
+    got, _ = oracle.count(p, p, p, edges, [L, L, L], contract=1)
+    mi, _ = oracle.brute_force(p, p, p, edges, [L, L, L])
+    assert not np.array_equal(got, mi)  # (a): the reference misses pairs across the partial cell
+    good = oracle.wrap_shifts
+    try:
+        oracle.wrap_shifts = lambda c1, cq, n, Lx: [-s for s in good(c1, cq, n, Lx)]
+        flipped, _ = oracle.brute_force_wrap(p, p, p, edges, [L, L, L])
+    finally:
+        oracle.wrap_shifts = good
+    assert not np.array_equal(got, flipped)  # (b)
+    right, _ = oracle.brute_force_wrap(p, p, p, edges, [L, L, L])
+    assert np.array_equal(got, right)
+
+
+def test_wrap_shifts_small_grids():
+    """The per-axis rule by hand: one cell is its own neighbour three times
+    (shifts -L, 0, +L); in two cells, cell 1 is reached from cell 0 directly
+    and across the wrap (shifts -L and 0)."""
+    assert oracle.wrap_shifts(0, 0, 1, 10.0) == [-10.0, 0.0, 10.0]
+    assert oracle.wrap_shifts(0, 1, 2, 10.0) == [-10.0, 0.0]
+    assert oracle.wrap_shifts(1, 0, 2, 10.0) == [0.0, 10.0]
+    assert oracle.wrap_shifts(0, 12, 13, 10.0) == [-10.0]
+    assert oracle.wrap_shifts(12, 0, 13, 10.0) == [10.0]
+    assert oracle.wrap_shifts(5, 8, 13, 10.0) == []

@@ -134,3 +134,20 @@ def test_compaction_sizes(lo, nb, npi, n3d):
     assert out.size == n3d * npi * npi
     # first selected bin is (0,0,0): its npi^2 pi bins come first, in (l, m) order
     assert np.array_equal(out[: npi * npi], np.arange(npi * npi))
+
+
+def test_compaction_known_answer():
+    """cudaNcorr.cu:88-105 with edges 0,1,2,3 and 2 pi bins: the selected rp
+    triples (by hand, as the threeD compaction) have dense triple indices
+    0, 4, 8, 13, 14, 17, 26; each is followed by its pi bins (l, m) in the
+    order (0,0) (0,1) (1,0) (1,1), i.e. dense index 4T + 2l + m."""
+    edges = np.array([0.0, 1.0, 2.0, 3.0])
+    npi = 2
+    dense = np.arange(27 * npi * npi, dtype=np.float64)
+    want = [4 * t + u for t in (0, 4, 8, 13, 14, 17, 26) for u in range(4)]
+    got = oracle.compact_rppi(edges, npi, dense).tolist()
+    assert got == want
+    assert got[4:8] == [16, 17, 18, 19]  # the second rp triple (0,1,1), past the first rp bin
+    # planted mistake: pi bins transposed (m major) -> a different order
+    bad = [4 * t + 2 * m + l for t in (0, 4, 8, 13, 14, 17, 26) for l in range(2) for m in range(2)]
+    assert bad != got
