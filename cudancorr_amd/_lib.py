@@ -27,7 +27,7 @@ EXPORTS = (
     "nc_create", "nc_destroy", "nc_last_error", "nc_catalog_create", "nc_catalog_wrap_device",
     "nc_catalog_destroy", "nc_catalog_size", "nc_hist_size", "nc_compact_size", "nc_hist_len", "nc_count",
     "nc_count_host", "nc_count_batched", "nc_last_timing", "nc_set_tuning", "nc_get_tuning", "nc_autotune",
-    "nc_mixed_classify",
+    "nc_mixed_classify", "nc_count_noalloc", "nc_noalloc_capacity",
 )
 
 
@@ -97,6 +97,9 @@ def lib():
                               ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(NcTuning)]
     L.nc_mixed_classify.argtypes = [dp, ctypes.c_int, ctypes.POINTER(ctypes.c_float), i64,
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    L.nc_count_noalloc.argtypes = [vp, ctypes.POINTER(NcParams), vp, i64, vp, i64, vp, i64, ctypes.c_int, dp, up]
+    L.nc_noalloc_capacity.argtypes = []
+    L.nc_noalloc_capacity.restype = i64
     _LIB = L
     return L
 
@@ -255,6 +258,19 @@ class Context:
         self._check(lib().nc_count_host(self.h, ctypes.byref(p), _dp(a1), a1.shape[0], _dp(a2), a2.shape[0],
                                         _dp(a3), a3.shape[0], int(path), _dp(hist),
                                         cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return hist[:n].reshape(shape), cnt[:n].reshape(shape)
+
+    def count_noalloc(self, d1, n1, d2, n2, d3, n3, edges, box, survey=False, path=NC_PATH_PLAIN):
+        """nc_count_noalloc on device buffers of double4 points (integer
+        device addresses; equal address and size = one catalogue).  Returns
+        (hist, counts) shaped (nb,nb,nb)."""
+        p, e = self.params(edges, box, survey)
+        shape = self._shape(e.size - 1, 0)
+        n = int(np.prod(shape))
+        hist = np.zeros(max(n, 1), dtype=np.float64)
+        cnt = np.zeros(hist.size, dtype=np.uint64)
+        self._check(lib().nc_count_noalloc(self.h, ctypes.byref(p), d1, int(n1), d2, int(n2), d3, int(n3), int(path),
+                                           _dp(hist), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
         return hist[:n].reshape(shape), cnt[:n].reshape(shape)
 
     def count_batched(self, terms, edges, box, survey=False, path=NC_PATH_MIXED, npi=0, ori=(0.0, 0.0, 0.0)):

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1954,6 +1955,15 @@ template __global__ void k_triplets<true, kHistT3>(KArgs);
 template __global__ void k_triplets<true, kHistDense>(KArgs);
 
 // ---------------------------------------------------------------------------
+// Scratch of nc_count_noalloc: device memory of a fixed size that the library
+// holds on every GPU it runs on until the process ends, so that entry point
+// allocates nothing.  In host code `noalloc_scratch` names a host-side
+// stand-in, not device memory: the device address is looked up per device
+// with hipGetSymbolAddress (noalloc_base) and only that address is used.
+constexpr size_t kNoAllocBytes = (size_t)256 << 20;
+static __device__ __align__(256) char noalloc_scratch[kNoAllocBytes];
+
+// ---------------------------------------------------------------------------
 // host side
 
 namespace {
@@ -1979,6 +1989,36 @@ void dfree(DBuf &b) {
     b.p = nullptr;
     b.bytes = 0;
 }
+
+// Where a count takes its device buffers: grown with hipMalloc and kept
+// (the default), or carved from a fixed arena that is never freed
+// (nc_count_noalloc: the static per-device scratch array).  An arena request
+// past its end fails with `exhausted` set and allocates nothing.
+struct Mem {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+    bool exhausted = false;
+    bool arena() const { return base != nullptr; }
+    size_t left() const { return size - std::min(size, (used + 255) & ~(size_t)255); }
+    hipError_t grow(DBuf &b, size_t need) {
+        if (!base) return dgrow(b, need);
+        if (need == 0) need = 16;
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at > size || need > size - at) {
+            exhausted = true;
+            return hipErrorOutOfMemory;
+        }
+        b.p = base + at;
+        b.bytes = need;
+        used = at + need;
+        return hipSuccess;
+    }
+};
+
+// Device buffers of one count (per context, reused across counts).
+struct CountBufs {
+    DBuf tables, terms, chunks, out, counter, errw, scratch, bound, wflags;
+};
 
 struct Geometry {
     double box[3];
@@ -2045,7 +2085,9 @@ struct nc_context {
     int ncu = 0;
     size_t lds_max = 0;  // LDS budget of a workgroup with two per CU
     size_t lds_hw = 0;   // the per-workgroup maximum
-    DBuf tables, terms, chunks, out, counter, errw, scratch, bound, wflags;
+    CountBufs buf;  // device buffers of a count, kept at their largest size
+    char *noalloc = nullptr;  // this device's noalloc_scratch (device address), set by nc_create
+    std::string noalloc_err;  // why it is missing, if it is
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     nc_timing timing{};
     int waves_per_group = 8;
@@ -2083,6 +2125,17 @@ int fail(nc_context *ctx, int code, const std::string &msg) {
         if (e_ != hipSuccess)                                                                           \
             return fail(ctx, e_ == hipErrorOutOfMemory ? NC_ERR_NOMEM : NC_ERR_HIP,                     \
                         std::string(#call) + ": " + hipGetErrorString(e_));                             \
+    } while (0)
+
+// A buffer request: on the arena, running past its end is NC_ERR_LIMIT.
+#define NC_MEM(ctx, m, b, bytes)                                                                       \
+    do {                                                                                                \
+        const hipError_t e2_ = (m).grow((b), (bytes));                                                  \
+        if (e2_ != hipSuccess && (m).exhausted)                                                         \
+            return fail(ctx, NC_ERR_LIMIT,                                                              \
+                        "the count needs more device memory than the no-alloc scratch holds (" +         \
+                            std::to_string((m).size >> 20) + " MiB)");                                  \
+        NC_HIP(ctx, e2_);                                                                               \
     } while (0)
 
 int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
@@ -2289,17 +2342,17 @@ struct CatGrid {
     bool is_p1;
 };
 
-int build_grid(nc_context *ctx, const Geometry &G, nc_catalog *c, bool check_unclamped) {
+int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_catalog *c, bool check_unclamped) {
     const int n = (int)c->n;
     const size_t nc = (size_t)G.ncells;
-    NC_HIP(ctx, dgrow(c->key, sizeof(uint32_t) * (size_t)n));
-    NC_HIP(ctx, dgrow(c->rank, sizeof(uint32_t) * (size_t)n));
-    NC_HIP(ctx, dgrow(c->cnt, sizeof(uint32_t) * nc));
-    NC_HIP(ctx, dgrow(c->start, sizeof(uint32_t) * (nc + 1)));
-    NC_HIP(ctx, dgrow(c->x, sizeof(double) * (size_t)n));
-    NC_HIP(ctx, dgrow(c->y, sizeof(double) * (size_t)n));
-    NC_HIP(ctx, dgrow(c->z, sizeof(double) * (size_t)n));
-    NC_HIP(ctx, dgrow(c->w, sizeof(double) * (size_t)n));
+    NC_MEM(ctx, m, c->key, sizeof(uint32_t) * (size_t)n);
+    NC_MEM(ctx, m, c->rank, sizeof(uint32_t) * (size_t)n);
+    NC_MEM(ctx, m, c->cnt, sizeof(uint32_t) * nc);
+    NC_MEM(ctx, m, c->start, sizeof(uint32_t) * (nc + 1));
+    NC_MEM(ctx, m, c->x, sizeof(double) * (size_t)n);
+    NC_MEM(ctx, m, c->y, sizeof(double) * (size_t)n);
+    NC_MEM(ctx, m, c->z, sizeof(double) * (size_t)n);
+    NC_MEM(ctx, m, c->w, sizeof(double) * (size_t)n);
     GridGeo g;
     for (int a = 0; a < 3; ++a) {
         g.cs[a] = G.cs[a];
@@ -2312,7 +2365,7 @@ int build_grid(nc_context *ctx, const Geometry &G, nc_catalog *c, bool check_unc
         const int nb = (n + 255) / 256;
         hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n, g,
                            check_unclamped ? 1 : 0, (uint32_t *)c->key.p, (uint32_t *)c->rank.p, (uint32_t *)c->cnt.p,
-                           (unsigned *)ctx->errw.p);
+                           (unsigned *)B.errw.p);
     }
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)c->cnt.p, (uint32_t *)c->start.p,
                        (int)nc);
@@ -2417,9 +2470,9 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     NC_HIP(ctx, hipGraphLaunch(p.count_exec, ctx->stream));
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     const int nterms = (int)p.weighted.size();
-    NC_HIP(ctx, hipMemcpyAsync(p.h_out, ctx->out.p, sizeof(unsigned long long) * (size_t)nterms * p.nbc,
+    NC_HIP(ctx, hipMemcpyAsync(p.h_out, ctx->buf.out.p, sizeof(unsigned long long) * (size_t)nterms * p.nbc,
                                hipMemcpyDeviceToHost, ctx->stream));
-    NC_HIP(ctx, hipMemcpyAsync(p.h_err, ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(p.h_err, ctx->buf.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms_b = 0.f, ms_c = 0.f;
     NC_HIP(ctx, hipEventElapsedTime(&ms_b, ctx->ev[0], ctx->ev[1]));
@@ -2443,8 +2496,13 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
 }
 
 int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &terms, int path, double *hist,
-              uint64_t *counts, bool use_graph) {
+              uint64_t *counts, bool use_graph, Mem &m) {
     if (!ctx) return NC_ERR_ARG;
+    // buffers: the context's own, or (arena) fresh ones carved from it; an
+    // arena count is never captured
+    CountBufs arena_bufs;
+    CountBufs &bufs = m.arena() ? arena_bufs : ctx->buf;
+    if (m.arena()) use_graph = false;
     if (!P || !hist) return fail(ctx, NC_ERR_ARG, "null params or output");
     if (path != NC_PATH_PLAIN && path != NC_PATH_MIXED) return fail(ctx, NC_ERR_ARG, "unknown path");
     for (const Term &t : terms)
@@ -2590,12 +2648,12 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         return o + (hm == kHistGlobal ? rank_bytes : 0);  // the rank table ends the tables
     };
     size_t off_sq, off_lut, off_ctab, off_lut32, off_t3;
-    NC_HIP(ctx, dgrow(ctx->tables, tab_bytes));
-    NC_HIP(ctx, hipMemcpyAsync(ctx->tables.p, tabs.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
-    NC_HIP(ctx, dgrow(ctx->errw, 16));
-    NC_HIP(ctx, dgrow(ctx->counter, 16));
-    NC_HIP(ctx, dgrow(ctx->bound, 16));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->bound.p, 0u, 4LL);
+    NC_MEM(ctx, m, bufs.tables, tab_bytes);
+    NC_HIP(ctx, hipMemcpyAsync(bufs.tables.p, tabs.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+    NC_MEM(ctx, m, bufs.errw, 16);
+    NC_MEM(ctx, m, bufs.counter, 16);
+    NC_MEM(ctx, m, bufs.bound, 16);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.bound.p, 0u, 4LL);
     for (const Term &t : terms)
         for (nc_catalog *c : {t.c1, t.c2, t.c3})
             if (c->n > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
@@ -2616,9 +2674,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         }
     }
     auto enqueue_builds = [&]() -> int {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->errw.p, 0u, 4LL);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.errw.p, 0u, 4LL);
         for (CatGrid &g : grids) {
-            const int r = build_grid(ctx, G, g.cat, G.survey && g.is_p1);
+            const int r = build_grid(ctx, m, bufs, G, g.cat, G.survey && g.is_p1);
             if (r) return r;
         }
         return NC_OK;
@@ -2635,15 +2693,15 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const int cb = (int)((G.ncells + 255) / 256);
     for (const Term &t : terms) {
         hipLaunchKernelGGL(k_nbr_bound, dim3(cb), dim3(256), 0, ctx->stream, (const uint32_t *)t.c1->start.p,
-                           (const uint32_t *)t.c2->start.p, gg, (unsigned *)ctx->bound.p);
+                           (const uint32_t *)t.c2->start.p, gg, (unsigned *)bufs.bound.p);
         if (t.c3 != t.c2)
             hipLaunchKernelGGL(k_nbr_bound, dim3(cb), dim3(256), 0, ctx->stream, (const uint32_t *)t.c1->start.p,
-                               (const uint32_t *)t.c3->start.p, gg, (unsigned *)ctx->bound.p);
+                               (const uint32_t *)t.c3->start.p, gg, (unsigned *)bufs.bound.p);
     }
     NC_HIP(ctx, hipGetLastError());
     unsigned host_flags[2] = {0, 0};
-    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    NC_HIP(ctx, hipMemcpyAsync(&host_flags[1], ctx->bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], bufs.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(&host_flags[1], bufs.bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     const int cap = (int)((std::max(1u, host_flags[1]) + 15u) & ~15u);
@@ -2681,16 +2739,16 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         B.r2min = G.sq[0];
         B.r2max = G.sq[G.nb];
         B.cap = 0;  // count only
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->bound.p, 0u, 1LL);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.bound.p, 0u, 1LL);
         for (const Term &t : terms) {
             const TermDesc d = term_desc(t);
             if (d.n1 == 0 || t.c2->n == 0 || t.c3->n == 0) continue;
             const int grid = (int)std::min<long long>((d.n1 + 3) / 4, (long long)ctx->ncu * 32);
-            hipLaunchKernelGGL(k_list_bound, dim3(grid), dim3(256), 0, ctx->stream, B, d, (unsigned *)ctx->bound.p);
+            hipLaunchKernelGGL(k_list_bound, dim3(grid), dim3(256), 0, ctx->stream, B, d, (unsigned *)bufs.bound.p);
         }
         NC_HIP(ctx, hipGetLastError());
         unsigned lb = 0;
-        NC_HIP(ctx, hipMemcpyAsync(&lb, ctx->bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        NC_HIP(ctx, hipMemcpyAsync(&lb, bufs.bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
         NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
         nmax = std::min(cap, (int)std::max(1u, lb));
     }
@@ -2748,19 +2806,6 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
                                                std::to_string(fa.maxThreadsPerBlock / kWave) +
                                                " waves this kernel variant is built for");
     }
-    int per_cu = ctx->wgs_per_cu;
-    if (per_cu <= 0) {
-        int occ = 0;
-        NC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, block, lds_bytes));
-        per_cu = std::max(1, occ);
-    }
-    const int grid = ctx->ncu * per_cu;
-    const long long nwaves = (long long)grid * W;
-    const long long slot_bytes = (long long)((2 * (size_t)cap * list_entry_bytes(lmode) + 255) & ~(size_t)255);
-    if ((double)nwaves * (double)slot_bytes > 64.0 * (1ULL << 30))
-        return fail(ctx, NC_ERR_LIMIT, "neighbour lists need more than 64 GiB of scratch");
-    NC_HIP(ctx, dgrow(ctx->scratch, (size_t)(nwaves * slot_bytes)));
-
     // --- terms and work queue
     std::vector<TermDesc> td(nterms);
     std::vector<Chunk> chunks;
@@ -2775,24 +2820,47 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         for (int b = 0; b < d.n1; b += ctx->chunk, ++nitems)
             if (nitems % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(d.n1, b + ctx->chunk), 0});
     }
-    NC_HIP(ctx, dgrow(ctx->terms, sizeof(TermDesc) * td.size()));
-    NC_HIP(ctx, dgrow(ctx->chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size())));
-    NC_HIP(ctx, dgrow(ctx->out, sizeof(unsigned long long) * (size_t)nterms * G.out_words));
+    NC_MEM(ctx, m, bufs.terms, sizeof(TermDesc) * td.size());
+    NC_MEM(ctx, m, bufs.chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size()));
+    NC_MEM(ctx, m, bufs.out, sizeof(unsigned long long) * (size_t)nterms * G.out_words);
     if (!td.empty())
-        NC_HIP(ctx, hipMemcpyAsync(ctx->terms.p, td.data(), sizeof(TermDesc) * td.size(), hipMemcpyHostToDevice,
+        NC_HIP(ctx, hipMemcpyAsync(bufs.terms.p, td.data(), sizeof(TermDesc) * td.size(), hipMemcpyHostToDevice,
                                    ctx->stream));
     if (!chunks.empty())
-        NC_HIP(ctx, hipMemcpyAsync(ctx->chunks.p, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice,
+        NC_HIP(ctx, hipMemcpyAsync(bufs.chunks.p, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice,
                                    ctx->stream));
+
+    // --- persistent grid and the per-wave list scratch (on an arena: as many
+    // workgroups as the rest of it holds lists for)
+    int per_cu = ctx->wgs_per_cu;
+    if (per_cu <= 0) {
+        int occ = 0;
+        NC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, block, lds_bytes));
+        per_cu = std::max(1, occ);
+    }
+    int grid = ctx->ncu * per_cu;
+    const long long slot_bytes = (long long)((2 * (size_t)cap * list_entry_bytes(lmode) + 255) & ~(size_t)255);
+    if (m.arena()) {
+        const long long fit = (long long)(m.left() / (size_t)slot_bytes) / W;
+        if (fit < 1)
+            return fail(ctx, NC_ERR_LIMIT, "the neighbour lists of one workgroup do not fit in the no-alloc scratch (" +
+                                               std::to_string(m.size >> 20) + " MiB)");
+        grid = (int)std::min<long long>(grid, fit);
+    }
+    const long long nwaves = (long long)grid * W;
+    if ((double)nwaves * (double)slot_bytes > 64.0 * (1ULL << 30))
+        return fail(ctx, NC_ERR_LIMIT, "neighbour lists need more than 64 GiB of scratch");
+    NC_MEM(ctx, m, bufs.scratch, (size_t)(nwaves * slot_bytes));
+
     KArgs A;
     std::memset(&A, 0, sizeof(A));
-    A.terms = (const TermDesc *)ctx->terms.p;
-    A.chunks = (const Chunk *)ctx->chunks.p;
+    A.terms = (const TermDesc *)bufs.terms.p;
+    A.chunks = (const Chunk *)bufs.chunks.p;
     A.nchunks = (int)chunks.size();
-    A.counter = (unsigned *)ctx->counter.p;
-    A.err = (unsigned *)ctx->errw.p;
-    A.out = (unsigned long long *)ctx->out.p;
-    A.scratch = (char *)ctx->scratch.p;
+    A.counter = (unsigned *)bufs.counter.p;
+    A.err = (unsigned *)bufs.errw.p;
+    A.out = (unsigned long long *)bufs.out.p;
+    A.scratch = (char *)bufs.scratch.p;
     A.slot_bytes = slot_bytes;
     A.cap = cap;
     A.nmax = hm == kHistDense ? nmax : cap;
@@ -2813,11 +2881,11 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.ori_x = G.ori_x;
     A.lut_scale = G.lut_scale;
     A.lut_m = G.lut_m;
-    A.sq = (const double *)((char *)ctx->tables.p + g_sq);
-    A.lut = (const uint16_t *)((char *)ctx->tables.p + g_lut);
-    A.ctab = (const int *)((char *)ctx->tables.p + g_ctab);
-    A.lut32 = (const uint2 *)((char *)ctx->tables.p + g_lut32);
-    A.t3 = (const uint16_t *)((char *)ctx->tables.p + g_t3);
+    A.sq = (const double *)((char *)bufs.tables.p + g_sq);
+    A.lut = (const uint16_t *)((char *)bufs.tables.p + g_lut);
+    A.ctab = (const int *)((char *)bufs.tables.p + g_ctab);
+    A.lut32 = (const uint2 *)((char *)bufs.tables.p + g_lut32);
+    A.t3 = (const uint16_t *)((char *)bufs.tables.p + g_t3);
     A.sigma32 = G.sigma32;
     A.lut32_lo = G.lut32_lo;
     A.lut32_shift = G.lut32_shift;
@@ -2893,7 +2961,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             As.rs_dry = 1;
             As.rs_cursor = (unsigned long long *)ctx->rs_cursor.p;
             As.rs_chunk_keys = (unsigned long long *)ctx->rs_chunk_keys.p;
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 4LL);
             hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, As.rs_cursor, 0ull, (long long)rs_nbk);
             hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, As.rs_chunk_keys, 0ull,
                                (long long)sample.size());
@@ -3018,7 +3086,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     auto round_args = [&](int r) {
         KArgs Ar = A;
         const Round &rd = rounds[r];
-        Ar.chunks = (const Chunk *)ctx->chunks.p + rd.first;
+        Ar.chunks = (const Chunk *)bufs.chunks.p + rd.first;
         Ar.nchunks = rd.n;
         Ar.rs_base = (const unsigned long long *)ctx->rs_base.p + (size_t)r * (rs_nbk + 1);
         Ar.rs_cursor = (unsigned long long *)ctx->rs_cursor.p + (size_t)r * rs_nbk;
@@ -3029,9 +3097,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const size_t bin_lds = (size_t)1 << rs_maxshift;
 
     auto enqueue_count = [&]() -> int {
-        hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->out.p, 0ull,
+        hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)bufs.out.p, 0ull,
                            (long long)nterms * G.out_words);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u, 4LL);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 4LL);
         if (A.nchunks > 0 && !stream) {
             void *args[] = {&A};
             if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
@@ -3044,7 +3112,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
                 KArgs Ar = round_args(r);
                 void *args[] = {&Ar};
                 if (r > 0)
-                    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->counter.p, 0u,
+                    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u,
                                        4LL);
                 if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
                     return NC_ERR_HIP;
@@ -3058,9 +3126,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (enqueue_count() != NC_OK) return fail(ctx, NC_ERR_HIP, "triplet kernel launch failed");
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     std::vector<unsigned long long> res((size_t)nterms * G.out_words);
-    NC_HIP(ctx, hipMemcpyAsync(res.data(), ctx->out.p, sizeof(unsigned long long) * res.size(), hipMemcpyDeviceToHost,
+    NC_HIP(ctx, hipMemcpyAsync(res.data(), bufs.out.p, sizeof(unsigned long long) * res.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
-    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], bufs.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms_b = 0.f, ms_c = 0.f;
     NC_HIP(ctx, hipEventElapsedTime(&ms_b, ctx->ev[0], ctx->ev[1]));
@@ -3146,6 +3214,23 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 // ---------------------------------------------------------------------------
 // C-ABI
 
+// Device address of noalloc_scratch on `device` (current): looked up once
+// per device with hipGetSymbolAddress and kept; each GPU holds its own copy.
+static std::mutex g_noalloc_mu;
+static std::vector<char *> g_noalloc_ptr;
+static hipError_t noalloc_base(int device, char **out) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if ((size_t)device >= g_noalloc_ptr.size()) g_noalloc_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_noalloc_ptr[(size_t)device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[(size_t)device] = static_cast<char *>(p);
+    }
+    *out = g_noalloc_ptr[(size_t)device];
+    return hipSuccess;
+}
+
 // The points of a catalogue that wraps caller-owned device memory
 // (nc_catalog_wrap_device) may be overwritten between counts, weights
 // included.  Whether all weights are 1 decides the histogram layout, the
@@ -3153,24 +3238,26 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 // from the current contents before every count over such catalogues (one
 // read of their weights and one synchronisation).  A captured plan keys on
 // these flags, so a changed weight status is never replayed.
-int refresh_weights(nc_context *ctx, const std::vector<Term> &terms) {
+int refresh_weights(nc_context *ctx, Mem &m, const std::vector<Term> &terms) {
     std::vector<nc_catalog *> w;
     for (const Term &t : terms)
         for (nc_catalog *c : {t.c1, t.c2, t.c3})
             if (c && !c->owns && c->n > 0 && std::find(w.begin(), w.end(), c) == w.end()) w.push_back(c);
     if (w.empty()) return NC_OK;
     NC_HIP(ctx, hipSetDevice(ctx->device));
-    NC_HIP(ctx, dgrow(ctx->wflags, sizeof(unsigned) * w.size()));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, (uint32_t *)ctx->wflags.p, 0u,
+    DBuf arena_flags;
+    DBuf &fl = m.arena() ? arena_flags : ctx->buf.wflags;
+    NC_MEM(ctx, m, fl, sizeof(unsigned) * w.size());
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, (uint32_t *)fl.p, 0u,
                        (long long)w.size());
     for (size_t i = 0; i < w.size(); ++i) {
         const int grid = (int)std::min<long long>((w[i]->n + 255) / 256, 1024);
         hipLaunchKernelGGL(k_unit_weight, dim3(grid), dim3(256), 0, ctx->stream, (const double4 *)w[i]->d_raw,
-                           (long long)w[i]->n, (unsigned *)ctx->wflags.p + i);
+                           (long long)w[i]->n, (unsigned *)fl.p + i);
     }
     NC_HIP(ctx, hipGetLastError());
     std::vector<unsigned> f(w.size());
-    NC_HIP(ctx, hipMemcpyAsync(f.data(), ctx->wflags.p, sizeof(unsigned) * f.size(), hipMemcpyDeviceToHost,
+    NC_HIP(ctx, hipMemcpyAsync(f.data(), fl.p, sizeof(unsigned) * f.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i < w.size(); ++i) w[i]->unit_w = f[i] == 0u;
@@ -3185,7 +3272,8 @@ int refresh_weights(nc_context *ctx, const std::vector<Term> &terms) {
 int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
                 uint64_t *counts, bool use_graph) {
     const int nterms = (int)t.size();
-    if (const int rc = refresh_weights(ctx, t)) return rc;
+    Mem m;  // the context's own buffers
+    if (const int rc = refresh_weights(ctx, m, t)) return rc;
     if (ctx->batch_split && path == NC_PATH_MIXED && params && params->bin_type == NC_BIN_THREED &&
         params->nedges >= 2 && params->nedges <= 41) {
         std::vector<int> iu, iw;
@@ -3206,7 +3294,8 @@ int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term
                 for (int i : idx) sub.push_back(t[i]);
                 h[g].resize(idx.size() * len);
                 if (counts) c[g].resize(idx.size() * len);
-                const int rc = run_count(ctx, params, sub, path, h[g].data(), counts ? c[g].data() : nullptr, false);
+                const int rc =
+                    run_count(ctx, params, sub, path, h[g].data(), counts ? c[g].data() : nullptr, false, m);
                 if (rc) return rc;
                 tm[g] = ctx->timing;
                 for (size_t k = 0; k < idx.size(); ++k) {
@@ -3226,7 +3315,7 @@ int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term
             return NC_OK;
         }
     }
-    return run_count(ctx, params, t, path, hist, counts, use_graph);
+    return run_count(ctx, params, t, path, hist, counts, use_graph, m);
 }
 
 extern "C" {
@@ -3244,6 +3333,10 @@ int nc_create(nc_context **out, int device) {
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return NC_ERR_HIP;
+    }
+    {
+        const hipError_t e = noalloc_base(device, &ctx->noalloc);
+        if (e != hipSuccess) ctx->noalloc_err = std::string("hipGetSymbolAddress(noalloc_scratch): ") + hipGetErrorString(e);
     }
     ctx->ncu = prop.multiProcessorCount;
     ctx->dev_name = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
@@ -3277,8 +3370,8 @@ int nc_destroy(nc_context *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     plan_drop(ctx);
-    for (DBuf *b : {&ctx->tables, &ctx->terms, &ctx->chunks, &ctx->out, &ctx->counter, &ctx->errw, &ctx->scratch,
-                    &ctx->bound, &ctx->wflags, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
+    for (DBuf *b : {&ctx->buf.tables, &ctx->buf.terms, &ctx->buf.chunks, &ctx->buf.out, &ctx->buf.counter,
+                    &ctx->buf.errw, &ctx->buf.scratch, &ctx->buf.bound, &ctx->buf.wflags, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
                     &ctx->rs_chunk_keys, &ctx->rs_keys, &ctx->rs_vals})
         dfree(*b);
     for (auto &e : ctx->ev)
@@ -3347,15 +3440,15 @@ int nc_catalog_wrap_device(nc_context *ctx, const double *d_xyzw, int64_t n, nc_
     c->owns = false;
     c->d_raw = d_xyzw;
     if (n > 0) {
-        hipError_t e = dgrow(ctx->errw, 16);
+        hipError_t e = dgrow(ctx->buf.errw, 16);
         unsigned flag = 0;
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->errw.p, 0u, 4LL);
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->buf.errw.p, 0u, 4LL);
             hipLaunchKernelGGL(k_unit_weight, dim3(1024), dim3(256), 0, ctx->stream, (const double4 *)d_xyzw,
-                               (long long)n, (unsigned *)ctx->errw.p);
+                               (long long)n, (unsigned *)ctx->buf.errw.p);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(&flag, ctx->errw.p, 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&flag, ctx->buf.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
             delete c;
@@ -3392,8 +3485,9 @@ int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalo
              double *hist, uint64_t *counts) {
     if (!ctx) return NC_ERR_ARG;
     std::vector<Term> t{{c1, c2, c3}};
-    if (const int rc = refresh_weights(ctx, t)) return rc;
-    return run_count(ctx, params, t, path, hist, counts, false);
+    Mem m;
+    if (const int rc = refresh_weights(ctx, m, t)) return rc;
+    return run_count(ctx, params, t, path, hist, counts, false, m);
 }
 
 int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, int64_t n1, const double *p2, int64_t n2,
@@ -3423,6 +3517,48 @@ int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, in
     if (c[0]) nc_catalog_destroy(c[0]);
     return rc;
 }
+
+int nc_count_noalloc(nc_context *ctx, const nc_params *params, const double *d_p1, int64_t n1, const double *d_p2,
+                     int64_t n2, const double *d_p3, int64_t n3, int path, double *hist, uint64_t *counts) {
+    if (!ctx) return NC_ERR_ARG;
+    if (!ctx->noalloc) return fail(ctx, NC_ERR_HIP, "no-alloc scratch unavailable: " + ctx->noalloc_err);
+    if (!params || params->bin_type != NC_BIN_THREED)
+        return fail(ctx, NC_ERR_ARG, "nc_count_noalloc counts threeD bins only");
+    const double *d[3] = {d_p1, d_p2, d_p3};
+    const int64_t n[3] = {n1, n2, n3};
+    for (int i = 0; i < 3; ++i) {
+        if (n[i] < 0 || (n[i] > 0 && !d[i])) return fail(ctx, NC_ERR_ARG, "bad catalogue");
+        if (n[i] > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
+    }
+    // catalogues wrapping the caller's points, on the stack (never registered
+    // with the context, never freed: their buffers live in the arena); the
+    // same pointer and size passed twice is one catalogue, as in nc_count_host
+    nc_catalog cat[3];
+    nc_catalog *c[3];
+    for (int i = 0; i < 3; ++i) {
+        c[i] = &cat[i];
+        for (int j = 0; j < i; ++j)
+            if (d[j] == d[i] && n[j] == n[i]) {
+                c[i] = c[j];
+                break;
+            }
+        if (c[i] != &cat[i]) continue;
+        cat[i].ctx = ctx;
+        cat[i].device = ctx->device;
+        cat[i].n = n[i];
+        cat[i].d_raw = d[i];
+        cat[i].owns = false;
+    }
+    Mem m;
+    m.base = ctx->noalloc;
+    m.size = kNoAllocBytes;
+    NC_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<Term> t{{c[0], c[1], c[2]}};
+    if (const int rc = refresh_weights(ctx, m, t)) return rc;
+    return run_count(ctx, params, t, path, hist, counts, false, m);
+}
+
+int64_t nc_noalloc_capacity(void) { return (int64_t)kNoAllocBytes; }
 
 int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1,
                      nc_catalog *const *c2, nc_catalog *const *c3, int path, double *hist, uint64_t *counts) {

@@ -146,6 +146,26 @@ int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalo
 int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, int64_t n1, const double *p2,
                   int64_t n2, const double *p3, int64_t n3, int path, double *hist, uint64_t *counts);
 
+/* Allocation-free form of nc_count.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The caller
+ * passes three device buffers of n double4 points (d_p1 == d_p2 with n1 == n2
+ * is one catalogue, as in nc_count_host) and host outputs as nc_count; threeD
+ * bins only.  Every device buffer of the count (cell lists, tables, work
+ * queue, histogram, neighbour lists) is carved from a fixed scratch array of
+ * nc_noalloc_capacity() bytes (256 MiB) that the library holds on each GPU it
+ * runs on until the process ends; the call makes no hipMalloc/hipFree and no
+ * other allocating device call.  A count that needs more than the scratch
+ * fails with NC_ERR_LIMIT (about 100 B per point plus 8 B per cell, then
+ * neighbour lists for as many persistent workgroups as the rest holds).
+ * All calls on one GPU share its scratch: calls that overlap on one GPU (from
+ * several contexts or threads) race; serialise them.  Never captured in a
+ * graph by the library. */
+int nc_count_noalloc(nc_context *ctx, const nc_params *params, const double *d_p1, int64_t n1, const double *d_p2,
+                     int64_t n2, const double *d_p3, int64_t n3, int path, double *hist, uint64_t *counts);
+/* Bytes of the scratch behind nc_count_noalloc. */
+int64_t nc_noalloc_capacity(void);
+
 /* Batched count (no counterpart in the reference, which makes one
  * calculateD1D2D3 call per term and per catalogue, main.cu:78-95): nterms
  * terms (c1[t], c2[t], c3[t]) with shared params run in one persistent launch
