@@ -146,12 +146,24 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long lon
         p[i] = v;
 }
 
+// c is the exact floor of v / cs: c*cs <= v < (c+1)*cs in exact arithmetic
+// (each FMA rounds v - c*cs once, which keeps its sign).  Fails for the
+// clamped index of a point at or past n*cs and for -cs < v < 0, which
+// truncation puts in cell 0.
+__device__ __forceinline__ bool exact_floor(double v, double cs, int c) {
+    return __builtin_fma(-(double)c, cs, v) >= 0.0 && __builtin_fma(-(double)(c + 1), cs, v) < 0.0;
+}
+
 // Cell key of every point (host clamp rule, cudaNcorr.cu:50-58) and its slot in
 // the cell.  Survey primaries are also checked without the clamp, as the
 // reference's device index does not clamp them (device.cu:84-87).
+// *inexact gets 1 when some point's cell index is not the exact floor of its
+// coordinate over the cell size (then no pair closer than a cell size is
+// guaranteed to lie in adjacent cells); with expect_exact the plan relied on
+// it, and err gets kErrReplan.
 __global__ void k_cell_keys(const double4 *__restrict__ raw, int n, GridGeo g, int check_unclamped,
                             uint32_t *__restrict__ key, uint32_t *__restrict__ rank, uint32_t *__restrict__ cnt,
-                            unsigned *__restrict__ err) {
+                            unsigned *__restrict__ err, unsigned *__restrict__ inexact, int expect_exact) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double4 p = raw[i];
@@ -159,6 +171,12 @@ __global__ void k_cell_keys(const double4 *__restrict__ raw, int n, GridGeo g, i
     int cx = axis_cell(p.x, g.cs[0], g.nc[0], true, bad);
     int cy = axis_cell(p.y, g.cs[1], g.nc[1], true, bad);
     int cz = axis_cell(p.z, g.cs[2], g.nc[2], true, bad);
+    const bool exact =
+        exact_floor(p.x, g.cs[0], cx) && exact_floor(p.y, g.cs[1], cy) && exact_floor(p.z, g.cs[2], cz);
+    if (__ballot(!exact) != 0ull && (threadIdx.x & (kWave - 1)) == 0) {  // lane 0 is active whenever any lane is
+        atomicOr(inexact, 1u);
+        if (expect_exact) atomicOr(err, kErrReplan);
+    }
     if (check_unclamped) {
         axis_cell(p.x, g.cs[0], g.nc[0], false, bad);
         axis_cell(p.y, g.cs[1], g.nc[1], false, bad);
@@ -329,13 +347,27 @@ struct CatRef {
 // 64-point blocks per step (both loads in flight together); the list order is
 // the reference's entry order either way.  Each list entry records a
 // neighbour entry s whose shift is its own (the middle one for a merged range).
-template <int LM>
+//
+// Selections (SEL) for the six-fold path of terms over one catalogue, where
+// pidx is the primary's index in the sorted catalogue and an entry is "last"
+// when its cell is the last one along some axis (box mode; surveys have
+// none):
+//   kSelAll    every entry into base;
+//   kSelSplit  entries that are not last and have index > pidx into base,
+//              last entries into base_b (*nb_out of them), and *nlo_out =
+//              the number of the remaining ones (not last, index < pidx);
+//   kSelLow    those remaining ones, into base from position `first` on.
+constexpr int kSelAll = 0, kSelSplit = 1, kSelLow = 2;
+
+template <int LM, int SEL = kSelAll>
 __device__ int build_list(const KArgs &A, const double *__restrict__ X, const double *__restrict__ Y,
                           const double *__restrict__ Z, const double *__restrict__ Wt, const uint32_t *__restrict__ st,
                           double px, double py, double pz, int cx, int cy, int cz, char *__restrict__ base, int lane,
-                          const double *sq, const uint16_t *lut) {
+                          const double *sq, const uint16_t *lut, int pidx = 0, char *__restrict__ base_b = nullptr,
+                          int *nb_out = nullptr, int *nlo_out = nullptr, int first = 0) {
     constexpr bool MIXED = LM == kListMixed;
-    int n = 0;
+    int n = SEL == kSelLow ? first : 0;
+    int nbl = 0, nlo = 0;
     // rppi survey: every triplet's line of sight L = 3a + d2 + d3 (a = p1 - ori,
     // |d2|, |d3| < smax) is within the angle th = asin(2 smax / 3|a|) of a, so
     // a side d of the list has, in any triplet, pi >= |ah.d| - th|d| and
@@ -357,7 +389,29 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
         }
     }
     const double pim = sqrt(A.pimax2) * (1.0 + 1e-9), r2m = A.r2max * (1.0 + 1e-9);
-    auto keep = [&](int j, bool v, double qx, double qy, double qz, double qw, int s) {
+    // one threeD list entry (exact bin k and sqrtf of its side) at position pos of the list at b
+    auto store = [&](char *b, int pos, int j, double d, double qx, double qy, double qz, double qw, int s) {
+        const int k = bin_of(d, A.lut_scale, lut, sq, A.nb);
+        const float f = __builtin_sqrtf((float)d);
+        if constexpr (MIXED) {
+            const MList ml = mlist(b, A.cap);
+            const double sg = A.sigma32;  // power of 2: the scaling is exact
+            const float rx = (float)((qx - px) * sg), ry = (float)((qy - py) * sg), rz = (float)((qz - pz) * sg);
+            ml.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
+            ml.KF[pos] = make_int2(k | (s << 8), __float_as_int(f));
+            ml.I[pos] = j;
+        } else {
+            Elem e;
+            e.x = qx;
+            e.y = qy;
+            e.z = qz;
+            e.k = k;
+            e.f = f;
+            reinterpret_cast<Elem *>(b)[pos] = e;
+            reinterpret_cast<double *>(b + (size_t)A.cap * sizeof(Elem))[pos] = qw;
+        }
+    };
+    auto keep = [&](int j, bool v, double qx, double qy, double qz, double qw, int s, bool last) {
         if constexpr (LM == kListRppiBox || LM == kListRppiSurvey) {
             // rppi: box keeps pi^2 < piMax^2 and rpMin^2 < rp^2 < rpMax^2
             // (device.cu:343-346), survey 0 < s^2 < sqrRMax (device.cu:454)
@@ -391,34 +445,32 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
             return;
         }
         const double d = sqrsep(px - qx, py - qy, pz - qz);
-        const bool pass = v & (d > A.r2min) & (d < A.r2max);
-        const unsigned long long m = __ballot(pass);
-        const int pos = n + lane_rank(m);
-        if (pass & (pos < A.cap)) {
-            const int k = bin_of(d, A.lut_scale, lut, sq, A.nb);
-            const float f = __builtin_sqrtf((float)d);
-            if constexpr (MIXED) {
-                const MList ml = mlist(base, A.cap);
-                const double sg = A.sigma32;  // power of 2: the scaling is exact
-                const float rx = (float)((qx - px) * sg), ry = (float)((qy - py) * sg), rz = (float)((qz - pz) * sg);
-                ml.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
-                ml.KF[pos] = make_int2(k | (s << 8), __float_as_int(f));
-                ml.I[pos] = j;
-            } else {
-                Elem e;
-                e.x = qx;
-                e.y = qy;
-                e.z = qz;
-                e.k = k;
-                e.f = f;
-                reinterpret_cast<Elem *>(base)[pos] = e;
-                reinterpret_cast<double *>(base + (size_t)A.cap * sizeof(Elem))[pos] = qw;
-            }
+        const bool in = v & (d > A.r2min) & (d < A.r2max);
+        if constexpr (SEL == kSelAll) {
+            const unsigned long long m = __ballot(in);
+            const int pos = n + lane_rank(m);
+            if (in & (pos < A.cap)) store(base, pos, j, d, qx, qy, qz, qw, s);
+            n += __popcll(m);
+        } else if constexpr (SEL == kSelSplit) {
+            const bool ph = in & !last & (j > pidx), pb = in & last;
+            const unsigned long long mh = __ballot(ph), mb = __ballot(pb);
+            const int posh = n + lane_rank(mh), posb = nbl + lane_rank(mb);
+            if (ph & (posh < A.cap)) store(base, posh, j, d, qx, qy, qz, qw, s);
+            if (pb & (posb < A.cap)) store(base_b, posb, j, d, qx, qy, qz, qw, s);
+            n += __popcll(mh);
+            nbl += __popcll(mb);
+            nlo += __popcll(__ballot(in & !last & (j < pidx)));
+        } else {
+            const bool pl = in & !last & (j < pidx);
+            const unsigned long long m = __ballot(pl);
+            const int pos = n + lane_rank(m);
+            if (pl & (pos < A.cap)) store(base, pos, j, d, qx, qy, qz, qw, s);
+            n += __popcll(m);
         }
-        n += __popcll(m);
     };
-    // points [beg, end) of the sorted catalogue, image shift (shx, shy, shz)
-    auto scan = [&](int beg, int end, double shx, double shy, double shz, int s) {
+    // points [beg, end) of the sorted catalogue, image shift (shx, shy, shz);
+    // an entry is last when the range is (last_all) or from index jl on
+    auto scan = [&](int beg, int end, double shx, double shy, double shz, int s, bool last_all, int jl) {
         for (int j0 = beg; j0 < end; j0 += 2 * kWave) {
             const int ja = j0 + lane, jb = ja + kWave;
             const bool va = ja < end, vb = jb < end;
@@ -438,10 +490,11 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                 bz = Z[jb] + shz;
                 if (!MIXED) bw = Wt[jb];
             }
-            keep(ja, va, ax, ay, az, aw, s);
-            if (j0 + kWave < end) keep(jb, vb, bx, by, bz, bw, s);
+            keep(ja, va, ax, ay, az, aw, s, last_all | (ja >= jl));
+            if (j0 + kWave < end) keep(jb, vb, bx, by, bz, bw, s, last_all | (jb >= jl));
         }
     };
+    constexpr int kNoLast = 0x7fffffff;
     const int nz = A.nc[2];
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy) {
@@ -452,7 +505,8 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                 if (ex < 0 || ex >= A.nc[0] || ey < 0 || ey >= A.nc[1]) continue;
                 const int row = nz * (ey + A.nc[1] * ex);
                 const int z0 = max(cz - 1, 0), z1 = min(cz + 1, nz - 1);
-                scan(uniform((int)st[row + z0]), uniform((int)st[row + z1 + 1]), 0.0, 0.0, 0.0, s0 + 1);
+                scan(uniform((int)st[row + z0]), uniform((int)st[row + z1 + 1]), 0.0, 0.0, 0.0, s0 + 1, false,
+                     kNoLast);
                 continue;
             }
             if (ex == A.nc[0]) { ex = 0; shx = A.box[0]; }
@@ -460,8 +514,12 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
             if (ex == -1) { ex = A.nc[0] - 1; shx = -A.box[0]; }
             if (ey == -1) { ey = A.nc[1] - 1; shy = -A.box[1]; }
             const int row = nz * (ey + A.nc[1] * ex);
+            const bool lastxy = ex == A.nc[0] - 1 || ey == A.nc[1] - 1;
             if (cz >= 1 && cz + 1 < nz) {
-                scan(uniform((int)st[row + cz - 1]), uniform((int)st[row + cz + 2]), shx, shy, 0.0, s0 + 1);
+                // the merged range reaches the last z cell when cz + 1 is it
+                const int jl = cz + 2 == nz ? uniform((int)st[row + nz - 1]) : kNoLast;
+                scan(uniform((int)st[row + cz - 1]), uniform((int)st[row + cz + 2]), shx, shy, 0.0, s0 + 1, lastxy,
+                     jl);
                 continue;
             }
             for (int dz = -1; dz <= 1; ++dz) {  // a z wrap: one cell at a time
@@ -469,9 +527,14 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                 double shz = 0.0;
                 if (ez == nz) { ez = 0; shz = A.box[2]; }
                 if (ez == -1) { ez = nz - 1; shz = -A.box[2]; }
-                scan(uniform((int)st[row + ez]), uniform((int)st[row + ez + 1]), shx, shy, shz, s0 + dz + 1);
+                scan(uniform((int)st[row + ez]), uniform((int)st[row + ez + 1]), shx, shy, shz, s0 + dz + 1,
+                     lastxy || ez == nz - 1, kNoLast);
             }
         }
+    if constexpr (SEL == kSelSplit) {
+        *nb_out = nbl;
+        *nlo_out = nlo;
+    }
     return n;
 }
 
@@ -589,18 +652,21 @@ __device__ __forceinline__ Sides sort3(float a, float b, float c) {
 }
 
 // Histogram update of one accepted pair at a precomputed slot (byte offset).
+// inc: ordered triplets the pair stands for (1; 2 with SYM; 6 on the six-fold
+// path); weighted sums carry that factor in the primary's weight instead.
 template <bool SYM, bool WEIGHTED, int HM>
-__device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1a, double wa, double w1b, double wb) {
+__device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1a, double wa, double w1b, double wb,
+                                         unsigned inc) {
     char *h = reinterpret_cast<char *>(t.hist) + off;
     if constexpr (HM == kHistDense) {
         static_assert(!WEIGHTED, "dense counts are for unit weights");
-        atomicAdd(reinterpret_cast<unsigned *>(h), SYM ? 2u : 1u);
+        atomicAdd(reinterpret_cast<unsigned *>(h), inc);
     } else if constexpr (WEIGHTED) {
         double v = w1a * wb;  // (w1*w2)*w3, device.cu:305
         if (SYM) v += w1b * wa;
         atomicAdd(reinterpret_cast<double *>(h), v);
     } else {
-        atomicAdd(reinterpret_cast<unsigned long long *>(h), SYM ? 2ull : 1ull);
+        atomicAdd(reinterpret_cast<unsigned long long *>(h), (unsigned long long)inc);
     }
 }
 
@@ -625,7 +691,7 @@ __device__ __forceinline__ int bin_any(const KArgs &A, const LdsTabs &t, double 
 template <bool SYM, bool WEIGHTED, int HM>
 __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, double ax, double ay, double az, int ka,
                                            float fa, double w1a, double wa, double bx, double by, double bz, int kb,
-                                           float fb, double w1b, double wb, bool ok) {
+                                           float fb, double w1b, double wb, bool ok, unsigned inc) {
     const double d = sqrsep(ax - bx, ay - by, az - bz);
     if (ok && d > A.r2min && d < A.r2max) {
         const int k = bin_any<false>(A, t, d);
@@ -634,7 +700,7 @@ __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, dou
         const float hi = fmaxf(fmaxf(fa, fb), f);
         const float mid = __builtin_amdgcn_fmed3f(fa, fb, f);
         if (lo + mid > hi)  // the reference rejects sqrtf(r1)+sqrtf(r2) <= sqrtf(r3), device.cu:173
-            hist_add<SYM, WEIGHTED, HM>(t, sorted_slot<HM>(A, t, ka, kb, k), w1a, wa, w1b, wb);
+            hist_add<SYM, WEIGHTED, HM>(t, sorted_slot<HM>(A, t, ka, kb, k), w1a, wa, w1b, wb, inc);
     }
 }
 
@@ -717,10 +783,11 @@ struct PlainPol {
     Elem eb;
     double wb, w1b;
     bool vb;
+    unsigned inc;
 
     __device__ PlainPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2, const char *l3, char *wlds,
-                        int lane_)
-        : A(A_), t(t_), w1(w1_), lane(lane_) {
+                        int lane_, unsigned inc_)
+        : A(A_), t(t_), w1(w1_), lane(lane_), inc(inc_) {
         C = A.chunk_elems;
         L2 = reinterpret_cast<const Elem *>(l2);
         W2 = reinterpret_cast<const double *>(l2 + (size_t)A.cap * sizeof(Elem));
@@ -812,9 +879,9 @@ struct PlainPol {
             if (acc[u]) {
                 const int slot = pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]);
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1b, wb, w1 * wa[u], wa[u]);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1b, wb, w1 * wa[u], wa[u], inc);
                 else
-                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1 * wa[u], wa[u], w1b, wb);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1 * wa[u], wa[u], w1b, wb, inc);
             }
     }
 
@@ -877,7 +944,7 @@ struct QueueEntry<kHistDense> {
 template <bool SYM, bool WEIGHTED, int HM>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
                             const CatRef &c2, const CatRef &c3, int cx, int cy, int cz,
-                            const typename QueueEntry<HM>::T *q, int qn, int lane) {
+                            const typename QueueEntry<HM>::T *q, int qn, int lane, unsigned inc) {
     const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
     wave_sync();
     for (int i = lane; i < qn; i += kWave) {
@@ -890,7 +957,7 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
         const double wa = WEIGHTED ? c2.w[ja] : 0.0, wb = WEIGHTED ? c3.w[jb] : 0.0;
         exact_pair<SYM, WEIGHTED, HM>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
                                         __int_as_float(ka.y), w1 * wa, wa, c3.x[jb] + sbx, c3.y[jb] + sby,
-                                        c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true);
+                                        c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true, inc);
     }
     wave_sync();
 }
@@ -933,10 +1000,12 @@ struct MixedPol {
     int kb, bidx;
     double wb, w1b;
     bool vb;
+    unsigned inc;
 
     __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
-                        int lane_, const CatRef &cat2, const CatRef &cat3, int cx_, int cy_, int cz_)
-        : A(A_), t(t_), c2(cat2), c3(cat3), l2(l2_), l3(l3_), cx(cx_), cy(cy_), cz(cz_), w1(w1_), lane(lane_) {
+                        int lane_, const CatRef &cat2, const CatRef &cat3, int cx_, int cy_, int cz_, unsigned inc_)
+        : A(A_), t(t_), c2(cat2), c3(cat3), l2(l2_), l3(l3_), cx(cx_), cy(cy_), cz(cz_), w1(w1_), lane(lane_),
+          inc(inc_) {
         C = A.chunk_elems;
         const int Cp = C + 4;
         m2 = mlist(l2, A.cap);
@@ -994,7 +1063,7 @@ struct MixedPol {
     }
 
     __device__ void drain() {
-        drain_queue<SYM, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane);
+        drain_queue<SYM, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane, inc);
         qn = 0;
     }
 
@@ -1034,9 +1103,9 @@ struct MixedPol {
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1b, wb, w1 * wa[u], wa[u]);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1b, wb, w1 * wa[u], wa[u], inc);
                 else
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1 * wa[u], wa[u], w1b, wb);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1 * wa[u], wa[u], w1b, wb, inc);
             }
         bool any = false;
 #pragma unroll
@@ -1136,15 +1205,68 @@ __device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long
 template <bool MIXED, int HM, bool SYM, bool WEIGHTED>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
-                                          const CatRef &c3, int cx, int cy, int cz) {
+                                          const CatRef &c3, int cx, int cy, int cz, unsigned inc) {
     if constexpr (MIXED) {
-        MixedPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz);
+        MixedPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz, inc);
         traverse(P, n2, n3);
         if (P.qn > 0) P.drain();
     } else {
-        PlainPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane);
+        PlainPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, inc);
         traverse(P, n2, n3);
     }
+}
+
+// One primary of a term over one catalogue on the six-fold path (TermDesc.tri).
+//
+// With every cell index the exact floor of coordinate / cell size, two points
+// closer than the cell size (rmax) lie in adjacent cells, and a wrap across
+// the periodic boundary only ever joins the last cell of an axis to the
+// first.  So for three points none of which lies in a last cell, each sees
+// the other two as unshifted neighbours whenever one of them sees the pair,
+// and all three compute the same three squared separations bit for bit
+// (sqrSep(a - b) == sqrSep(b - a)), the same sorted bins and the same FP32
+// triangle test: the reference counts such a triple in all six orders or in
+// none.  It is evaluated once, from its lowest-index point, and counted six
+// times.  A triple with a point in a last cell may be seen from one primary
+// only (the partial last cell of a box that is not a whole number of cells:
+// device.cu:89-122), so each of its primaries counts its own two orders, as
+// the reference does.  A primary in a last cell therefore takes the ordinary
+// path (the caller); any other primary counts
+//   pairs of H (not last, index > p)                 x 6
+//   pairs of (H + L) x B  (L: not last, index < p;
+//                          B: entries in a last cell) x 2
+//   pairs of B                                        x 2
+// which is its own share of the reference's ordered count.  Weighted sums
+// carry the factor in the primary's weight.  Returns false on a list overflow.
+template <bool MIXED, int HM, bool WEIGHTED>
+__device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T, int p, double px, double py,
+                            double pz, double pw, int cx, int cy, int cz, char *l2, char *l3, char *wlds, int lane,
+                            unsigned long long &wpairs) {
+    constexpr int LM = MIXED ? kListMixed : kListPlain;
+    int nb = 0, nlo = 0;
+    const int nh = uniform(build_list<LM, kSelSplit>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2,
+                                                     lane, t.sq, t.lut, p, l3, &nb, &nlo));
+    const int nbu = uniform(nb), nlou = uniform(nlo);
+    if (nh + nbu + nlou > A.nmax) return false;
+    wave_sync();
+    const CatRef c = {T.x2, T.y2, T.z2, T.w2};
+    if constexpr (HM == kHistDense)
+        wpairs += 3ull * (unsigned long long)nh * (unsigned long long)max(nh - 1, 0) +
+                  2ull * (unsigned long long)(nh + nlou) * (unsigned long long)nbu +
+                  (unsigned long long)nbu * (unsigned long long)max(nbu - 1, 0);
+    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw * 3.0, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz, 6u);
+    if (nbu > 0) {
+        int nrow = nh;
+        if (nlou > 0) {
+            nrow = uniform(build_list<LM, kSelLow>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane,
+                                                   t.sq, t.lut, p, nullptr, nullptr, nullptr, nh));
+            wave_sync();
+        }
+        if (nrow > 0)
+            pair_loop<MIXED, HM, false, WEIGHTED>(A, t, pw * 2.0, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
+        if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
+    }
+    return true;
 }
 
 // Persistent triplet kernel.  Workgroups pull (term, primary range) chunks
@@ -1242,6 +1364,28 @@ k_triplets(KArgs A) {
                 if (lane == 0) atomicOr(A.err, kErrDomain);
                 continue;
             }
+            // the six-fold path, unless the primary lies in a last cell (box)
+            if (T.tri && (A.survey || (cx != A.nc[0] - 1 && cy != A.nc[1] - 1 && cz != A.nc[2] - 1))) {
+                bool ok;
+                if constexpr (HM == kHistDense) {
+                    ok = true;
+                    if (T.weighted) {  // never planned so
+                        if (lane == 0) atomicOr(A.err, kErrMode);
+                    } else {
+                        ok = tri_primary<MIXED, HM, false>(A, t, T, p, px, py, pz, pw, cx, cy, cz, l2, l3, wlds, lane,
+                                                           wpairs);
+                    }
+                } else if (T.weighted) {
+                    ok = tri_primary<MIXED, HM, true>(A, t, T, p, px, py, pz, pw, cx, cy, cz, l2, l3, wlds, lane,
+                                                      wpairs);
+                } else {
+                    ok = tri_primary<MIXED, HM, false>(A, t, T, p, px, py, pz, pw, cx, cy, cz, l2, l3, wlds, lane,
+                                                       wpairs);
+                }
+                if (!ok && lane == 0) atomicOr(A.err, kErrOverflow);
+                wave_sync();
+                continue;
+            }
             const int n2 =
                 build_list<MIXED ? kListMixed : kListPlain>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz,
                                                             l2, lane, t.sq, t.lut);
@@ -1271,16 +1415,17 @@ k_triplets(KArgs A) {
             if (T.sym) {
                 if (T.weighted) {
                     if constexpr (HM != kHistDense)
-                        pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                        pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz, 2u);
                 } else {
-                    pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz, 2u);
                 }
             } else {
                 if (T.weighted) {
                     if constexpr (HM != kHistDense)
-                        pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                        pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz,
+                                                          1u);
                 } else {
-                    pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz, 1u);
                 }
             }
             wave_sync();
@@ -2018,6 +2163,7 @@ struct Mem {
 // Device buffers of one count (per context, reused across counts).
 struct CountBufs {
     DBuf tables, terms, chunks, out, counter, errw, scratch, bound, wflags;
+    DBuf cflags;  // per catalogue of a count: some cell index is not an exact floor
 };
 
 struct Geometry {
@@ -2056,6 +2202,7 @@ struct nc_catalog {
     const double *d_raw = nullptr;
     bool owns = false;
     bool unit_w = true;
+    bool exact = false;  // last build: every cell index the exact floor of coordinate / cell size
     DBuf raw, key, rank, cnt, start, x, y, z, w;
 };
 
@@ -2071,7 +2218,7 @@ struct BatchPlan {
     unsigned long long *h_out = nullptr;
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
-    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0;
+    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0, tri_terms = 0;
     long long nbc = 0;  // output words per term
 };
 
@@ -2104,6 +2251,7 @@ struct nc_context {
     // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
     int rppi_stream = -1;  // rppi key stream: 1 on, 0 off (direct atomics), -1 box on, survey off
     bool batch_split = true;  // unit-weight terms of a mixed-weight batch as their own launch
+    bool tri = true;          // NC_TRI=0: terms over one catalogue never use the six-fold symmetry
     double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
     double rs_round_keys = 0.0;  // NC_RPPI_ROUND_KEYS: keys per round (tests of several rounds)
     DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_chunk_keys, rs_keys, rs_vals;
@@ -2340,9 +2488,11 @@ int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
 struct CatGrid {
     nc_catalog *cat;
     bool is_p1;
+    bool expect_exact;  // a captured build: the plan counted this catalogue as exactly gridded
 };
 
-int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_catalog *c, bool check_unclamped) {
+int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_catalog *c, bool check_unclamped,
+               unsigned *inexact, bool expect_exact) {
     const int n = (int)c->n;
     const size_t nc = (size_t)G.ncells;
     NC_MEM(ctx, m, c->key, sizeof(uint32_t) * (size_t)n);
@@ -2365,7 +2515,7 @@ int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_cata
         const int nb = (n + 255) / 256;
         hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n, g,
                            check_unclamped ? 1 : 0, (uint32_t *)c->key.p, (uint32_t *)c->rank.p, (uint32_t *)c->cnt.p,
-                           (unsigned *)B.errw.p);
+                           (unsigned *)B.errw.p, inexact, expect_exact ? 1 : 0);
     }
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)c->cnt.p, (uint32_t *)c->start.p,
                        (int)nc);
@@ -2403,7 +2553,7 @@ TermDesc term_desc(const Term &t) {
     d.n1 = (int)t.c1->n;
     d.sym = t.c2 == t.c3;
     d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
-    d.pad = 0;
+    d.tri = 0;  // set by run_count once the builds have shown exact gridding
     return d;
 }
 
@@ -2484,8 +2634,9 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     ctx->timing.hist_layout = p.hist_layout;
     ctx->timing.lds_bytes = p.lds_bytes;
     ctx->timing.path = p.path;
+    ctx->timing.tri_terms = p.tri_terms;
     if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
-    if (*p.h_err & kErrOverflow) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
+    if (*p.h_err & (kErrOverflow | kErrReplan)) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
         plan_drop(ctx);
         retry = true;
         return NC_OK;
@@ -2670,13 +2821,18 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
                     found = true;
                     g.is_p1 = g.is_p1 || r == 0;
                 }
-            if (!found) grids.push_back({cs[r], r == 0});
+            if (!found) grids.push_back({cs[r], r == 0, false});
         }
     }
+    NC_MEM(ctx, m, bufs.cflags, sizeof(unsigned) * grids.size());
     auto enqueue_builds = [&]() -> int {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.errw.p, 0u, 4LL);
-        for (CatGrid &g : grids) {
-            const int r = build_grid(ctx, m, bufs, G, g.cat, G.survey && g.is_p1);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, (uint32_t *)bufs.cflags.p, 0u,
+                           (long long)grids.size());
+        for (size_t gi = 0; gi < grids.size(); ++gi) {
+            CatGrid &g = grids[gi];
+            const int r = build_grid(ctx, m, bufs, G, g.cat, G.survey && g.is_p1, (unsigned *)bufs.cflags.p + gi,
+                                     g.expect_exact);
             if (r) return r;
         }
         return NC_OK;
@@ -2700,11 +2856,28 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     }
     NC_HIP(ctx, hipGetLastError());
     unsigned host_flags[2] = {0, 0};
+    std::vector<unsigned> cflags(grids.size());
     NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], bufs.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     NC_HIP(ctx, hipMemcpyAsync(&host_flags[1], bufs.bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NC_HIP(ctx, hipMemcpyAsync(cflags.data(), bufs.cflags.p, sizeof(unsigned) * cflags.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     const int cap = (int)((std::max(1u, host_flags[1]) + 15u) & ~15u);
+    for (size_t gi = 0; gi < grids.size(); ++gi) grids[gi].cat->exact = cflags[gi] == 0u;
+    // Terms over one catalogue (c1 == c2 == c3) count every unordered triple
+    // once when its gridding is exact (threeD only).
+    std::vector<char> tri(nterms, 0);
+    int ntri = 0;
+    for (int i = 0; i < nterms; ++i) {
+        const Term &t = terms[i];
+        tri[i] = ctx->tri && !G.rppi && t.c1 == t.c2 && t.c2 == t.c3 && t.c1->exact && t.c1->n > 0;
+        ntri += tri[i];
+    }
+    for (CatGrid &g : grids) {
+        g.expect_exact = false;
+        for (int i = 0; i < nterms; ++i) g.expect_exact = g.expect_exact || (tri[i] && terms[i].c1 == g.cat);
+    }
 
     // --- histogram layout: dense u32 counts when every term has unit weights,
     // they fit and no count can wrap between merges (one chunk adds at most
@@ -2721,12 +2894,15 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // Longest list the launch may meet: the 27-cell candidate bound, or, when
     // only the exact lengths can admit the dense layout, the longest list
     // k_list_bound finds (one extra pass over the primaries).
+    // A primary adds at most n(n-1) to the counts (n: its list length), three
+    // times that on the six-fold path.
     const double kWrap = 2147483648.0;
+    const double mult = ntri > 0 ? 3.0 : 1.0;
     const bool dense_cand = !G.rppi && (mixed || G.lut_single_step) && all_unit;
     int nmax = cap;
     // (the mixed path takes the dense layout over a whole CU, the plain path
     // only within half of one)
-    if (dense_cand && (double)ctx->chunk * (double)cap * (double)cap >= kWrap &&
+    if (dense_cand && (double)ctx->chunk * mult * (double)cap * (double)cap >= kWrap &&
         fits(kHistDense, mixed ? ctx->lds_hw : ctx->lds_max)) {
         KArgs B;
         std::memset(&B, 0, sizeof(B));
@@ -2752,7 +2928,12 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
         nmax = std::min(cap, (int)std::max(1u, lb));
     }
-    const bool dense_ok = dense_cand && (double)ctx->chunk * (double)nmax * (double)nmax < kWrap;
+    // primaries per work item: the tuned chunk, cut where the dense layout
+    // needs it so that one work item cannot add 2^31 (merges follow work items)
+    int qchunk = ctx->chunk;
+    const double per_primary = mult * (double)nmax * (double)nmax;
+    const bool dense_ok = dense_cand && per_primary < kWrap;
+    if (dense_ok && (double)qchunk * per_primary >= kWrap) qchunk = std::max(1, (int)((kWrap - 1.0) / per_primary));
     auto choose = [&](size_t budget) {
         if (G.rppi) return kHistGlobal;
         if (dense_ok && fits(kHistDense, budget)) return kHistDense;
@@ -2817,8 +2998,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         d = term_desc(t);
         weighted[i] = d.weighted;
         if (t.c2->n == 0 || t.c3->n == 0) continue;
-        for (int b = 0; b < d.n1; b += ctx->chunk, ++nitems)
-            if (nitems % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(d.n1, b + ctx->chunk), 0});
+        d.tri = tri[i];
+        for (int b = 0; b < d.n1; b += qchunk, ++nitems)
+            if (nitems % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(d.n1, b + qchunk), 0});
     }
     NC_MEM(ctx, m, bufs.terms, sizeof(TermDesc) * td.size());
     NC_MEM(ctx, m, bufs.chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size()));
@@ -3141,6 +3323,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.hist_layout = stream ? kHistStream : hm;
     ctx->timing.lds_bytes = (int32_t)lds_bytes;
     ctx->timing.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
+    ctx->timing.tri_terms = ntri;
     // key stream: keep the region sizes; after an overflow (counted, but
     // through direct atomics) scale the buckets that overflowed by what this
     // run reserved, and capture no graph so that the next call uses that
@@ -3188,6 +3371,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.hist_layout = stream ? kHistStream : hm;
         p.lds_bytes = (int)lds_bytes;
         p.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
+        p.tri_terms = ntri;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
                       hipSuccess &&
                   hipHostMalloc((void **)&p.h_err, 16) == hipSuccess;
@@ -3312,6 +3496,7 @@ int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term
             ctx->timing.hist_layout = tm[0].hist_layout;
             ctx->timing.lds_bytes = tm[0].lds_bytes;
             ctx->timing.path = tm[0].path;
+            ctx->timing.tri_terms = tm[0].tri_terms + tm[1].tri_terms;
             return NC_OK;
         }
     }
@@ -3352,6 +3537,7 @@ int nc_create(nc_context **out, int device) {
         if (v > 0.0 && v <= 1.0) ctx->rs_scale = v;
     }
     if (const char *e = std::getenv("NC_BATCH_SPLIT")) ctx->batch_split = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("NC_TRI")) ctx->tri = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
         const unsigned long long v = std::strtoull(e, nullptr, 10);
         if (v > 0 && v <= (1ull << 31)) ctx->dense_flush = v;
@@ -3371,7 +3557,7 @@ int nc_destroy(nc_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     plan_drop(ctx);
     for (DBuf *b : {&ctx->buf.tables, &ctx->buf.terms, &ctx->buf.chunks, &ctx->buf.out, &ctx->buf.counter,
-                    &ctx->buf.errw, &ctx->buf.scratch, &ctx->buf.bound, &ctx->buf.wflags, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
+                    &ctx->buf.errw, &ctx->buf.scratch, &ctx->buf.bound, &ctx->buf.wflags, &ctx->buf.cflags, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
                     &ctx->rs_chunk_keys, &ctx->rs_keys, &ctx->rs_vals})
         dfree(*b);
     for (auto &e : ctx->ev)

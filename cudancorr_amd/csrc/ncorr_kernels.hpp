@@ -18,6 +18,9 @@ constexpr int kWave = 64;
 constexpr unsigned kErrDomain = 1u;
 constexpr unsigned kErrOverflow = 2u;
 constexpr unsigned kErrMode = 4u;  // a weighted term in a dense-count launch
+// a catalogue planned as exactly gridded (every cell index the exact floor of
+// coordinate / cell size) no longer is: a replayed plan must be rebuilt
+constexpr unsigned kErrReplan = 8u;
 constexpr int kMaxWavesPerGroup = 8;
 
 // One compacted neighbour of a primary: the image position used by the
@@ -38,7 +41,10 @@ struct TermDesc {
     int n1;
     int sym;       // c2 == c3: count unordered (p2,p3) pairs once, doubled
     int weighted;  // accumulate (w1*w2)*w3 in FP64, else integer counts
-    int pad;
+    // c1 == c2 == c3, exactly gridded: every unordered triple of points outside
+    // the last cells is evaluated once, from its lowest-index point, and
+    // counted six times (see k_triplets)
+    int tri;
 };
 
 struct Chunk {

@@ -92,6 +92,9 @@ typedef struct {
                             workgroup per CU) */
     int32_t path;        /* nc_path that ran: NC_PATH_MIXED was asked for but NC_PATH_PLAIN ran when the
                             bins are too narrow for its guard bands; rppi counts always run NC_PATH_PLAIN */
+    int32_t tri_terms;   /* terms over one catalogue (c1 == c2 == c3) whose unordered triples away from the
+                            last cells were each evaluated once and counted six times (every cell index the
+                            exact floor of coordinate / cell size; threeD; NC_TRI=0 turns it off) */
 } nc_timing;
 
 /* Create a context on HIP device `device` (replaces the implicit CUDA context
@@ -102,7 +105,9 @@ typedef struct {
  *   NC_RPPI_ROUND_KEYS=<n>    key-stream keys per round (default: 60% of free device memory)
  *   NC_RPPI_REGION_SCALE=<f>  key-stream regions at f (0 < f <= 1) of the sampled need
  *   NC_DENSE_FLUSH_PAIRS=<n>  dense counts merged once n pairs were added (default and max 2^31)
- *   NC_BATCH_SPLIT=0          nc_count_batched keeps mixed-weight batches in one launch */
+ *   NC_BATCH_SPLIT=0          nc_count_batched keeps mixed-weight batches in one launch
+ *   NC_TRI=0                  terms over one catalogue count each ordered triple's (p2, p3) pair
+ *                             once per primary (no six-fold symmetry) */
 int nc_create(nc_context **ctx, int device);
 /* Catalogues of a destroyed context may still be passed to nc_catalog_destroy
  * (in any order); nothing else. */

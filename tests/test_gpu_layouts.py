@@ -1,8 +1,8 @@
 """Workgroup histogram layouts of the triplet kernel, each against the oracle.
 
-Unit-weight launches count into dense nb^3 32-bit words (layout 2) unless a
-chunk could add 2^31 pairs before a merge; then, and for weighted terms, into
-the sorted slots through the slot table (layout 1).  Every layout must give the
+Unit-weight launches count into dense nb^3 32-bit words (layout 2), with work
+items cut so that none can add 2^31 pairs before a merge; weighted terms count
+into the sorted slots through the slot table (layout 1).  Every layout must give the
 same, bit-exact counts; the dense counts must survive being merged after every
 chunk (NC_DENSE_FLUSH_PAIRS set low in a fresh context).
 """
@@ -40,10 +40,11 @@ def test_dense_and_slot_table_layouts(ctx, path):
         _, got = ctx.count(p, p, p, edges, box, path=path)
         assert ctx.timing().hist_layout == 2
         assert np.array_equal(got, want)
-        # chunks of 2^20 primaries could add more than 2^31 pairs: slot table
+        # chunks of 2^20 primaries could add more than 2^31 pairs between
+        # merges: the work items are cut instead, and the dense layout stays
         ctx.set_tuning(-1, 1 << 20, -1, 0)
         _, got = ctx.count(p, p, p, edges, box, path=path)
-        assert ctx.timing().hist_layout == 1
+        assert ctx.timing().hist_layout == 2
         assert np.array_equal(got, want)
     finally:
         ctx.set_tuning(*before)
