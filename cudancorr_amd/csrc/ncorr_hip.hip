@@ -354,9 +354,10 @@ struct CatRef {
 // none):
 //   kSelAll    every entry into base;
 //   kSelSplit  entries that are not last and have index > pidx into base,
-//              last entries into base_b (*nb_out of them), and *nlo_out =
-//              the number of the remaining ones (not last, index < pidx);
-//   kSelLow    those remaining ones, into base from position `first` on.
+//              last entries into base_b (*nb_out of them);
+//   kSelLow    the remaining ones (not last, index < pidx), into base from
+//              position `first` on.
+// Each selection scans only the neighbour rows that can hold its entries.
 constexpr int kSelAll = 0, kSelSplit = 1, kSelLow = 2;
 
 template <int LM, int SEL = kSelAll>
@@ -364,10 +365,10 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                           const double *__restrict__ Z, const double *__restrict__ Wt, const uint32_t *__restrict__ st,
                           double px, double py, double pz, int cx, int cy, int cz, char *__restrict__ base, int lane,
                           const double *sq, const uint16_t *lut, int pidx = 0, char *__restrict__ base_b = nullptr,
-                          int *nb_out = nullptr, int *nlo_out = nullptr, int first = 0) {
+                          int *nb_out = nullptr, int first = 0) {
     constexpr bool MIXED = LM == kListMixed;
     int n = SEL == kSelLow ? first : 0;
-    int nbl = 0, nlo = 0;
+    int nbl = 0;
     // rppi survey: every triplet's line of sight L = 3a + d2 + d3 (a = p1 - ori,
     // |d2|, |d3| < smax) is within the angle th = asin(2 smax / 3|a|) of a, so
     // a side d of the list has, in any triplet, pi >= |ah.d| - th|d| and
@@ -459,7 +460,6 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
             if (pb & (posb < A.cap)) store(base_b, posb, j, d, qx, qy, qz, qw, s);
             n += __popcll(mh);
             nbl += __popcll(mb);
-            nlo += __popcll(__ballot(in & !last & (j < pidx)));
         } else {
             const bool pl = in & !last & (j < pidx);
             const unsigned long long m = __ballot(pl);
@@ -495,6 +495,22 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
         }
     };
     constexpr int kNoLast = 0x7fffffff;
+    // The part of a range [beg, end) of neighbour row `rc` that a selection
+    // needs: rows whose (x, y) cell lies before the primary's (rc 0) hold
+    // lower indices only, the primary's own row (rc 1) splits at pidx (its
+    // last entries, in the last z cell, lie above pidx: the primary is not in
+    // a last cell), later rows (rc 2) hold higher indices only.  Last entries
+    // are the whole range (last_all) or those from jl on.
+    auto scan_sel = [&](int beg, int end, double shx, double shy, double shz, int s, bool last_all, int jl, int rc) {
+        if constexpr (SEL == kSelSplit) {
+            if (rc == 0 && !last_all) beg = jl < end ? jl : end;  // only its last entries
+            if (rc == 1) beg = max(beg, pidx + 1);
+        } else if constexpr (SEL == kSelLow) {
+            if (rc == 2 || (rc == 0 && last_all)) return;
+            end = min(end, rc == 1 ? pidx : jl);
+        }
+        scan(beg, end, shx, shy, shz, s, last_all, jl);
+    };
     const int nz = A.nc[2];
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy) {
@@ -505,8 +521,9 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                 if (ex < 0 || ex >= A.nc[0] || ey < 0 || ey >= A.nc[1]) continue;
                 const int row = nz * (ey + A.nc[1] * ex);
                 const int z0 = max(cz - 1, 0), z1 = min(cz + 1, nz - 1);
-                scan(uniform((int)st[row + z0]), uniform((int)st[row + z1 + 1]), 0.0, 0.0, 0.0, s0 + 1, false,
-                     kNoLast);
+                const int rc = ex < cx || (ex == cx && ey < cy) ? 0 : ex == cx && ey == cy ? 1 : 2;
+                scan_sel(uniform((int)st[row + z0]), uniform((int)st[row + z1 + 1]), 0.0, 0.0, 0.0, s0 + 1, false,
+                         kNoLast, rc);
                 continue;
             }
             if (ex == A.nc[0]) { ex = 0; shx = A.box[0]; }
@@ -515,11 +532,12 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
             if (ey == -1) { ey = A.nc[1] - 1; shy = -A.box[1]; }
             const int row = nz * (ey + A.nc[1] * ex);
             const bool lastxy = ex == A.nc[0] - 1 || ey == A.nc[1] - 1;
+            const int rc = ex < cx || (ex == cx && ey < cy) ? 0 : ex == cx && ey == cy ? 1 : 2;
             if (cz >= 1 && cz + 1 < nz) {
                 // the merged range reaches the last z cell when cz + 1 is it
                 const int jl = cz + 2 == nz ? uniform((int)st[row + nz - 1]) : kNoLast;
-                scan(uniform((int)st[row + cz - 1]), uniform((int)st[row + cz + 2]), shx, shy, 0.0, s0 + 1, lastxy,
-                     jl);
+                scan_sel(uniform((int)st[row + cz - 1]), uniform((int)st[row + cz + 2]), shx, shy, 0.0, s0 + 1,
+                         lastxy, jl, rc);
                 continue;
             }
             for (int dz = -1; dz <= 1; ++dz) {  // a z wrap: one cell at a time
@@ -527,14 +545,11 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
                 double shz = 0.0;
                 if (ez == nz) { ez = 0; shz = A.box[2]; }
                 if (ez == -1) { ez = nz - 1; shz = -A.box[2]; }
-                scan(uniform((int)st[row + ez]), uniform((int)st[row + ez + 1]), shx, shy, shz, s0 + dz + 1,
-                     lastxy || ez == nz - 1, kNoLast);
+                scan_sel(uniform((int)st[row + ez]), uniform((int)st[row + ez + 1]), shx, shy, shz, s0 + dz + 1,
+                         lastxy || ez == nz - 1, kNoLast, rc);
             }
         }
-    if constexpr (SEL == kSelSplit) {
-        *nb_out = nbl;
-        *nlo_out = nlo;
-    }
+    if constexpr (SEL == kSelSplit) *nb_out = nbl;
     return n;
 }
 
@@ -1243,29 +1258,28 @@ __device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T,
                             double pz, double pw, int cx, int cy, int cz, char *l2, char *l3, char *wlds, int lane,
                             unsigned long long &wpairs) {
     constexpr int LM = MIXED ? kListMixed : kListPlain;
-    int nb = 0, nlo = 0;
+    int nb = 0;
     const int nh = uniform(build_list<LM, kSelSplit>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2,
-                                                     lane, t.sq, t.lut, p, l3, &nb, &nlo));
-    const int nbu = uniform(nb), nlou = uniform(nlo);
-    if (nh + nbu + nlou > A.nmax) return false;
+                                                     lane, t.sq, t.lut, p, l3, &nb));
+    const int nbu = uniform(nb);
+    if (nh + nbu > A.nmax) return false;
     wave_sync();
     const CatRef c = {T.x2, T.y2, T.z2, T.w2};
-    if constexpr (HM == kHistDense)
-        wpairs += 3ull * (unsigned long long)nh * (unsigned long long)max(nh - 1, 0) +
-                  2ull * (unsigned long long)(nh + nlou) * (unsigned long long)nbu +
-                  (unsigned long long)nbu * (unsigned long long)max(nbu - 1, 0);
     if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw * 3.0, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz, 6u);
+    int nrow = nh;
     if (nbu > 0) {
-        int nrow = nh;
-        if (nlou > 0) {
-            nrow = uniform(build_list<LM, kSelLow>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane,
-                                                   t.sq, t.lut, p, nullptr, nullptr, nullptr, nh));
-            wave_sync();
-        }
+        nrow = uniform(build_list<LM, kSelLow>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
+                                               t.lut, p, nullptr, nullptr, nh));
+        if (nrow + nbu > A.nmax) return false;
+        wave_sync();
         if (nrow > 0)
             pair_loop<MIXED, HM, false, WEIGHTED>(A, t, pw * 2.0, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
         if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
     }
+    if constexpr (HM == kHistDense)
+        wpairs += 3ull * (unsigned long long)nh * (unsigned long long)max(nh - 1, 0) +
+                  2ull * (unsigned long long)nrow * (unsigned long long)nbu +
+                  (unsigned long long)nbu * (unsigned long long)max(nbu - 1, 0);
     return true;
 }
 
