@@ -18,9 +18,10 @@ timed region; the timed region is bracketed by hipDeviceSynchronize.
 
 Survey workloads (--workload survey, survey-small): the four terms DDD, DDR,
 DRR, RRR of a cap/shell survey (weighted data, unit-weight randoms) as in
-source/main.cu:72-95.  The primaries are split into --slices contiguous
-slices; each slice is one nc_count_batched call over its four terms (mixed)
-or four nc_count calls (plain), and one step is all slices.  Binned triplets
+source/main.cu:72-95.  Each term's primaries are split into --slices slices of
+their cell-sorted order (nc_set_primary_slice, the same catalogue handles);
+each slice is one nc_count_batched call over the four terms (mixed) or four
+nc_count calls (plain), and one step is all slices.  Binned triplets
 of the weighted terms come from an untimed unit-weight run of the same terms;
 weighted sums of the mixed path are compared with the plain FP64 path.
 
@@ -210,18 +211,15 @@ def run_survey(args):
     K = max(1, args.slices)
     du = d.copy()
     du[:, 3] = 1.0
-
-    def slices(a):
-        b = np.linspace(0, a.shape[0], K + 1).astype(int)
-        return [ctx.catalog(a[i:j]) for i, j in zip(b[:-1], b[1:])]
-
     D, R, DU = ctx.catalog(d), ctx.catalog(r), ctx.catalog(du)
-    Ds, Rs, DUs = slices(d), slices(r), slices(du)
     names = ["DDD", "DDR", "DRR", "RRR"]
 
     def terms(k, unit=False):
-        dk, dd = (DUs[k], DU) if unit else (Ds[k], D)
-        return [(dk, dd, dd), (dk, dd, R), (dk, R, R), (Rs[k], R, R)]
+        # slice k of K of each term's cell-sorted primaries (nc_set_primary_slice),
+        # on the same catalogue handles: DDD and RRR keep their six-fold symmetry
+        ctx.set_primary_slice(k, K)
+        dd = DU if unit else D
+        return [(dd, dd, dd), (dd, dd, R), (dd, R, R), (R, R, R)]
 
     device_sync(hip)
 
@@ -291,6 +289,7 @@ def run_survey(args):
                  "triplets_per_s": trip / wall_p}
         log(f"plain: wall {wall_p:.2f} s, device {dev_p / 1e3:.2f} s (count kernels {cms_p / 1e3:.2f} s)")
 
+    ctx.set_primary_slice(0, 1)
     achieved = trip * FLOPS_PER_TRIPLET / (cms_m * 1e-3) / 1e12
     cpu = None if args.no_cpu else cpu_baseline(
         [r], edges, box, survey=True,
@@ -322,7 +321,7 @@ def run_survey(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    for c in [D, R, DU] + Ds + Rs + DUs:
+    for c in (D, R, DU):
         c.close()
     ctx.close()
 

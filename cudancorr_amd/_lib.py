@@ -18,6 +18,8 @@ NC_PATH_PLAIN = 0
 NC_PATH_MIXED = 1
 NC_BIN_THREED = 0
 NC_BIN_RPPI = 1
+NC_ACCUM_FP64 = 0
+NC_ACCUM_EXACT = 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "invalid bin edges", 3: "point outside the cell grid",
           4: "out of memory", 5: "HIP error", 6: "size limit"}
@@ -27,7 +29,7 @@ EXPORTS = (
     "nc_create", "nc_destroy", "nc_last_error", "nc_catalog_create", "nc_catalog_wrap_device",
     "nc_catalog_destroy", "nc_catalog_size", "nc_hist_size", "nc_compact_size", "nc_hist_len", "nc_count",
     "nc_count_host", "nc_count_batched", "nc_last_timing", "nc_set_tuning", "nc_get_tuning", "nc_autotune",
-    "nc_mixed_classify", "nc_count_noalloc", "nc_noalloc_capacity",
+    "nc_mixed_classify", "nc_count_noalloc", "nc_noalloc_capacity", "nc_set_primary_slice", "nc_set_accumulation",
 )
 
 
@@ -100,6 +102,8 @@ def lib():
     L.nc_count_noalloc.argtypes = [vp, ctypes.POINTER(NcParams), vp, i64, vp, i64, vp, i64, ctypes.c_int, dp, up]
     L.nc_noalloc_capacity.argtypes = []
     L.nc_noalloc_capacity.restype = i64
+    L.nc_set_primary_slice.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.nc_set_accumulation.argtypes = [vp, ctypes.c_int]
     _LIB = L
     return L
 
@@ -308,6 +312,15 @@ class Context:
 
     def set_tuning(self, waves_per_group=0, chunk=0, wgs_per_cu=-1, list_chunk=0):
         self._check(lib().nc_set_tuning(self.h, waves_per_group, chunk, wgs_per_cu, list_chunk))
+
+    def set_accumulation(self, mode):
+        """nc_set_accumulation: NC_ACCUM_FP64 (0, default) or NC_ACCUM_EXACT (1)."""
+        self._check(lib().nc_set_accumulation(self.h, int(mode)))
+
+    def set_primary_slice(self, slice_, nslices):
+        """nc_set_primary_slice: later counts take slice `slice_` of `nslices`
+        of each term's cell-sorted primaries ((0, 1): all)."""
+        self._check(lib().nc_set_primary_slice(self.h, int(slice_), int(nslices)))
 
     def get_tuning(self):
         a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

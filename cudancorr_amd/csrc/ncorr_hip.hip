@@ -7,9 +7,10 @@
 // Design (not a port): instead of one thread per unsorted primary walking
 // per-cell device allocations, the library
 //   1. builds a cell list on the device: FP64 cell key per point
-//      (cudaNcorr.cu:50-58), a counting sort by key (atomic rank + exclusive
-//      scan of the per-cell counts) and an SoA scatter, so neighbour cells are
-//      contiguous, coalesced ranges;
+//      (cudaNcorr.cu:50-58), a stable counting sort by key (per-cell counts,
+//      exclusive scan, then each point placed after the points of its cell
+//      with a smaller index) and an SoA scatter, so neighbour cells are
+//      contiguous, coalesced ranges in an order that is the same on every run;
 //   2. runs one persistent triplet kernel: each workgroup pulls
 //      (term, primary tile) items from a device work queue, each wave takes one
 //      primary at a time, compacts the neighbours within (rmin, rmax) of its
@@ -216,13 +217,34 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ cnt,
     if (t == 1023) start[ncells] = part[1023];
 }
 
-__global__ void k_scatter(const double4 *__restrict__ raw, int n, const uint32_t *__restrict__ key,
-                          const uint32_t *__restrict__ rank, const uint32_t *__restrict__ start, double *__restrict__ x,
-                          double *__restrict__ y, double *__restrict__ z, double *__restrict__ w) {
+// The counting sort by cell key, made stable: k_place puts every point index
+// into its cell's range at the slot its atomic rank gave it (an order that
+// varies from run to run); k_stable_scatter then moves each point to position
+// start + (number of points of its cell with a smaller index), i.e. the cell's
+// points in input order, and writes the SoA arrays there.  The order within a
+// cell is thus the same on every run, and so is every list, work split and
+// floating-point summation order that follows from it.  Each position scans
+// its cell once: O(sum of squared cell occupancies), parallel over positions.
+__global__ void k_place(int n, const uint32_t *__restrict__ key, const uint32_t *__restrict__ rank,
+                        const uint32_t *__restrict__ start, uint32_t *__restrict__ perm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t d = start[key[i]] + rank[i];
-    const double4 p = raw[i];
+    perm[start[key[i]] + rank[i]] = (uint32_t)i;
+}
+
+__global__ void k_stable_scatter(const double4 *__restrict__ raw, int n, const uint32_t *__restrict__ key,
+                                 const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
+                                 double *__restrict__ x, double *__restrict__ y, double *__restrict__ z,
+                                 double *__restrict__ w) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t me = perm[q];
+    const uint32_t c = key[me];
+    const int b = (int)start[c], e = (int)start[c + 1];
+    int r = 0;
+    for (int j = b; j < e; ++j) r += perm[j] < me ? 1 : 0;
+    const int d = b + r;
+    const double4 p = raw[me];
     x[d] = p.x;
     y[d] = p.y;
     z[d] = p.z;
@@ -253,9 +275,24 @@ __global__ void k_nbr_bound(const uint32_t *__restrict__ st1, const uint32_t *__
     atomicMax(out, tot);
 }
 
-__global__ void k_unit_weight(const double4 *__restrict__ raw, long long n, unsigned *__restrict__ flag) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        if (raw[i].w != 1.0) atomicOr(flag, 1u);
+// Weight statistics of a catalogue into s[0..3]: s[0] |= 1 when some weight
+// is not 1, s[1] |= 1 when some weight is not finite, and the 64-bit word at
+// s + 2 gets the largest |w| as its bit pattern (non-negative doubles order
+// as their bits).
+__global__ void k_weight_stats(const double4 *__restrict__ raw, long long n, unsigned *__restrict__ s) {
+    unsigned nonunit = 0u, nonfinite = 0u;
+    unsigned long long mx = 0ull;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double w = raw[i].w;
+        nonunit |= w != 1.0 ? 1u : 0u;
+        if (__builtin_isfinite(w))
+            mx = max(mx, (unsigned long long)__double_as_longlong(fabs(w)));
+        else
+            nonfinite = 1u;
+    }
+    if (__ballot(nonunit != 0u) != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(s, 1u);
+    if (__ballot(nonfinite != 0u) != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(s + 1, 1u);
+    atomicMax(reinterpret_cast<unsigned long long *>(s + 2), mx);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,6 +322,8 @@ struct LdsTabs {
     const uint2 *lut32;
     const uint16_t *t3;
     const uint16_t *rk;  // rppi key stream: rank of a dense rp triple
+    double xscale;       // kHistExact: 2^K, the fixed-point scale of the weight products
+    int xhi;             // kHistExact: byte offset of the high words after the low words
 };
 
 // Neighbour-list flavours: threeD plain (Elem + weight), threeD mixed (MList),
@@ -592,12 +631,19 @@ __global__ __launch_bounds__(256) void k_list_bound(KArgs A, TermDesc T, unsigne
 constexpr int kHistCtab = 0, kHistT3 = 1, kHistDense = 2;
 constexpr int kHistGlobal = 3;  // rppi: the dense histogram in global memory, no LDS copy
 constexpr int kHistStream = 4;  // rppi: key stream binned per bucket in LDS (reported layout)
+// kHistExact: the sorted slots of kHistT3 as 128-bit fixed-point integers (a
+// low-word array, then a high-word array): weight products are truncated to
+// multiples of 2^-K and summed exactly, so the sums do not depend on the order
+// of the additions (nc_set_accumulation NC_ACCUM_EXACT).
+constexpr int kHistExact = 5;
+// layouts addressed through the t3 slot table
+__host__ __device__ constexpr bool t3_like(int hm) { return hm == kHistT3 || hm == kHistExact; }
 
 // Byte offset in the LDS histogram of the slot of bins (ka, kb, k), k in
 // [-1, nb] (slots of k outside [0, nb) are never added).
 template <int HM>
 __device__ __forceinline__ int sorted_slot(const KArgs &A, const LdsTabs &t, int ka, int kb, int k) {
-    if constexpr (HM == kHistT3) return t.t3[__mul24(__mul24(ka, A.nb) + kb, A.nb + 2) + k + 1];
+    if constexpr (t3_like(HM)) return t.t3[__mul24(__mul24(ka, A.nb) + kb, A.nb + 2) + k + 1];
     if constexpr (HM == kHistDense) return 4 * (__mul24(__mul24(ka, A.nb) + kb, A.nb) + k);
     const int klo = min(min(ka, kb), k);
     const int khi = max(max(ka, kb), k);
@@ -612,13 +658,13 @@ __device__ __forceinline__ int sorted_slot(const KArgs &A, const LdsTabs &t, int
 // offset row + lane + 4k; kHistCtab, the plain bins.
 template <int HM>
 __device__ __forceinline__ int row_bin(const KArgs &A, int k) {
-    if constexpr (HM == kHistT3) return 2 * (__mul24(k, A.nb * (A.nb + 2)) + 1);
+    if constexpr (t3_like(HM)) return 2 * (__mul24(k, A.nb * (A.nb + 2)) + 1);
     if constexpr (HM == kHistDense) return 4 * __mul24(k, A.nb * A.nb);
     return k;
 }
 template <int HM>
 __device__ __forceinline__ int lane_bin(const KArgs &A, int k) {
-    if constexpr (HM == kHistT3) return 2 * __mul24(k, A.nb + 2);
+    if constexpr (t3_like(HM)) return 2 * __mul24(k, A.nb + 2);
     if constexpr (HM == kHistDense) return 4 * __mul24(k, A.nb);
     return k;
 }
@@ -639,14 +685,14 @@ __device__ __forceinline__ int shl_add(int a, int b) {
 // or k (kHistCtab).
 template <int HM>
 __device__ __forceinline__ int pair_slot_x(const KArgs &A, const LdsTabs &t, int kr, int kl, int kx) {
-    if constexpr (HM == kHistT3)
+    if constexpr (t3_like(HM))
         return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(t.t3) + kr + kl + kx);
     if constexpr (HM == kHistDense) return kr + kl + kx;
     return sorted_slot<kHistCtab>(A, t, kr, kl, kx);
 }
 template <int HM>
 __device__ __forceinline__ int pair_slot(const KArgs &A, const LdsTabs &t, int kr, int kl, int k) {
-    if constexpr (HM == kHistT3)
+    if constexpr (t3_like(HM))
         return *reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(t.t3) + kr + kl + 2 * k);
     if constexpr (HM == kHistDense) return kr + kl + 4 * k;
     return sorted_slot<kHistCtab>(A, t, kr, kl, k);
@@ -667,18 +713,40 @@ __device__ __forceinline__ Sides sort3(float a, float b, float c) {
 }
 
 // Histogram update of one accepted pair at a precomputed slot (byte offset).
-// inc: ordered triplets the pair stands for (1; 2 with SYM; 6 on the six-fold
-// path); weighted sums carry that factor in the primary's weight instead.
+// inc: ordered triplets the pair stands for.  With the primary p1 (weight w1),
+// the pair's row entry a and lane entry b (w1a = w1*wa, w1b = w1*wb), the
+// reference adds one product (p1*p2)*p3 per order (device.cu:305):
+//   inc 1  (p1; a, b)                          (w1 wa) wb
+//   inc 2  and (p1; b, a)                      + (w1 wb) wa
+//   inc 6  and the four orders led by a or b:  2 [(w1 wa) wb + (w1 wb) wa + (wa wb) w1]
+// (the products of orders that differ by swapping the first two factors are
+// equal).  Counts add inc.
+__device__ __forceinline__ long long fixed_point(double p, double scale) { return (long long)(p * scale); }
 template <bool SYM, bool WEIGHTED, int HM>
-__device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1a, double wa, double w1b, double wb,
-                                         unsigned inc) {
+__device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1, double w1a, double wa, double w1b,
+                                         double wb, unsigned inc) {
     char *h = reinterpret_cast<char *>(t.hist) + off;
     if constexpr (HM == kHistDense) {
         static_assert(!WEIGHTED, "dense counts are for unit weights");
         atomicAdd(reinterpret_cast<unsigned *>(h), inc);
+    } else if constexpr (WEIGHTED && HM == kHistExact) {
+        // 128-bit sum: the low word's wrap (from the value it had) carries
+        // into the high word, whatever the order of the additions
+        long long f = fixed_point(w1a * wb, t.xscale);
+        if (inc != 1) {
+            f += fixed_point(w1b * wa, t.xscale);
+            if (inc == 6) f = 2 * (f + fixed_point((wa * wb) * w1, t.xscale));
+        }
+        const unsigned long long u = (unsigned long long)f;
+        const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long *>(h), u);
+        const long long carry = (f < 0 ? -1 : 0) + (old + u < old ? 1 : 0);
+        if (carry != 0) atomicAdd(reinterpret_cast<unsigned long long *>(h + t.xhi), (unsigned long long)carry);
     } else if constexpr (WEIGHTED) {
-        double v = w1a * wb;  // (w1*w2)*w3, device.cu:305
-        if (SYM) v += w1b * wa;
+        double v = w1a * wb;
+        if (inc != 1) {
+            v += w1b * wa;
+            if (inc == 6) v = 2.0 * (v + (wa * wb) * w1);
+        }
         atomicAdd(reinterpret_cast<double *>(h), v);
     } else {
         atomicAdd(reinterpret_cast<unsigned long long *>(h), (unsigned long long)inc);
@@ -705,8 +773,8 @@ __device__ __forceinline__ int bin_any(const KArgs &A, const LdsTabs &t, double 
 // with the correctly rounded sqrtf.
 template <bool SYM, bool WEIGHTED, int HM>
 __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, double ax, double ay, double az, int ka,
-                                           float fa, double w1a, double wa, double bx, double by, double bz, int kb,
-                                           float fb, double w1b, double wb, bool ok, unsigned inc) {
+                                           float fa, double w1, double w1a, double wa, double bx, double by, double bz,
+                                           int kb, float fb, double w1b, double wb, bool ok, unsigned inc) {
     const double d = sqrsep(ax - bx, ay - by, az - bz);
     if (ok && d > A.r2min && d < A.r2max) {
         const int k = bin_any<false>(A, t, d);
@@ -715,7 +783,7 @@ __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, dou
         const float hi = fmaxf(fmaxf(fa, fb), f);
         const float mid = __builtin_amdgcn_fmed3f(fa, fb, f);
         if (lo + mid > hi)  // the reference rejects sqrtf(r1)+sqrtf(r2) <= sqrtf(r3), device.cu:173
-            hist_add<SYM, WEIGHTED, HM>(t, sorted_slot<HM>(A, t, ka, kb, k), w1a, wa, w1b, wb, inc);
+            hist_add<SYM, WEIGHTED, HM>(t, sorted_slot<HM>(A, t, ka, kb, k), w1, w1a, wa, w1b, wb, inc);
     }
 }
 
@@ -894,9 +962,9 @@ struct PlainPol {
             if (acc[u]) {
                 const int slot = pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]);
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1b, wb, w1 * wa[u], wa[u], inc);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1, w1b, wb, w1 * wa[u], wa[u], inc);
                 else
-                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1 * wa[u], wa[u], w1b, wb, inc);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1, w1 * wa[u], wa[u], w1b, wb, inc);
             }
     }
 
@@ -971,7 +1039,7 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
         neighbour(A, cx, cy, cz, kb.x >> 8, sbx, sby, sbz);
         const double wa = WEIGHTED ? c2.w[ja] : 0.0, wb = WEIGHTED ? c3.w[jb] : 0.0;
         exact_pair<SYM, WEIGHTED, HM>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
-                                        __int_as_float(ka.y), w1 * wa, wa, c3.x[jb] + sbx, c3.y[jb] + sby,
+                                        __int_as_float(ka.y), w1, w1 * wa, wa, c3.x[jb] + sbx, c3.y[jb] + sby,
                                         c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true, inc);
     }
     wave_sync();
@@ -992,7 +1060,7 @@ template <bool SYM, bool WEIGHTED, int HM>
 struct MixedPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 4;
-    static constexpr int kShift = HM == kHistDense ? 2 : HM == kHistT3 ? 1 : 0;  // bin units of the layout
+    static constexpr int kShift = HM == kHistDense ? 2 : t3_like(HM) ? 1 : 0;  // bin units of the layout
     static __device__ __forceinline__ float E_(uint2 e) { return __uint_as_float(e.y); }
     const KArgs &A;
     const LdsTabs &t;
@@ -1118,9 +1186,9 @@ struct MixedPol {
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1b, wb, w1 * wa[u], wa[u], inc);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1, w1b, wb, w1 * wa[u], wa[u], inc);
                 else
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1 * wa[u], wa[u], w1b, wb, inc);
+                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1, w1 * wa[u], wa[u], w1b, wb, inc);
             }
         bool any = false;
 #pragma unroll
@@ -1188,8 +1256,20 @@ template <int HM>
 __device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long long *hist,
                            unsigned long long *pairs) {
     __syncthreads();
-    unsigned long long *out = A.out + (long long)term * A.nbc;
-    if constexpr (HM == kHistDense) {
+    unsigned long long *out = A.out + (long long)term * A.out_stride;
+    if constexpr (HM == kHistExact) {
+        // 128-bit merge: low words with the carry of their wrap into the high words
+        for (int i = threadIdx.x; i < A.nbc; i += blockDim.x) {
+            const unsigned long long lo = hist[i], hi = hist[A.nbc + i];
+            if ((lo | hi) != 0ull) {
+                const unsigned long long old = atomicAdd(&out[i], lo);
+                const unsigned long long up = hi + (old + lo < old ? 1ull : 0ull);
+                if (up != 0ull) atomicAdd(&out[A.nbc + i], up);
+                hist[i] = 0ull;
+                hist[A.nbc + i] = 0ull;
+            }
+        }
+    } else if constexpr (HM == kHistDense) {
         unsigned *h32 = reinterpret_cast<unsigned *>(hist);
         const int nb = A.nb, nb3 = nb * nb * nb;
         for (int i = threadIdx.x; i < nb3; i += blockDim.x) {
@@ -1251,8 +1331,8 @@ __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, doub
 //   pairs of (H + L) x B  (L: not last, index < p;
 //                          B: entries in a last cell) x 2
 //   pairs of B                                        x 2
-// which is its own share of the reference's ordered count.  Weighted sums
-// carry the factor in the primary's weight.  Returns false on a list overflow.
+// which is its own share of the reference's ordered count (hist_add forms the
+// weight products of those orders).  Returns false on a list overflow.
 template <bool MIXED, int HM, bool WEIGHTED>
 __device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T, int p, double px, double py,
                             double pz, double pw, int cx, int cy, int cz, char *l2, char *l3, char *wlds, int lane,
@@ -1265,7 +1345,7 @@ __device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T,
     if (nh + nbu > A.nmax) return false;
     wave_sync();
     const CatRef c = {T.x2, T.y2, T.z2, T.w2};
-    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw * 3.0, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz, 6u);
+    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz, 6u);
     int nrow = nh;
     if (nbu > 0) {
         nrow = uniform(build_list<LM, kSelLow>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
@@ -1273,7 +1353,7 @@ __device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T,
         if (nrow + nbu > A.nmax) return false;
         wave_sync();
         if (nrow > 0)
-            pair_loop<MIXED, HM, false, WEIGHTED>(A, t, pw * 2.0, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
+            pair_loop<MIXED, HM, false, WEIGHTED>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
         if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
     }
     if constexpr (HM == kHistDense)
@@ -1312,6 +1392,8 @@ k_triplets(KArgs A) {
     t.ctab = ctab;
     t.lut32 = lut32;
     t.t3 = t3;
+    t.xscale = A.xscale;
+    t.xhi = A.nbc * 8;
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int i = tid; i < A.nslots; i += nthr) t.hist[i] = 0ull;
     if (tid == 0) s_pairs = 0ull;
@@ -1321,7 +1403,7 @@ k_triplets(KArgs A) {
         t.lut = A.lut;
         // entry .x (edges below the cell) in the layout's bin units: bytes
         // of a dense count (4), of a t3 entry (2), or bins (ctab)
-        constexpr int kshift = HM == kHistDense ? 2 : HM == kHistT3 ? 1 : 0;
+        constexpr int kshift = HM == kHistDense ? 2 : t3_like(HM) ? 1 : 0;
         for (int i = tid; i < A.lut32_n; i += nthr) {
             uint2 v = A.lut32[i];
             v.x <<= kshift;
@@ -1331,7 +1413,7 @@ k_triplets(KArgs A) {
         for (int i = tid; i <= A.nb; i += nthr) sq[i] = A.sq[i];
         for (int i = tid; i <= A.lut_m; i += nthr) lut[i] = A.lut[i];
     }
-    if constexpr (HM == kHistT3)
+    if constexpr (t3_like(HM))
         for (int i = tid; i < A.nb * A.nb * (A.nb + 2); i += nthr) t3[i] = A.t3[i];
     else if constexpr (HM == kHistCtab)
         for (int i = tid; i < A.nb * A.nb; i += nthr) ctab[i] = A.ctab[i];
@@ -2112,6 +2194,8 @@ template __global__ void k_triplets<false, kHistDense>(KArgs);
 template __global__ void k_triplets<true, kHistCtab>(KArgs);
 template __global__ void k_triplets<true, kHistT3>(KArgs);
 template __global__ void k_triplets<true, kHistDense>(KArgs);
+template __global__ void k_triplets<false, kHistExact>(KArgs);
+template __global__ void k_triplets<true, kHistExact>(KArgs);
 
 // ---------------------------------------------------------------------------
 // Scratch of nc_count_noalloc: device memory of a fixed size that the library
@@ -2131,6 +2215,8 @@ struct DBuf {
     void *p = nullptr;
     size_t bytes = 0;
 };
+
+constexpr int kNoExact = -100000;  // no exact layout (see convert_out)
 
 hipError_t dgrow(DBuf &b, size_t need) {
     if (need == 0) need = 16;
@@ -2217,7 +2303,9 @@ struct nc_catalog {
     bool owns = false;
     bool unit_w = true;
     bool exact = false;  // last build: every cell index the exact floor of coordinate / cell size
-    DBuf raw, key, rank, cnt, start, x, y, z, w;
+    double wmax = 0.0;     // largest finite |w|
+    bool wfinite = true;   // every weight finite
+    DBuf raw, key, rank, perm, cnt, start, x, y, z, w;
 };
 
 // Captured launch sequence of nc_count_batched: replayed while the same
@@ -2233,7 +2321,8 @@ struct BatchPlan {
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
     int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0, tri_terms = 0;
-    long long nbc = 0;  // output words per term
+    int xk = kNoExact;  // exact layout: fixed-point scale exponent
+    long long nbc = 0;  // output words per term (slots; twice as many u64 words in the exact layout)
 };
 
 struct nc_context {
@@ -2259,6 +2348,7 @@ struct nc_context {
     // u32 counts from wrapping; NC_DENSE_FLUSH_PAIRS lowers it for tests)
     unsigned long long dense_flush = 1ull << 31;
     int sample_stride = 1;    // autotuner: queue only every n-th work item
+    int slice = 0, nslices = 1;  // nc_set_primary_slice: primaries of slice `slice` of `nslices` only
     bool autotuning = false;  // in nc_autotune's timed runs
     std::string dev_name;
     // rppi key stream: buffers, and the per-bucket capacities last sized for
@@ -2266,6 +2356,7 @@ struct nc_context {
     int rppi_stream = -1;  // rppi key stream: 1 on, 0 off (direct atomics), -1 box on, survey off
     bool batch_split = true;  // unit-weight terms of a mixed-weight batch as their own launch
     bool tri = true;          // NC_TRI=0: terms over one catalogue never use the six-fold symmetry
+    int accum = NC_ACCUM_FP64;  // nc_set_accumulation
     double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
     double rs_round_keys = 0.0;  // NC_RPPI_ROUND_KEYS: keys per round (tests of several rounds)
     DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_chunk_keys, rs_keys, rs_vals;
@@ -2511,6 +2602,7 @@ int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_cata
     const size_t nc = (size_t)G.ncells;
     NC_MEM(ctx, m, c->key, sizeof(uint32_t) * (size_t)n);
     NC_MEM(ctx, m, c->rank, sizeof(uint32_t) * (size_t)n);
+    NC_MEM(ctx, m, c->perm, sizeof(uint32_t) * (size_t)n);
     NC_MEM(ctx, m, c->cnt, sizeof(uint32_t) * nc);
     NC_MEM(ctx, m, c->start, sizeof(uint32_t) * (nc + 1));
     NC_MEM(ctx, m, c->x, sizeof(double) * (size_t)n);
@@ -2535,8 +2627,10 @@ int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_cata
                        (int)nc);
     if (n > 0) {
         const int nb = (n + 255) / 256;
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n,
-                           (const uint32_t *)c->key.p, (const uint32_t *)c->rank.p, (const uint32_t *)c->start.p,
+        hipLaunchKernelGGL(k_place, dim3(nb), dim3(256), 0, ctx->stream, n, (const uint32_t *)c->key.p,
+                           (const uint32_t *)c->rank.p, (const uint32_t *)c->start.p, (uint32_t *)c->perm.p);
+        hipLaunchKernelGGL(k_stable_scatter, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n,
+                           (const uint32_t *)c->key.p, (const uint32_t *)c->start.p, (const uint32_t *)c->perm.p,
                            (double *)c->x.p, (double *)c->y.p, (double *)c->z.p, (double *)c->w.p);
     }
     NC_HIP(ctx, hipGetLastError());
@@ -2580,9 +2674,20 @@ void plan_drop(nc_context *ctx) {
     p = BatchPlan();
 }
 
+// A 128-bit fixed-point sum (high word, low word) in units of 2^-k, rounded
+// once to double.
+double fixed_to_double(unsigned long long lo, unsigned long long hi, int k) {
+    const __int128 v = (__int128)(((unsigned __int128)hi << 64) | (unsigned __int128)lo);
+    return std::ldexp((double)v, -k);
+}
+
 // Compact sorted-bin results -> dense nb^3 histogram (device.cu:198 layout).
+// Exact layout (xk != kNoExact): per term nbc low words then nbc high words,
+// weighted sums in units of 2^-xk, counts in the low words.
 void convert_out(int nb, long long nbc, int rppi, int nterms, const std::vector<int> &weighted,
-                 const unsigned long long *res, double *hist, uint64_t *counts) {
+                 const unsigned long long *res, double *hist, uint64_t *counts, int xk = kNoExact) {
+    const bool exact = xk != kNoExact;
+    const long long stride = exact ? 2 * nbc : nbc;
     if (rppi) {  // already the dense layout
         for (int i = 0; i < nterms; ++i) {
             const unsigned long long *r = res + (size_t)i * nbc;
@@ -2606,13 +2711,15 @@ void convert_out(int nb, long long nbc, int rppi, int nterms, const std::vector<
         uint64_t *cn = counts ? counts + (size_t)i * nh : nullptr;
         std::fill(h, h + nh, 0.0);
         if (cn) std::fill(cn, cn + nh, (uint64_t)0);
-        const unsigned long long *r = res + (size_t)i * nbc;
+        const unsigned long long *r = res + (size_t)i * stride;
         int c = 0;
         for (int a = 0; a < nb; ++a)
             for (int b = a; b < nb; ++b)
                 for (int k = b; k < nb; ++k, ++c) {
                     const size_t di = (size_t)k + (size_t)b * nb + (size_t)a * nb * nb;
-                    if (weighted[i]) {
+                    if (weighted[i] && exact) {
+                        h[di] = fixed_to_double(r[c], r[nbc + c], xk);
+                    } else if (weighted[i]) {
                         double v;
                         std::memcpy(&v, &r[c], 8);
                         h[di] = v;
@@ -2634,7 +2741,8 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     NC_HIP(ctx, hipGraphLaunch(p.count_exec, ctx->stream));
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     const int nterms = (int)p.weighted.size();
-    NC_HIP(ctx, hipMemcpyAsync(p.h_out, ctx->buf.out.p, sizeof(unsigned long long) * (size_t)nterms * p.nbc,
+    const long long ostride = p.xk != kNoExact ? 2 * p.nbc : p.nbc;
+    NC_HIP(ctx, hipMemcpyAsync(p.h_out, ctx->buf.out.p, sizeof(unsigned long long) * (size_t)nterms * ostride,
                                hipMemcpyDeviceToHost, ctx->stream));
     NC_HIP(ctx, hipMemcpyAsync(p.h_err, ctx->buf.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2656,7 +2764,7 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
         return NC_OK;
     }
     if (*p.h_err & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
-    convert_out(p.nb, p.nbc, p.rppi, nterms, p.weighted, p.h_out, hist, counts);
+    convert_out(p.nb, p.nbc, p.rppi, nterms, p.weighted, p.h_out, hist, counts, p.xk);
     return NC_OK;
 }
 
@@ -2689,7 +2797,29 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     pkey.push_back((double)G.npi);
     pkey.push_back(G.ori_x);
     pkey.insert(pkey.end(), G.edges.begin(), G.edges.end());
-    const std::vector<int> tkey = {path, ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
+    // exact accumulation of weighted sums (nc_set_accumulation): every weight
+    // product is truncated to a multiple of 2^-xk with xk set so that the
+    // largest product any weighted term can form stays below 2^60 units
+    // (below 2^63 for the six orders of a triple together)
+    bool all_unit = true;
+    for (const Term &t : terms) all_unit = all_unit && t.c1->unit_w && t.c2->unit_w && t.c3->unit_w;
+    const bool exact = ctx->accum == NC_ACCUM_EXACT && !G.rppi && !all_unit;
+    int xk = kNoExact;
+    if (exact) {
+        double pmax = 0.0;
+        for (const Term &t : terms) {
+            if (t.c1->unit_w && t.c2->unit_w && t.c3->unit_w) continue;
+            if (!t.c1->wfinite || !t.c2->wfinite || !t.c3->wfinite)
+                return fail(ctx, NC_ERR_ARG, "exact accumulation needs finite weights");
+            pmax = std::max(pmax, t.c1->wmax * t.c2->wmax * t.c3->wmax);
+        }
+        int e = 0;
+        if (pmax > 0.0) std::frexp(pmax, &e);  // pmax < 2^e
+        xk = std::min(std::max(60 - e, -900), 900);
+    }
+    pkey.push_back((double)xk);
+    const std::vector<int> tkey = {path,       ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems,
+                                   ctx->slice, ctx->nslices};
     if (use_graph && ctx->plan.valid && ctx->plan.uids == uids && ctx->plan.wkey == wkey && ctx->plan.pkey == pkey &&
         ctx->plan.tkey == tkey) {
         bool retry = false;
@@ -2789,7 +2919,10 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (!G.t3.empty()) std::memcpy(tabs.data() + g_t3, G.t3.data(), sizeof(uint16_t) * G.t3.size());
     const size_t nb3 = (size_t)G.nb * G.nb * G.nb;
     auto hist_size = [&](int hm) {
-        return hm == kHistGlobal ? (size_t)0 : hm == kHistDense ? al16(nb3 * 4) : al16((size_t)G.nbc * 8);
+        return hm == kHistGlobal    ? (size_t)0
+               : hm == kHistDense   ? al16(nb3 * 4)
+               : hm == kHistExact   ? al16((size_t)G.nbc * 16)
+                                    : al16((size_t)G.nbc * 8);
     };
     auto lds_tables = [&](int hm, size_t &o_sq, size_t &o_lut, size_t &o_ctab, size_t &o_lut32, size_t &o_t3) {
         size_t o = 0;
@@ -2803,7 +2936,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             o_lut32 = o;
             o = al16(o + sizeof(uint2) * G.lut32.size());
         }
-        if (hm == kHistT3) {
+        if (t3_like(hm)) {
             o_t3 = o;
             o = al16(o + sizeof(uint16_t) * G.t3.size());
         } else if (hm == kHistCtab) {
@@ -2897,8 +3030,6 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // they fit and no count can wrap between merges (one chunk adds at most
     // chunk * cap^2 pairs); else the slot table; else ctab.  The plain path
     // needs the single-step bin lookup for either of the first two.
-    bool all_unit = true;
-    for (const Term &t : terms) all_unit = all_unit && t.c1->unit_w && t.c2->unit_w && t.c3->unit_w;
     // The budget is first half the CU's LDS (two workgroups per CU); a layout
     // that only fits the whole of it runs one workgroup per CU.
     auto fits = [&](int hm, size_t budget) {
@@ -2956,7 +3087,12 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     };
     size_t budget = ctx->lds_max;
     int hm = choose(budget);
-    if (mixed && dense_ok && hm != kHistDense && fits(kHistDense, ctx->lds_hw)) {
+    if (exact) {
+        if (G.t3.empty() || !(mixed || G.lut_single_step))
+            return fail(ctx, NC_ERR_LIMIT, "exact accumulation supports at most 35 bins per side");
+        hm = kHistExact;
+        if (!fits(hm, budget)) budget = ctx->lds_hw;
+    } else if (mixed && dense_ok && hm != kHistDense && fits(kHistDense, ctx->lds_hw)) {
         // mixed path: dense counts over a whole CU beat the sorted-slot
         // layouts at half of it (25 bins, survey randoms: 4405 against 5428 ms)
         budget = ctx->lds_hw;
@@ -2991,7 +3127,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
          (const void *)k_triplets<true, kHistDense>}};
     const void *kfn = G.rppi ? (G.survey ? (stream ? (const void *)k_rppi<true, true> : (const void *)k_rppi<true, false>)
                                          : (stream ? (const void *)k_rppi<false, true> : (const void *)k_rppi<false, false>))
-                             : kfns[mixed ? 1 : 0][hm];
+                      : hm == kHistExact ? (mixed ? (const void *)k_triplets<true, kHistExact>
+                                                  : (const void *)k_triplets<false, kHistExact>)
+                                         : kfns[mixed ? 1 : 0][hm];
     {
         // some variants are compiled for smaller workgroups (more VGPRs per lane)
         hipFuncAttributes fa;
@@ -3013,12 +3151,16 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         weighted[i] = d.weighted;
         if (t.c2->n == 0 || t.c3->n == 0) continue;
         d.tri = tri[i];
-        for (int b = 0; b < d.n1; b += qchunk, ++nitems)
-            if (nitems % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(d.n1, b + qchunk), 0});
+        // the primary slice: positions [p0, p1) of the cell-sorted catalogue
+        const int p0 = (int)((long long)d.n1 * ctx->slice / ctx->nslices);
+        const int p1 = (int)((long long)d.n1 * (ctx->slice + 1) / ctx->nslices);
+        for (int b = p0; b < p1; b += qchunk, ++nitems)
+            if (nitems % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(p1, b + qchunk), 0});
     }
     NC_MEM(ctx, m, bufs.terms, sizeof(TermDesc) * td.size());
     NC_MEM(ctx, m, bufs.chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size()));
-    NC_MEM(ctx, m, bufs.out, sizeof(unsigned long long) * (size_t)nterms * G.out_words);
+    const long long ostride = exact ? 2 * G.out_words : G.out_words;  // u64 words of out per term
+    NC_MEM(ctx, m, bufs.out, sizeof(unsigned long long) * (size_t)nterms * ostride);
     if (!td.empty())
         NC_HIP(ctx, hipMemcpyAsync(bufs.terms.p, td.data(), sizeof(TermDesc) * td.size(), hipMemcpyHostToDevice,
                                    ctx->stream));
@@ -3089,7 +3231,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.lut32_n = (int)G.lut32.size();
     A.g32 = G.g32;
     A.tri_t = G.tri_t;
-    A.nslots = hm == kHistDense ? (int)((nb3 + 1) / 2) : hm == kHistGlobal ? 0 : G.nbc;
+    A.nslots = hm == kHistDense ? (int)((nb3 + 1) / 2) : hm == kHistGlobal ? 0 : hm == kHistExact ? 2 * G.nbc : G.nbc;
+    A.out_stride = (int)ostride;
+    A.xscale = exact ? std::ldexp(1.0, xk) : 0.0;
     A.flush_pairs = ctx->dense_flush;
     A.off_sq = (int)hist_bytes;
     A.off_lut = (int)(hist_bytes + off_lut);
@@ -3294,7 +3438,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
 
     auto enqueue_count = [&]() -> int {
         hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)bufs.out.p, 0ull,
-                           (long long)nterms * G.out_words);
+                           (long long)nterms * ostride);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 4LL);
         if (A.nchunks > 0 && !stream) {
             void *args[] = {&A};
@@ -3321,7 +3465,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     };
     if (enqueue_count() != NC_OK) return fail(ctx, NC_ERR_HIP, "triplet kernel launch failed");
     NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-    std::vector<unsigned long long> res((size_t)nterms * G.out_words);
+    std::vector<unsigned long long> res((size_t)nterms * ostride);
     NC_HIP(ctx, hipMemcpyAsync(res.data(), bufs.out.p, sizeof(unsigned long long) * res.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
     NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], bufs.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -3367,7 +3511,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
     if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
 
-    convert_out(G.nb, G.out_words, G.rppi, nterms, weighted, res.data(), hist, counts);
+    convert_out(G.nb, G.out_words, G.rppi, nterms, weighted, res.data(), hist, counts, xk);
 
     // --- capture the launch sequence for replay (builds, then count); any
     // failure leaves the eager result valid and no plan cached
@@ -3380,6 +3524,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.weighted = weighted;
         p.nb = G.nb;
         p.nbc = G.out_words;
+        p.xk = xk;
         p.rppi = G.rppi;
         p.nkernel = A.nchunks > 0 ? (stream ? 2 * nrounds : 1) : 0;
         p.hist_layout = stream ? kHistStream : hm;
@@ -3445,20 +3590,25 @@ int refresh_weights(nc_context *ctx, Mem &m, const std::vector<Term> &terms) {
     NC_HIP(ctx, hipSetDevice(ctx->device));
     DBuf arena_flags;
     DBuf &fl = m.arena() ? arena_flags : ctx->buf.wflags;
-    NC_MEM(ctx, m, fl, sizeof(unsigned) * w.size());
+    NC_MEM(ctx, m, fl, 4 * sizeof(unsigned) * w.size());
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, (uint32_t *)fl.p, 0u,
-                       (long long)w.size());
+                       4LL * (long long)w.size());
     for (size_t i = 0; i < w.size(); ++i) {
         const int grid = (int)std::min<long long>((w[i]->n + 255) / 256, 1024);
-        hipLaunchKernelGGL(k_unit_weight, dim3(grid), dim3(256), 0, ctx->stream, (const double4 *)w[i]->d_raw,
-                           (long long)w[i]->n, (unsigned *)fl.p + i);
+        hipLaunchKernelGGL(k_weight_stats, dim3(grid), dim3(256), 0, ctx->stream, (const double4 *)w[i]->d_raw,
+                           (long long)w[i]->n, (unsigned *)fl.p + 4 * i);
     }
     NC_HIP(ctx, hipGetLastError());
-    std::vector<unsigned> f(w.size());
+    std::vector<unsigned> f(4 * w.size());
     NC_HIP(ctx, hipMemcpyAsync(f.data(), fl.p, sizeof(unsigned) * f.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (size_t i = 0; i < w.size(); ++i) w[i]->unit_w = f[i] == 0u;
+    for (size_t i = 0; i < w.size(); ++i) {
+        w[i]->unit_w = f[4 * i] == 0u;
+        w[i]->wfinite = f[4 * i + 1] == 0u;
+        const unsigned long long b = (unsigned long long)f[4 * i + 2] | ((unsigned long long)f[4 * i + 3] << 32);
+        std::memcpy(&w[i]->wmax, &b, 8);
+    }
     return NC_OK;
 }
 
@@ -3606,11 +3756,14 @@ int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog
     c->uid = ctx->next_uid++;
     c->n = n;
     c->owns = true;
-    for (int64_t i = 0; i < n; ++i)
-        if (xyzw[4 * i + 3] != 1.0) {
-            c->unit_w = false;
-            break;
-        }
+    for (int64_t i = 0; i < n; ++i) {
+        const double w = xyzw[4 * i + 3];
+        c->unit_w = c->unit_w && w == 1.0;
+        if (std::isfinite(w))
+            c->wmax = std::max(c->wmax, std::fabs(w));
+        else
+            c->wfinite = false;
+    }
     hipError_t e = dgrow(c->raw, sizeof(double) * 4 * (size_t)n);
     if (e == hipSuccess && n > 0)
         e = hipMemcpyAsync(c->raw.p, xyzw, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
@@ -3639,23 +3792,7 @@ int nc_catalog_wrap_device(nc_context *ctx, const double *d_xyzw, int64_t n, nc_
     c->n = n;
     c->owns = false;
     c->d_raw = d_xyzw;
-    if (n > 0) {
-        hipError_t e = dgrow(ctx->buf.errw, 16);
-        unsigned flag = 0;
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)ctx->buf.errw.p, 0u, 4LL);
-            hipLaunchKernelGGL(k_unit_weight, dim3(1024), dim3(256), 0, ctx->stream, (const double4 *)d_xyzw,
-                               (long long)n, (unsigned *)ctx->buf.errw.p);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipMemcpyAsync(&flag, ctx->buf.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) {
-            delete c;
-            return fail(ctx, NC_ERR_HIP, hipGetErrorString(e));
-        }
-        c->unit_w = flag == 0;
-    }
+    // weight statistics: derived from the current contents before every count
     c->device = ctx->device;
     ctx->cats.push_back(c);
     *cat = c;
@@ -3674,7 +3811,7 @@ int nc_catalog_destroy(nc_catalog *c) {
     } else {
         (void)hipSetDevice(c->device);
     }
-    for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->cnt, &c->start, &c->x, &c->y, &c->z, &c->w}) dfree(*b);
+    for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->perm, &c->cnt, &c->start, &c->x, &c->y, &c->z, &c->w}) dfree(*b);
     delete c;
     return NC_OK;
 }
@@ -3829,6 +3966,21 @@ int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_c
     return NC_OK;
 }
 
+int nc_set_accumulation(nc_context *ctx, int mode) {
+    if (!ctx) return NC_ERR_ARG;
+    if (mode != NC_ACCUM_FP64 && mode != NC_ACCUM_EXACT) return fail(ctx, NC_ERR_ARG, "unknown accumulation mode");
+    ctx->accum = mode;
+    return NC_OK;
+}
+
+int nc_set_primary_slice(nc_context *ctx, int slice, int nslices) {
+    if (!ctx) return NC_ERR_ARG;
+    if (nslices < 1 || slice < 0 || slice >= nslices) return fail(ctx, NC_ERR_ARG, "need 0 <= slice < nslices");
+    ctx->slice = slice;
+    ctx->nslices = nslices;
+    return NC_OK;
+}
+
 int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu, int *list_chunk) {
     if (!ctx) return NC_ERR_ARG;
     if (list_chunk) *list_chunk = ctx->chunk_elems;
@@ -3882,6 +4034,9 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     const long long items = std::max<long long>(1, nprim / 16);
     ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
     ctx->autotuning = true;
+    const int saved_slice[2] = {ctx->slice, ctx->nslices};  // the sample is drawn from all primaries
+    ctx->slice = 0;
+    ctx->nslices = 1;
     nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
     int rc = NC_OK;
     // 16-wave workgroups only where one workgroup per CU is forced anyway
@@ -3929,6 +4084,8 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
 done:
     ctx->sample_stride = 1;
     ctx->autotuning = false;
+    ctx->slice = saved_slice[0];
+    ctx->nslices = saved_slice[1];
     plan_drop(ctx);
     if (rc) {
         ctx->waves_per_group = saved[0];

@@ -3,7 +3,8 @@
 // Data layout in HBM (per catalogue, per count geometry):
 //   raw    : n x double4 {x,y,z,w}, as handed over by the caller
 //   key    : n x u32 cell key  z + nz*(y + ny*x)      (cudaNcorr.cu:50-58)
-//   rank   : n x u32 slot of the point inside its cell
+//   rank   : n x u32 arrival slot of the point inside its cell (atomic)
+//   perm   : n x u32 point indices in cell order, then placed stably (by index)
 //   start  : ncells+1 x u32 exclusive scan of the per-cell counts
 //   x,y,z,w: n x double each, points sorted by cell key (SoA, coalesced)
 // Per persistent wave, a scratch slot holds the compacted neighbour lists of
@@ -85,6 +86,8 @@ struct KArgs {
     double r2min, r2max;
     int nb, nbc;
     int nslots;                   // u64 words of the LDS histogram (nbc compact slots, or nb^3 u32 dense counts)
+    int out_stride;               // u64 words of out per term: nbc, or 2*nbc (exact layout: low words, high words)
+    double xscale;                // exact layout: 2^K, the fixed-point scale of the weight products
     // rppi binning (cudaNcorr.cu:159-216): npi unit-width pi bins; nbc is then
     // the dense nb^3 * npi^2 output length per term
     int npi;

@@ -199,6 +199,36 @@ int nc_last_timing(const nc_context *ctx, nc_timing *out);
 int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu, int list_chunk);
 int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu, int *list_chunk);
 
+/* Accumulation of weighted sums (no counterpart in the reference, which adds
+ * every FP64 product (w1*w2)*w3 with a global FP64 atomic, device.cu:305-306,
+ * in a run-dependent order).
+ *   NC_ACCUM_FP64   (default) per-workgroup FP64 sums in LDS, merged into FP64
+ *                   totals: the reference's arithmetic, order not reproducible.
+ *   NC_ACCUM_EXACT  every product, computed in FP64 in the reference's order, is
+ *                   truncated to a multiple of 2^-K and summed exactly as a
+ *                   128-bit integer, then rounded once to double.  K is the
+ *                   largest integer with every product of a weighted term below
+ *                   2^(60-K) (from the catalogues' largest |w|), so each product
+ *                   is off by less than 2^-K: relative 2^(e-60)/|p| for a
+ *                   product p when the largest possible one is below 2^e
+ *                   (about 5e-14 for lognormal(0, 0.3) weights).  Results are
+ *                   identical on every run and on both arithmetic paths.  Needs
+ *                   finite weights and at most 35 bins; threeD only (rppi keeps
+ *                   FP64).  Unit-weight terms count exactly either way. */
+enum nc_accum { NC_ACCUM_FP64 = 0, NC_ACCUM_EXACT = 1 };
+int nc_set_accumulation(nc_context *ctx, int mode);
+
+/* Primary slices (no counterpart in the reference, whose kernel takes every
+ * primary of p1 in one launch, cudaNcorr.cu:383-389): later counts on ctx
+ * take, for every term, only the primaries of slice `slice` of `nslices`
+ * equal slices of c1 in cell-sorted order; the nslices results add up to the
+ * full count (exactly for counts, to rounding for weighted sums).  The
+ * catalogues stay the same handles, so a term over one catalogue keeps its
+ * six-fold symmetry.  For long counts split into steps (progress,
+ * checkpoints) and for dividing one count between devices.  (0, 1) = all
+ * primaries, the default. */
+int nc_set_primary_slice(nc_context *ctx, int slice, int nslices);
+
 /* Autotuner (no counterpart in the reference): times the triplet kernel (the
  * better of two runs) on a strided sample of the work items of these terms,
  * sized by a first run to about 80 ms, for every candidate of

@@ -159,3 +159,41 @@ def test_batched_boxes_replayed(ctx):
             assert np.array_equal(g, w), f"replay {rep}"
     for c in cats:
         c.close()
+
+
+@pytest.mark.parametrize("nslices", [3, 7])
+def test_primary_slices_add_up(ctx, nslices):
+    """nc_set_primary_slice: the slices of every term's cell-sorted primaries,
+    on the same catalogue handles, add up to the full count (exactly for
+    counts, within 1e-12 for weighted sums), and the six-fold path stays on."""
+    _log(f"tri primary slices {nslices}")
+    d, r = survey_cap(2500, 4000, seed=37)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 40.0, 10)
+    D, R = ctx.catalog(d), ctx.catalog(r)
+    terms = [(D, D, D), (D, D, R), (D, R, R), (R, R, R)]
+    try:
+        for path in PATHS:
+            h_full, c_full = ctx.count_batched(terms, edges, box, survey=True, path=path)
+            h_sum = np.zeros_like(h_full)
+            c_sum = np.zeros_like(c_full)
+            for k in range(nslices):
+                ctx.set_primary_slice(k, nslices)
+                h, c = ctx.count_batched(terms, edges, box, survey=True, path=path)
+                assert ctx.timing().tri_terms == 2  # DDD and RRR
+                h_sum += h
+                c_sum += c
+            ctx.set_primary_slice(0, 1)
+            assert np.array_equal(c_sum[3], c_full[3])
+            want_rrr, _ = oracle.count(r, r, r, edges, box, survey=True)
+            assert np.array_equal(c_full[3], want_rrr)
+            for t in range(3):
+                nz = h_full[t] != 0
+                assert np.array_equal(h_sum[t][~nz], h_full[t][~nz])
+                assert np.max(np.abs(h_sum[t][nz] / h_full[t][nz] - 1.0)) <= 1e-12
+    finally:
+        ctx.set_primary_slice(0, 1)
+        D.close()
+        R.close()
