@@ -326,6 +326,129 @@ def run_survey(args):
     ctx.close()
 
 
+def config4_subset(args, hip):
+    """The large weighted survey (4M Neyman-Scott data with lognormal weights +
+    20M uniform randoms, rmax=50, 25 bins/side, DDD+DDR+DRR+RRR as
+    source/main.cu:72-95) on a stated subset: primary slices --config4-slices
+    of 32 in cell-sorted order (nc_set_primary_slice), all four terms per slice
+    in one nc_count_batched call.  Legs: mixed path with FP64 accumulation (the
+    measured one), plain FP64 path (four nc_count calls per slice), and the
+    exact fixed-point accumulation (NC_ACCUM_EXACT on the plain path) as the
+    FP64 reference of the weighted sums.  Binned triplets of the weighted terms
+    from an untimed unit-weight run.  The CPU oracle times a sample of RRR
+    primaries against all randoms, checked bit for bit on the GPU."""
+    from cudancorr_amd import NC_ACCUM_EXACT, NC_ACCUM_FP64, NC_PATH_MIXED, NC_PATH_PLAIN, Context
+
+    slices = [int(x) for x in args.config4_slices.split(",")]
+    K = 32
+    t0 = time.perf_counter()
+    d, r, edges, box, _, desc = survey_workload("survey")
+    log(f"config 4: generated {d.shape[0]} data + {r.shape[0]} randoms in {time.perf_counter() - t0:.1f}s")
+    names = ["DDD", "DDR", "DRR", "RRR"]
+    du = d.copy()
+    du[:, 3] = 1.0
+    out = {"workload": desc + f"; primary slices {slices} of {K} (cell-sorted order) of every term",
+           "slices": slices, "nslices": K}
+    with Context(0) as ctx:
+        D, R, DU = ctx.catalog(d), ctx.catalog(r), ctx.catalog(du)
+        terms = [(D, D, D), (D, D, R), (D, R, R), (R, R, R)]
+        unit_terms = [(DU, DU, DU), (DU, DU, R), (DU, R, R)]
+        tune = ctx.autotune(terms, edges, box, survey=True, path=NC_PATH_MIXED)
+        log(f"config 4: autotune mixed {tune}")
+
+        def leg(path, batched=True, exact=False, tl=terms):
+            ctx.set_accumulation(NC_ACCUM_EXACT if exact else NC_ACCUM_FP64)
+            hist = cnt = None
+            ms = 0.0
+            device_sync(hip)
+            t1 = time.perf_counter()
+            for k in slices:
+                ctx.set_primary_slice(k, K)
+                if batched:
+                    h, c = ctx.count_batched(tl, edges, box, survey=True, path=path)
+                    ms += ctx.timing().ms_count
+                else:
+                    hc = []
+                    for tm in tl:
+                        hc.append(ctx.count(*tm, edges, box, survey=True, path=path))
+                        ms += ctx.timing().ms_count
+                    h, c = np.stack([a for a, _ in hc]), np.stack([b for _, b in hc])
+                hist = h if hist is None else hist + h
+                cnt = c if cnt is None else cnt + c
+            device_sync(hip)
+            wall = time.perf_counter() - t1
+            ctx.set_primary_slice(0, 1)
+            ctx.set_accumulation(NC_ACCUM_FP64)
+            return hist, cnt, wall, ms
+
+        leg(NC_PATH_MIXED, tl=[terms[3]])  # warm-up (one term)
+        hm_, cm_, wall_m, ms_m = leg(NC_PATH_MIXED)
+        tri_terms = ctx.timing().tri_terms
+        log(f"config 4: mixed {wall_m:.2f} s (count kernels {ms_m / 1e3:.2f} s)")
+        _, cu, _, _ = leg(NC_PATH_MIXED, tl=unit_terms)
+        trip_terms = [int(c.sum()) for c in cu] + [int(cm_[3].sum())]
+        trip = int(sum(trip_terms))
+        log(f"config 4: binned triplets {dict(zip(names, trip_terms))}")
+        tune_p = ctx.autotune(terms, edges, box, survey=True, path=NC_PATH_PLAIN)
+        log(f"config 4: autotune plain {tune_p}")
+        hx, cx, wall_x, ms_x = leg(NC_PATH_PLAIN, exact=True)
+        log(f"config 4: exact reference (plain path) {wall_x:.2f} s")
+        hp, cp, wall_p, ms_p = leg(NC_PATH_PLAIN, batched=False)
+        log(f"config 4: plain x4 per slice {wall_p:.2f} s")
+        if not (np.array_equal(cm_[3], cp[3]) and np.array_equal(cm_[3], cx[3])):
+            raise SystemExit("config 4: RRR counts differ between legs")
+
+        def rel(a, b):
+            nz = b != 0
+            if np.any(a[~nz] != 0):
+                raise SystemExit("config 4: weight in bins the exact reference leaves empty")
+            return float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz]), initial=0.0))
+
+        err_m = {n: rel(hm_[i], hx[i]) for i, n in enumerate(names[:3])}
+        err_p = {n: rel(hp[i], hx[i]) for i, n in enumerate(names[:3])}
+        tot_m = {n: float(abs(hm_[i].sum() - hx[i].sum()) / abs(hx[i].sum())) for i, n in enumerate(names[:3])}
+        if max(err_m.values()) > 1e-9 or max(err_p.values()) > 1e-9:
+            raise SystemExit(f"config 4: weighted sums off the exact reference by more than 1e-9: {err_m} {err_p}")
+        cpu = None
+        if not args.no_cpu:
+            from oracle import oracle
+
+            threads = min(16, len(os.sched_getaffinity(0)))
+            rng = np.random.default_rng(4)
+            m = 128
+            while True:
+                sample = r[np.sort(rng.choice(r.shape[0], size=m, replace=False))]
+                t1 = time.perf_counter()
+                c, _ = oracle.count(sample, r, r, edges, box, survey=True, nthreads=threads)
+                dt = time.perf_counter() - t1
+                if dt > 3.0 or m >= 4096:
+                    break
+                m *= 2
+            g = ctx.count(sample, R, R, edges, box, survey=True)[1]
+            if not np.array_equal(g, c):
+                raise SystemExit("config 4: the GPU count of the CPU sample differs from the oracle")
+            cpu = {"value": float(c.sum()) / dt, "unit": "binned triplets/s", "cores": threads, "kind": "port",
+                   "sample": f"{m} random RRR primaries against all {r.shape[0]} randoms, {int(c.sum())} "
+                             f"triplets in {dt:.2f} s", "gpu_counts_bit_exact_on_sample": True}
+        for c in (D, R, DU):
+            c.close()
+    out.update({
+        "binned_triplets": trip, "binned_triplets_per_term": dict(zip(names, trip_terms)),
+        "mixed": {"wall_s": wall_m, "count_kernel_ms": ms_m, "triplets_per_s": trip / wall_m},
+        "plain": {"wall_s": wall_p, "count_kernel_ms": ms_p, "triplets_per_s": trip / wall_p},
+        "speedup_vs_plain": wall_p / wall_m,
+        "exact_reference": {"wall_s": wall_x, "count_kernel_ms": ms_x,
+                            "what": "plain FP64 path with NC_ACCUM_EXACT: the FP64 products (w1*w2)*w3 summed "
+                                    "exactly in 128-bit fixed point (each truncated by < 2^-K, K from the "
+                                    "largest weights), rounded once"},
+        "error": {"unweighted": "RRR counts bit-exact (mixed = plain = exact reference)",
+                  "weighted_rel_max_vs_exact": err_m, "weighted_rel_max_plain_vs_exact": err_p,
+                  "weighted_total_rel_vs_exact": tot_m},
+        "six_fold_terms": tri_terms, "tuning": {"mixed": tune, "plain": tune_p}, "cpu_baseline": cpu,
+    })
+    return out
+
+
 def run_rppi_survey(args):
     """rppi binning of the survey workload (500k weighted data + 2.5M randoms,
     cap/shell), rp edges 0..40 in 20 bins, 20 unit pi bins: DDD, DDR, DRR, RRR
@@ -525,6 +648,10 @@ def main():
     ap.add_argument("--boxes", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true", help="skip the 64-call plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-config4", action="store_true",
+                    help="boxes: skip the block on a subset of the large weighted survey")
+    ap.add_argument("--config4-slices", default="8,24",
+                    help="boxes: primary slices (of 32) of the large weighted survey block")
     ap.add_argument("--atomic-rate", action="store_true",
                     help="only measure scattered global atomic adds per second and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -645,10 +772,12 @@ def main():
                      "kernel_ms_median": count_ms_med},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     for c in dev:
         c.close()
     ctx.close()
+    if not args.no_config4:
+        out["config4"] = config4_subset(args, hip)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -597,12 +597,13 @@ __device__ int build_list(const KArgs &A, const double *__restrict__ X, const do
 // dense layout's no-wrap rule needs chunk * (list length)^2 < 2^31; the
 // 27-cell candidate bound is several times the list length, too loose for
 // dense catalogues.  One wave per primary, grid-stride.
-__global__ __launch_bounds__(256) void k_list_bound(KArgs A, TermDesc T, unsigned *__restrict__ out) {
+// Primaries [p0, p1) of the sorted catalogue (the primary slice counted).
+__global__ __launch_bounds__(256) void k_list_bound(KArgs A, TermDesc T, int p0, int p1, unsigned *__restrict__ out) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
     const int nwaves = (int)(gridDim.x * blockDim.x / kWave);
     int best = 0;
-    for (int p = wave; p < T.n1; p += nwaves) {
+    for (int p = p0 + wave; p < p1; p += nwaves) {
         const double px = T.x1[p], py = T.y1[p], pz = T.z1[p];
         bool bad = false;
         const bool clamp = !A.survey;
@@ -3064,8 +3065,13 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         for (const Term &t : terms) {
             const TermDesc d = term_desc(t);
             if (d.n1 == 0 || t.c2->n == 0 || t.c3->n == 0) continue;
-            const int grid = (int)std::min<long long>((d.n1 + 3) / 4, (long long)ctx->ncu * 32);
-            hipLaunchKernelGGL(k_list_bound, dim3(grid), dim3(256), 0, ctx->stream, B, d, (unsigned *)bufs.bound.p);
+            // only the primaries of the slice counted (the same split as the work queue)
+            const int p0 = (int)((long long)d.n1 * ctx->slice / ctx->nslices);
+            const int p1 = (int)((long long)d.n1 * (ctx->slice + 1) / ctx->nslices);
+            if (p1 <= p0) continue;
+            const int grid = (int)std::min<long long>((p1 - p0 + 3) / 4, (long long)ctx->ncu * 32);
+            hipLaunchKernelGGL(k_list_bound, dim3(grid), dim3(256), 0, ctx->stream, B, d, p0, p1,
+                               (unsigned *)bufs.bound.p);
         }
         NC_HIP(ctx, hipGetLastError());
         unsigned lb = 0;
