@@ -2240,14 +2240,23 @@ void dfree(DBuf &b) {
 // (the default), or carved from a fixed arena that is never freed
 // (nc_count_noalloc: the static per-device scratch array).  An arena request
 // past its end fails with `exhausted` set and allocates nothing.
+// Test knob (NC_FAIL_DEVICE_ALLOC=n at nc_create): the context's n-th device
+// allocation from then on (counting from 0) fails as out of memory, so the
+// error paths can be driven.  *fail < 0: off.
+hipError_t dgrow_checked(long long *fail, DBuf &b, size_t need) {
+    if (fail && *fail >= 0 && (need == 0 ? 16 : need) > b.bytes && (*fail)-- == 0) return hipErrorOutOfMemory;
+    return dgrow(b, need);
+}
+
 struct Mem {
     char *base = nullptr;
     size_t size = 0, used = 0;
     bool exhausted = false;
+    long long *fail = nullptr;  // the context's allocation-failure countdown
     bool arena() const { return base != nullptr; }
     size_t left() const { return size - std::min(size, (used + 255) & ~(size_t)255); }
     hipError_t grow(DBuf &b, size_t need) {
-        if (!base) return dgrow(b, need);
+        if (!base) return dgrow_checked(fail, b, need);
         if (need == 0) need = 16;
         const size_t at = (used + 255) & ~(size_t)255;
         if (at > size || need > size - at) {
@@ -2358,6 +2367,7 @@ struct nc_context {
     bool batch_split = true;  // unit-weight terms of a mixed-weight batch as their own launch
     bool tri = true;          // NC_TRI=0: terms over one catalogue never use the six-fold symmetry
     int accum = NC_ACCUM_FP64;  // nc_set_accumulation
+    long long fail_alloc = -1;  // NC_FAIL_DEVICE_ALLOC (test knob; see dgrow_checked)
     double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
     double rs_round_keys = 0.0;  // NC_RPPI_ROUND_KEYS: keys per round (tests of several rounds)
     DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_chunk_keys, rs_keys, rs_vals;
@@ -3627,6 +3637,7 @@ int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term
                 uint64_t *counts, bool use_graph) {
     const int nterms = (int)t.size();
     Mem m;  // the context's own buffers
+    m.fail = &ctx->fail_alloc;
     if (const int rc = refresh_weights(ctx, m, t)) return rc;
     if (ctx->batch_split && path == NC_PATH_MIXED && params && params->bin_type == NC_BIN_THREED &&
         params->nedges >= 2 && params->nedges <= 41) {
@@ -3708,6 +3719,7 @@ int nc_create(nc_context **out, int device) {
     }
     if (const char *e = std::getenv("NC_BATCH_SPLIT")) ctx->batch_split = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_TRI")) ctx->tri = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("NC_FAIL_DEVICE_ALLOC")) ctx->fail_alloc = std::strtoll(e, nullptr, 10);
     if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
         const unsigned long long v = std::strtoull(e, nullptr, 10);
         if (v > 0 && v <= (1ull << 31)) ctx->dense_flush = v;
@@ -3770,7 +3782,7 @@ int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog
         else
             c->wfinite = false;
     }
-    hipError_t e = dgrow(c->raw, sizeof(double) * 4 * (size_t)n);
+    hipError_t e = dgrow_checked(&ctx->fail_alloc, c->raw, sizeof(double) * 4 * (size_t)n);
     if (e == hipSuccess && n > 0)
         e = hipMemcpyAsync(c->raw.p, xyzw, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -3829,6 +3841,7 @@ int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalo
     if (!ctx) return NC_ERR_ARG;
     std::vector<Term> t{{c1, c2, c3}};
     Mem m;
+    m.fail = &ctx->fail_alloc;
     if (const int rc = refresh_weights(ctx, m, t)) return rc;
     return run_count(ctx, params, t, path, hist, counts, false, m);
 }

@@ -107,7 +107,9 @@ typedef struct {
  *   NC_DENSE_FLUSH_PAIRS=<n>  dense counts merged once n pairs were added (default and max 2^31)
  *   NC_BATCH_SPLIT=0          nc_count_batched keeps mixed-weight batches in one launch
  *   NC_TRI=0                  terms over one catalogue count each ordered triple's (p2, p3) pair
- *                             once per primary (no six-fold symmetry) */
+ *                             once per primary (no six-fold symmetry)
+ *   NC_FAIL_DEVICE_ALLOC=<n>  test knob: the context's n-th device allocation from creation on
+ *                             (counting from 0) fails as out of memory (NC_ERR_NOMEM) */
 int nc_create(nc_context **ctx, int device);
 /* Catalogues of a destroyed context may still be passed to nc_catalog_destroy
  * (in any order); nothing else. */

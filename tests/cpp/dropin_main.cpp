@@ -170,6 +170,12 @@ int main(int argc, char **argv) {
         if (m == "count" && argc >= 12) return mode_count(argc, argv, false);
         if (m == "countbins" && argc >= 12) return mode_count(argc, argv, true);
         if (m == "rppi" && argc >= 13) return mode_rppi(argc, argv);
+    } catch (const std::invalid_argument &e) {
+        std::printf("EXCEPTION invalid_argument %s\n", e.what());
+        return 3;
+    } catch (const std::runtime_error &e) {
+        std::printf("EXCEPTION runtime_error %s\n", e.what());
+        return 3;
     } catch (const std::exception &e) {
         std::printf("EXCEPTION %s\n", e.what());
         return 3;

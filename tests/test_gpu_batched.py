@@ -1,9 +1,10 @@
 """nc_count_batched: one persistent launch over many terms, captured as HIP
 graphs after the first call and replayed while the same terms come back.
 
-Every replay is checked against per-term eager counts (nc_count, plain FP64),
+Every replay is checked against the oracle (and the eager per-term counts),
 with host work and device synchronisations between replays; the four survey
-terms DDD/DDR/DRR/RRR in one batch are checked against the oracle.
+terms DDD/DDR/DRR/RRR in one batch are checked against the oracle, also on
+clustered data at 25 bins up to rmax 50.
 """
 import ctypes
 
@@ -29,7 +30,9 @@ def test_batched_boxes_replays(ctx):
     box = [L, L, L]
     pts = [neyman_scott_box(n, 40, 4.0, L, seed=100 + n) for n in (150, 400, 900)]
     cats = [ctx.catalog(p) for p in pts]
-    want = np.stack([ctx.count(c, c, c, edges, box, path=NC_PATH_PLAIN)[1] for c in cats])
+    want = np.stack([oracle.count(p, p, p, edges, box)[0] for p in pts])  # the oracle, per box
+    eager = np.stack([ctx.count(c, c, c, edges, box, path=NC_PATH_PLAIN)[1] for c in cats])
+    assert np.array_equal(eager, want)
     terms = [(c, c, c) for c in cats]
     for rep in range(4):
         hist, cnt = ctx.count_batched(terms, edges, box, path=NC_PATH_MIXED)
@@ -213,3 +216,31 @@ def test_wrapped_device_catalogue_weights_change(ctx):
         C.close()
     finally:
         hip.hipFree(dev)
+
+
+def test_batched_survey_four_terms_clustered_25_bins(ctx):
+    """The large survey's shape at a size the oracle covers: clustered data
+    (Neyman-Scott, sigma 4) with lognormal weights and uniform randoms in a
+    shrunk cap, rmax 50 in 25 bins, the four terms in one batch on both paths,
+    replayed: counts bit-exact, weighted sums within 1e-9 of the oracle."""
+    print("batched survey clustered 25 bins", flush=True)
+    d, r = survey_cap(2500, 6000, seed=29, clustered=True, sigma=4.0)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.15
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 50.0, 25)
+    want = [oracle.count(a, b, c, edges, box, survey=True) for a, b, c in ((d, d, d), (d, d, r), (d, r, r), (r, r, r))]
+    assert want[3][0].sum() > 10000
+    D, R = ctx.catalog(d), ctx.catalog(r)
+    terms = [(D, D, D), (D, D, R), (D, R, R), (R, R, R)]
+    for path in (NC_PATH_PLAIN, NC_PATH_MIXED):
+        for rep in range(2):
+            hist, cnt = ctx.count_batched(terms, edges, box, survey=True, path=path)
+            assert np.array_equal(cnt[3], want[3][0]), f"path {path} replay {rep}"
+            for i in range(3):
+                ws = want[i][1]
+                nz = ws != 0
+                assert np.array_equal(hist[i][~nz], ws[~nz])
+                assert np.max(np.abs(hist[i][nz] / ws[nz] - 1.0)) <= 1e-9
+    D.close()
+    R.close()

@@ -90,3 +90,48 @@ def test_dropin_non_diagonal_bin_position(dropin, tmp_path, precision):
                                     [1.5, 3.5, 3.5], [2.5, 2.5, 2.5], [2.5, 2.5, 3.5], [2.5, 3.5, 3.5],
                                     [3.5, 3.5, 3.5]]
     assert rows[:, 0].tolist() == [0, 0, 6, 0, 0, 0, 0, 0, 0]
+
+
+def test_dropin_allocation_failure_is_runtime_error(dropin, tmp_path):
+    """A device allocation failing inside calculateD1D2D3 (the library's test
+    knob NC_FAIL_DEVICE_ALLOC=0 fails the context's first one) reaches the
+    caller as std::runtime_error naming the failed call, as the reference's
+    gpuErrchk does (gpuerrchk.h:9-14)."""
+    p = uniform_box(500, 50.0, seed=3)
+    f = str(tmp_path / "P.bin")
+    p.tofile(f)
+    args = [dropin, "count", f, f, f, "0", "0", "50.0", "50.0", "50.0"] + [repr(float(e)) for e in edges_linear(0.0, 5.0, 5)]
+    print("dropin allocation failure", flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, NC_FAIL_DEVICE_ALLOC="0"))
+    assert out.returncode == 3, out.stdout + out.stderr
+    assert out.stdout.startswith("EXCEPTION runtime_error calculateD1D2D3:"), out.stdout
+    assert "out of memory" in out.stdout
+
+
+@pytest.mark.parametrize("nth", [0, 3, 9])
+def test_allocation_failure_leaves_context_usable(monkeypatch, nth):
+    """NC_FAIL_DEVICE_ALLOC=n fails the n-th device allocation of a fresh
+    context (catalogue upload or a buffer of the count): the call returns
+    NC_ERR_NOMEM, and the next calls on the same context count correctly."""
+    from cudancorr_amd import Context, NcError
+
+    monkeypatch.setenv("NC_FAIL_DEVICE_ALLOC", str(nth))
+    p = uniform_box(3000, 80.0, seed=5)
+    edges = edges_linear(0.0, 8.0, 8)
+    box = [80.0] * 3
+    want, _ = oracle.count(p, p, p, edges, box)
+    print("allocation failure at", nth, flush=True)
+    with Context(0) as c:
+        failed = False
+        for _ in range(12):  # the countdown passes the failing allocation within these calls
+            try:
+                _, got = c.count(p, p, p, edges, box)
+            except NcError as e:
+                assert e.code == 4 and "out of memory" in str(e)
+                failed = True
+                continue
+            assert np.array_equal(got, want)
+        assert failed
+        _, got = c.count(p, p, p, edges, box)
+        assert np.array_equal(got, want)

@@ -105,7 +105,7 @@ def test_batched_replays(ctx):
     q[:, 2] = np.mod(q[:, 2], 60.0)
     P, Q = ctx.catalog(p), ctx.catalog(q)
     terms = [(P, P, P), (P, P, Q), (Q, Q, Q)]
-    want = [ctx.count(*t, edges, box, npi=npi)[1] for t in terms]
+    want = [oracle.count_rppi(a, b, c, edges, npi, box)[0] for a, b, c in ((p, p, p), (p, p, q), (q, q, q))]
     for rep in range(3):
         _, cnt = ctx.count_batched(terms, edges, box, path=NC_PATH_PLAIN, npi=npi)
         for i in range(3):
