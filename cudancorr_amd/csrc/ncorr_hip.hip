@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
@@ -400,11 +401,12 @@ struct CatRef {
 constexpr int kSelAll = 0, kSelSplit = 1, kSelLow = 2;
 
 template <int LM, int SEL = kSelAll>
-__device__ int build_list(const KArgs &A, const double *__restrict__ X, const double *__restrict__ Y,
-                          const double *__restrict__ Z, const double *__restrict__ Wt, const uint32_t *__restrict__ st,
-                          double px, double py, double pz, int cx, int cy, int cz, char *__restrict__ base, int lane,
-                          const double *sq, const uint16_t *lut, int pidx = 0, char *__restrict__ base_b = nullptr,
-                          int *nb_out = nullptr, int first = 0) {
+__device__ __forceinline__ int build_list(const KArgs &A, const double *__restrict__ X,
+                                          const double *__restrict__ Y, const double *__restrict__ Z,
+                                          const double *__restrict__ Wt, const uint32_t *__restrict__ st, double px,
+                                          double py, double pz, int cx, int cy, int cz, char *__restrict__ base,
+                                          int lane, const double *sq, const uint16_t *lut, int pidx = 0,
+                                          char *__restrict__ base_b = nullptr, int *nb_out = nullptr, int first = 0) {
     constexpr bool MIXED = LM == kListMixed;
     int n = SEL == kSelLow ? first : 0;
     int nbl = 0;
@@ -714,43 +716,40 @@ __device__ __forceinline__ Sides sort3(float a, float b, float c) {
 }
 
 // Histogram update of one accepted pair at a precomputed slot (byte offset).
-// inc: ordered triplets the pair stands for.  With the primary p1 (weight w1),
+// ORD: ordered triplets the pair stands for.  With the primary p1 (weight w1),
 // the pair's row entry a and lane entry b (w1a = w1*wa, w1b = w1*wb), the
 // reference adds one product (p1*p2)*p3 per order (device.cu:305):
-//   inc 1  (p1; a, b)                          (w1 wa) wb
-//   inc 2  and (p1; b, a)                      + (w1 wb) wa
-//   inc 6  and the four orders led by a or b:  2 [(w1 wa) wb + (w1 wb) wa + (wa wb) w1]
+//   ORD 1  (p1; a, b)                          (w1 wa) wb
+//   ORD 2  and (p1; b, a)                      + (w1 wb) wa
+//   ORD 6  and the four orders led by a or b:  2 [(w1 wa) wb + (w1 wb) wa + (wa wb) w1]
 // (the products of orders that differ by swapping the first two factors are
-// equal).  Counts add inc.
+// equal).  Counts add ORD.
 __device__ __forceinline__ long long fixed_point(double p, double scale) { return (long long)(p * scale); }
-template <bool SYM, bool WEIGHTED, int HM>
+template <int ORD, bool WEIGHTED, int HM>
 __device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1, double w1a, double wa, double w1b,
-                                         double wb, unsigned inc) {
+                                         double wb) {
+    static_assert(ORD == 1 || ORD == 2 || ORD == 6, "orders per pair");
     char *h = reinterpret_cast<char *>(t.hist) + off;
     if constexpr (HM == kHistDense) {
         static_assert(!WEIGHTED, "dense counts are for unit weights");
-        atomicAdd(reinterpret_cast<unsigned *>(h), inc);
+        atomicAdd(reinterpret_cast<unsigned *>(h), (unsigned)ORD);
     } else if constexpr (WEIGHTED && HM == kHistExact) {
         // 128-bit sum: the low word's wrap (from the value it had) carries
         // into the high word, whatever the order of the additions
         long long f = fixed_point(w1a * wb, t.xscale);
-        if (inc != 1) {
-            f += fixed_point(w1b * wa, t.xscale);
-            if (inc == 6) f = 2 * (f + fixed_point((wa * wb) * w1, t.xscale));
-        }
+        if constexpr (ORD != 1) f += fixed_point(w1b * wa, t.xscale);
+        if constexpr (ORD == 6) f = 2 * (f + fixed_point((wa * wb) * w1, t.xscale));
         const unsigned long long u = (unsigned long long)f;
         const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long *>(h), u);
         const long long carry = (f < 0 ? -1 : 0) + (old + u < old ? 1 : 0);
         if (carry != 0) atomicAdd(reinterpret_cast<unsigned long long *>(h + t.xhi), (unsigned long long)carry);
     } else if constexpr (WEIGHTED) {
         double v = w1a * wb;
-        if (inc != 1) {
-            v += w1b * wa;
-            if (inc == 6) v = 2.0 * (v + (wa * wb) * w1);
-        }
+        if constexpr (ORD != 1) v += w1b * wa;
+        if constexpr (ORD == 6) v = 2.0 * (v + (wa * wb) * w1);
         atomicAdd(reinterpret_cast<double *>(h), v);
     } else {
-        atomicAdd(reinterpret_cast<unsigned long long *>(h), (unsigned long long)inc);
+        atomicAdd(reinterpret_cast<unsigned long long *>(h), (unsigned long long)ORD);
     }
 }
 
@@ -772,10 +771,10 @@ __device__ __forceinline__ int bin_any(const KArgs &A, const LdsTabs &t, double 
 // Exact FP64 evaluation of one pair (device.cu:299-307 restated): r23 from
 // the image positions, strict range filter, exact bin, FP32 triangle test
 // with the correctly rounded sqrtf.
-template <bool SYM, bool WEIGHTED, int HM>
+template <int ORD, bool WEIGHTED, int HM>
 __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, double ax, double ay, double az, int ka,
                                            float fa, double w1, double w1a, double wa, double bx, double by, double bz,
-                                           int kb, float fb, double w1b, double wb, bool ok, unsigned inc) {
+                                           int kb, float fb, double w1b, double wb, bool ok) {
     const double d = sqrsep(ax - bx, ay - by, az - bz);
     if (ok && d > A.r2min && d < A.r2max) {
         const int k = bin_any<false>(A, t, d);
@@ -784,7 +783,7 @@ __device__ __forceinline__ void exact_pair(const KArgs &A, const LdsTabs &t, dou
         const float hi = fmaxf(fmaxf(fa, fb), f);
         const float mid = __builtin_amdgcn_fmed3f(fa, fb, f);
         if (lo + mid > hi)  // the reference rejects sqrtf(r1)+sqrtf(r2) <= sqrtf(r3), device.cu:173
-            hist_add<SYM, WEIGHTED, HM>(t, sorted_slot<HM>(A, t, ka, kb, k), w1, w1a, wa, w1b, wb, inc);
+            hist_add<ORD, WEIGHTED, HM>(t, sorted_slot<HM>(A, t, ka, kb, k), w1, w1a, wa, w1b, wb);
     }
 }
 
@@ -852,7 +851,7 @@ __device__ __forceinline__ void traverse(Pol &P, int n2, int n3) {
 // FP32 triangle test with the hardware sqrt (<= 1 ulp) -- redone with the
 // correctly rounded sqrtf when the slack is within 8 ulp of the largest side
 // -- and the histogram updates.
-template <bool SYM, bool WEIGHTED, int HM>
+template <bool SYM, int ORD, bool WEIGHTED, int HM>
 struct PlainPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 2;
@@ -867,11 +866,10 @@ struct PlainPol {
     Elem eb;
     double wb, w1b;
     bool vb;
-    unsigned inc;
 
     __device__ PlainPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2, const char *l3, char *wlds,
-                        int lane_, unsigned inc_)
-        : A(A_), t(t_), w1(w1_), lane(lane_), inc(inc_) {
+                        int lane_)
+        : A(A_), t(t_), w1(w1_), lane(lane_) {
         C = A.chunk_elems;
         L2 = reinterpret_cast<const Elem *>(l2);
         W2 = reinterpret_cast<const double *>(l2 + (size_t)A.cap * sizeof(Elem));
@@ -963,9 +961,9 @@ struct PlainPol {
             if (acc[u]) {
                 const int slot = pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]);
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1, w1b, wb, w1 * wa[u], wa[u], inc);
+                    hist_add<ORD, WEIGHTED, HM>(t, slot, w1, w1b, wb, w1 * wa[u], wa[u]);
                 else
-                    hist_add<SYM, WEIGHTED, HM>(t, slot, w1, w1 * wa[u], wa[u], w1b, wb, inc);
+                    hist_add<ORD, WEIGHTED, HM>(t, slot, w1, w1 * wa[u], wa[u], w1b, wb);
             }
     }
 
@@ -1025,10 +1023,10 @@ struct QueueEntry<kHistDense> {
 // Mixed path, deferred exact rechecks: every queued (S2 entry, S3 entry) pair
 // is evaluated with the FP64 image positions (catalogue point + periodic
 // shift of its neighbour entry, rebuilt as in build_list), one pair per lane.
-template <bool SYM, bool WEIGHTED, int HM>
+template <int ORD, bool WEIGHTED, int HM>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
                             const CatRef &c2, const CatRef &c3, int cx, int cy, int cz,
-                            const typename QueueEntry<HM>::T *q, int qn, int lane, unsigned inc) {
+                            const typename QueueEntry<HM>::T *q, int qn, int lane) {
     const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
     wave_sync();
     for (int i = lane; i < qn; i += kWave) {
@@ -1039,9 +1037,9 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
         neighbour(A, cx, cy, cz, ka.x >> 8, sax, say, saz);
         neighbour(A, cx, cy, cz, kb.x >> 8, sbx, sby, sbz);
         const double wa = WEIGHTED ? c2.w[ja] : 0.0, wb = WEIGHTED ? c3.w[jb] : 0.0;
-        exact_pair<SYM, WEIGHTED, HM>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
+        exact_pair<ORD, WEIGHTED, HM>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
                                         __int_as_float(ka.y), w1, w1 * wa, wa, c3.x[jb] + sbx, c3.y[jb] + sby,
-                                        c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true, inc);
+                                        c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true);
     }
     wave_sync();
 }
@@ -1057,7 +1055,7 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
 // the error bound (make_mixed_tables), is queued for the exact FP64
 // evaluation.
 // Rows: float4 {r, s} + bin + weight in LDS; lanes in registers.
-template <bool SYM, bool WEIGHTED, int HM>
+template <bool SYM, int ORD, bool WEIGHTED, int HM>
 struct MixedPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 4;
@@ -1084,12 +1082,10 @@ struct MixedPol {
     int kb, bidx;
     double wb, w1b;
     bool vb;
-    unsigned inc;
 
     __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
-                        int lane_, const CatRef &cat2, const CatRef &cat3, int cx_, int cy_, int cz_, unsigned inc_)
-        : A(A_), t(t_), c2(cat2), c3(cat3), l2(l2_), l3(l3_), cx(cx_), cy(cy_), cz(cz_), w1(w1_), lane(lane_),
-          inc(inc_) {
+                        int lane_, const CatRef &cat2, const CatRef &cat3, int cx_, int cy_, int cz_)
+        : A(A_), t(t_), c2(cat2), c3(cat3), l2(l2_), l3(l3_), cx(cx_), cy(cy_), cz(cz_), w1(w1_), lane(lane_) {
         C = A.chunk_elems;
         const int Cp = C + 4;
         m2 = mlist(l2, A.cap);
@@ -1147,7 +1143,7 @@ struct MixedPol {
     }
 
     __device__ void drain() {
-        drain_queue<SYM, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane, inc);
+        drain_queue<ORD, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane);
         qn = 0;
     }
 
@@ -1187,9 +1183,9 @@ struct MixedPol {
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
                 if (SWAP)
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1, w1b, wb, w1 * wa[u], wa[u], inc);
+                    hist_add<ORD, WEIGHTED, HM>(t, slot[u], w1, w1b, wb, w1 * wa[u], wa[u]);
                 else
-                    hist_add<SYM, WEIGHTED, HM>(t, slot[u], w1, w1 * wa[u], wa[u], w1b, wb, inc);
+                    hist_add<ORD, WEIGHTED, HM>(t, slot[u], w1, w1 * wa[u], wa[u], w1b, wb);
             }
         bool any = false;
 #pragma unroll
@@ -1254,8 +1250,8 @@ struct MixedPol {
 // Merge the workgroup's LDS histogram into the term's global compact
 // histogram and clear it.  Dense counts go to the sorted slot of their bins.
 template <int HM>
-__device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long long *hist,
-                           unsigned long long *pairs) {
+__device__ __forceinline__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long long *hist,
+                                           unsigned long long *pairs) {
     __syncthreads();
     unsigned long long *out = A.out + (long long)term * A.out_stride;
     if constexpr (HM == kHistExact) {
@@ -1298,16 +1294,18 @@ __device__ void flush_hist(const KArgs &A, int term, int weighted, unsigned long
     __syncthreads();
 }
 
-template <bool MIXED, int HM, bool SYM, bool WEIGHTED>
+// ORD: ordered triplets per evaluated pair (hist_add); by default 2 for a
+// symmetric traversal (p2 and p3 from one list) and 1 otherwise.
+template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? 2 : 1>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
-                                          const CatRef &c3, int cx, int cy, int cz, unsigned inc) {
+                                          const CatRef &c3, int cx, int cy, int cz) {
     if constexpr (MIXED) {
-        MixedPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz, inc);
+        MixedPol<SYM, ORD, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz);
         traverse(P, n2, n3);
         if (P.qn > 0) P.drain();
     } else {
-        PlainPol<SYM, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, inc);
+        PlainPol<SYM, ORD, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane);
         traverse(P, n2, n3);
     }
 }
@@ -1335,9 +1333,9 @@ __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, doub
 // which is its own share of the reference's ordered count (hist_add forms the
 // weight products of those orders).  Returns false on a list overflow.
 template <bool MIXED, int HM, bool WEIGHTED>
-__device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T, int p, double px, double py,
-                            double pz, double pw, int cx, int cy, int cz, char *l2, char *l3, char *wlds, int lane,
-                            unsigned long long &wpairs) {
+__device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T, int p,
+                                            double px, double py, double pz, double pw, int cx, int cy, int cz,
+                                            char *l2, char *l3, char *wlds, int lane, unsigned long long &wpairs) {
     constexpr int LM = MIXED ? kListMixed : kListPlain;
     int nb = 0;
     const int nh = uniform(build_list<LM, kSelSplit>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2,
@@ -1346,7 +1344,7 @@ __device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T,
     if (nh + nbu > A.nmax) return false;
     wave_sync();
     const CatRef c = {T.x2, T.y2, T.z2, T.w2};
-    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz, 6u);
+    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, 6>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz);
     int nrow = nh;
     if (nbu > 0) {
         nrow = uniform(build_list<LM, kSelLow>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
@@ -1354,8 +1352,8 @@ __device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T,
         if (nrow + nbu > A.nmax) return false;
         wave_sync();
         if (nrow > 0)
-            pair_loop<MIXED, HM, false, WEIGHTED>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
-        if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz, 2u);
+            pair_loop<MIXED, HM, false, WEIGHTED, 2>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz);
+        if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz);
     }
     if constexpr (HM == kHistDense)
         wpairs += 3ull * (unsigned long long)nh * (unsigned long long)max(nh - 1, 0) +
@@ -1374,9 +1372,18 @@ __device__ bool tri_primary(const KArgs &A, const LdsTabs &t, const TermDesc &T,
 // The unit-weight mixed variant is held to 96 VGPRs (five waves per SIMD):
 // its pair loop is latency bound, and with four-wave workgroups of about
 // 31 KB five of them share a CU (20 waves instead of 16).
-template <bool MIXED, int HM>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MIXED && HM == kHistDense ? 5 : 4))) void
+// WT: the launch holds terms with non-unit weights.  Such a variant carries
+// the six-fold path for those terms only (its unit-weight terms take the
+// two-fold path); a WT = false variant carries no weighted code.  Both
+// six-fold variants in one kernel would not fit its 128 VGPRs.
+// The mixed exact-accumulation variant (128-bit slot adds) gets 168 VGPRs
+// (three waves per SIMD, workgroups of at most 12 waves) rather than spill.
+template <bool MIXED, int HM, bool WT>
+__global__ __launch_bounds__(MIXED && HM == kHistExact ? 768 : 1024) __attribute__((
+    amdgpu_waves_per_eu(MIXED && HM == kHistDense ? 5 : MIXED && HM == kHistExact ? 3 : 4))) void
 k_triplets(KArgs A) {
+    static_assert(!(WT && HM == kHistDense), "dense counts are for unit weights");
+    static_assert(WT || HM != kHistExact, "exact accumulation is for weighted terms");
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
@@ -1463,21 +1470,11 @@ k_triplets(KArgs A) {
             }
             // the six-fold path, unless the primary lies in a last cell (box)
             if (T.tri && (A.survey || (cx != A.nc[0] - 1 && cy != A.nc[1] - 1 && cz != A.nc[2] - 1))) {
-                bool ok;
-                if constexpr (HM == kHistDense) {
-                    ok = true;
-                    if (T.weighted) {  // never planned so
-                        if (lane == 0) atomicOr(A.err, kErrMode);
-                    } else {
-                        ok = tri_primary<MIXED, HM, false>(A, t, T, p, px, py, pz, pw, cx, cy, cz, l2, l3, wlds, lane,
-                                                           wpairs);
-                    }
-                } else if (T.weighted) {
-                    ok = tri_primary<MIXED, HM, true>(A, t, T, p, px, py, pz, pw, cx, cy, cz, l2, l3, wlds, lane,
-                                                      wpairs);
+                bool ok = true;
+                if (T.weighted != (int)WT) {  // never planned so
+                    if (lane == 0) atomicOr(A.err, kErrMode);
                 } else {
-                    ok = tri_primary<MIXED, HM, false>(A, t, T, p, px, py, pz, pw, cx, cy, cz, l2, l3, wlds, lane,
-                                                       wpairs);
+                    ok = tri_primary<MIXED, HM, WT>(A, t, T, p, px, py, pz, pw, cx, cy, cz, l2, l3, wlds, lane, wpairs);
                 }
                 if (!ok && lane == 0) atomicOr(A.err, kErrOverflow);
                 wave_sync();
@@ -1501,28 +1498,29 @@ k_triplets(KArgs A) {
             const char *lb = T.sym ? l2 : l3;
             const CatRef c2 = {T.x2, T.y2, T.z2, T.w2};
             const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3};
-            if constexpr (HM == kHistDense) {
+            if constexpr (!WT) {
                 if (T.weighted) {  // never planned so
                     if (lane == 0) atomicOr(A.err, kErrMode);
                     continue;
                 }
+            }
+            if constexpr (HM == kHistDense) {
                 wpairs += T.sym ? (unsigned long long)n2u * (unsigned long long)(n2u - 1)
                                 : (unsigned long long)n2u * (unsigned long long)n3u;
             }
             if (T.sym) {
                 if (T.weighted) {
-                    if constexpr (HM != kHistDense)
-                        pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz, 2u);
+                    if constexpr (WT)
+                        pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
                 } else {
-                    pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz, 2u);
+                    pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
                 }
             } else {
                 if (T.weighted) {
-                    if constexpr (HM != kHistDense)
-                        pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz,
-                                                          1u);
+                    if constexpr (WT)
+                        pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
                 } else {
-                    pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz, 1u);
+                    pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
                 }
             }
             wave_sync();
@@ -2189,14 +2187,18 @@ __global__ __launch_bounds__(1024) void k_rppi_bin(KArgs A) {
     }
 }
 
-template __global__ void k_triplets<false, kHistCtab>(KArgs);
-template __global__ void k_triplets<false, kHistT3>(KArgs);
-template __global__ void k_triplets<false, kHistDense>(KArgs);
-template __global__ void k_triplets<true, kHistCtab>(KArgs);
-template __global__ void k_triplets<true, kHistT3>(KArgs);
-template __global__ void k_triplets<true, kHistDense>(KArgs);
-template __global__ void k_triplets<false, kHistExact>(KArgs);
-template __global__ void k_triplets<true, kHistExact>(KArgs);
+template __global__ void k_triplets<false, kHistCtab, false>(KArgs);
+template __global__ void k_triplets<false, kHistT3, false>(KArgs);
+template __global__ void k_triplets<false, kHistDense, false>(KArgs);
+template __global__ void k_triplets<true, kHistCtab, false>(KArgs);
+template __global__ void k_triplets<true, kHistT3, false>(KArgs);
+template __global__ void k_triplets<true, kHistDense, false>(KArgs);
+template __global__ void k_triplets<false, kHistCtab, true>(KArgs);
+template __global__ void k_triplets<false, kHistT3, true>(KArgs);
+template __global__ void k_triplets<true, kHistCtab, true>(KArgs);
+template __global__ void k_triplets<true, kHistT3, true>(KArgs);
+template __global__ void k_triplets<false, kHistExact, true>(KArgs);
+template __global__ void k_triplets<true, kHistExact, true>(KArgs);
 
 // ---------------------------------------------------------------------------
 // Scratch of nc_count_noalloc: device memory of a fixed size that the library
@@ -2360,6 +2362,9 @@ struct nc_context {
     int sample_stride = 1;    // autotuner: queue only every n-th work item
     int slice = 0, nslices = 1;  // nc_set_primary_slice: primaries of slice `slice` of `nslices` only
     bool autotuning = false;  // in nc_autotune's timed runs
+    // longest list of each launch's terms (keyed by their catalogues), found
+    // once per nc_autotune call: its runs share the inputs
+    std::map<std::vector<uint64_t>, int> tune_list_bound;
     std::string dev_name;
     // rppi key stream: buffers, and the per-bucket capacities last sized for
     // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
@@ -2774,7 +2779,7 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
         retry = true;
         return NC_OK;
     }
-    if (*p.h_err & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
+    if (*p.h_err & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: a term does not match the weighting of its kernel variant");
     convert_out(p.nb, p.nbc, p.rppi, nterms, p.weighted, p.h_out, hist, counts, p.xk);
     return NC_OK;
 }
@@ -3027,9 +3032,12 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // once when its gridding is exact (threeD only).
     std::vector<char> tri(nterms, 0);
     int ntri = 0;
+    // A launch with weighted terms runs its unit-weight terms on the two-fold
+    // path (its kernel variant carries the six-fold path for weighted terms).
     for (int i = 0; i < nterms; ++i) {
         const Term &t = terms[i];
-        tri[i] = ctx->tri && !G.rppi && t.c1 == t.c2 && t.c2 == t.c3 && t.c1->exact && t.c1->n > 0;
+        tri[i] = ctx->tri && !G.rppi && t.c1 == t.c2 && t.c2 == t.c3 && t.c1->exact && t.c1->n > 0 &&
+                 (all_unit || !t.c1->unit_w);
         ntri += tri[i];
     }
     for (CatGrid &g : grids) {
@@ -3058,8 +3066,11 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     int nmax = cap;
     // (the mixed path takes the dense layout over a whole CU, the plain path
     // only within half of one)
-    if (dense_cand && (double)ctx->chunk * mult * (double)cap * (double)cap >= kWrap &&
-        fits(kHistDense, mixed ? ctx->lds_hw : ctx->lds_max)) {
+    const bool need_bound = dense_cand && (double)ctx->chunk * mult * (double)cap * (double)cap >= kWrap &&
+                            fits(kHistDense, mixed ? ctx->lds_hw : ctx->lds_max);
+    if (need_bound && ctx->autotuning && ctx->tune_list_bound.count(uids)) {
+        nmax = ctx->tune_list_bound[uids];
+    } else if (need_bound) {
         KArgs B;
         std::memset(&B, 0, sizeof(B));
         for (int a = 0; a < 3; ++a) {
@@ -3088,6 +3099,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         NC_HIP(ctx, hipMemcpyAsync(&lb, bufs.bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
         NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
         nmax = std::min(cap, (int)std::max(1u, lb));
+        if (ctx->autotuning) ctx->tune_list_bound[uids] = nmax;
     }
     // primaries per work item: the tuned chunk, cut where the dense layout
     // needs it so that one work item cannot add 2^31 (merges follow work items)
@@ -3136,16 +3148,20 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const size_t wave_bytes = wave_lds(C, hm);
     const size_t lds_bytes = lds_tab + W * wave_bytes;
     if (lds_bytes > budget) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
-    const void *kfns[2][3] = {
-        {(const void *)k_triplets<false, kHistCtab>, (const void *)k_triplets<false, kHistT3>,
-         (const void *)k_triplets<false, kHistDense>},
-        {(const void *)k_triplets<true, kHistCtab>, (const void *)k_triplets<true, kHistT3>,
-         (const void *)k_triplets<true, kHistDense>}};
+    // [weighted terms present][mixed][layout]
+    const void *kfns[2][2][3] = {
+        {{(const void *)k_triplets<false, kHistCtab, false>, (const void *)k_triplets<false, kHistT3, false>,
+          (const void *)k_triplets<false, kHistDense, false>},
+         {(const void *)k_triplets<true, kHistCtab, false>, (const void *)k_triplets<true, kHistT3, false>,
+          (const void *)k_triplets<true, kHistDense, false>}},
+        {{(const void *)k_triplets<false, kHistCtab, true>, (const void *)k_triplets<false, kHistT3, true>, nullptr},
+         {(const void *)k_triplets<true, kHistCtab, true>, (const void *)k_triplets<true, kHistT3, true>, nullptr}}};
     const void *kfn = G.rppi ? (G.survey ? (stream ? (const void *)k_rppi<true, true> : (const void *)k_rppi<true, false>)
                                          : (stream ? (const void *)k_rppi<false, true> : (const void *)k_rppi<false, false>))
-                      : hm == kHistExact ? (mixed ? (const void *)k_triplets<true, kHistExact>
-                                                  : (const void *)k_triplets<false, kHistExact>)
-                                         : kfns[mixed ? 1 : 0][hm];
+                      : hm == kHistExact ? (mixed ? (const void *)k_triplets<true, kHistExact, true>
+                                                  : (const void *)k_triplets<false, kHistExact, true>)
+                                         : kfns[all_unit ? 0 : 1][mixed ? 1 : 0][hm];
+    if (kfn == nullptr) return fail(ctx, NC_ERR_LIMIT, "no kernel variant for the planned layout");
     {
         // some variants are compiled for smaller workgroups (more VGPRs per lane)
         hipFuncAttributes fa;
@@ -3525,7 +3541,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     }
     if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
-    if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: weighted term in a dense-count launch");
+    if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: a term does not match the weighting of its kernel variant");
 
     convert_out(G.nb, G.out_words, G.rppi, nterms, weighted, res.data(), hist, counts, xk);
 
@@ -4053,6 +4069,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     const long long items = std::max<long long>(1, nprim / 16);
     ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
     ctx->autotuning = true;
+    ctx->tune_list_bound.clear();
     const int saved_slice[2] = {ctx->slice, ctx->nslices};  // the sample is drawn from all primaries
     ctx->slice = 0;
     ctx->nslices = 1;
@@ -4103,6 +4120,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
 done:
     ctx->sample_stride = 1;
     ctx->autotuning = false;
+    ctx->tune_list_bound.clear();
     ctx->slice = saved_slice[0];
     ctx->nslices = saved_slice[1];
     plan_drop(ctx);

@@ -18,7 +18,7 @@ namespace nck {
 constexpr int kWave = 64;
 constexpr unsigned kErrDomain = 1u;
 constexpr unsigned kErrOverflow = 2u;
-constexpr unsigned kErrMode = 4u;  // a weighted term in a dense-count launch
+constexpr unsigned kErrMode = 4u;  // a term the kernel variant was not planned for (weighting)
 // a catalogue planned as exactly gridded (every cell index the exact floor of
 // coordinate / cell size) no longer is: a replayed plan must be rebuilt
 constexpr unsigned kErrReplan = 8u;

@@ -221,7 +221,7 @@ enum nc_accum { NC_ACCUM_FP64 = 0, NC_ACCUM_EXACT = 1 };
 int nc_set_accumulation(nc_context *ctx, int mode);
 
 /* Primary slices (no counterpart in the reference, whose kernel takes every
- * primary of p1 in one launch, cudaNcorr.cu:383-389): later counts on ctx
+ * primary of p1 in one launch, cudaNcorr.cu:327-345): later counts on ctx
  * take, for every term, only the primaries of slice `slice` of `nslices`
  * equal slices of c1 in cell-sorted order; the nslices results add up to the
  * full count (exactly for counts, to rounding for weighted sums).  The

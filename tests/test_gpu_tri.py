@@ -124,6 +124,37 @@ def test_survey_terms_weighted(ctx):
                 assert np.max(np.abs(h[nz] / ws[nz] - 1.0)) <= 1e-9, name
 
 
+def test_mixed_weight_launch_six_fold_for_weighted_terms(monkeypatch):
+    """NC_BATCH_SPLIT=0: DDD (weighted) and RRR (unit weights) in one launch.
+    Its kernel variant carries the six-fold path for weighted terms only, so
+    DDD takes it and RRR the two-fold path; both equal the oracle."""
+    from cudancorr_amd import Context
+
+    _log("tri mixed-weight launch")
+    monkeypatch.setenv("NC_BATCH_SPLIT", "0")
+    d, r = survey_cap(2500, 4000, seed=41)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 40.0, 10)
+    wc_d, ws_d = oracle.count(d, d, d, edges, box, survey=True)
+    wc_r, _ = oracle.count(r, r, r, edges, box, survey=True)
+    with Context(0) as c:
+        D, R = c.catalog(d), c.catalog(r)
+        try:
+            for path in PATHS:
+                h, cnt = c.count_batched([(D, D, D), (R, R, R)], edges, box, survey=True, path=path)
+                t = c.timing()
+                assert t.hist_layout in (0, 1) and t.tri_terms == 1, path  # one sorted-slot launch
+                assert np.array_equal(cnt[1], wc_r), path
+                nz = ws_d != 0
+                assert np.array_equal(h[0][~nz], ws_d[~nz])
+                assert np.max(np.abs(h[0][nz] / ws_d[nz] - 1.0)) <= 1e-9, path
+        finally:
+            D.close()
+            R.close()
+
+
 def test_dense_merges_under_six_fold_counts(monkeypatch):
     """Dense 32-bit counts with increments of 6, merged after every work item
     (NC_DENSE_FLUSH_PAIRS=1): the same counts."""
@@ -182,7 +213,9 @@ def test_primary_slices_add_up(ctx, nslices):
             for k in range(nslices):
                 ctx.set_primary_slice(k, nslices)
                 h, c = ctx.count_batched(terms, edges, box, survey=True, path=path)
-                assert ctx.timing().tri_terms == 2  # DDD and RRR
+                # DDD and RRR; the plain path keeps the batch in one launch,
+                # where the unit-weight RRR takes the two-fold path
+                assert ctx.timing().tri_terms == (2 if path == NC_PATH_MIXED else 1)
                 h_sum += h
                 c_sum += c
             ctx.set_primary_slice(0, 1)
