@@ -43,7 +43,8 @@ class NcTiming(ctypes.Structure):
     _fields_ = [("ms_build", ctypes.c_float), ("ms_count", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("n_kernel", ctypes.c_int64),
                 ("hist_layout", ctypes.c_int32),
-                ("lds_bytes", ctypes.c_int32), ("path", ctypes.c_int32), ("tri_terms", ctypes.c_int32)]
+                ("lds_bytes", ctypes.c_int32), ("path", ctypes.c_int32), ("tri_terms", ctypes.c_int32),
+                ("swap_terms", ctypes.c_int32)]
 
 
 class NcTuning(ctypes.Structure):

@@ -716,40 +716,45 @@ __device__ __forceinline__ Sides sort3(float a, float b, float c) {
 }
 
 // Histogram update of one accepted pair at a precomputed slot (byte offset).
-// ORD: ordered triplets the pair stands for.  With the primary p1 (weight w1),
-// the pair's row entry a and lane entry b (w1a = w1*wa, w1b = w1*wb), the
-// reference adds one product (p1*p2)*p3 per order (device.cu:305):
-//   ORD 1  (p1; a, b)                          (w1 wa) wb
-//   ORD 2  and (p1; b, a)                      + (w1 wb) wa
-//   ORD 6  and the four orders led by a or b:  2 [(w1 wa) wb + (w1 wb) wa + (wa wb) w1]
+// ORD: the ordered triplets the pair stands for.  With the primary p1 (weight
+// w1), the pair's list-2 entry a and list-3 entry b (w1a = w1*wa, w1b =
+// w1*wb), the reference adds one product (p1*p2)*p3 per order (device.cu:305):
+//   kOrdOne   (p1; a, b)                          (w1 wa) wb
+//   kOrdSym   and (p1; b, a)                      + (w1 wb) wa
+//   kOrdSix   and the four orders led by a or b:  2 [(w1 wa) wb + (w1 wb) wa + (wa wb) w1]
+//   kOrdSwap  (p1; a, b) and (a; p1, b)           2 (w1 wa) wb
 // (the products of orders that differ by swapping the first two factors are
-// equal).  Counts add ORD.
+// equal).  Counts add the number of orders.
+constexpr int kOrdOne = 1, kOrdSym = 2, kOrdSix = 6, kOrdSwap = 12;
+constexpr int ord_count(int ord) { return ord == kOrdSwap ? 2 : ord; }
 __device__ __forceinline__ long long fixed_point(double p, double scale) { return (long long)(p * scale); }
 template <int ORD, bool WEIGHTED, int HM>
 __device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1, double w1a, double wa, double w1b,
                                          double wb) {
-    static_assert(ORD == 1 || ORD == 2 || ORD == 6, "orders per pair");
+    static_assert(ORD == kOrdOne || ORD == kOrdSym || ORD == kOrdSix || ORD == kOrdSwap, "orders per pair");
     char *h = reinterpret_cast<char *>(t.hist) + off;
     if constexpr (HM == kHistDense) {
         static_assert(!WEIGHTED, "dense counts are for unit weights");
-        atomicAdd(reinterpret_cast<unsigned *>(h), (unsigned)ORD);
+        atomicAdd(reinterpret_cast<unsigned *>(h), (unsigned)ord_count(ORD));
     } else if constexpr (WEIGHTED && HM == kHistExact) {
         // 128-bit sum: the low word's wrap (from the value it had) carries
         // into the high word, whatever the order of the additions
         long long f = fixed_point(w1a * wb, t.xscale);
-        if constexpr (ORD != 1) f += fixed_point(w1b * wa, t.xscale);
-        if constexpr (ORD == 6) f = 2 * (f + fixed_point((wa * wb) * w1, t.xscale));
+        if constexpr (ORD == kOrdSym || ORD == kOrdSix) f += fixed_point(w1b * wa, t.xscale);
+        if constexpr (ORD == kOrdSix) f = 2 * (f + fixed_point((wa * wb) * w1, t.xscale));
+        if constexpr (ORD == kOrdSwap) f *= 2;
         const unsigned long long u = (unsigned long long)f;
         const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long *>(h), u);
         const long long carry = (f < 0 ? -1 : 0) + (old + u < old ? 1 : 0);
         if (carry != 0) atomicAdd(reinterpret_cast<unsigned long long *>(h + t.xhi), (unsigned long long)carry);
     } else if constexpr (WEIGHTED) {
         double v = w1a * wb;
-        if constexpr (ORD != 1) v += w1b * wa;
-        if constexpr (ORD == 6) v = 2.0 * (v + (wa * wb) * w1);
+        if constexpr (ORD == kOrdSym || ORD == kOrdSix) v += w1b * wa;
+        if constexpr (ORD == kOrdSix) v = 2.0 * (v + (wa * wb) * w1);
+        if constexpr (ORD == kOrdSwap) v *= 2.0;
         atomicAdd(reinterpret_cast<double *>(h), v);
     } else {
-        atomicAdd(reinterpret_cast<unsigned long long *>(h), (unsigned long long)ORD);
+        atomicAdd(reinterpret_cast<unsigned long long *>(h), (unsigned long long)ord_count(ORD));
     }
 }
 
@@ -1294,9 +1299,9 @@ __device__ __forceinline__ void flush_hist(const KArgs &A, int term, int weighte
     __syncthreads();
 }
 
-// ORD: ordered triplets per evaluated pair (hist_add); by default 2 for a
-// symmetric traversal (p2 and p3 from one list) and 1 otherwise.
-template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? 2 : 1>
+// ORD: ordered triplets per evaluated pair (hist_add); by default kOrdSym for
+// a symmetric traversal (p2 and p3 from one list) and kOrdOne otherwise.
+template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? kOrdSym : kOrdOne>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
                                           const CatRef &c3, int cx, int cy, int cz) {
@@ -1344,7 +1349,7 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
     if (nh + nbu > A.nmax) return false;
     wave_sync();
     const CatRef c = {T.x2, T.y2, T.z2, T.w2};
-    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, 6>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz);
+    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, kOrdSix>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz);
     int nrow = nh;
     if (nbu > 0) {
         nrow = uniform(build_list<LM, kSelLow>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
@@ -1352,7 +1357,7 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
         if (nrow + nbu > A.nmax) return false;
         wave_sync();
         if (nrow > 0)
-            pair_loop<MIXED, HM, false, WEIGHTED, 2>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz);
+            pair_loop<MIXED, HM, false, WEIGHTED, kOrdSym>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz);
         if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz);
     }
     if constexpr (HM == kHistDense)
@@ -1469,7 +1474,7 @@ k_triplets(KArgs A) {
                 continue;
             }
             // the six-fold path, unless the primary lies in a last cell (box)
-            if (T.tri && (A.survey || (cx != A.nc[0] - 1 && cy != A.nc[1] - 1 && cz != A.nc[2] - 1))) {
+            if (T.tri == kTriSix && (A.survey || (cx != A.nc[0] - 1 && cy != A.nc[1] - 1 && cz != A.nc[2] - 1))) {
                 bool ok = true;
                 if (T.weighted != (int)WT) {  // never planned so
                     if (lane == 0) atomicOr(A.err, kErrMode);
@@ -1480,13 +1485,19 @@ k_triplets(KArgs A) {
                 wave_sync();
                 continue;
             }
-            const int n2 =
-                build_list<MIXED ? kListMixed : kListPlain>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz,
-                                                            l2, lane, t.sq, t.lut);
-            const int n3 = T.sym ? n2
-                                 : build_list<MIXED ? kListMixed : kListPlain>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py,
-                                                                               pz, cx, cy, cz, l3,
-                                                     lane, t.sq, t.lut);
+            // kTriSwap (surveys: no last cells): list 2 holds the entries of
+            // index above the primary's
+            constexpr int LM = MIXED ? kListMixed : kListPlain;
+            int n2;
+            if (T.tri == kTriSwap) {
+                int nb_none = 0;
+                n2 = build_list<LM, kSelSplit>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
+                                               t.lut, p, l3, &nb_none);
+            } else {
+                n2 = build_list<LM>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq, t.lut);
+            }
+            const int n3 =
+                T.sym ? n2 : build_list<LM>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3, lane, t.sq, t.lut);
             // wave-uniform lengths keep the pair loops' control in scalar registers
             const int n2u = uniform(n2), n3u = uniform(n3);
             if (n2u > A.nmax || n3u > A.nmax) {
@@ -1505,8 +1516,8 @@ k_triplets(KArgs A) {
                 }
             }
             if constexpr (HM == kHistDense) {
-                wpairs += T.sym ? (unsigned long long)n2u * (unsigned long long)(n2u - 1)
-                                : (unsigned long long)n2u * (unsigned long long)n3u;
+                const unsigned long long a2 = (unsigned long long)n2u, a3 = (unsigned long long)n3u;
+                wpairs += T.tri == kTriSwap ? 2ull * a2 * a3 : T.sym ? a2 * (a2 - 1ull) : a2 * a3;
             }
             if (T.sym) {
                 if (T.weighted) {
@@ -1515,6 +1526,13 @@ k_triplets(KArgs A) {
                 } else {
                     pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
                 }
+            } else if (T.tri == kTriSwap) {
+                // (the planner takes this only for terms of the variant's weighting)
+                if (T.weighted == (int)WT)
+                    pair_loop<MIXED, HM, false, WT, kOrdSwap>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy,
+                                                              cz);
+                else if (lane == 0)
+                    atomicOr(A.err, kErrMode);
             } else {
                 if (T.weighted) {
                     if constexpr (WT)
@@ -2332,7 +2350,7 @@ struct BatchPlan {
     unsigned long long *h_out = nullptr;
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
-    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0, tri_terms = 0;
+    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0, tri_terms = 0, swap_terms = 0;
     int xk = kNoExact;  // exact layout: fixed-point scale exponent
     long long nbc = 0;  // output words per term (slots; twice as many u64 words in the exact layout)
 };
@@ -2773,6 +2791,7 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     ctx->timing.lds_bytes = p.lds_bytes;
     ctx->timing.path = p.path;
     ctx->timing.tri_terms = p.tri_terms;
+    ctx->timing.swap_terms = p.swap_terms;
     if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (*p.h_err & (kErrOverflow | kErrReplan)) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
         plan_drop(ctx);
@@ -3029,16 +3048,19 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     const int cap = (int)((std::max(1u, host_flags[1]) + 15u) & ~15u);
     for (size_t gi = 0; gi < grids.size(); ++gi) grids[gi].cat->exact = cflags[gi] == 0u;
     // Terms over one catalogue (c1 == c2 == c3) count every unordered triple
-    // once when its gridding is exact (threeD only).
-    std::vector<char> tri(nterms, 0);
-    int ntri = 0;
-    // A launch with weighted terms runs its unit-weight terms on the two-fold
-    // path (its kernel variant carries the six-fold path for weighted terms).
+    // once when its gridding is exact (threeD only); survey terms with c1 ==
+    // c2 != c3 every unordered (p1, p2) pair.  A launch with weighted terms
+    // runs its unit-weight terms on the plain two-fold path (its kernel
+    // variant carries these paths for weighted terms only).
+    std::vector<char> tri(nterms, kTriNone);
+    int ntri = 0, nswap = 0;
     for (int i = 0; i < nterms; ++i) {
         const Term &t = terms[i];
-        tri[i] = ctx->tri && !G.rppi && t.c1 == t.c2 && t.c2 == t.c3 && t.c1->exact && t.c1->n > 0 &&
-                 (all_unit || !t.c1->unit_w);
-        ntri += tri[i];
+        const bool own = all_unit || !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
+        const bool one = ctx->tri && !G.rppi && own && t.c1 == t.c2 && t.c1->exact && t.c1->n > 0;
+        tri[i] = !one ? kTriNone : t.c2 == t.c3 ? kTriSix : G.survey ? kTriSwap : kTriNone;
+        ntri += tri[i] == kTriSix;
+        nswap += tri[i] == kTriSwap;
     }
     for (CatGrid &g : grids) {
         g.expect_exact = false;
@@ -3059,9 +3081,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     // only the exact lengths can admit the dense layout, the longest list
     // k_list_bound finds (one extra pass over the primaries).
     // A primary adds at most n(n-1) to the counts (n: its list length), three
-    // times that on the six-fold path.
+    // times that on the six-fold path, twice on the pair-swap path.
     const double kWrap = 2147483648.0;
-    const double mult = ntri > 0 ? 3.0 : 1.0;
+    const double mult = ntri > 0 ? 3.0 : nswap > 0 ? 2.0 : 1.0;
     const bool dense_cand = !G.rppi && (mixed || G.lut_single_step) && all_unit;
     int nmax = cap;
     // (the mixed path takes the dense layout over a whole CU, the plain path
@@ -3514,6 +3536,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.lds_bytes = (int32_t)lds_bytes;
     ctx->timing.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
     ctx->timing.tri_terms = ntri;
+    ctx->timing.swap_terms = nswap;
     // key stream: keep the region sizes; after an overflow (counted, but
     // through direct atomics) scale the buckets that overflowed by what this
     // run reserved, and capture no graph so that the next call uses that
@@ -3563,6 +3586,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.lds_bytes = (int)lds_bytes;
         p.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
         p.tri_terms = ntri;
+        p.swap_terms = nswap;
         bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
                       hipSuccess &&
                   hipHostMalloc((void **)&p.h_err, 16) == hipSuccess;
@@ -3694,6 +3718,7 @@ int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term
             ctx->timing.lds_bytes = tm[0].lds_bytes;
             ctx->timing.path = tm[0].path;
             ctx->timing.tri_terms = tm[0].tri_terms + tm[1].tri_terms;
+            ctx->timing.swap_terms = tm[0].swap_terms + tm[1].swap_terms;
             return NC_OK;
         }
     }

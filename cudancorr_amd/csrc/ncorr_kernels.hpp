@@ -33,6 +33,8 @@ struct __attribute__((aligned(16))) Elem {
     float f;
 };
 
+constexpr int kTriNone = 0, kTriSix = 1, kTriSwap = 2;  // TermDesc::tri
+
 struct TermDesc {
     const double *x1, *y1, *z1, *w1;
     const double *x2, *y2, *z2, *w2;
@@ -42,9 +44,12 @@ struct TermDesc {
     int n1;
     int sym;       // c2 == c3: count unordered (p2,p3) pairs once, doubled
     int weighted;  // accumulate (w1*w2)*w3 in FP64, else integer counts
-    // c1 == c2 == c3, exactly gridded: every unordered triple of points outside
-    // the last cells is evaluated once, from its lowest-index point, and
-    // counted six times (see k_triplets)
+    // kTriSix: c1 == c2 == c3, exactly gridded: every unordered triple of
+    // points outside the last cells is evaluated once, from its lowest-index
+    // point, and counted six times (see k_triplets).
+    // kTriSwap: survey, c1 == c2 != c3, exactly gridded: the triplets
+    // (p1, p2, p3) and (p2, p1, p3) are evaluated once, from the lower index
+    // of p1 and p2, and counted twice.
     int tri;
 };
 

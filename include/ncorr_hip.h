@@ -95,6 +95,9 @@ typedef struct {
     int32_t tri_terms;   /* terms over one catalogue (c1 == c2 == c3) whose unordered triples away from the
                             last cells were each evaluated once and counted six times (every cell index the
                             exact floor of coordinate / cell size; threeD; NC_TRI=0 turns it off) */
+    int32_t swap_terms;  /* survey terms whose first two catalogues are one (c1 == c2 != c3, DDR): the
+                            triplets (p1, p2, p3) and (p2, p1, p3) were evaluated once, from the lower
+                            index, and counted twice (the same conditions as tri_terms) */
 } nc_timing;
 
 /* Create a context on HIP device `device` (replaces the implicit CUDA context

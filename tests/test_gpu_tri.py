@@ -103,8 +103,8 @@ def test_inexact_gridding_falls_back(ctx):
 
 
 def test_survey_terms_weighted(ctx):
-    """DDD (weighted) and RRR take the six-fold path; DDR and DRR do not (their
-    catalogues differ).  Counts exact, weighted sums within 1e-9."""
+    """DDD (weighted) and RRR take the six-fold path, DDR the pair-swap path;
+    DRR neither.  Counts exact, weighted sums within 1e-9."""
     _log("tri survey terms")
     d, r = survey_cap(3000, 5000, seed=23)
     for c in (d, r):
@@ -115,6 +115,7 @@ def test_survey_terms_weighted(ctx):
         for name, (a, b, c) in {"DDD": (d, d, d), "RRR": (r, r, r), "DDR": (d, d, r)}.items():
             h, cnt = ctx.count(a, b, c, edges, box, survey=True, path=path)
             assert ctx.timing().tri_terms == (1 if name != "DDR" else 0)
+            assert ctx.timing().swap_terms == (1 if name == "DDR" else 0)
             wc, ws = oracle.count(a, b, c, edges, box, survey=True)
             if name == "RRR":
                 assert np.array_equal(cnt, wc)
@@ -153,6 +154,54 @@ def test_mixed_weight_launch_six_fold_for_weighted_terms(monkeypatch):
         finally:
             D.close()
             R.close()
+
+
+@pytest.mark.parametrize("unit", [False, True])
+def test_survey_pair_swap_terms(ctx, ctx_notri, unit):
+    """Survey terms with c1 == c2 != c3 (DDR): the triplets (p1, p2, p3) and
+    (p2, p1, p3) are evaluated once, from the lower index, and counted twice
+    (weighted: twice the product (w1 w2) w3).  Weighted sums within 1e-9,
+    counts bit-exact, against the oracle and against NC_TRI=0; a clustered
+    catalogue so that many pairs of D lie in one cell."""
+    _log(f"tri pair swap unit={unit}")
+    d, r = survey_cap(2500, 4000, seed=43, clustered=True, sigma=3.0, n_children=50)
+    for c in (d, r):
+        c[:, :3] = 1000.0 + (c[:, :3] - 1000.0) * 0.12
+    if unit:
+        d[:, 3] = 1.0
+    box, _ = prep_cat_survey(d, r)
+    edges = edges_linear(0.0, 40.0, 10)
+    wc, ws = oracle.count(d, d, r, edges, box, survey=True)
+    assert wc.sum() > 0 and np.all(wc % 2 == 0)
+    for path in PATHS:
+        h, cnt = ctx.count(d, d, r, edges, box, survey=True, path=path)
+        t = ctx.timing()
+        assert t.swap_terms == 1 and t.tri_terms == 0, path
+        h0, cnt0 = ctx_notri.count(d, d, r, edges, box, survey=True, path=path)
+        assert ctx_notri.timing().swap_terms == 0
+        if unit:
+            assert np.array_equal(cnt, wc) and np.array_equal(cnt0, wc), path
+        else:
+            nz = ws != 0
+            for got in (h, h0):
+                assert np.array_equal(got[~nz], ws[~nz])
+                assert np.max(np.abs(got[nz] / ws[nz] - 1.0)) <= 1e-9, path
+
+
+def test_pair_swap_off_in_boxes(ctx):
+    """A box term with c1 == c2 != c3 keeps the ordered path (a wrap or a last
+    cell can break the p1/p2 symmetry) and equals the oracle."""
+    _log("tri pair swap box")
+    L = 110.0
+    a = neyman_scott_box(60, 30, 3.0, L, seed=47)
+    b = uniform_box(2500, L, seed=48)
+    edges = edges_linear(1.0, 20.0, 8)
+    box = np.array([L, L, L])
+    want, _ = oracle.count(a, a, b, edges, box)
+    for path in PATHS:
+        _, got = ctx.count(a, a, b, edges, box, path=path)
+        assert ctx.timing().swap_terms == 0
+        assert np.array_equal(got, want), path
 
 
 def test_dense_merges_under_six_fold_counts(monkeypatch):
@@ -216,6 +265,7 @@ def test_primary_slices_add_up(ctx, nslices):
                 # DDD and RRR; the plain path keeps the batch in one launch,
                 # where the unit-weight RRR takes the two-fold path
                 assert ctx.timing().tri_terms == (2 if path == NC_PATH_MIXED else 1)
+                assert ctx.timing().swap_terms == 1  # DDR
                 h_sum += h
                 c_sum += c
             ctx.set_primary_slice(0, 1)
