@@ -717,8 +717,10 @@ def main():
     plain = None
     tune_p = None
     if not args.no_plain:
-        tune_p = ctx.autotune(terms, edges, box, path=NC_PATH_PLAIN)
-        log(f"autotune plain: {tune_p}")
+        # tuned on a launch of the kind this leg runs: one box, the largest
+        big = dev[int(np.argmax([c.shape[0] for c in cats]))]
+        tune_p = ctx.autotune([(big, big, big)], edges, box, path=NC_PATH_PLAIN)
+        log(f"autotune plain (largest box): {tune_p}")
         for c in dev[:1]:
             ctx.count(c, c, c, edges, box, path=NC_PATH_PLAIN)
         device_sync(hip)

@@ -684,6 +684,18 @@ __device__ __forceinline__ int shl_add(int a, int b) {
     }
 }
 
+// A wave-uniform 32-bit value held in a VGPR (the compiler would keep it in a
+// scalar register).  Inside the pair loops the scalar registers run out, and
+// a constant spilled to a VGPR lane costs a v_readlane_b32 (a VALU issue slot)
+// at every use; a VGPR operand costs nothing.
+template <typename T>
+__device__ __forceinline__ T in_vgpr(T x) {
+    static_assert(sizeof(T) == 4, "32-bit values");
+    unsigned r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(__builtin_bit_cast(unsigned, x)));
+    return __builtin_bit_cast(T, r);
+}
+
 // The same with the third bin in layout units: 4k (kHistDense), 2k (kHistT3)
 // or k (kHistCtab).
 template <int HM>
@@ -1104,11 +1116,12 @@ struct MixedPol {
         qn = 0;
         row_first = 0;
         lb = t.lut32 - A.lut32_c0;
-        lo32 = A.lut32_lo;
-        sh32 = A.lut32_shift;
-        g32 = A.g32;
-        tri_t = A.tri_t;
-        nbx = (unsigned)A.nb << kShift;
+        // the pair loop's constants as VGPR operands
+        lo32 = in_vgpr(A.lut32_lo);
+        sh32 = in_vgpr(A.lut32_shift);
+        g32 = in_vgpr(A.g32);
+        tri_t = in_vgpr(A.tri_t);
+        nbx = in_vgpr((unsigned)A.nb << kShift);
     }
 
     __device__ void stage(bool from3, int first, int last) {
@@ -1392,6 +1405,7 @@ k_triplets(KArgs A) {
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
+    __shared__ int s_seg;  // segments drained so far, counted from this workgroup's own
     __shared__ unsigned long long s_pairs;
     LdsTabs t;
     t.hist = reinterpret_cast<unsigned long long *>(lds);
@@ -1409,7 +1423,10 @@ k_triplets(KArgs A) {
     t.xhi = A.nbc * 8;
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int i = tid; i < A.nslots; i += nthr) t.hist[i] = 0ull;
-    if (tid == 0) s_pairs = 0ull;
+    if (tid == 0) {
+        s_pairs = 0ull;
+        s_seg = 0;
+    }
     if constexpr (MIXED) {
         // FP64 tables are only read by the rare exact rechecks: global memory
         t.sq = A.sq;
@@ -1438,9 +1455,23 @@ k_triplets(KArgs A) {
     int cur = -1, cur_weighted = 0;
     unsigned long long wpairs = 0ull;  // pairs this wave added in the current chunk
     __syncthreads();
+    const int home = (int)(blockIdx.x % (unsigned)A.nseg);
     for (;;) {
         if (tid == 0) {
-            s_chunk = (int)atomicAdd(A.counter, 1u);
+            // the next chunk of the first segment, from this workgroup's own
+            // on, that is not drained (counters only grow: a drained segment
+            // stays drained)
+            int ci = A.nchunks;
+            for (int k = s_seg; k < A.nseg; ++k) {
+                const int sg = home + k < A.nseg ? home + k : home + k - A.nseg;
+                const unsigned v = atomicAdd(A.counter + sg, 1u);
+                if (v < (unsigned)(A.seg[sg + 1] - A.seg[sg])) {
+                    ci = A.seg[sg] + (int)v;
+                    break;
+                }
+                s_seg = k + 1;
+            }
+            s_chunk = ci;
             s_next = 0;
         }
         __syncthreads();
@@ -2324,6 +2355,10 @@ struct Geometry {
 
 }  // namespace
 
+// The autotuner times contiguous runs of work items, so that the workgroups
+// running at one time share neighbourhoods (in L2) as in a full count.
+constexpr long long kSampleRun = 64;
+
 struct nc_catalog {
     nc_context *ctx = nullptr;  // null once the context is destroyed
     int device = 0;
@@ -2377,7 +2412,7 @@ struct nc_context {
     // dense counts are merged once this many pairs were added (< 2^31 keeps
     // u32 counts from wrapping; NC_DENSE_FLUSH_PAIRS lowers it for tests)
     unsigned long long dense_flush = 1ull << 31;
-    int sample_stride = 1;    // autotuner: queue only every n-th work item
+    int sample_stride = 1;    // autotuner: queue only every n-th run of kSampleRun work items
     int slice = 0, nslices = 1;  // nc_set_primary_slice: primaries of slice `slice` of `nslices` only
     bool autotuning = false;  // in nc_autotune's timed runs
     // longest list of each launch's terms (keyed by their catalogues), found
@@ -2984,7 +3019,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     NC_MEM(ctx, m, bufs.tables, tab_bytes);
     NC_HIP(ctx, hipMemcpyAsync(bufs.tables.p, tabs.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
     NC_MEM(ctx, m, bufs.errw, 16);
-    NC_MEM(ctx, m, bufs.counter, 16);
+    // 16 queue counters (zeroed before every launch), then the queue's
+    // kMaxSeg + 1 segment bounds
+    NC_MEM(ctx, m, bufs.counter, sizeof(unsigned) * (16 + kMaxSeg + 1));
     NC_MEM(ctx, m, bufs.bound, 16);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.bound.p, 0u, 4LL);
     for (const Term &t : terms)
@@ -3209,7 +3246,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         const int p0 = (int)((long long)d.n1 * ctx->slice / ctx->nslices);
         const int p1 = (int)((long long)d.n1 * (ctx->slice + 1) / ctx->nslices);
         for (int b = p0; b < p1; b += qchunk, ++nitems)
-            if (nitems % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(p1, b + qchunk), 0});
+            if ((nitems / kSampleRun) % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(p1, b + qchunk), 0});
     }
     NC_MEM(ctx, m, bufs.terms, sizeof(TermDesc) * td.size());
     NC_MEM(ctx, m, bufs.chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size()));
@@ -3250,6 +3287,33 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.chunks = (const Chunk *)bufs.chunks.p;
     A.nchunks = (int)chunks.size();
     A.counter = (unsigned *)bufs.counter.p;
+    // Queue segments, one per XCD (workgroup b runs on XCD b % 8): each XCD
+    // works through its own run of cell-ordered chunks, so the neighbourhoods
+    // its primaries scan stay in its L2.  The chunk list is cut at equal
+    // estimated work (primaries x |c2| x |c3|); drained segments are helped
+    // with by the other XCDs.
+    std::vector<int> seg(kMaxSeg + 1, 0);  // host copy, kept (like chunks) until the copy has run
+    {
+        const int nseg = std::max(1, std::min(kMaxSeg, grid));
+        std::vector<double> cw(chunks.size());
+        double wtot = 0.0;
+        for (size_t c = 0; c < chunks.size(); ++c) {
+            const Term &t = terms[chunks[c].term];
+            cw[c] = (double)(chunks[c].end - chunks[c].begin) * (double)t.c2->n * (double)t.c3->n;
+            wtot += cw[c];
+        }
+        int sgi = 1;
+        double acc = 0.0;
+        for (size_t c = 0; c < chunks.size() && sgi < nseg; ++c) {
+            acc += cw[c];
+            while (sgi < nseg && acc >= wtot * sgi / nseg) seg[sgi++] = (int)c + 1;
+        }
+        while (sgi <= nseg) seg[sgi++] = (int)chunks.size();
+        A.nseg = nseg;
+        A.seg = (const int *)(A.counter + 16);
+        NC_HIP(ctx, hipMemcpyAsync((void *)A.seg, seg.data(), sizeof(int) * (size_t)(nseg + 1), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    }
     A.err = (unsigned *)bufs.errw.p;
     A.out = (unsigned long long *)bufs.out.p;
     A.scratch = (char *)bufs.scratch.p;
@@ -3355,7 +3419,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             As.rs_dry = 1;
             As.rs_cursor = (unsigned long long *)ctx->rs_cursor.p;
             As.rs_chunk_keys = (unsigned long long *)ctx->rs_chunk_keys.p;
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 4LL);
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 16LL);
             hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, As.rs_cursor, 0ull, (long long)rs_nbk);
             hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, As.rs_chunk_keys, 0ull,
                                (long long)sample.size());
@@ -3493,7 +3557,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     auto enqueue_count = [&]() -> int {
         hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)bufs.out.p, 0ull,
                            (long long)nterms * ostride);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 4LL);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 16LL);
         if (A.nchunks > 0 && !stream) {
             void *args[] = {&A};
             if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
@@ -3507,7 +3571,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
                 void *args[] = {&Ar};
                 if (r > 0)
                     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u,
-                                       4LL);
+                                       16LL);
                 if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
                     return NC_ERR_HIP;
                 if (Ar.rs_nitems > 0 && hipLaunchKernel((const void *)k_rppi_bin, dim3(Ar.rs_nitems), dim3(1024),
@@ -3669,17 +3733,20 @@ int refresh_weights(nc_context *ctx, Mem &m, const std::vector<Term> &terms) {
 }
 
 // A count over terms, as nc_count_batched runs it.  Unit-weight terms beside
-// weighted ones (the survey's RRR beside DDD, DDR, DRR): on the mixed path the
-// unit-weight ones may take the dense u32 layout, which the weighted ones
-// cannot, so they run as a launch of their own whenever a dense histogram
-// could fit in a CU's LDS.  Those two launches are not captured in a graph.
+// weighted ones (the survey's RRR beside DDD, DDR, DRR) may take the dense u32
+// layout, which the weighted ones cannot, and the six-fold path, which a
+// launch with weighted terms keeps for those: they run as a launch of their
+// own whenever a dense histogram could fit in a CU's LDS, on either path (so
+// both paths give every primary the same share of the triplets, and the
+// primary slices of the two paths agree).  Those two launches are not
+// captured in a graph.
 int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
                 uint64_t *counts, bool use_graph) {
     const int nterms = (int)t.size();
     Mem m;  // the context's own buffers
     m.fail = &ctx->fail_alloc;
     if (const int rc = refresh_weights(ctx, m, t)) return rc;
-    if (ctx->batch_split && path == NC_PATH_MIXED && params && params->bin_type == NC_BIN_THREED &&
+    if (ctx->batch_split && params && params->bin_type == NC_BIN_THREED &&
         params->nedges >= 2 && params->nedges <= 41) {
         std::vector<int> iu, iw;
         for (int i = 0; i < nterms; ++i) {

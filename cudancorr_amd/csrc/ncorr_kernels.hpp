@@ -72,10 +72,19 @@ struct RBItem {
     int pad;
 };
 
+// Work-queue segments: one per XCD (the chip's eight L2 domains).
+constexpr int kMaxSeg = 8;
+
 struct KArgs {
     const TermDesc *terms;
     const Chunk *chunks;
     int nchunks;
+    // chunks [seg[s], seg[s+1]) form segment s, pulled through counter[s];
+    // workgroup b starts on segment b % nseg (its XCD) and moves on to the
+    // next segment once its own is drained (seg in device memory: a runtime
+    // index into an array of the kernel arguments would copy them to scratch)
+    int nseg;
+    const int *seg;
     unsigned *counter;
     unsigned *err;
     unsigned long long *out;  // nterms x nbc, u64 counts or f64 sums (bit pattern)

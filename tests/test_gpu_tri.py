@@ -262,9 +262,7 @@ def test_primary_slices_add_up(ctx, nslices):
             for k in range(nslices):
                 ctx.set_primary_slice(k, nslices)
                 h, c = ctx.count_batched(terms, edges, box, survey=True, path=path)
-                # DDD and RRR; the plain path keeps the batch in one launch,
-                # where the unit-weight RRR takes the two-fold path
-                assert ctx.timing().tri_terms == (2 if path == NC_PATH_MIXED else 1)
+                assert ctx.timing().tri_terms == 2  # DDD and RRR (its own launch on both paths)
                 assert ctx.timing().swap_terms == 1  # DDR
                 h_sum += h
                 c_sum += c
