@@ -379,14 +379,12 @@ struct CatRef {
 // whose squared separation lies in (r2min, r2max) into the list at `base`.
 // Returns the list length (entries beyond cap are not stored).
 //
-// The 27 neighbour entries are visited in the reference's x-major order
-// (cudaNcorr.cu:37-48); cells are numbered z-fastest, so the three entries
-// dz = -1, 0, 1 of one (dx, dy) are consecutive cells and, when none of them
-// wraps (box) or leaves the grid (survey: those are skipped), one contiguous
-// point range with one image shift.  Such a range is scanned as a whole, two
-// 64-point blocks per step (both loads in flight together); the list order is
-// the reference's entry order either way.  Each list entry records a
-// neighbour entry s whose shift is its own (the middle one for a merged range).
+// The 27 neighbour entries follow the reference's x-major order
+// (cudaNcorr.cu:37-48), and so does the list.  Their cell ranges are laid end
+// to end and scanned as one sequence, 256 candidates per step (four 64-point
+// blocks with their loads in flight together), so that short cells leave no
+// lanes idle and a primary waits for few memory round trips.  Each list entry
+// records its neighbour entry s (0..26), which fixes its image shift.
 //
 // Selections (SEL) for the six-fold path of terms over one catalogue, where
 // pidx is the primary's index in the sorted catalogue and an entry is "last"
@@ -509,87 +507,105 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             n += __popcll(m);
         }
     };
-    // points [beg, end) of the sorted catalogue, image shift (shx, shy, shz);
-    // an entry is last when the range is (last_all) or from index jl on
-    auto scan = [&](int beg, int end, double shx, double shy, double shz, int s, bool last_all, int jl) {
-        for (int j0 = beg; j0 < end; j0 += 2 * kWave) {
-            const int ja = j0 + lane, jb = ja + kWave;
-            const bool va = ja < end, vb = jb < end;
-            double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0, bx = 0.0, by = 0.0, bz = 0.0, bw = 0.0;
-            // image position: point + shift (device.cu:282-284)
-            // (weights only for the plain list: the mixed list reads them from
-            // the catalogue where they are needed)
-            if (va) {
-                ax = X[ja] + shx;
-                ay = Y[ja] + shy;
-                az = Z[ja] + shz;
-                if (!MIXED) aw = Wt[ja];
-            }
-            if (vb) {
-                bx = X[jb] + shx;
-                by = Y[jb] + shy;
-                bz = Z[jb] + shz;
-                if (!MIXED) bw = Wt[jb];
-            }
-            keep(ja, va, ax, ay, az, aw, s, last_all | (ja >= jl));
-            if (j0 + kWave < end) keep(jb, vb, bx, by, bz, bw, s, last_all | (jb >= jl));
-        }
-    };
-    constexpr int kNoLast = 0x7fffffff;
-    // The part of a range [beg, end) of neighbour row `rc` that a selection
-    // needs: rows whose (x, y) cell lies before the primary's (rc 0) hold
-    // lower indices only, the primary's own row (rc 1) splits at pidx (its
-    // last entries, in the last z cell, lie above pidx: the primary is not in
-    // a last cell), later rows (rc 2) hold higher indices only.  Last entries
-    // are the whole range (last_all) or those from jl on.
-    auto scan_sel = [&](int beg, int end, double shx, double shy, double shz, int s, bool last_all, int jl, int rc) {
-        if constexpr (SEL == kSelSplit) {
-            if (rc == 0 && !last_all) beg = jl < end ? jl : end;  // only its last entries
-            if (rc == 1) beg = max(beg, pidx + 1);
-        } else if constexpr (SEL == kSelLow) {
-            if (rc == 2 || (rc == 0 && last_all)) return;
-            end = min(end, rc == 1 ? pidx : jl);
-        }
-        scan(beg, end, shx, shy, shz, s, last_all, jl);
-    };
-    const int nz = A.nc[2];
-    for (int dx = -1; dx <= 1; ++dx)
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int s0 = (dx + 1) * 9 + (dy + 1) * 3;  // entry of dz = -1
-            int ex = cx + dx, ey = cy + dy;
-            double shx = 0.0, shy = 0.0;
+    // The 27 neighbour entries, one per lane (lane e: entry e of the
+    // reference's x-major order, cudaNcorr.cu:37-48): the entry's cell after
+    // the one periodic wrap (box: image shift of +-L per wrapped axis,
+    // device.cu:89-122) or nothing (survey: outside the grid), its range of
+    // the sorted catalogue cut to the selection, and whether it is a last
+    // cell.  All 27 ranges load at once.
+    int cbeg = 0, clen = 0, cpk = 0;  // cpk: shift signs (2 bits per axis) | last << 6
+    if (lane < 27) {
+        const int d[3] = {lane / 9 - 1, (lane / 3) % 3 - 1, lane % 3 - 1};
+        const int c0[3] = {cx, cy, cz};
+        int e[3];
+        bool in = true, last = false;
+        for (int a = 0; a < 3; ++a) {
+            e[a] = c0[a] + d[a];
             if (A.survey) {
-                if (ex < 0 || ex >= A.nc[0] || ey < 0 || ey >= A.nc[1]) continue;
-                const int row = nz * (ey + A.nc[1] * ex);
-                const int z0 = max(cz - 1, 0), z1 = min(cz + 1, nz - 1);
-                const int rc = ex < cx || (ex == cx && ey < cy) ? 0 : ex == cx && ey == cy ? 1 : 2;
-                scan_sel(uniform((int)st[row + z0]), uniform((int)st[row + z1 + 1]), 0.0, 0.0, 0.0, s0 + 1, false,
-                         kNoLast, rc);
-                continue;
-            }
-            if (ex == A.nc[0]) { ex = 0; shx = A.box[0]; }
-            if (ey == A.nc[1]) { ey = 0; shy = A.box[1]; }
-            if (ex == -1) { ex = A.nc[0] - 1; shx = -A.box[0]; }
-            if (ey == -1) { ey = A.nc[1] - 1; shy = -A.box[1]; }
-            const int row = nz * (ey + A.nc[1] * ex);
-            const bool lastxy = ex == A.nc[0] - 1 || ey == A.nc[1] - 1;
-            const int rc = ex < cx || (ex == cx && ey < cy) ? 0 : ex == cx && ey == cy ? 1 : 2;
-            if (cz >= 1 && cz + 1 < nz) {
-                // the merged range reaches the last z cell when cz + 1 is it
-                const int jl = cz + 2 == nz ? uniform((int)st[row + nz - 1]) : kNoLast;
-                scan_sel(uniform((int)st[row + cz - 1]), uniform((int)st[row + cz + 2]), shx, shy, 0.0, s0 + 1,
-                         lastxy, jl, rc);
-                continue;
-            }
-            for (int dz = -1; dz <= 1; ++dz) {  // a z wrap: one cell at a time
-                int ez = cz + dz;
-                double shz = 0.0;
-                if (ez == nz) { ez = 0; shz = A.box[2]; }
-                if (ez == -1) { ez = nz - 1; shz = -A.box[2]; }
-                scan_sel(uniform((int)st[row + ez]), uniform((int)st[row + ez + 1]), shx, shy, shz, s0 + dz + 1,
-                         lastxy || ez == nz - 1, kNoLast, rc);
+                in = in && e[a] >= 0 && e[a] < A.nc[a];
+            } else {
+                if (e[a] == A.nc[a]) {
+                    e[a] = 0;
+                    cpk |= 1 << (2 * a);  // +L
+                } else if (e[a] == -1) {
+                    e[a] = A.nc[a] - 1;
+                    cpk |= 2 << (2 * a);  // -L
+                }
+                last = last || e[a] == A.nc[a] - 1;
             }
         }
+        if (in) {
+            const int cell = e[2] + A.nc[2] * (e[1] + A.nc[1] * e[0]);
+            int b = (int)st[cell], end = (int)st[cell + 1];
+            if constexpr (SEL == kSelSplit)
+                if (!last) b = max(b, pidx + 1);  // last cells whole (into base_b), others above pidx
+            if constexpr (SEL == kSelLow) end = last ? b : min(end, pidx);  // not last, below pidx
+            cbeg = b;
+            clen = max(end - b, 0);
+            cpk |= (int)last << 6;
+        }
+    }
+    // the entries' ranges laid end to end: exclusive offsets (lanes 27..31
+    // hold the total, which no candidate reaches)
+    int incl = clen;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const int cexcl = incl - clen;
+    const int total = uniform(__shfl(incl, 26));
+    // Candidates v of [0, total), four 64-lane blocks per step with all their
+    // loads in flight; a lane finds its entry by a binary search over the
+    // offsets of lanes 0..31.
+    constexpr int kB = 2;
+    for (int v0 = 0; v0 < total; v0 += kB * kWave) {
+        int jj[kB], pk[kB], ent[kB];
+        bool ok[kB];
+        double qx[kB], qy[kB], qz[kB], qw[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            const int v = v0 + u * kWave + lane;
+            ok[u] = v < total;
+            int r = 0, er = 0;
+#pragma unroll
+            for (int st2 = 16; st2 >= 1; st2 >>= 1) {
+                const int x = __shfl(cexcl, r + st2);
+                if (x <= v) {
+                    r += st2;
+                    er = x;
+                }
+            }
+            // every lane shuffles (a shuffle reads 0 from a lane outside the
+            // EXEC mask, so none may sit under the lane's own validity test);
+            // the index of an invalid lane is never used
+            ent[u] = r;
+            jj[u] = __shfl(cbeg, r) + (v - er);
+            pk[u] = __shfl(cpk, r);
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            qx[u] = qy[u] = qz[u] = qw[u] = 0.0;
+            if (ok[u]) {
+                // image position: point + shift (device.cu:282-284); weights
+                // only for the lists that carry them (the mixed list reads them
+                // from the catalogue where they are needed)
+                qx[u] = X[jj[u]];
+                qy[u] = Y[jj[u]];
+                qz[u] = Z[jj[u]];
+                if (!MIXED) qw[u] = Wt[jj[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            if (v0 + u * kWave >= total) break;  // wave-uniform
+            const int p = pk[u];
+            const double sx = (p & 1) ? A.box[0] : (p & 2) ? -A.box[0] : 0.0;
+            const double sy = (p & 4) ? A.box[1] : (p & 8) ? -A.box[1] : 0.0;
+            const double sz = (p & 16) ? A.box[2] : (p & 32) ? -A.box[2] : 0.0;
+            keep(jj[u], ok[u], qx[u] + sx, qy[u] + sy, qz[u] + sz, qw[u], ent[u], (p & 64) != 0);
+        }
+    }
     if constexpr (SEL == kSelSplit) *nb_out = nbl;
     return n;
 }
