@@ -156,8 +156,17 @@ __device__ __forceinline__ bool exact_floor(double v, double cs, int c) {
     return __builtin_fma(-(double)c, cs, v) >= 0.0 && __builtin_fma(-(double)(c + 1), cs, v) < 0.0;
 }
 
-// Cell key of every point (host clamp rule, cudaNcorr.cu:50-58) and its slot in
-// the cell.  Survey primaries are also checked without the clamp, as the
+// z slab of a point of cell c along z: trunc(kSlabs * v / cs) - kSlabs * c
+// (the scaling by kSlabs, a power of 2, is exact), clamped to [0, kSlabs):
+// the clamp of a point at n*cs into cell n-1 and the truncation of
+// -cs < v < 0 into cell 0 leave such points outside their cell's range.
+__device__ __forceinline__ int slab_of(double v, double cs, int c) {
+    const int s = (int)((v / cs) * (double)kSlabs) - kSlabs * c;
+    return min(max(s, 0), kSlabs - 1);
+}
+
+// Slab key of every point (cell by the host clamp rule, cudaNcorr.cu:50-58,
+// then its z slab) and its slot in the slab.  Survey primaries are also checked without the clamp, as the
 // reference's device index does not clamp them (device.cu:84-87).
 // *inexact gets 1 when some point's cell index is not the exact floor of its
 // coordinate over the cell size (then no pair closer than a cell size is
@@ -188,12 +197,13 @@ __global__ void k_cell_keys(const double4 *__restrict__ raw, int n, GridGeo g, i
         atomicOr(err, kErrDomain);
         cx = cy = cz = 0;
     }
-    const uint32_t k = (uint32_t)(cz + g.nc[2] * (cy + g.nc[1] * cx));
+    const uint32_t k = (uint32_t)((cz + g.nc[2] * (cy + g.nc[1] * cx)) * kSlabs + (bad ? 0 : slab_of(p.z, g.cs[2], cz)));
     key[i] = k;
     rank[i] = atomicAdd(&cnt[k], 1u);
 }
 
-// Exclusive scan of the per-cell counts in one workgroup of 1024 threads.
+// Exclusive scan of n counts in one workgroup of 1024 threads (start[n] gets
+// the total): the tile sums of the multi-block scan below.
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ start,
                                                int ncells) {
     __shared__ uint32_t part[1024];
@@ -218,14 +228,82 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ cnt,
     if (t == 1023) start[ncells] = part[1023];
 }
 
-// The counting sort by cell key, made stable: k_place puts every point index
-// into its cell's range at the slot its atomic rank gave it (an order that
+// Multi-block exclusive scan of the slab counts, ntiles tiles of kScanTile
+// counts (zero past the last slab): k_scan_tiles sums every tile, k_scan
+// scans the sums into pstart, k_scan_apply scans each tile from its offset.
+// 256 threads per tile, 16 consecutive counts per thread.
+__global__ __launch_bounds__(256) void k_scan_tiles(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ part) {
+    const uint4 *c = reinterpret_cast<const uint4 *>(cnt + (size_t)blockIdx.x * kScanTile);
+    uint32_t s = 0;
+    for (int j = threadIdx.x; j < kScanTile / 4; j += 256) {
+        const uint4 v = c[j];
+        s += v.x + v.y + v.z + v.w;
+    }
+    for (int o = kWave / 2; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ uint32_t ws[256 / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < 256 / kWave; ++w) t += ws[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t *__restrict__ cnt,
+                                                    const uint32_t *__restrict__ pstart, uint32_t *__restrict__ start,
+                                                    int ntiles) {
+    static_assert(kScanTile == 256 * 16, "16 counts per thread");
+    const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)t * 16;
+    const uint4 *c = reinterpret_cast<const uint4 *>(cnt + base);
+    uint32_t v[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 q = c[k];
+        v[4 * k] = q.x;
+        v[4 * k + 1] = q.y;
+        v[4 * k + 2] = q.z;
+        v[4 * k + 3] = q.w;
+    }
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) tot += v[k];
+    uint32_t inc = tot;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    __shared__ uint32_t ws[256 / kWave];
+    if (lane == kWave - 1) ws[wave] = inc;
+    __syncthreads();
+    uint32_t run = pstart[blockIdx.x] + inc - tot;
+    for (int w = 0; w < wave; ++w) run += ws[w];
+    uint4 *o = reinterpret_cast<uint4 *>(start + base);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint4 q;
+        q.x = run;
+        run += v[4 * k];
+        q.y = run;
+        run += v[4 * k + 1];
+        q.z = run;
+        run += v[4 * k + 2];
+        q.w = run;
+        run += v[4 * k + 3];
+        o[k] = q;
+    }
+    if (blockIdx.x == 0 && t == 0) start[(size_t)ntiles * kScanTile] = pstart[ntiles];
+}
+
+// The counting sort by slab key, made stable: k_place puts every point index
+// into its slab's range at the slot its atomic rank gave it (an order that
 // varies from run to run); k_stable_scatter then moves each point to position
-// start + (number of points of its cell with a smaller index), i.e. the cell's
+// start + (number of points of its slab with a smaller index), i.e. the slab's
 // points in input order, and writes the SoA arrays there.  The order within a
-// cell is thus the same on every run, and so is every list, work split and
+// slab is thus the same on every run, and so is every list, work split and
 // floating-point summation order that follows from it.  Each position scans
-// its cell once: O(sum of squared cell occupancies), parallel over positions.
+// its slab once: O(sum of squared slab occupancies), parallel over positions.
 __global__ void k_place(int n, const uint32_t *__restrict__ key, const uint32_t *__restrict__ rank,
                         const uint32_t *__restrict__ start, uint32_t *__restrict__ perm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,7 +336,7 @@ __global__ void k_nbr_bound(const uint32_t *__restrict__ st1, const uint32_t *__
                             unsigned *__restrict__ out) {
     const int ncells = g.nc[0] * g.nc[1] * g.nc[2];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells || st1[c + 1] == st1[c]) return;
+    if (c >= ncells || st1[(c + 1) * kSlabs] == st1[c * kSlabs]) return;
     const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1], cx = c / (g.nc[2] * g.nc[1]);
     unsigned tot = 0;
     for (int s = 0; s < 27; ++s) {
@@ -271,7 +349,7 @@ __global__ void k_nbr_bound(const uint32_t *__restrict__ st1, const uint32_t *__
             ez = ez == g.nc[2] ? 0 : (ez == -1 ? g.nc[2] - 1 : ez);
         }
         const int k = ez + g.nc[2] * (ey + g.nc[1] * ex);
-        tot += st2[k + 1] - st2[k];
+        tot += st2[(k + 1) * kSlabs] - st2[k * kSlabs];
     }
     atomicMax(out, tot);
 }
@@ -381,7 +459,7 @@ struct CatRef {
 //
 // The 27 neighbour entries follow the reference's x-major order
 // (cudaNcorr.cu:37-48), and so does the list.  Their cell ranges are laid end
-// to end and scanned as one sequence, 256 candidates per step (four 64-point
+// to end and scanned as one sequence, 128 candidates per step (two 64-point
 // blocks with their loads in flight together), so that short cells leave no
 // lanes idle and a primary waits for few memory round trips.  Each list entry
 // records its neighbour entry s (0..26), which fixes its image shift.
@@ -396,13 +474,38 @@ struct CatRef {
 //   kSelLow    the remaining ones (not last, index < pidx), into base from
 //              position `first` on.
 // Each selection scans only the neighbour rows that can hold its entries.
+//
+// cut (threeD, exactly gridded catalogue): a neighbour cell's range is cut
+// to its z slabs within reach of the primary.  With (ax, ay) the distance from
+// the primary to the cell's x-y extent (image position), a point of the cell
+// can lie within R of the primary only if |z - pz| <= h = sqrt(R^2 - ax^2 -
+// ay^2), and the slabs holding such z are a contiguous run.  R = sqrt(prune_r2)
+// exceeds rmax by far more than the rounding of the cell and slab bounds, so
+// every skipped point fails the range filter below; the lists are the same as
+// without pruning.
 constexpr int kSelAll = 0, kSelSplit = 1, kSelLow = 2;
 
-template <int LM, int SEL = kSelAll>
+// Exact bin of a squared separation d with r2min < d < r2max for a mixed
+// list entry: the FP32 lookup of the pair loop (LDS) on d scaled by sigma^2
+// (exact) and rounded once to FP32, an error far inside the guard band g32;
+// within the band of the edge, the FP64 lookup (global tables, rare).  KS:
+// the shift of the table's .x.
+template <int KS>
+__device__ __forceinline__ int mixed_entry_bin(const KArgs &A, const uint2 *lb32, double d, const double *sq,
+                                               const uint16_t *lut) {
+    const float ds = (float)(d * (A.sigma32 * A.sigma32));
+    const uint2 e = lb32[__float_as_uint(__builtin_amdgcn_fmed3f(ds, A.lut32_lo, 4.0f)) >> A.lut32_shift];
+    const float dE = ds - __uint_as_float(e.y);
+    if (fabsf(dE) <= A.g32) return bin_of(d, A.lut_scale, lut, sq, A.nb);
+    return (int)(e.x >> KS) - (dE < 0.0f ? 1 : 0);
+}
+
+template <int LM, int SEL = kSelAll, int KS = 0>
 __device__ __forceinline__ int build_list(const KArgs &A, const double *__restrict__ X,
                                           const double *__restrict__ Y, const double *__restrict__ Z,
-                                          const double *__restrict__ Wt, const uint32_t *__restrict__ st, double px,
-                                          double py, double pz, int cx, int cy, int cz, char *__restrict__ base,
+                                          const double *__restrict__ Wt, const uint32_t *__restrict__ st, bool cut,
+                                          double px, double py, double pz, int cx, int cy, int cz,
+                                          char *__restrict__ base,
                                           int lane, const double *sq, const uint16_t *lut, int pidx = 0,
                                           char *__restrict__ base_b = nullptr, int *nb_out = nullptr, int first = 0) {
     constexpr bool MIXED = LM == kListMixed;
@@ -429,9 +532,12 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
         }
     }
     const double pim = sqrt(A.pimax2) * (1.0 + 1e-9), r2m = A.r2max * (1.0 + 1e-9);
+    // the mixed path's FP32 lookup in LDS (entry c - c0 at lb32[c])
+    extern __shared__ __attribute__((aligned(16))) char lds_tables[];
+    const uint2 *lb32 = MIXED ? reinterpret_cast<const uint2 *>(lds_tables + A.off_lut32) - A.lut32_c0 : nullptr;
     // one threeD list entry (exact bin k and sqrtf of its side) at position pos of the list at b
     auto store = [&](char *b, int pos, int j, double d, double qx, double qy, double qz, double qw, int s) {
-        const int k = bin_of(d, A.lut_scale, lut, sq, A.nb);
+        const int k = MIXED ? mixed_entry_bin<KS>(A, lb32, d, sq, lut) : bin_of(d, A.lut_scale, lut, sq, A.nb);
         const float f = __builtin_sqrtf((float)d);
         if constexpr (MIXED) {
             const MList ml = mlist(b, A.cap);
@@ -495,8 +601,9 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             const bool ph = in & !last & (j > pidx), pb = in & last;
             const unsigned long long mh = __ballot(ph), mb = __ballot(pb);
             const int posh = n + lane_rank(mh), posb = nbl + lane_rank(mb);
-            if (ph & (posh < A.cap)) store(base, posh, j, d, qx, qy, qz, qw, s);
-            if (pb & (posb < A.cap)) store(base_b, posb, j, d, qx, qy, qz, qw, s);
+            // the two lists take disjoint entries: one store path
+            const bool sh = ph & (posh < A.cap), sb = pb & (posb < A.cap);
+            if (sh | sb) store(sh ? base : base_b, sh ? posh : posb, j, d, qx, qy, qz, qw, s);
             n += __popcll(mh);
             nbl += __popcll(mb);
         } else {
@@ -518,6 +625,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
         const int d[3] = {lane / 9 - 1, (lane / 3) % 3 - 1, lane % 3 - 1};
         const int c0[3] = {cx, cy, cz};
         int e[3];
+        double sh[3] = {0.0, 0.0, 0.0};
         bool in = true, last = false;
         for (int a = 0; a < 3; ++a) {
             e[a] = c0[a] + d[a];
@@ -527,16 +635,40 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
                 if (e[a] == A.nc[a]) {
                     e[a] = 0;
                     cpk |= 1 << (2 * a);  // +L
+                    sh[a] = A.box[a];
                 } else if (e[a] == -1) {
                     e[a] = A.nc[a] - 1;
                     cpk |= 2 << (2 * a);  // -L
+                    sh[a] = -A.box[a];
                 }
                 last = last || e[a] == A.nc[a] - 1;
             }
         }
         if (in) {
             const int cell = e[2] + A.nc[2] * (e[1] + A.nc[1] * e[0]);
-            int b = (int)st[cell], end = (int)st[cell + 1];
+            int s0 = 0, s1 = kSlabs;  // the slabs [s0, s1) of the cell
+            if constexpr (LM == kListPlain || LM == kListMixed) {
+                if (cut) {
+                    const double pc[2] = {px, py};
+                    double dxy2 = 0.0;
+                    for (int a = 0; a < 2; ++a) {
+                        const double lo = (double)e[a] * A.cs[a] + sh[a], hi = lo + A.cs[a];
+                        const double dd = fmax(fmax(lo - pc[a], pc[a] - hi), 0.0);
+                        dxy2 = __builtin_fma(dd, dd, dxy2);
+                    }
+                    if (!(dxy2 <= A.prune_r2)) {
+                        s0 = s1 = 0;
+                    } else {
+                        const double h = sqrt(A.prune_r2 - dxy2);
+                        const double z0 = (double)e[2] * A.cs[2] + sh[2], iw = (double)kSlabs / A.cs[2];
+                        const double slo = floor((pz - h - z0) * iw), shi = floor((pz + h - z0) * iw);
+                        s0 = (int)fmin(fmax(slo, 0.0), (double)kSlabs);
+                        s1 = (int)fmin(fmax(shi + 1.0, 0.0), (double)kSlabs);
+                        s1 = max(s1, s0);
+                    }
+                }
+            }
+            int b = (int)st[cell * kSlabs + s0], end = (int)st[cell * kSlabs + s1];
             if constexpr (SEL == kSelSplit)
                 if (!last) b = max(b, pidx + 1);  // last cells whole (into base_b), others above pidx
             if constexpr (SEL == kSelLow) end = last ? b : min(end, pidx);  // not last, below pidx
@@ -555,7 +687,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
     }
     const int cexcl = incl - clen;
     const int total = uniform(__shfl(incl, 26));
-    // Candidates v of [0, total), four 64-lane blocks per step with all their
+    // Candidates v of [0, total), two 64-lane blocks per step with all their
     // loads in flight; a lane finds its entry by a binary search over the
     // offsets of lanes 0..31.
     constexpr int kB = 2;
@@ -629,10 +761,10 @@ __global__ __launch_bounds__(256) void k_list_bound(KArgs A, TermDesc T, int p0,
         const int cy = axis_cell(py, A.cs[1], A.nc[1], clamp, bad);
         const int cz = axis_cell(pz, A.cs[2], A.nc[2], clamp, bad);
         if (bad) continue;  // the triplet kernel reports it
-        const int n2 = build_list<kListPlain>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, nullptr, lane,
+        const int n2 = build_list<kListPlain>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, nullptr, lane,
                                               nullptr, nullptr);
         const int n3 = T.sym ? n2
-                             : build_list<kListPlain>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz,
+                             : build_list<kListPlain>(A, T.x3, T.y3, T.z3, T.w3, T.st3, T.prune3, px, py, pz, cx, cy, cz,
                                                       nullptr, lane, nullptr, nullptr);
         best = max(best, max(n2, n3));
     }
@@ -657,6 +789,10 @@ constexpr int kHistStream = 4;  // rppi: key stream binned per bucket in LDS (re
 constexpr int kHistExact = 5;
 // layouts addressed through the t3 slot table
 __host__ __device__ constexpr bool t3_like(int hm) { return hm == kHistT3 || hm == kHistExact; }
+// The mixed path's FP32 lookup (lut32, copied into LDS) holds in its entries'
+// .x the bin count in the layout's units: bytes of a dense count (4), of a t3
+// entry (2), or bins (ctab); the shift of .x.
+__host__ __device__ constexpr int lut32_units(int hm) { return hm == kHistDense ? 2 : t3_like(hm) ? 1 : 0; }
 
 // Byte offset in the LDS histogram of the slot of bins (ka, kb, k), k in
 // [-1, nb] (slots of k outside [0, nb) are never added).
@@ -1372,7 +1508,7 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
                                             char *l2, char *l3, char *wlds, int lane, unsigned long long &wpairs) {
     constexpr int LM = MIXED ? kListMixed : kListPlain;
     int nb = 0;
-    const int nh = uniform(build_list<LM, kSelSplit>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2,
+    const int nh = uniform(build_list<LM, kSelSplit, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, l2,
                                                      lane, t.sq, t.lut, p, l3, &nb));
     const int nbu = uniform(nb);
     if (nh + nbu > A.nmax) return false;
@@ -1381,7 +1517,7 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
     if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, kOrdSix>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz);
     int nrow = nh;
     if (nbu > 0) {
-        nrow = uniform(build_list<LM, kSelLow>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
+        nrow = uniform(build_list<LM, kSelLow, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
                                                t.lut, p, nullptr, nullptr, nh));
         if (nrow + nbu > A.nmax) return false;
         wave_sync();
@@ -1447,9 +1583,8 @@ k_triplets(KArgs A) {
         // FP64 tables are only read by the rare exact rechecks: global memory
         t.sq = A.sq;
         t.lut = A.lut;
-        // entry .x (edges below the cell) in the layout's bin units: bytes
-        // of a dense count (4), of a t3 entry (2), or bins (ctab)
-        constexpr int kshift = HM == kHistDense ? 2 : t3_like(HM) ? 1 : 0;
+        // entry .x (edges below the cell) in the layout's bin units
+        constexpr int kshift = lut32_units(HM);
         for (int i = tid; i < A.lut32_n; i += nthr) {
             uint2 v = A.lut32[i];
             v.x <<= kshift;
@@ -1538,13 +1673,16 @@ k_triplets(KArgs A) {
             int n2;
             if (T.tri == kTriSwap) {
                 int nb_none = 0;
-                n2 = build_list<LM, kSelSplit>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
+                n2 = build_list<LM, kSelSplit, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
                                                t.lut, p, l3, &nb_none);
             } else {
-                n2 = build_list<LM>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq, t.lut);
+                n2 = build_list<LM, kSelAll, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx,
+                                                              cy, cz, l2, lane, t.sq, t.lut);
             }
             const int n3 =
-                T.sym ? n2 : build_list<LM>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3, lane, t.sq, t.lut);
+                T.sym ? n2
+                      : build_list<LM, kSelAll, lut32_units(HM)>(A, T.x3, T.y3, T.z3, T.w3, T.st3, T.prune3, px, py, pz,
+                                                                 cx, cy, cz, l3, lane, t.sq, t.lut);
             // wave-uniform lengths keep the pair loops' control in scalar registers
             const int n2u = uniform(n2), n3u = uniform(n3);
             if (n2u > A.nmax || n3u > A.nmax) {
@@ -2163,10 +2301,10 @@ __global__ __launch_bounds__(SURVEY && STREAM ? 512 : 1024)
             // p2 and p3 from one catalogue: each unordered pair once (survey: both
             // orders evaluated in it, the line-of-sight sum being order dependent)
             const bool sym = T.sym;
-            const int n2 = build_list<LM>(A, T.x2, T.y2, T.z2, T.w2, T.st2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
+            const int n2 = build_list<LM>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
                                           t.lut);
             const int n3 = T.sym ? n2
-                                 : build_list<LM>(A, T.x3, T.y3, T.z3, T.w3, T.st3, px, py, pz, cx, cy, cz, l3, lane,
+                                 : build_list<LM>(A, T.x3, T.y3, T.z3, T.w3, T.st3, T.prune3, px, py, pz, cx, cy, cz, l3, lane,
                                                   t.sq, t.lut);
             const int n2u = uniform(n2), n3u = uniform(n3);
             if (n2u > A.cap || n3u > A.cap) {
@@ -2386,7 +2524,7 @@ struct nc_catalog {
     bool exact = false;  // last build: every cell index the exact floor of coordinate / cell size
     double wmax = 0.0;     // largest finite |w|
     bool wfinite = true;   // every weight finite
-    DBuf raw, key, rank, perm, cnt, start, x, y, z, w;
+    DBuf raw, key, rank, perm, cnt, start, scan, x, y, z, w;
 };
 
 // Captured launch sequence of nc_count_batched: replayed while the same
@@ -2639,7 +2777,7 @@ int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
         G.nc[a] = (int)q;
         G.ncells *= G.nc[a];
     }
-    if (G.ncells > (1LL << 27)) return fail(ctx, NC_ERR_LIMIT, "cell grid larger than 2^27 cells");
+    if (G.ncells > (1LL << 24)) return fail(ctx, NC_ERR_LIMIT, "cell grid larger than 2^24 cells");
     // lower-bound bin lookup over d*scale (see bin_of / bin_any): the table
     // is refined until one step over the squared edges always suffices.
     const double r2max = G.sq[G.nb];
@@ -2684,12 +2822,15 @@ struct CatGrid {
 int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_catalog *c, bool check_unclamped,
                unsigned *inexact, bool expect_exact) {
     const int n = (int)c->n;
-    const size_t nc = (size_t)G.ncells;
+    // slab counts, padded to whole scan tiles (the padding stays zero)
+    const size_t ntiles = ((size_t)G.ncells * kSlabs + kScanTile - 1) / kScanTile;
+    const size_t ns = ntiles * kScanTile;
     NC_MEM(ctx, m, c->key, sizeof(uint32_t) * (size_t)n);
     NC_MEM(ctx, m, c->rank, sizeof(uint32_t) * (size_t)n);
     NC_MEM(ctx, m, c->perm, sizeof(uint32_t) * (size_t)n);
-    NC_MEM(ctx, m, c->cnt, sizeof(uint32_t) * nc);
-    NC_MEM(ctx, m, c->start, sizeof(uint32_t) * (nc + 1));
+    NC_MEM(ctx, m, c->cnt, sizeof(uint32_t) * ns);
+    NC_MEM(ctx, m, c->start, sizeof(uint32_t) * (ns + 1));
+    NC_MEM(ctx, m, c->scan, sizeof(uint32_t) * (2 * ntiles + 1));  // tile sums, then their scan
     NC_MEM(ctx, m, c->x, sizeof(double) * (size_t)n);
     NC_MEM(ctx, m, c->y, sizeof(double) * (size_t)n);
     NC_MEM(ctx, m, c->z, sizeof(double) * (size_t)n);
@@ -2700,16 +2841,19 @@ int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_cata
         g.nc[a] = G.nc[a];
     }
     g.survey = G.survey;
-    const int fb = (int)std::min<long long>((G.ncells + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, ctx->stream, (uint32_t *)c->cnt.p, 0u, (long long)nc);
+    const int fb = (int)std::min<long long>(((long long)ns + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, ctx->stream, (uint32_t *)c->cnt.p, 0u, (long long)ns);
     if (n > 0) {
         const int nb = (n + 255) / 256;
         hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n, g,
                            check_unclamped ? 1 : 0, (uint32_t *)c->key.p, (uint32_t *)c->rank.p, (uint32_t *)c->cnt.p,
                            (unsigned *)B.errw.p, inexact, expect_exact ? 1 : 0);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)c->cnt.p, (uint32_t *)c->start.p,
-                       (int)nc);
+    uint32_t *part = (uint32_t *)c->scan.p, *pstart = part + ntiles;
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream, (const uint32_t *)c->cnt.p, part);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)part, pstart, (int)ntiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream, (const uint32_t *)c->cnt.p,
+                       (const uint32_t *)pstart, (uint32_t *)c->start.p, (int)ntiles);
     if (n > 0) {
         const int nb = (n + 255) / 256;
         hipLaunchKernelGGL(k_place, dim3(nb), dim3(256), 0, ctx->stream, n, (const uint32_t *)c->key.p,
@@ -2747,6 +2891,7 @@ TermDesc term_desc(const Term &t) {
     d.sym = t.c2 == t.c3;
     d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
     d.tri = 0;  // set by run_count once the builds have shown exact gridding
+    d.prune2 = d.prune3 = 0;  // likewise
     return d;
 }
 
@@ -3115,10 +3260,21 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         ntri += tri[i] == kTriSix;
         nswap += tri[i] == kTriSwap;
     }
+    // Catalogues whose lists skip slabs beyond rmax (prune2 / prune3) and the
+    // six-fold primaries rely on exact gridding: a replayed plan checks it.
+    auto prunes = [&](const nc_catalog *c) { return !G.rppi && c->exact; };
     for (CatGrid &g : grids) {
         g.expect_exact = false;
-        for (int i = 0; i < nterms; ++i) g.expect_exact = g.expect_exact || (tri[i] && terms[i].c1 == g.cat);
+        for (int i = 0; i < nterms; ++i)
+            g.expect_exact = g.expect_exact || (tri[i] && terms[i].c1 == g.cat) ||
+                             (prunes(g.cat) && (terms[i].c2 == g.cat || terms[i].c3 == g.cat));
     }
+    // slab pruning radius: rmax plus a margin far above the rounding of the
+    // cell and slab bounds (and of a point's slab) at these coordinates
+    double bmax = 0.0;
+    for (int a = 0; a < 3; ++a) bmax = std::max(bmax, std::fabs(G.box[a]));
+    const double prune_r = G.edges[G.nb] * (1.0 + 1e-9) + 1e-9 * bmax;
+    const double prune_r2 = prune_r * prune_r;
 
     // --- histogram layout: dense u32 counts when every term has unit weights,
     // they fit and no count can wrap between merges (one chunk adds at most
@@ -3156,10 +3312,13 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         B.survey = G.survey;
         B.r2min = G.sq[0];
         B.r2max = G.sq[G.nb];
+        B.prune_r2 = prune_r2;
         B.cap = 0;  // count only
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.bound.p, 0u, 1LL);
         for (const Term &t : terms) {
-            const TermDesc d = term_desc(t);
+            TermDesc d = term_desc(t);
+            d.prune2 = prunes(t.c2);
+            d.prune3 = prunes(t.c3);
             if (d.n1 == 0 || t.c2->n == 0 || t.c3->n == 0) continue;
             // only the primaries of the slice counted (the same split as the work queue)
             const int p0 = (int)((long long)d.n1 * ctx->slice / ctx->nslices);
@@ -3258,6 +3417,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         weighted[i] = d.weighted;
         if (t.c2->n == 0 || t.c3->n == 0) continue;
         d.tri = tri[i];
+        d.prune2 = prunes(t.c2);
+        d.prune3 = prunes(t.c3);
         // the primary slice: positions [p0, p1) of the cell-sorted catalogue
         const int p0 = (int)((long long)d.n1 * ctx->slice / ctx->nslices);
         const int p1 = (int)((long long)d.n1 * (ctx->slice + 1) / ctx->nslices);
@@ -3345,6 +3506,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.survey = G.survey;
     A.r2min = G.sq[0];
     A.r2max = G.sq[G.nb];
+    A.prune_r2 = prune_r2;
     A.nb = G.nb;
     A.nbc = (int)G.out_words;
     A.npi = G.npi;
@@ -3953,7 +4115,8 @@ int nc_catalog_destroy(nc_catalog *c) {
     } else {
         (void)hipSetDevice(c->device);
     }
-    for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->perm, &c->cnt, &c->start, &c->x, &c->y, &c->z, &c->w}) dfree(*b);
+    for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->perm, &c->cnt, &c->start, &c->scan, &c->x, &c->y, &c->z, &c->w})
+        dfree(*b);
     delete c;
     return NC_OK;
 }

@@ -2,11 +2,13 @@
 //
 // Data layout in HBM (per catalogue, per count geometry):
 //   raw    : n x double4 {x,y,z,w}, as handed over by the caller
-//   key    : n x u32 cell key  z + nz*(y + ny*x)      (cudaNcorr.cu:50-58)
-//   rank   : n x u32 arrival slot of the point inside its cell (atomic)
-//   perm   : n x u32 point indices in cell order, then placed stably (by index)
-//   start  : ncells+1 x u32 exclusive scan of the per-cell counts
-//   x,y,z,w: n x double each, points sorted by cell key (SoA, coalesced)
+//   key    : n x u32 slab key (z + nz*(y + ny*x)) * kSlabs + slab, the cell of
+//            cudaNcorr.cu:50-58 and the point's z slab within it
+//   rank   : n x u32 arrival slot of the point inside its slab (atomic)
+//   perm   : n x u32 point indices in slab order, then placed stably (by index)
+//   start  : (ncells * kSlabs rounded up to kScanTile) + 1 x u32, exclusive scan
+//            of the per-slab counts; cell c is [start[c*kSlabs], start[(c+1)*kSlabs])
+//   x,y,z,w: n x double each, points sorted by slab key (SoA, coalesced)
 // Per persistent wave, a scratch slot holds the compacted neighbour lists of
 // the primary it is working on (Elem + weight per entry).
 #pragma once
@@ -23,6 +25,13 @@ constexpr unsigned kErrMode = 4u;  // a term the kernel variant was not planned 
 // coordinate / cell size) no longer is: a replayed plan must be rebuilt
 constexpr unsigned kErrReplan = 8u;
 constexpr int kMaxWavesPerGroup = 8;
+
+// Every cell is cut into kSlabs equal slabs along z, and its points are
+// sorted by slab, so that a neighbour list can skip the slabs of a neighbour
+// cell that lie beyond rmax of the primary.
+constexpr int kSlabs = 8;
+// Entries per workgroup of the multi-block scan of the slab counts.
+constexpr int kScanTile = 4096;
 
 // One compacted neighbour of a primary: the image position used by the
 // reference (point + periodic shift, FP64), the exact bin of its squared
@@ -51,6 +60,9 @@ struct TermDesc {
     // (p1, p2, p3) and (p2, p1, p3) are evaluated once, from the lower index
     // of p1 and p2, and counted twice.
     int tri;
+    // prune2 / prune3: the catalogue is exactly gridded (threeD), so its lists
+    // may skip the slabs of a neighbour cell beyond rmax of the primary
+    int prune2, prune3;
 };
 
 struct Chunk {
@@ -98,6 +110,7 @@ struct KArgs {
     int nc[3];
     int survey;
     double r2min, r2max;
+    double prune_r2;  // slab pruning: squared radius, rmax with a margin above any rounding of the bounds
     int nb, nbc;
     int nslots;                   // u64 words of the LDS histogram (nbc compact slots, or nb^3 u32 dense counts)
     int out_stride;               // u64 words of out per term: nbc, or 2*nbc (exact layout: low words, high words)

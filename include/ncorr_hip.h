@@ -166,7 +166,7 @@ int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, in
  * nc_noalloc_capacity() bytes (256 MiB) that the library holds on each GPU it
  * runs on until the process ends; the call makes no hipMalloc/hipFree and no
  * other allocating device call.  A count that needs more than the scratch
- * fails with NC_ERR_LIMIT (about 100 B per point plus 8 B per cell, then
+ * fails with NC_ERR_LIMIT (about 100 B per point plus 64 B per cell, then
  * neighbour lists for as many persistent workgroups as the rest holds).
  * All calls on one GPU share its scratch: calls that overlap on one GPU (from
  * several contexts or threads) race; serialise them.  Never captured in a
