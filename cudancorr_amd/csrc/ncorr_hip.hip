@@ -922,6 +922,18 @@ __device__ __forceinline__ void hist_add(const LdsTabs &t, int off, double w1, d
     }
 }
 
+// FP64 accumulation (not the exact layout): the pair loops stage each row's
+// weight already multiplied by w1 and the number of orders the pair stands
+// for, so an accepted pair adds one product (c w1 w_row) w_lane.  For FP32
+// weights every order's product (w1 w2) w3 is the correctly rounded w1 w2 w3,
+// and c w1 w_row is exact (c = 1, 2, 6), so the one rounding of this product
+// matches them; for wider weights the two differ by an ulp or two.
+template <bool WEIGHTED, int HM>
+constexpr bool pre_weighted() { return WEIGHTED && HM != kHistExact; }
+__device__ __forceinline__ void hist_add_value(const LdsTabs &t, int off, double v) {
+    atomicAdd(reinterpret_cast<double *>(reinterpret_cast<char *>(t.hist) + off), v);
+}
+
 // Bin of a squared separation (exact for r2min < d < r2max; any value in
 // [0, nb] otherwise).  STEP1: the lookup table is fine enough that one step
 // over the squared edges always suffices (checked on the host).
@@ -1030,7 +1042,7 @@ struct PlainPol {
     const double *W2, *W3;
     Elem *la;
     double *lw;
-    double w1;
+    double w1, wrow;  // wrow: the staged rows' weight factor (pre_weighted)
     int C, lane, row_first;
     Elem eb;
     double wb, w1b;
@@ -1039,6 +1051,7 @@ struct PlainPol {
     __device__ PlainPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2, const char *l3, char *wlds,
                         int lane_)
         : A(A_), t(t_), w1(w1_), lane(lane_) {
+        wrow = w1 * (double)ord_count(ORD);
         C = A.chunk_elems;
         L2 = reinterpret_cast<const Elem *>(l2);
         W2 = reinterpret_cast<const double *>(l2 + (size_t)A.cap * sizeof(Elem));
@@ -1059,7 +1072,7 @@ struct PlainPol {
             if (i < last - first) {
                 e = L[first + i];
                 e.k = row_bin<HM>(A, e.k);
-                if (WEIGHTED) w = W[first + i];
+                if (WEIGHTED) w = pre_weighted<WEIGHTED, HM>() ? wrow * W[first + i] : W[first + i];
             } else {  // padding: far away, never in range
                 e.x = e.y = e.z = 1e150;
                 e.k = 0;
@@ -1085,7 +1098,7 @@ struct PlainPol {
             eb.k = 0;
             eb.f = 0.0f;
         }
-        w1b = w1 * wb;
+        if constexpr (!pre_weighted<WEIGHTED, HM>()) w1b = w1 * wb;
     }
 
     // U pairs: rows ea[u] (weights wa[u]) with the lane entry.  SWAP: the rows
@@ -1129,7 +1142,9 @@ struct PlainPol {
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
                 const int slot = pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]);
-                if (SWAP)
+                if constexpr (pre_weighted<WEIGHTED, HM>())
+                    hist_add_value(t, slot, wa[u] * wb);
+                else if (SWAP)
                     hist_add<ORD, WEIGHTED, HM>(t, slot, w1, w1b, wb, w1 * wa[u], wa[u]);
                 else
                     hist_add<ORD, WEIGHTED, HM>(t, slot, w1, w1 * wa[u], wa[u], w1b, wb);
@@ -1250,6 +1265,7 @@ struct MixedPol {
     float4 rb;
     int kb, bidx;
     double wb, w1b;
+    double wrow;  // the staged rows' weight factor (pre_weighted)
     bool vb;
 
     __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
@@ -1274,6 +1290,7 @@ struct MixedPol {
         g32 = in_vgpr(A.g32);
         tri_t = in_vgpr(A.tri_t);
         nbx = in_vgpr((unsigned)A.nb << kShift);
+        wrow = w1 * (double)ord_count(ORD);
     }
 
     __device__ void stage(bool from3, int first, int last) {
@@ -1287,7 +1304,7 @@ struct MixedPol {
             if (i < last - first) {
                 r = m.R[first + i];
                 k = row_bin<HM>(A, m.KF[first + i].x & 0xff);
-                if (WEIGHTED) w = c.w[m.I[first + i]];
+                if (WEIGHTED) w = pre_weighted<WEIGHTED, HM>() ? wrow * c.w[m.I[first + i]] : c.w[m.I[first + i]];
             }
             la[i] = r;
             lk[i] = k;
@@ -1309,7 +1326,7 @@ struct MixedPol {
             kb = lane_bin<HM>(A, m.KF[bidx].x & 0xff);
             if (WEIGHTED) wb = (from3 ? c3 : c2).w[m.I[bidx]];
         }
-        w1b = w1 * wb;
+        if constexpr (!pre_weighted<WEIGHTED, HM>()) w1b = w1 * wb;
     }
 
     __device__ void drain() {
@@ -1352,7 +1369,9 @@ struct MixedPol {
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
-                if (SWAP)
+                if constexpr (pre_weighted<WEIGHTED, HM>())
+                    hist_add_value(t, slot[u], wa[u] * wb);
+                else if (SWAP)
                     hist_add<ORD, WEIGHTED, HM>(t, slot[u], w1, w1b, wb, w1 * wa[u], wa[u]);
                 else
                     hist_add<ORD, WEIGHTED, HM>(t, slot[u], w1, w1 * wa[u], wa[u], w1b, wb);
