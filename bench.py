@@ -337,7 +337,7 @@ def config4_subset(args, hip):
     FP64 reference of the weighted sums.  Binned triplets of the weighted terms
     from an untimed unit-weight run.  The CPU oracle times a sample of RRR
     primaries against all randoms, checked bit for bit on the GPU."""
-    from cudancorr_amd import NC_ACCUM_EXACT, NC_ACCUM_FP64, NC_PATH_MIXED, NC_PATH_PLAIN, Context
+    from cudancorr_amd import NC_ACCUM_EXACT, NC_ACCUM_FP32, NC_ACCUM_FP64, NC_PATH_MIXED, NC_PATH_PLAIN, Context
 
     slices = [int(x) for x in args.config4_slices.split(",")]
     K = 32
@@ -356,8 +356,8 @@ def config4_subset(args, hip):
         tune = ctx.autotune(terms, edges, box, survey=True, path=NC_PATH_MIXED)
         log(f"config 4: autotune mixed {tune}")
 
-        def leg(path, batched=True, exact=False, tl=terms):
-            ctx.set_accumulation(NC_ACCUM_EXACT if exact else NC_ACCUM_FP64)
+        def leg(path, batched=True, exact=False, tl=terms, acc=None):
+            ctx.set_accumulation(acc if acc is not None else NC_ACCUM_EXACT if exact else NC_ACCUM_FP64)
             hist = cnt = None
             ms = 0.0
             device_sync(hip)
@@ -393,9 +393,11 @@ def config4_subset(args, hip):
         log(f"config 4: autotune plain {tune_p}")
         hx, cx, wall_x, ms_x = leg(NC_PATH_PLAIN, exact=True)
         log(f"config 4: exact reference (plain path) {wall_x:.2f} s")
+        hf_, cf_, wall_f, ms_f = leg(NC_PATH_MIXED, acc=NC_ACCUM_FP32)
+        log(f"config 4: mixed with FP32 accumulation {wall_f:.2f} s")
         hp, cp, wall_p, ms_p = leg(NC_PATH_PLAIN, batched=False)
         log(f"config 4: plain x4 per slice {wall_p:.2f} s")
-        if not (np.array_equal(cm_[3], cp[3]) and np.array_equal(cm_[3], cx[3])):
+        if not (np.array_equal(cm_[3], cp[3]) and np.array_equal(cm_[3], cx[3]) and np.array_equal(cm_[3], cf_[3])):
             raise SystemExit("config 4: RRR counts differ between legs")
 
         def rel(a, b):
@@ -406,9 +408,11 @@ def config4_subset(args, hip):
 
         err_m = {n: rel(hm_[i], hx[i]) for i, n in enumerate(names[:3])}
         err_p = {n: rel(hp[i], hx[i]) for i, n in enumerate(names[:3])}
+        err_f = {n: rel(hf_[i], hx[i]) for i, n in enumerate(names[:3])}
         tot_m = {n: float(abs(hm_[i].sum() - hx[i].sum()) / abs(hx[i].sum())) for i, n in enumerate(names[:3])}
-        if max(err_m.values()) > 1e-9 or max(err_p.values()) > 1e-9:
-            raise SystemExit(f"config 4: weighted sums off the exact reference by more than 1e-9: {err_m} {err_p}")
+        if max(err_m.values()) > 1e-9 or max(err_p.values()) > 1e-9 or max(err_f.values()) > 1e-9:
+            raise SystemExit(f"config 4: weighted sums off the exact reference by more than 1e-9: {err_m} {err_p} "
+                             f"{err_f}")
         cpu = None
         if not args.no_cpu:
             from oracle import oracle
@@ -437,6 +441,11 @@ def config4_subset(args, hip):
         "mixed": {"wall_s": wall_m, "count_kernel_ms": ms_m, "triplets_per_s": trip / wall_m},
         "plain": {"wall_s": wall_p, "count_kernel_ms": ms_p, "triplets_per_s": trip / wall_p},
         "speedup_vs_plain": wall_p / wall_m,
+        "mixed_fp32_accumulation": {
+            "wall_s": wall_f, "count_kernel_ms": ms_f, "triplets_per_s": trip / wall_f,
+            "weighted_rel_max_vs_exact": err_f,
+            "what": "NC_ACCUM_FP32: FP32 float-float weight products, FP32 running sums with exact TwoSum "
+                    "compensation in FP64, folded into FP64 totals (the default accumulates FP64 products)"},
         "exact_reference": {"wall_s": wall_x, "count_kernel_ms": ms_x,
                             "what": "plain FP64 path with NC_ACCUM_EXACT: the FP64 products (w1*w2)*w3 summed "
                                     "exactly in 128-bit fixed point (each truncated by < 2^-K, K from the "

@@ -20,6 +20,7 @@ NC_BIN_THREED = 0
 NC_BIN_RPPI = 1
 NC_ACCUM_FP64 = 0
 NC_ACCUM_EXACT = 1
+NC_ACCUM_FP32 = 2
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "invalid bin edges", 3: "point outside the cell grid",
           4: "out of memory", 5: "HIP error", 6: "size limit"}
@@ -315,7 +316,7 @@ class Context:
         self._check(lib().nc_set_tuning(self.h, waves_per_group, chunk, wgs_per_cu, list_chunk))
 
     def set_accumulation(self, mode):
-        """nc_set_accumulation: NC_ACCUM_FP64 (0, default) or NC_ACCUM_EXACT (1)."""
+        """nc_set_accumulation: NC_ACCUM_FP64 (0, default), NC_ACCUM_EXACT (1) or NC_ACCUM_FP32 (2)."""
         self._check(lib().nc_set_accumulation(self.h, int(mode)))
 
     def set_primary_slice(self, slice_, nslices):

@@ -354,16 +354,25 @@ __global__ void k_nbr_bound(const uint32_t *__restrict__ st1, const uint32_t *__
     atomicMax(out, tot);
 }
 
+// An FP32 weight for the FP32 accumulation (NC_ACCUM_FP32): a value FP32
+// holds exactly, 0 or of magnitude in [2^-40, 2^40], so that products of
+// three such weights (and six times them) stay normal FP32 numbers.
+__host__ __device__ inline bool fp32_weight(double w) {
+    const double a = w < 0.0 ? -w : w;
+    return (double)(float)w == w && (w == 0.0 || (a >= 0x1p-40 && a <= 0x1p40));
+}
+
 // Weight statistics of a catalogue into s[0..3]: s[0] |= 1 when some weight
-// is not 1, s[1] |= 1 when some weight is not finite, and the 64-bit word at
-// s + 2 gets the largest |w| as its bit pattern (non-negative doubles order
-// as their bits).
+// is not 1, s[1] |= 1 when some weight is not finite and |= 2 when some weight
+// is not an FP32 weight (fp32_weight), and the 64-bit word at s + 2 gets the
+// largest |w| as its bit pattern (non-negative doubles order as their bits).
 __global__ void k_weight_stats(const double4 *__restrict__ raw, long long n, unsigned *__restrict__ s) {
-    unsigned nonunit = 0u, nonfinite = 0u;
+    unsigned nonunit = 0u, nonfinite = 0u, nonf32 = 0u;
     unsigned long long mx = 0ull;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const double w = raw[i].w;
         nonunit |= w != 1.0 ? 1u : 0u;
+        nonf32 |= fp32_weight(w) ? 0u : 2u;
         if (__builtin_isfinite(w))
             mx = max(mx, (unsigned long long)__double_as_longlong(fabs(w)));
         else
@@ -371,6 +380,7 @@ __global__ void k_weight_stats(const double4 *__restrict__ raw, long long n, uns
     }
     if (__ballot(nonunit != 0u) != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(s, 1u);
     if (__ballot(nonfinite != 0u) != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(s + 1, 1u);
+    if (__ballot(nonf32 != 0u) != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(s + 1, 2u);
     atomicMax(reinterpret_cast<unsigned long long *>(s + 2), mx);
 }
 
@@ -787,8 +797,27 @@ constexpr int kHistStream = 4;  // rppi: key stream binned per bucket in LDS (re
 // multiples of 2^-K and summed exactly, so the sums do not depend on the order
 // of the additions (nc_set_accumulation NC_ACCUM_EXACT).
 constexpr int kHistExact = 5;
-// layouts addressed through the t3 slot table
-__host__ __device__ constexpr bool t3_like(int hm) { return hm == kHistT3 || hm == kHistExact; }
+// kHistF32: the sorted slots of kHistT3 as FP32 running sums with FP64
+// compensation (nc_set_accumulation NC_ACCUM_FP32): an array of nbc FP64
+// compensations at the slot offsets of kHistT3, then (at byte xhi = 8 nbc) an
+// array of nbc FP32 sums.  Each accepted pair's weight product is formed in
+// FP32 as a float-float pair (hi, lo) and hi is added to the FP32 sum with a
+// returning LDS atomic; the rounding error of that addition (TwoSum of the old
+// sum and hi, exact) and lo go to the compensation.  Sum + compensation is the
+// exact sum of the float-float products up to the FP64 rounding of the
+// compensation's own additions; the two are folded into the FP64 totals at
+// every merge.
+constexpr int kHistF32 = 6;
+// kHistF32C: the layout of kHistF32 with the slot offsets of kHistCtab (the
+// reference's index arithmetic), for bin counts whose t3 table and FP32 sums
+// do not fit in LDS together.
+constexpr int kHistF32C = 7;
+__host__ __device__ constexpr bool is_f32(int hm) { return hm == kHistF32 || hm == kHistF32C; }
+// layouts addressed through the t3 slot table, or through ctab
+__host__ __device__ constexpr bool t3_like(int hm) { return hm == kHistT3 || hm == kHistExact || hm == kHistF32; }
+__host__ __device__ constexpr bool ctab_like(int hm) { return hm == kHistCtab || hm == kHistF32C; }
+// layouts whose merges are also forced by a count of pairs added (flush_pairs)
+__host__ __device__ constexpr bool counts_pairs(int hm) { return hm == kHistDense || is_f32(hm); }
 // The mixed path's FP32 lookup (lut32, copied into LDS) holds in its entries'
 // .x the bin count in the layout's units: bytes of a dense count (4), of a t3
 // entry (2), or bins (ctab); the shift of .x.
@@ -934,6 +963,41 @@ __device__ __forceinline__ void hist_add_value(const LdsTabs &t, int off, double
     atomicAdd(reinterpret_cast<double *>(reinterpret_cast<char *>(t.hist) + off), v);
 }
 
+// kHistF32: the row's weight factor c w1 w_row as a float-float pair (its
+// bits in the FP64 word the rows are staged in) times the lane's FP32 weight:
+// the product (hi, lo) from an FMA, hi into the FP32 sum, the exact rounding
+// error of that sum (TwoSum on the value the atomic returns) and lo into the
+// FP64 compensation.  -ffp-contract=off keeps the TwoSum's operations apart.
+__device__ __forceinline__ double f32_pair(double x) {
+    const float hi = (float)x;
+    return __builtin_bit_cast(double, make_float2(hi, (float)(x - (double)hi)));
+}
+// The U pairs of a pair-loop group together: every returning atomic is issued
+// before the first sum it returns is used, so their latencies overlap.
+template <int U>
+__device__ __forceinline__ void hist_add_f32(const LdsTabs &t, const int (&off)[U], const double (&wrow)[U], float wl,
+                                             const bool (&acc)[U]) {
+    char *h = reinterpret_cast<char *>(t.hist);
+    float ph[U], pl[U], old[U] = {};
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const float2 wr = __builtin_bit_cast(float2, wrow[u]);
+        ph[u] = wr.x * wl;
+        pl[u] = __builtin_fmaf(wr.y, wl, __builtin_fmaf(wr.x, wl, -ph[u]));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (acc[u]) old[u] = atomicAdd(reinterpret_cast<float *>(h + t.xhi + (off[u] >> 1)), ph[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (acc[u]) {
+            const float sum = old[u] + ph[u];
+            const float bv = sum - old[u];
+            const float err = (old[u] - (sum - bv)) + (ph[u] - bv);
+            atomicAdd(reinterpret_cast<double *>(h + off[u]), (double)err + (double)pl[u]);
+        }
+}
+
 // Bin of a squared separation (exact for r2min < d < r2max; any value in
 // [0, nb] otherwise).  STEP1: the lookup table is fine enough that one step
 // over the squared edges always suffices (checked on the host).
@@ -1046,6 +1110,7 @@ struct PlainPol {
     int C, lane, row_first;
     Elem eb;
     double wb, w1b;
+    float wbf;  // the lane's weight in FP32 (kHistF32: FP32 weights)
     bool vb;
 
     __device__ PlainPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2, const char *l3, char *wlds,
@@ -1073,6 +1138,7 @@ struct PlainPol {
                 e = L[first + i];
                 e.k = row_bin<HM>(A, e.k);
                 if (WEIGHTED) w = pre_weighted<WEIGHTED, HM>() ? wrow * W[first + i] : W[first + i];
+                if (is_f32(HM)) w = f32_pair(w);
             } else {  // padding: far away, never in range
                 e.x = e.y = e.z = 1e150;
                 e.k = 0;
@@ -1099,6 +1165,7 @@ struct PlainPol {
             eb.f = 0.0f;
         }
         if constexpr (!pre_weighted<WEIGHTED, HM>()) w1b = w1 * wb;
+        wbf = (float)wb;
     }
 
     // U pairs: rows ea[u] (weights wa[u]) with the lane entry.  SWAP: the rows
@@ -1119,7 +1186,7 @@ struct PlainPol {
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if constexpr (HM != kHistCtab) {
+            if constexpr (!ctab_like(HM)) {
                 k[u] += d[u] >= t.sq[k[u] + 1] ? 1 : 0;
             } else {
                 while (k[u] < A.nb - 1 && d[u] >= t.sq[k[u] + 1]) ++k[u];
@@ -1137,6 +1204,13 @@ struct PlainPol {
                 const Sides e = sort3(ea[u].f, eb.f, __builtin_sqrtf(fd));
                 acc[u] = e.lo + e.mid > e.hi;
             }
+        }
+        if constexpr (WEIGHTED && is_f32(HM)) {
+            int slot[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) slot[u] = acc[u] ? pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]) : 0;
+            hist_add_f32<U>(t, slot, wa, wbf, acc);
+            return;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -1266,6 +1340,7 @@ struct MixedPol {
     int kb, bidx;
     double wb, w1b;
     double wrow;  // the staged rows' weight factor (pre_weighted)
+    float wbf;    // the lane's weight in FP32 (kHistF32: FP32 weights)
     bool vb;
 
     __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
@@ -1305,6 +1380,7 @@ struct MixedPol {
                 r = m.R[first + i];
                 k = row_bin<HM>(A, m.KF[first + i].x & 0xff);
                 if (WEIGHTED) w = pre_weighted<WEIGHTED, HM>() ? wrow * c.w[m.I[first + i]] : c.w[m.I[first + i]];
+                if (is_f32(HM)) w = f32_pair(w);
             }
             la[i] = r;
             lk[i] = k;
@@ -1327,6 +1403,7 @@ struct MixedPol {
             if (WEIGHTED) wb = (from3 ? c3 : c2).w[m.I[bidx]];
         }
         if constexpr (!pre_weighted<WEIGHTED, HM>()) w1b = w1 * wb;
+        wbf = (float)wb;
     }
 
     __device__ void drain() {
@@ -1366,9 +1443,10 @@ struct MixedPol {
             acc[u] = ok[u] & valid & !amb_b & !amb_t;
             slot[u] = pair_slot_x<HM>(A, t, ka[u], kb, kx);
         }
+        if constexpr (WEIGHTED && is_f32(HM)) hist_add_f32<U>(t, slot, wa, wbf, acc);
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (acc[u]) {
+            if (acc[u] && !(WEIGHTED && is_f32(HM))) {
                 if constexpr (pre_weighted<WEIGHTED, HM>())
                     hist_add_value(t, slot[u], wa[u] * wb);
                 else if (SWAP)
@@ -1455,6 +1533,24 @@ __device__ __forceinline__ void flush_hist(const KArgs &A, int term, int weighte
                 hist[A.nbc + i] = 0ull;
             }
         }
+    } else if constexpr (is_f32(HM)) {
+        // FP32 sum + FP64 compensation, folded into the FP64 total
+        // (a unit-weight term adds u64 counts in the compensation words)
+        double *comp = reinterpret_cast<double *>(hist);
+        float *sum = reinterpret_cast<float *>(reinterpret_cast<char *>(hist) + (size_t)A.nbc * 8);
+        for (int i = threadIdx.x; i < A.nbc; i += blockDim.x) {
+            const float s = sum[i];
+            const unsigned long long c = hist[i];
+            if (s != 0.0f || c != 0ull) {
+                if (weighted)
+                    unsafeAtomicAdd(reinterpret_cast<double *>(&out[i]), (double)s + comp[i]);
+                else
+                    atomicAdd(&out[i], c);
+                sum[i] = 0.0f;
+                hist[i] = 0ull;
+            }
+        }
+        if (threadIdx.x == 0) *pairs = 0ull;
     } else if constexpr (HM == kHistDense) {
         unsigned *h32 = reinterpret_cast<unsigned *>(hist);
         const int nb = A.nb, nb3 = nb * nb * nb;
@@ -1544,7 +1640,7 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
             pair_loop<MIXED, HM, false, WEIGHTED, kOrdSym>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz);
         if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz);
     }
-    if constexpr (HM == kHistDense)
+    if constexpr (counts_pairs(HM))
         wpairs += 3ull * (unsigned long long)nh * (unsigned long long)max(nh - 1, 0) +
                   2ull * (unsigned long long)nrow * (unsigned long long)nbu +
                   (unsigned long long)nbu * (unsigned long long)max(nbu - 1, 0);
@@ -1572,7 +1668,7 @@ __global__ __launch_bounds__(MIXED && HM == kHistExact ? 768 : 1024) __attribute
     amdgpu_waves_per_eu(MIXED && HM == kHistDense ? 5 : MIXED && HM == kHistExact ? 3 : 4))) void
 k_triplets(KArgs A) {
     static_assert(!(WT && HM == kHistDense), "dense counts are for unit weights");
-    static_assert(WT || HM != kHistExact, "exact accumulation is for weighted terms");
+    static_assert(WT || (HM != kHistExact && !is_f32(HM)), "exact and FP32 accumulation are for weighted terms");
     extern __shared__ __attribute__((aligned(16))) char lds[];
     __shared__ int s_chunk;
     __shared__ int s_next;
@@ -1615,7 +1711,7 @@ k_triplets(KArgs A) {
     }
     if constexpr (t3_like(HM))
         for (int i = tid; i < A.nb * A.nb * (A.nb + 2); i += nthr) t3[i] = A.t3[i];
-    else if constexpr (HM == kHistCtab)
+    else if constexpr (ctab_like(HM))
         for (int i = tid; i < A.nb * A.nb; i += nthr) ctab[i] = A.ctab[i];
     const int lane = tid & (kWave - 1), wave = tid / kWave, nw = nthr / kWave;
     char *wlds = lds + A.off_wave + wave * A.wave_bytes;
@@ -1651,7 +1747,7 @@ k_triplets(KArgs A) {
         if (ci >= A.nchunks) break;
         const Chunk ch = A.chunks[ci];
         const TermDesc T = A.terms[ch.term];
-        const bool full = HM == kHistDense && uniform((int)(s_pairs >= A.flush_pairs)) != 0;
+        const bool full = counts_pairs(HM) && uniform((int)(s_pairs >= A.flush_pairs)) != 0;
         if (ch.term != cur || full) {
             if (cur >= 0) flush_hist<HM>(A, cur, cur_weighted, t.hist, &s_pairs);
             cur = ch.term;
@@ -1719,7 +1815,7 @@ k_triplets(KArgs A) {
                     continue;
                 }
             }
-            if constexpr (HM == kHistDense) {
+            if constexpr (counts_pairs(HM)) {
                 const unsigned long long a2 = (unsigned long long)n2u, a3 = (unsigned long long)n3u;
                 wpairs += T.tri == kTriSwap ? 2ull * a2 * a3 : T.sym ? a2 * (a2 - 1ull) : a2 * a3;
             }
@@ -1747,7 +1843,7 @@ k_triplets(KArgs A) {
             }
             wave_sync();
         }
-        if constexpr (HM == kHistDense) {
+        if constexpr (counts_pairs(HM)) {
             atomicAdd(&s_pairs, lane == 0 ? wpairs : 0ull);  // every lane: no branch on lane == 0
             wpairs = 0ull;
         }
@@ -2421,6 +2517,10 @@ template __global__ void k_triplets<true, kHistCtab, true>(KArgs);
 template __global__ void k_triplets<true, kHistT3, true>(KArgs);
 template __global__ void k_triplets<false, kHistExact, true>(KArgs);
 template __global__ void k_triplets<true, kHistExact, true>(KArgs);
+template __global__ void k_triplets<false, kHistF32, true>(KArgs);
+template __global__ void k_triplets<true, kHistF32, true>(KArgs);
+template __global__ void k_triplets<false, kHistF32C, true>(KArgs);
+template __global__ void k_triplets<true, kHistF32C, true>(KArgs);
 
 // ---------------------------------------------------------------------------
 // Scratch of nc_count_noalloc: device memory of a fixed size that the library
@@ -2543,6 +2643,7 @@ struct nc_catalog {
     bool exact = false;  // last build: every cell index the exact floor of coordinate / cell size
     double wmax = 0.0;     // largest finite |w|
     bool wfinite = true;   // every weight finite
+    bool wf32 = true;      // every weight an FP32 weight (fp32_weight)
     DBuf raw, key, rank, perm, cnt, start, scan, x, y, z, w;
 };
 
@@ -3040,7 +3141,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     for (const Term &t : terms) uids.insert(uids.end(), {t.c1->uid, t.c2->uid, t.c3->uid});
     // weight status of each catalogue (wrapped ones may change it between calls)
     std::vector<int> wkey;
-    for (const Term &t : terms) wkey.push_back((t.c1->unit_w ? 1 : 0) | (t.c2->unit_w ? 2 : 0) | (t.c3->unit_w ? 4 : 0));
+    for (const Term &t : terms)
+        wkey.push_back((t.c1->unit_w ? 1 : 0) | (t.c2->unit_w ? 2 : 0) | (t.c3->unit_w ? 4 : 0) |
+                       (t.c1->wf32 ? 8 : 0) | (t.c2->wf32 ? 16 : 0) | (t.c3->wf32 ? 32 : 0));
     std::vector<double> pkey(P->box, P->box + 3);
     pkey.push_back((double)G.survey);
     pkey.push_back((double)G.rppi);
@@ -3068,6 +3171,16 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         xk = std::min(std::max(60 - e, -900), 900);
     }
     pkey.push_back((double)xk);
+    // FP32 accumulation of weighted sums (nc_set_accumulation): FP32 weight
+    // products and running sums with FP64 compensation, for FP32 weights only
+    const bool f32acc = ctx->accum == NC_ACCUM_FP32 && !G.rppi && !all_unit;
+    if (f32acc)
+        for (const Term &t : terms)
+            if (!(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w) && !(t.c1->wf32 && t.c2->wf32 && t.c3->wf32))
+                return fail(ctx, NC_ERR_ARG,
+                            "FP32 accumulation needs FP32 weights (values FP32 holds exactly, 0 or of magnitude in "
+                            "[2^-40, 2^40])");
+    pkey.push_back((double)ctx->accum);
     const std::vector<int> tkey = {path,       ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems,
                                    ctx->slice, ctx->nslices};
     if (use_graph && ctx->plan.valid && ctx->plan.uids == uids && ctx->plan.wkey == wkey && ctx->plan.pkey == pkey &&
@@ -3172,6 +3285,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         return hm == kHistGlobal    ? (size_t)0
                : hm == kHistDense   ? al16(nb3 * 4)
                : hm == kHistExact   ? al16((size_t)G.nbc * 16)
+               : is_f32(hm)         ? al16((size_t)G.nbc * 12)
                                     : al16((size_t)G.nbc * 8);
     };
     auto lds_tables = [&](int hm, size_t &o_sq, size_t &o_lut, size_t &o_ctab, size_t &o_lut32, size_t &o_t3) {
@@ -3189,7 +3303,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         if (t3_like(hm)) {
             o_t3 = o;
             o = al16(o + sizeof(uint16_t) * G.t3.size());
-        } else if (hm == kHistCtab) {
+        } else if (ctab_like(hm)) {
             o_ctab = o;
             o = al16(o + sizeof(int) * G.ctab.size());
         }
@@ -3373,6 +3487,13 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
             return fail(ctx, NC_ERR_LIMIT, "exact accumulation supports at most 35 bins per side");
         hm = kHistExact;
         if (!fits(hm, budget)) budget = ctx->lds_hw;
+    } else if (f32acc) {
+        if (G.t3.empty() || !(mixed || G.lut_single_step))
+            return fail(ctx, NC_ERR_LIMIT, "FP32 accumulation supports at most 35 bins per side");
+        // the t3 table where it fits beside the FP32 sums, else ctab
+        hm = fits(kHistF32, budget) || (!fits(kHistF32C, budget) && fits(kHistF32, ctx->lds_hw)) ? kHistF32
+                                                                                                 : kHistF32C;
+        if (!fits(hm, budget)) budget = ctx->lds_hw;
     } else if (mixed && dense_ok && hm != kHistDense && fits(kHistDense, ctx->lds_hw)) {
         // mixed path: dense counts over a whole CU beat the sorted-slot
         // layouts at half of it (25 bins, survey randoms: 4405 against 5428 ms)
@@ -3413,6 +3534,10 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
                                          : (stream ? (const void *)k_rppi<false, true> : (const void *)k_rppi<false, false>))
                       : hm == kHistExact ? (mixed ? (const void *)k_triplets<true, kHistExact, true>
                                                   : (const void *)k_triplets<false, kHistExact, true>)
+                      : hm == kHistF32   ? (mixed ? (const void *)k_triplets<true, kHistF32, true>
+                                                  : (const void *)k_triplets<false, kHistF32, true>)
+                      : hm == kHistF32C  ? (mixed ? (const void *)k_triplets<true, kHistF32C, true>
+                                                  : (const void *)k_triplets<false, kHistF32C, true>)
                                          : kfns[all_unit ? 0 : 1][mixed ? 1 : 0][hm];
     if (kfn == nullptr) return fail(ctx, NC_ERR_LIMIT, "no kernel variant for the planned layout");
     {
@@ -3546,10 +3671,17 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.lut32_n = (int)G.lut32.size();
     A.g32 = G.g32;
     A.tri_t = G.tri_t;
-    A.nslots = hm == kHistDense ? (int)((nb3 + 1) / 2) : hm == kHistGlobal ? 0 : hm == kHistExact ? 2 * G.nbc : G.nbc;
+    A.nslots = hm == kHistDense    ? (int)((nb3 + 1) / 2)
+               : hm == kHistGlobal ? 0
+               : hm == kHistExact  ? 2 * G.nbc
+               : is_f32(hm)        ? (int)(hist_size(hm) / 8)
+                                   : G.nbc;
     A.out_stride = (int)ostride;
     A.xscale = exact ? std::ldexp(1.0, xk) : 0.0;
-    A.flush_pairs = ctx->dense_flush;
+    // FP32 accumulation: merged at the first work-item boundary after 2^22
+    // pairs, which bounds the FP64 rounding of the compensations (see the
+    // header) far below the FP32 products' own 2^-46
+    A.flush_pairs = is_f32(hm) ? std::min<unsigned long long>(ctx->dense_flush, 1ull << 22) : ctx->dense_flush;
     A.off_sq = (int)hist_bytes;
     A.off_lut = (int)(hist_bytes + off_lut);
     A.off_ctab = (int)(hist_bytes + off_ctab);
@@ -3922,7 +4054,8 @@ int refresh_weights(nc_context *ctx, Mem &m, const std::vector<Term> &terms) {
     NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (size_t i = 0; i < w.size(); ++i) {
         w[i]->unit_w = f[4 * i] == 0u;
-        w[i]->wfinite = f[4 * i + 1] == 0u;
+        w[i]->wfinite = (f[4 * i + 1] & 1u) == 0u;
+        w[i]->wf32 = (f[4 * i + 1] & 2u) == 0u;
         const unsigned long long b = (unsigned long long)f[4 * i + 2] | ((unsigned long long)f[4 * i + 3] << 32);
         std::memcpy(&w[i]->wmax, &b, 8);
     }
@@ -4082,6 +4215,7 @@ int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog
     for (int64_t i = 0; i < n; ++i) {
         const double w = xyzw[4 * i + 3];
         c->unit_w = c->unit_w && w == 1.0;
+        c->wf32 = c->wf32 && fp32_weight(w);
         if (std::isfinite(w))
             c->wmax = std::max(c->wmax, std::fabs(w));
         else
@@ -4293,7 +4427,8 @@ int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_c
 
 int nc_set_accumulation(nc_context *ctx, int mode) {
     if (!ctx) return NC_ERR_ARG;
-    if (mode != NC_ACCUM_FP64 && mode != NC_ACCUM_EXACT) return fail(ctx, NC_ERR_ARG, "unknown accumulation mode");
+    if (mode != NC_ACCUM_FP64 && mode != NC_ACCUM_EXACT && mode != NC_ACCUM_FP32)
+        return fail(ctx, NC_ERR_ARG, "unknown accumulation mode");
     ctx->accum = mode;
     return NC_OK;
 }

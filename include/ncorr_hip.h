@@ -219,8 +219,24 @@ int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *
  *                   (about 5e-14 for lognormal(0, 0.3) weights).  Results are
  *                   identical on every run and on both arithmetic paths.  Needs
  *                   finite weights and at most 35 bins; threeD only (rppi keeps
- *                   FP64).  Unit-weight terms count exactly either way. */
-enum nc_accum { NC_ACCUM_FP64 = 0, NC_ACCUM_EXACT = 1 };
+ *                   FP64).  Unit-weight terms count exactly either way.
+ *   NC_ACCUM_FP32   weight products in FP32 and per-workgroup FP32 running sums
+ *                   with FP64 compensation, folded into FP64 totals at every
+ *                   merge.  Each pair's product (c w1 w2) w3 is formed as an
+ *                   FP32 pair hi + lo (relative error below 2^-46); hi goes to
+ *                   the slot's FP32 sum with a returning LDS atomic, and the
+ *                   exact rounding error of that addition (TwoSum) plus lo to
+ *                   the slot's FP64 compensation.  Merges come at least every
+ *                   2^22 pairs a workgroup adds (at work-item boundaries), so
+ *                   the compensation's own FP64 rounding stays below
+ *                   m^2 2^-77 of a slot's sum for m additions to the slot
+ *                   between merges.  Needs FP32 weights: values FP32 holds
+ *                   exactly, 0 or of magnitude in [2^-40, 2^40]
+ *                   (NC_ERR_ARG otherwise); at most 35 bins; threeD only.
+ *                   On MI355X an FP64 product issues at the rate of an FP32
+ *                   one, so this mode costs more instructions per pair than
+ *                   NC_ACCUM_FP64 and is not the default. */
+enum nc_accum { NC_ACCUM_FP64 = 0, NC_ACCUM_EXACT = 1, NC_ACCUM_FP32 = 2 };
 int nc_set_accumulation(nc_context *ctx, int mode);
 
 /* Primary slices (no counterpart in the reference, whose kernel takes every
