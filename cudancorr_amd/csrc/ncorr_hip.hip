@@ -697,6 +697,11 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
     }
     const int cexcl = incl - clen;
     const int total = uniform(__shfl(incl, 26));
+    // most primaries have no wrapped (shifted) and no last neighbour cell:
+    // then the per-candidate shift and flag are skipped (wave-uniform tests;
+    // q + 0.0 would be q in every later use)
+    const bool any_shift = __ballot((cpk & 63) != 0) != 0ull;
+    const bool any_flag = __ballot(cpk != 0) != 0ull;
     // Candidates v of [0, total), two 64-lane blocks per step with all their
     // loads in flight; a lane finds its entry by a binary search over the
     // offsets of lanes 0..31.
@@ -723,7 +728,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             // the index of an invalid lane is never used
             ent[u] = r;
             jj[u] = __shfl(cbeg, r) + (v - er);
-            pk[u] = __shfl(cpk, r);
+            pk[u] = any_flag ? __shfl(cpk, r) : 0;
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
@@ -742,10 +747,12 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
         for (int u = 0; u < kB; ++u) {
             if (v0 + u * kWave >= total) break;  // wave-uniform
             const int p = pk[u];
-            const double sx = (p & 1) ? A.box[0] : (p & 2) ? -A.box[0] : 0.0;
-            const double sy = (p & 4) ? A.box[1] : (p & 8) ? -A.box[1] : 0.0;
-            const double sz = (p & 16) ? A.box[2] : (p & 32) ? -A.box[2] : 0.0;
-            keep(jj[u], ok[u], qx[u] + sx, qy[u] + sy, qz[u] + sz, qw[u], ent[u], (p & 64) != 0);
+            if (any_shift) {
+                qx[u] += (p & 1) ? A.box[0] : (p & 2) ? -A.box[0] : 0.0;
+                qy[u] += (p & 4) ? A.box[1] : (p & 8) ? -A.box[1] : 0.0;
+                qz[u] += (p & 16) ? A.box[2] : (p & 32) ? -A.box[2] : 0.0;
+            }
+            keep(jj[u], ok[u], qx[u], qy[u], qz[u], qw[u], ent[u], (p & 64) != 0);
         }
     }
     if constexpr (SEL == kSelSplit) *nb_out = nbl;
