@@ -132,11 +132,6 @@ __device__ __forceinline__ int neighbour(const KArgs &A, int cx, int cy, int cz,
 // ---------------------------------------------------------------------------
 // cell-list build
 
-struct GridGeo {
-    double cs[3];
-    int nc[3];
-    int survey;
-};
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -354,13 +349,6 @@ __global__ void k_nbr_bound(const uint32_t *__restrict__ st1, const uint32_t *__
     atomicMax(out, tot);
 }
 
-// An FP32 weight for the FP32 accumulation (NC_ACCUM_FP32): a value FP32
-// holds exactly, 0 or of magnitude in [2^-40, 2^40], so that products of
-// three such weights (and six times them) stay normal FP32 numbers.
-__host__ __device__ inline bool fp32_weight(double w) {
-    const double a = w < 0.0 ? -w : w;
-    return (double)(float)w == w && (w == 0.0 || (a >= 0x1p-40 && a <= 0x1p40));
-}
 
 // Weight statistics of a catalogue into s[0..3]: s[0] |= 1 when some weight
 // is not 1, s[1] |= 1 when some weight is not finite and |= 2 when some weight
@@ -401,7 +389,6 @@ __global__ void k_weight_stats(const double4 *__restrict__ raw, long long n, uns
 // its neighbour entry's shift, exactly as the list build computed it)
 // k = exact bin of |p1-q|^2, f = sqrtf(float(|p1-q|^2)) as in device.cu:173.
 
-constexpr int kQueueCap = 320;  // deferred exact rechecks per wave (mixed path), >= 4*64 + 64
 
 struct LdsTabs {
     unsigned long long *hist;
@@ -415,17 +402,7 @@ struct LdsTabs {
     int xhi;             // kHistExact: byte offset of the high words after the low words
 };
 
-// Neighbour-list flavours: threeD plain (Elem + weight), threeD mixed (MList),
-// rppi box and rppi survey (RElem).
-constexpr int kListPlain = 0, kListMixed = 1, kListRppiBox = 2, kListRppiSurvey = 3;
 
-// One rppi list entry: image position, to the primary rp^2 (box) or the
-// squared 3D separation (survey) r, then q, and the weight.  Box: q holds the
-// side's FP32 root sqrtf(rp^2), rp bin and pi bin floor(sqrtf(pi^2)) packed
-// (rp_side_pack), so a pair only evaluates its third side; survey: x - ori.x.
-struct RElem {
-    double x, y, z, r, q, w;
-};
 __device__ __forceinline__ double rp_side_pack(float f, int k, int p) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)(k | (p << 8)) << 32) |
                                             (unsigned long long)__float_as_uint(f)));
@@ -438,11 +415,6 @@ __device__ __forceinline__ void rp_side_unpack(double q, float &f, int &k, int &
     p = (int)(kp >> 8);
 }
 
-// Bytes of one list entry in a wave's scratch slot.
-__host__ __device__ constexpr size_t list_entry_bytes(int lm) {
-    return lm == kListMixed ? sizeof(float4) + sizeof(int2) + sizeof(int)
-                            : lm == kListPlain ? sizeof(Elem) + sizeof(double) : sizeof(RElem);
-}
 
 // Arrays of a mixed-path list inside a wave's scratch slot.
 struct MList {
@@ -788,47 +760,6 @@ __global__ __launch_bounds__(256) void k_list_bound(KArgs A, TermDesc T, int p0,
     if (lane == 0) atomicMax(out, (unsigned)best);
 }
 
-// LDS histogram layouts of the triplet kernel (HM):
-//   kHistCtab   compact C(nb+2,3) sorted slots (u64 counts / f64 sums), slot
-//               index through ctab; the plain path's FP64 bin lookup may take
-//               several steps;
-//   kHistT3     compact sorted slots, slot byte offset from the t3 table;
-//   kHistDense  dense nb^3 u32 counts indexed (ka, kb, k) unsorted, folded
-//               into the sorted slots when flushed (unit-weight terms only).
-// kHistT3 and kHistDense also take the single-step FP64 bin lookup.
-constexpr int kHistCtab = 0, kHistT3 = 1, kHistDense = 2;
-constexpr int kHistGlobal = 3;  // rppi: the dense histogram in global memory, no LDS copy
-constexpr int kHistStream = 4;  // rppi: key stream binned per bucket in LDS (reported layout)
-// kHistExact: the sorted slots of kHistT3 as 128-bit fixed-point integers (a
-// low-word array, then a high-word array): weight products are truncated to
-// multiples of 2^-K and summed exactly, so the sums do not depend on the order
-// of the additions (nc_set_accumulation NC_ACCUM_EXACT).
-constexpr int kHistExact = 5;
-// kHistF32: the sorted slots of kHistT3 as FP32 running sums with FP64
-// compensation (nc_set_accumulation NC_ACCUM_FP32): an array of nbc FP64
-// compensations at the slot offsets of kHistT3, then (at byte xhi = 8 nbc) an
-// array of nbc FP32 sums.  Each accepted pair's weight product is formed in
-// FP32 as a float-float pair (hi, lo) and hi is added to the FP32 sum with a
-// returning LDS atomic; the rounding error of that addition (TwoSum of the old
-// sum and hi, exact) and lo go to the compensation.  Sum + compensation is the
-// exact sum of the float-float products up to the FP64 rounding of the
-// compensation's own additions; the two are folded into the FP64 totals at
-// every merge.
-constexpr int kHistF32 = 6;
-// kHistF32C: the layout of kHistF32 with the slot offsets of kHistCtab (the
-// reference's index arithmetic), for bin counts whose t3 table and FP32 sums
-// do not fit in LDS together.
-constexpr int kHistF32C = 7;
-__host__ __device__ constexpr bool is_f32(int hm) { return hm == kHistF32 || hm == kHistF32C; }
-// layouts addressed through the t3 slot table, or through ctab
-__host__ __device__ constexpr bool t3_like(int hm) { return hm == kHistT3 || hm == kHistExact || hm == kHistF32; }
-__host__ __device__ constexpr bool ctab_like(int hm) { return hm == kHistCtab || hm == kHistF32C; }
-// layouts whose merges are also forced by a count of pairs added (flush_pairs)
-__host__ __device__ constexpr bool counts_pairs(int hm) { return hm == kHistDense || is_f32(hm); }
-// The mixed path's FP32 lookup (lut32, copied into LDS) holds in its entries'
-// .x the bin count in the layout's units: bytes of a dense count (4), of a t3
-// entry (2), or bins (ctab); the shift of .x.
-__host__ __device__ constexpr int lut32_units(int hm) { return hm == kHistDense ? 2 : t3_like(hm) ? 1 : 0; }
 
 // Byte offset in the LDS histogram of the slot of bins (ka, kb, k), k in
 // [-1, nb] (slots of k outside [0, nb) are never added).
@@ -1936,15 +1867,6 @@ __device__ __forceinline__ bool rppi_cell_sides(const KArgs &A, double r1, float
     return true;
 }
 
-// Per-wave staging of the rppi key stream: up to kStage keys (and values);
-// a row adds at most 2*64, so the wave flushes once it holds more than
-// kStage - 128.  kStreamBuckets bounds the buckets of one term.  Per bucket
-// of its current term a wave holds a reserved run [cur, end) of its region
-// (taken rs_blk slots at a time, so the bucket cursors see one atomic per
-// run, not one per flush); slots it leaves unused get kRsPad.
-constexpr int kStage = 512;
-constexpr int kStreamBuckets = 128;
-constexpr unsigned kRsPad = 0xffffffffu;
 // Views of a wave's stream state, after its staged keys: the flush
 // permutation, then per bucket of the current term (nbm slots, the launch's
 // largest bucket count rounded up to 8) counts and run positions.
@@ -1969,12 +1891,6 @@ struct StreamLds {
         cap = q + 5 * nbm;
     }
 };
-__host__ __device__ constexpr size_t stream_lds_offset(bool weighted) {
-    return ((size_t)kStage * 4 + (weighted ? (size_t)kStage * 8 : 0) + (size_t)kStage * 2 + 15) & ~(size_t)15;
-}
-__host__ __device__ constexpr size_t stream_wave_bytes(bool weighted, int nbm) {
-    return stream_lds_offset(weighted) + (size_t)kStage * 2 + (size_t)nbm * 60;
-}
 
 // Exclusive prefix sum over the 64 lanes of a wave; *total: the sum.
 __device__ __forceinline__ unsigned wave_excl_scan(unsigned v, int lane, unsigned *total) {
@@ -2535,2047 +2451,9 @@ template __global__ void k_triplets<true, kHistF32C, true>(KArgs);
 // allocates nothing.  In host code `noalloc_scratch` names a host-side
 // stand-in, not device memory: the device address is looked up per device
 // with hipGetSymbolAddress (noalloc_base) and only that address is used.
-constexpr size_t kNoAllocBytes = (size_t)256 << 20;
 static __device__ __align__(256) char noalloc_scratch[kNoAllocBytes];
 
-// ---------------------------------------------------------------------------
-// host side
-
-namespace {
-
-struct DBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
-constexpr int kNoExact = -100000;  // no exact layout (see convert_out)
-
-hipError_t dgrow(DBuf &b, size_t need) {
-    if (need == 0) need = 16;
-    if (need <= b.bytes) return hipSuccess;
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
-    hipError_t e = hipMalloc(&b.p, need);
-    if (e == hipSuccess) b.bytes = need;
-    return e;
-}
-
-void dfree(DBuf &b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
-}
-
-// Where a count takes its device buffers: grown with hipMalloc and kept
-// (the default), or carved from a fixed arena that is never freed
-// (nc_count_noalloc: the static per-device scratch array).  An arena request
-// past its end fails with `exhausted` set and allocates nothing.
-// Test knob (NC_FAIL_DEVICE_ALLOC=n at nc_create): the context's n-th device
-// allocation from then on (counting from 0) fails as out of memory, so the
-// error paths can be driven.  *fail < 0: off.
-hipError_t dgrow_checked(long long *fail, DBuf &b, size_t need) {
-    if (fail && *fail >= 0 && (need == 0 ? 16 : need) > b.bytes && (*fail)-- == 0) return hipErrorOutOfMemory;
-    return dgrow(b, need);
-}
-
-struct Mem {
-    char *base = nullptr;
-    size_t size = 0, used = 0;
-    bool exhausted = false;
-    long long *fail = nullptr;  // the context's allocation-failure countdown
-    bool arena() const { return base != nullptr; }
-    size_t left() const { return size - std::min(size, (used + 255) & ~(size_t)255); }
-    hipError_t grow(DBuf &b, size_t need) {
-        if (!base) return dgrow_checked(fail, b, need);
-        if (need == 0) need = 16;
-        const size_t at = (used + 255) & ~(size_t)255;
-        if (at > size || need > size - at) {
-            exhausted = true;
-            return hipErrorOutOfMemory;
-        }
-        b.p = base + at;
-        b.bytes = need;
-        used = at + need;
-        return hipSuccess;
-    }
-};
-
-// Device buffers of one count (per context, reused across counts).
-struct CountBufs {
-    DBuf tables, terms, chunks, out, counter, errw, scratch, bound, wflags;
-    DBuf cflags;  // per catalogue of a count: some cell index is not an exact floor
-};
-
-struct Geometry {
-    double box[3];
-    double cs[3];
-    int nc[3];
-    long long ncells;
-    int survey;
-    int nb, nbc;
-    int rppi = 0, npi = 0;      // rppi binning: npi unit-width pi bins
-    double ori_x = 0.0;         // survey origin (x only reaches the device in the reference)
-    long long out_words = 0;    // output words per term: nbc compact bins, or nb^3 * npi^2 (rppi)
-    std::vector<double> edges, sq;
-    std::vector<uint16_t> lut;
-    double lut_scale;
-    int lut_m;
-    std::vector<int> ctab;
-    bool lut_single_step = false;  // one step over the edges always suffices after the lookup
-    // mixed path: guard bands and the FP32 lookup (valid when mixed_ok)
-    bool mixed_ok = false;
-    std::vector<uint2> lut32;
-    double sigma32 = 0.0;
-    float lut32_lo = 0.f;
-    int lut32_shift = 0, lut32_c0 = 0;
-    float g32 = 0.f, tri_t = 0.f;
-    std::vector<uint16_t> t3;  // (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
-};
-
-}  // namespace
-
-// The autotuner times contiguous runs of work items, so that the workgroups
-// running at one time share neighbourhoods (in L2) as in a full count.
-constexpr long long kSampleRun = 64;
-
-struct nc_catalog {
-    nc_context *ctx = nullptr;  // null once the context is destroyed
-    int device = 0;
-    uint64_t uid = 0;  // unique per handle (graph plans key on it)
-    int64_t n = 0;
-    const double *d_raw = nullptr;
-    bool owns = false;
-    bool unit_w = true;
-    bool exact = false;  // last build: every cell index the exact floor of coordinate / cell size
-    double wmax = 0.0;     // largest finite |w|
-    bool wfinite = true;   // every weight finite
-    bool wf32 = true;      // every weight an FP32 weight (fp32_weight)
-    DBuf raw, key, rank, perm, cnt, start, scan, x, y, z, w;
-};
-
-// Captured launch sequence of nc_count_batched: replayed while the same
-// terms, params, path and tuning are passed again.
-struct BatchPlan {
-    bool valid = false;
-    std::vector<uint64_t> uids;
-    std::vector<int> wkey;  // per term: unit-weight flags of c1, c2, c3 (bits 0-2)
-    std::vector<double> pkey;
-    std::vector<int> tkey;
-    hipGraphExec_t build_exec = nullptr, count_exec = nullptr;
-    unsigned long long *h_out = nullptr;
-    unsigned *h_err = nullptr;
-    std::vector<int> weighted;
-    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0, tri_terms = 0, swap_terms = 0;
-    int xk = kNoExact;  // exact layout: fixed-point scale exponent
-    long long nbc = 0;  // output words per term (slots; twice as many u64 words in the exact layout)
-};
-
-struct nc_context {
-    int device = 0;
-    BatchPlan plan;
-    uint64_t next_uid = 1;
-    std::vector<nc_catalog *> cats;  // live catalogues, detached by nc_destroy
-    hipStream_t stream = nullptr;
-    std::string err;
-    int ncu = 0;
-    size_t lds_max = 0;  // LDS budget of a workgroup with two per CU
-    size_t lds_hw = 0;   // the per-workgroup maximum
-    CountBufs buf;  // device buffers of a count, kept at their largest size
-    char *noalloc = nullptr;  // this device's noalloc_scratch (device address), set by nc_create
-    std::string noalloc_err;  // why it is missing, if it is
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    nc_timing timing{};
-    int waves_per_group = 8;
-    int chunk = 64;
-    int wgs_per_cu = 0;
-    int chunk_elems = 128;
-    // dense counts are merged once this many pairs were added (< 2^31 keeps
-    // u32 counts from wrapping; NC_DENSE_FLUSH_PAIRS lowers it for tests)
-    unsigned long long dense_flush = 1ull << 31;
-    int sample_stride = 1;    // autotuner: queue only every n-th run of kSampleRun work items
-    int slice = 0, nslices = 1;  // nc_set_primary_slice: primaries of slice `slice` of `nslices` only
-    bool autotuning = false;  // in nc_autotune's timed runs
-    // longest list of each launch's terms (keyed by their catalogues), found
-    // once per nc_autotune call: its runs share the inputs
-    std::map<std::vector<uint64_t>, int> tune_list_bound;
-    std::string dev_name;
-    // rppi key stream: buffers, and the per-bucket capacities last sized for
-    // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
-    int rppi_stream = -1;  // rppi key stream: 1 on, 0 off (direct atomics), -1 box on, survey off
-    bool batch_split = true;  // unit-weight terms of a mixed-weight batch as their own launch
-    bool tri = true;          // NC_TRI=0: terms over one catalogue never use the six-fold symmetry
-    int accum = NC_ACCUM_FP64;  // nc_set_accumulation
-    long long fail_alloc = -1;  // NC_FAIL_DEVICE_ALLOC (test knob; see dgrow_checked)
-    double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)
-    double rs_round_keys = 0.0;  // NC_RPPI_ROUND_KEYS: keys per round (tests of several rounds)
-    DBuf rs_tab, rs_cursor, rs_base, rs_items, rs_sample, rs_chunk_keys, rs_keys, rs_vals;
-    std::vector<double> rs_key;
-    std::vector<unsigned long long> rs_caps;
-    std::vector<double> rs_cest;  // estimated keys per work item
-};
-
-namespace {
-
-int fail(nc_context *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define NC_HIP(ctx, call)                                                                               \
-    do {                                                                                                \
-        hipError_t e_ = (call);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return fail(ctx, e_ == hipErrorOutOfMemory ? NC_ERR_NOMEM : NC_ERR_HIP,                     \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                             \
-    } while (0)
-
-// A buffer request: on the arena, running past its end is NC_ERR_LIMIT.
-#define NC_MEM(ctx, m, b, bytes)                                                                       \
-    do {                                                                                                \
-        const hipError_t e2_ = (m).grow((b), (bytes));                                                  \
-        if (e2_ != hipSuccess && (m).exhausted)                                                         \
-            return fail(ctx, NC_ERR_LIMIT,                                                              \
-                        "the count needs more device memory than the no-alloc scratch holds (" +         \
-                            std::to_string((m).size >> 20) + " MiB)");                                  \
-        NC_HIP(ctx, e2_);                                                                               \
-    } while (0)
-
-int64_t choose_compact(int nb) { return (int64_t)nb * (nb + 1) * (nb + 2) / 6; }
-
-// Guard band, collinearity bound and FP32 lookup of the mixed path.
-//
-// Let u = 2^-24, R = rmax, A, B the FP64 image positions of two list entries,
-// P the primary.  Stored: r = fl32(A - P) per component (|error| <= u*R, as
-// |A - P| < R), s = fl32(|r|^2).  Per pair: dot = fl32(r_a . r_b) and
-// d32 = fl32(s_a + s_b - 2*dot).  Then, with S = |A-P|^2 and D = (A-P).(B-P)
-// exact,  |s - S| <= 6.5uR^2,  |dot - D| <= 6.5uR^2,  and
-//     |d32 - |A-B|^2| <= 32uR^2,
-// while the reference's FP64 |A-B|^2 is within ~1e-15 R^2 of the exact value.
-// Every squared edge E (rounded to FP32, error <= u*R^2) gets the guard band
-// g32 = 80uR^2 (2x the sum): d32 outside it is on the same side of E as the
-// reference's value.
-//
-// Triangle test.  The reference rejects fl(f1 + f2) <= f3 on sorted
-// f = sqrtf(float(r^2)); with exact lengths a = |A-P|, b = |B-P|, c = |A-B|,
-// it accepts whenever min(a + b - c, c - |a - b|) > 20uR.  Both slacks are at
-// least ab*sin^2(t)/(2(a+b)) = (S_a S_b - D^2)/(2ab(a+b)) and ab(a+b) <=
-// R(S_a + S_b), so  S_a S_b - D^2 > 40uR^2 (S_a + S_b)  is sufficient.  The
-// FP32 value s_a*s_b - dot^2 is within 28uR^4 of the exact one, so
-// s_a*s_b - dot^2 > 80uR^2 (s_a + s_b) + 64uR^4  (both terms doubled)
-// suffices; with s_a, s_b <= R^2 (1 + 8u) (list entries lie within R) the
-// one constant  tri_t = 80uR^2 * 2R^2 (1 + 1e-5) + 64uR^4  bounds the right
-// side for every pair.  A pair with s_a*s_b - dot^2 > tri_t is accepted,
-// any other re-evaluated exactly.
-//
-// Scaling: s32 = sigma^2 is a power of 4 and the lists hold sigma*(A - P)
-// (rounded once, from the exact FP64 product), so every FP32 quantity above
-// is the unscaled one times a power of 2, with the same rounding; the
-// thresholds g32, tri_t and the edges are stored scaled (by s32, s32^2,
-// s32).
-//
-// Lookup.  For a positive float x, bits(x) >> sh is monotone in x: cell c
-// holds the floats [val(c << sh), val((c+1) << sh)), 2^(23-sh) log-spaced
-// cells per octave, fine near zero where squared edges crowd and coarse where
-// they are sparse.  The kernel clamps d32' = s32*d32 to [val(c0 << sh), 4]
-// (cell c0 thereby takes every smaller value, negative ones included, and
-// the cell c1 of 4.0, above the scaled rmax^2 < 4, every larger one) and
-// reads entry c - c0 = {number of edges whose bands lie wholly below the
-// cell, the edge s32*E of the one band that meets the cell as float bits,
-// +inf if none}: the bin is that number, minus one when d32' < E.  c0 is at most 2^w - 1 cells below c1 and at least the cell of the
-// smallest normal float.  The smallest table (w = 8..13) for which some
-// cell density (8..4096 per octave) leaves no cell meeting two bands is kept;
-// bins narrower than the bands, or no such table, disable the path.
-static void make_mixed_tables(Geometry &G) {
-    G.mixed_ok = false;
-    const double u = std::ldexp(1.0, -24);
-    const double R = G.edges.back();
-    const int ne = G.nb + 1;
-    const double g = 80.0 * u * R * R;
-    std::vector<float> E(ne);
-    for (int k = 0; k < ne; ++k) E[k] = (float)G.sq[k];
-    for (int k = 0; k + 1 < ne; ++k)
-        if (!((double)E[k] + g < (double)E[k + 1] - g)) return;  // bins narrower than the guard band
-    // s32 = 4^kk with s32 * rmax^2 in [1, 4)
-    const int kk = -(int)std::floor(std::log2(G.sq[G.nb]) / 2.0);
-    const double s = std::ldexp(1.0, 2 * kk);
-    std::vector<double> Es(ne);
-    for (int k = 0; k < ne; ++k) Es[k] = (double)E[k] * s;  // exact: s is a power of 4
-    const double gs = g * s;
-    auto fbits = [](float x) {
-        uint32_t b;
-        std::memcpy(&b, &x, 4);
-        return b;
-    };
-    auto fval = [](uint32_t b) {
-        float x;
-        std::memcpy(&x, &b, 4);
-        return (double)x;
-    };
-    const uint32_t top = fbits(4.0f);  // the kernel's upper clamp, above s32 * rmax^2
-    for (int w = 8; w <= 13; ++w)
-        for (int p = 3; p <= 12; ++p) {
-            const int sh = 23 - p;
-            const int64_t c1 = top >> sh;
-            const int64_t c0 = std::max<int64_t>(c1 - ((int64_t)1 << w) + 1, (int64_t)(0x00800000u >> sh) + 1);
-            const int n = (int)(c1 - c0 + 1);
-            std::vector<uint2> lut(n);
-            bool ok = n >= 2;
-            // cells and bands both ascend: edges [0, below) lie under the
-            // cell, [below, over) meet it
-            int below = 0, over = 0;
-            for (int j = 0; j < n && ok; ++j) {
-                const uint32_t c = (uint32_t)(c0 + j);
-                const double a = j == 0 ? -INFINITY : fval(c << sh) * (1.0 - 1e-6);
-                const double b = j == n - 1 ? INFINITY : fval((c + 1) << sh) * (1.0 + 1e-6);
-                while (below < ne && Es[below] + gs < a) ++below;
-                over = std::max(over, below);
-                while (over < ne && Es[over] - gs <= b) ++over;
-                if (over - below > 1) ok = false;
-                const float edge = over > below ? (float)Es[below] : INFINITY;
-                lut[j] = make_uint2((uint32_t)below, fbits(edge));
-            }
-            if (!ok) continue;
-            G.lut32 = std::move(lut);
-            G.lut32_shift = sh;
-            G.lut32_c0 = (int)c0;
-            G.lut32_lo = (float)fval((uint32_t)c0 << sh);
-            G.sigma32 = std::ldexp(1.0, kk);
-            G.g32 = (float)gs;
-            const double R2s = R * R * s;
-            G.tri_t = std::nextafter((float)(80.0 * u * R2s * 2.0 * R2s * (1.0 + 1e-5) + 64.0 * u * R2s * R2s),
-                                     INFINITY);
-            G.mixed_ok = true;
-            return;
-        }
-}
-
-static void make_slot_table(Geometry &G) {
-    const int nb = G.nb;
-    G.t3.assign((size_t)nb * nb * (nb + 2), 0);
-    for (int a = 0; a < nb; ++a)
-        for (int b = 0; b < nb; ++b)
-            for (int k = 0; k < nb; ++k) {
-                const int lo = std::min(std::min(a, b), k), hi = std::max(std::max(a, b), k);
-                const int mid = a + b + k - lo - hi;
-                G.t3[((size_t)a * nb + b) * (nb + 2) + k + 1] =
-                    (uint16_t)((G.ctab[lo * nb + mid] + (hi - mid)) * sizeof(unsigned long long));
-            }
-}
-
-// Geometry of the reference's threeD constructor (cudaNcorr.cu:118-157) plus
-// the lookup tables of the kernels.
-int make_geometry(nc_context *ctx, const nc_params *P, Geometry &G) {
-    if (!P || !P->edges || P->nedges < 2) return fail(ctx, NC_ERR_BINS, "need at least two bin edges");
-    const int ne = P->nedges;
-    G.edges.assign(P->edges, P->edges + ne);
-    if (!(G.edges[0] >= 0.0)) return fail(ctx, NC_ERR_BINS, "first bin edge must be >= 0");
-    for (int i = 1; i < ne; ++i)
-        if (!(G.edges[i] > G.edges[i - 1])) return fail(ctx, NC_ERR_BINS, "bin edges must be strictly increasing");
-    G.nb = ne - 1;
-    if (G.nb > 40) return fail(ctx, NC_ERR_LIMIT, "at most 40 bins per side are supported");
-    G.nbc = (int)choose_compact(G.nb);
-    G.out_words = G.nbc;
-    G.sq.resize(ne);
-    for (int i = 0; i < ne; ++i) G.sq[i] = G.edges[i] * G.edges[i];  // cudaNcorr.cu:124-127
-    G.survey = P->survey ? 1 : 0;
-    if (P->bin_type != NC_BIN_THREED && P->bin_type != NC_BIN_RPPI) return fail(ctx, NC_ERR_ARG, "unknown bin type");
-    double cell[3];
-    for (int a = 0; a < 3; ++a) cell[a] = G.edges[ne - 1];  // cellSize = rmax, cudaNcorr.cu:137
-    if (P->bin_type == NC_BIN_RPPI) {
-        // rppi constructor, cudaNcorr.cu:159-216: pi bins 0..npi, cells
-        // (rpMax, rpMax, piMax) in a box, sqrtf(rpMax^2 + piMax^2) (FP32) in a survey
-        if (P->num_pi_bins < 1 || P->num_pi_bins > 4096) return fail(ctx, NC_ERR_BINS, "need 1..4096 pi bins");
-        G.rppi = 1;
-        G.npi = P->num_pi_bins;
-        G.ori_x = P->ori[0];
-        const double rpmax = G.edges[ne - 1], pimax = (double)G.npi;
-        const double rmax = (double)sqrtf((float)(rpmax * rpmax + pimax * pimax));
-        for (int a = 0; a < 3; ++a) cell[a] = G.survey ? rmax : (a == 2 ? pimax : rpmax);
-        G.out_words = (long long)G.nb * G.nb * G.nb * G.npi * G.npi;
-        if (G.out_words > (1LL << 27)) return fail(ctx, NC_ERR_LIMIT, "rppi histogram larger than 2^27 bins");
-    }
-    G.ncells = 1;
-    for (int a = 0; a < 3; ++a) {
-        G.box[a] = P->box[a];
-        G.cs[a] = cell[a];
-        const double q = std::ceil(G.box[a] / G.cs[a]);
-        if (!(q >= 1.0 && q <= 1048576.0)) return fail(ctx, NC_ERR_ARG, "box must be positive and finite");
-        G.nc[a] = (int)q;
-        G.ncells *= G.nc[a];
-    }
-    if (G.ncells > (1LL << 24)) return fail(ctx, NC_ERR_LIMIT, "cell grid larger than 2^24 cells");
-    // lower-bound bin lookup over d*scale (see bin_of / bin_any): the table
-    // is refined until one step over the squared edges always suffices.
-    const double r2max = G.sq[G.nb];
-    for (int M = 2048;; M *= 2) {
-        G.lut_m = M;
-        G.lut_scale = (double)M / r2max;
-        G.lut.assign(M + 1, 0);
-        int worst = 0;
-        for (int j = 0; j <= M; ++j) {
-            int k = 0;
-            for (int e = 1; e < G.nb; ++e)
-                if (G.sq[e] * G.lut_scale * (1.0 + 1e-12) <= (double)(j - 1)) k = e;
-            G.lut[j] = (uint16_t)k;
-            int steps = 0;
-            for (int e = k + 1; e < G.nb; ++e)
-                if (G.sq[e] <= (double)(j + 1) / G.lut_scale * (1.0 + 1e-9)) ++steps;
-            worst = std::max(worst, steps);
-        }
-        G.lut_single_step = worst <= 1;
-        if (G.lut_single_step || M >= 8192) break;
-    }
-    // compact index of sorted (i <= j <= k) bins in lexicographic order
-    G.ctab.assign((size_t)G.nb * G.nb, 0);
-    int c = 0;
-    for (int i = 0; i < G.nb; ++i)
-        for (int j = i; j < G.nb; ++j) {
-            G.ctab[i * G.nb + j] = c;
-            c += G.nb - j;
-        }
-    if (G.rppi) return NC_OK;  // plain FP64 path only, histogram in global memory
-    make_mixed_tables(G);
-    if ((size_t)G.nbc * sizeof(unsigned long long) <= 65535) make_slot_table(G);  // byte offsets fit 16 bits
-    return NC_OK;
-}
-
-struct CatGrid {
-    nc_catalog *cat;
-    bool is_p1;
-    bool expect_exact;  // a captured build: the plan counted this catalogue as exactly gridded
-};
-
-int build_grid(nc_context *ctx, Mem &m, CountBufs &B, const Geometry &G, nc_catalog *c, bool check_unclamped,
-               unsigned *inexact, bool expect_exact) {
-    const int n = (int)c->n;
-    // slab counts, padded to whole scan tiles (the padding stays zero)
-    const size_t ntiles = ((size_t)G.ncells * kSlabs + kScanTile - 1) / kScanTile;
-    const size_t ns = ntiles * kScanTile;
-    NC_MEM(ctx, m, c->key, sizeof(uint32_t) * (size_t)n);
-    NC_MEM(ctx, m, c->rank, sizeof(uint32_t) * (size_t)n);
-    NC_MEM(ctx, m, c->perm, sizeof(uint32_t) * (size_t)n);
-    NC_MEM(ctx, m, c->cnt, sizeof(uint32_t) * ns);
-    NC_MEM(ctx, m, c->start, sizeof(uint32_t) * (ns + 1));
-    NC_MEM(ctx, m, c->scan, sizeof(uint32_t) * (2 * ntiles + 1));  // tile sums, then their scan
-    NC_MEM(ctx, m, c->x, sizeof(double) * (size_t)n);
-    NC_MEM(ctx, m, c->y, sizeof(double) * (size_t)n);
-    NC_MEM(ctx, m, c->z, sizeof(double) * (size_t)n);
-    NC_MEM(ctx, m, c->w, sizeof(double) * (size_t)n);
-    GridGeo g;
-    for (int a = 0; a < 3; ++a) {
-        g.cs[a] = G.cs[a];
-        g.nc[a] = G.nc[a];
-    }
-    g.survey = G.survey;
-    const int fb = (int)std::min<long long>(((long long)ns + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(256), 0, ctx->stream, (uint32_t *)c->cnt.p, 0u, (long long)ns);
-    if (n > 0) {
-        const int nb = (n + 255) / 256;
-        hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n, g,
-                           check_unclamped ? 1 : 0, (uint32_t *)c->key.p, (uint32_t *)c->rank.p, (uint32_t *)c->cnt.p,
-                           (unsigned *)B.errw.p, inexact, expect_exact ? 1 : 0);
-    }
-    uint32_t *part = (uint32_t *)c->scan.p, *pstart = part + ntiles;
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream, (const uint32_t *)c->cnt.p, part);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)part, pstart, (int)ntiles);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(256), 0, ctx->stream, (const uint32_t *)c->cnt.p,
-                       (const uint32_t *)pstart, (uint32_t *)c->start.p, (int)ntiles);
-    if (n > 0) {
-        const int nb = (n + 255) / 256;
-        hipLaunchKernelGGL(k_place, dim3(nb), dim3(256), 0, ctx->stream, n, (const uint32_t *)c->key.p,
-                           (const uint32_t *)c->rank.p, (const uint32_t *)c->start.p, (uint32_t *)c->perm.p);
-        hipLaunchKernelGGL(k_stable_scatter, dim3(nb), dim3(256), 0, ctx->stream, (const double4 *)c->d_raw, n,
-                           (const uint32_t *)c->key.p, (const uint32_t *)c->start.p, (const uint32_t *)c->perm.p,
-                           (double *)c->x.p, (double *)c->y.p, (double *)c->z.p, (double *)c->w.p);
-    }
-    NC_HIP(ctx, hipGetLastError());
-    return NC_OK;
-}
-
-struct Term {
-    nc_catalog *c1, *c2, *c3;
-};
-
-// Device view of a term: the cell-sorted arrays of its three catalogues.
-TermDesc term_desc(const Term &t) {
-    TermDesc d;
-    d.x1 = (const double *)t.c1->x.p;
-    d.y1 = (const double *)t.c1->y.p;
-    d.z1 = (const double *)t.c1->z.p;
-    d.w1 = (const double *)t.c1->w.p;
-    d.x2 = (const double *)t.c2->x.p;
-    d.y2 = (const double *)t.c2->y.p;
-    d.z2 = (const double *)t.c2->z.p;
-    d.w2 = (const double *)t.c2->w.p;
-    d.st2 = (const uint32_t *)t.c2->start.p;
-    d.x3 = (const double *)t.c3->x.p;
-    d.y3 = (const double *)t.c3->y.p;
-    d.z3 = (const double *)t.c3->z.p;
-    d.w3 = (const double *)t.c3->w.p;
-    d.st3 = (const uint32_t *)t.c3->start.p;
-    d.n1 = (int)t.c1->n;
-    d.sym = t.c2 == t.c3;
-    d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
-    d.tri = 0;  // set by run_count once the builds have shown exact gridding
-    d.prune2 = d.prune3 = 0;  // likewise
-    return d;
-}
-
-void plan_drop(nc_context *ctx) {
-    BatchPlan &p = ctx->plan;
-    if (p.build_exec) (void)hipGraphExecDestroy(p.build_exec);
-    if (p.count_exec) (void)hipGraphExecDestroy(p.count_exec);
-    if (p.h_out) (void)hipHostFree(p.h_out);
-    if (p.h_err) (void)hipHostFree(p.h_err);
-    p = BatchPlan();
-}
-
-// A 128-bit fixed-point sum (high word, low word) in units of 2^-k, rounded
-// once to double.
-double fixed_to_double(unsigned long long lo, unsigned long long hi, int k) {
-    const __int128 v = (__int128)(((unsigned __int128)hi << 64) | (unsigned __int128)lo);
-    return std::ldexp((double)v, -k);
-}
-
-// Compact sorted-bin results -> dense nb^3 histogram (device.cu:198 layout).
-// Exact layout (xk != kNoExact): per term nbc low words then nbc high words,
-// weighted sums in units of 2^-xk, counts in the low words.
-void convert_out(int nb, long long nbc, int rppi, int nterms, const std::vector<int> &weighted,
-                 const unsigned long long *res, double *hist, uint64_t *counts, int xk = kNoExact) {
-    const bool exact = xk != kNoExact;
-    const long long stride = exact ? 2 * nbc : nbc;
-    if (rppi) {  // already the dense layout
-        for (int i = 0; i < nterms; ++i) {
-            const unsigned long long *r = res + (size_t)i * nbc;
-            double *h = hist + (size_t)i * nbc;
-            uint64_t *cn = counts ? counts + (size_t)i * nbc : nullptr;
-            for (long long k = 0; k < nbc; ++k) {
-                if (weighted[i]) {
-                    std::memcpy(&h[k], &r[k], 8);
-                } else {
-                    h[k] = (double)r[k];
-                    if (cn) cn[k] = r[k];
-                }
-            }
-            if (weighted[i] && cn) std::fill(cn, cn + nbc, (uint64_t)0);
-        }
-        return;
-    }
-    const size_t nh = (size_t)nb * nb * nb;
-    for (int i = 0; i < nterms; ++i) {
-        double *h = hist + (size_t)i * nh;
-        uint64_t *cn = counts ? counts + (size_t)i * nh : nullptr;
-        std::fill(h, h + nh, 0.0);
-        if (cn) std::fill(cn, cn + nh, (uint64_t)0);
-        const unsigned long long *r = res + (size_t)i * stride;
-        int c = 0;
-        for (int a = 0; a < nb; ++a)
-            for (int b = a; b < nb; ++b)
-                for (int k = b; k < nb; ++k, ++c) {
-                    const size_t di = (size_t)k + (size_t)b * nb + (size_t)a * nb * nb;
-                    if (weighted[i] && exact) {
-                        h[di] = fixed_to_double(r[c], r[nbc + c], xk);
-                    } else if (weighted[i]) {
-                        double v;
-                        std::memcpy(&v, &r[c], 8);
-                        h[di] = v;
-                    } else {
-                        h[di] = (double)r[c];
-                        if (cn) cn[di] = r[c];
-                    }
-                }
-    }
-}
-
-// Replay a captured plan: cell-list builds graph, triplet graph, copy-back.
-int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
-    BatchPlan &p = ctx->plan;
-    retry = false;
-    NC_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    NC_HIP(ctx, hipGraphLaunch(p.build_exec, ctx->stream));
-    NC_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    NC_HIP(ctx, hipGraphLaunch(p.count_exec, ctx->stream));
-    NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-    const int nterms = (int)p.weighted.size();
-    const long long ostride = p.xk != kNoExact ? 2 * p.nbc : p.nbc;
-    NC_HIP(ctx, hipMemcpyAsync(p.h_out, ctx->buf.out.p, sizeof(unsigned long long) * (size_t)nterms * ostride,
-                               hipMemcpyDeviceToHost, ctx->stream));
-    NC_HIP(ctx, hipMemcpyAsync(p.h_err, ctx->buf.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    float ms_b = 0.f, ms_c = 0.f;
-    NC_HIP(ctx, hipEventElapsedTime(&ms_b, ctx->ev[0], ctx->ev[1]));
-    NC_HIP(ctx, hipEventElapsedTime(&ms_c, ctx->ev[1], ctx->ev[3]));
-    ctx->timing.ms_build = ms_b;
-    ctx->timing.ms_count = ms_c;
-    ctx->timing.ms_total = ms_b + ms_c;
-    ctx->timing.n_kernel = p.nkernel;
-    ctx->timing.hist_layout = p.hist_layout;
-    ctx->timing.lds_bytes = p.lds_bytes;
-    ctx->timing.path = p.path;
-    ctx->timing.tri_terms = p.tri_terms;
-    ctx->timing.swap_terms = p.swap_terms;
-    if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
-    if (*p.h_err & (kErrOverflow | kErrReplan)) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
-        plan_drop(ctx);
-        retry = true;
-        return NC_OK;
-    }
-    if (*p.h_err & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: a term does not match the weighting of its kernel variant");
-    convert_out(p.nb, p.nbc, p.rppi, nterms, p.weighted, p.h_out, hist, counts, p.xk);
-    return NC_OK;
-}
-
-int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &terms, int path, double *hist,
-              uint64_t *counts, bool use_graph, Mem &m) {
-    if (!ctx) return NC_ERR_ARG;
-    // buffers: the context's own, or (arena) fresh ones carved from it; an
-    // arena count is never captured
-    CountBufs arena_bufs;
-    CountBufs &bufs = m.arena() ? arena_bufs : ctx->buf;
-    if (m.arena()) use_graph = false;
-    if (!P || !hist) return fail(ctx, NC_ERR_ARG, "null params or output");
-    if (path != NC_PATH_PLAIN && path != NC_PATH_MIXED) return fail(ctx, NC_ERR_ARG, "unknown path");
-    for (const Term &t : terms)
-        if (!t.c1 || !t.c2 || !t.c3 || t.c1->ctx != ctx || t.c2->ctx != ctx || t.c3->ctx != ctx)
-            return fail(ctx, NC_ERR_ARG, "null catalogue or catalogue of another context");
-    Geometry G;
-    int rc = make_geometry(ctx, P, G);
-    if (rc) return rc;
-    NC_HIP(ctx, hipSetDevice(ctx->device));
-    // --- graph plan: key = catalogues, params, path, tuning
-    std::vector<uint64_t> uids;
-    for (const Term &t : terms) uids.insert(uids.end(), {t.c1->uid, t.c2->uid, t.c3->uid});
-    // weight status of each catalogue (wrapped ones may change it between calls)
-    std::vector<int> wkey;
-    for (const Term &t : terms)
-        wkey.push_back((t.c1->unit_w ? 1 : 0) | (t.c2->unit_w ? 2 : 0) | (t.c3->unit_w ? 4 : 0) |
-                       (t.c1->wf32 ? 8 : 0) | (t.c2->wf32 ? 16 : 0) | (t.c3->wf32 ? 32 : 0));
-    std::vector<double> pkey(P->box, P->box + 3);
-    pkey.push_back((double)G.survey);
-    pkey.push_back((double)G.rppi);
-    pkey.push_back((double)G.npi);
-    pkey.push_back(G.ori_x);
-    pkey.insert(pkey.end(), G.edges.begin(), G.edges.end());
-    // exact accumulation of weighted sums (nc_set_accumulation): every weight
-    // product is truncated to a multiple of 2^-xk with xk set so that the
-    // largest product any weighted term can form stays below 2^60 units
-    // (below 2^63 for the six orders of a triple together)
-    bool all_unit = true;
-    for (const Term &t : terms) all_unit = all_unit && t.c1->unit_w && t.c2->unit_w && t.c3->unit_w;
-    const bool exact = ctx->accum == NC_ACCUM_EXACT && !G.rppi && !all_unit;
-    int xk = kNoExact;
-    if (exact) {
-        double pmax = 0.0;
-        for (const Term &t : terms) {
-            if (t.c1->unit_w && t.c2->unit_w && t.c3->unit_w) continue;
-            if (!t.c1->wfinite || !t.c2->wfinite || !t.c3->wfinite)
-                return fail(ctx, NC_ERR_ARG, "exact accumulation needs finite weights");
-            pmax = std::max(pmax, t.c1->wmax * t.c2->wmax * t.c3->wmax);
-        }
-        int e = 0;
-        if (pmax > 0.0) std::frexp(pmax, &e);  // pmax < 2^e
-        xk = std::min(std::max(60 - e, -900), 900);
-    }
-    pkey.push_back((double)xk);
-    // FP32 accumulation of weighted sums (nc_set_accumulation): FP32 weight
-    // products and running sums with FP64 compensation, for FP32 weights only
-    const bool f32acc = ctx->accum == NC_ACCUM_FP32 && !G.rppi && !all_unit;
-    if (f32acc)
-        for (const Term &t : terms)
-            if (!(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w) && !(t.c1->wf32 && t.c2->wf32 && t.c3->wf32))
-                return fail(ctx, NC_ERR_ARG,
-                            "FP32 accumulation needs FP32 weights (values FP32 holds exactly, 0 or of magnitude in "
-                            "[2^-40, 2^40])");
-    pkey.push_back((double)ctx->accum);
-    const std::vector<int> tkey = {path,       ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems,
-                                   ctx->slice, ctx->nslices};
-    if (use_graph && ctx->plan.valid && ctx->plan.uids == uids && ctx->plan.wkey == wkey && ctx->plan.pkey == pkey &&
-        ctx->plan.tkey == tkey) {
-        bool retry = false;
-        rc = plan_replay(ctx, hist, counts, retry);
-        if (rc || !retry) return rc;
-    }
-    plan_drop(ctx);
-    const int nterms = (int)terms.size();
-    auto al16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
-    const int W = ctx->waves_per_group;
-    // the mixed lookup table must leave LDS room for the smallest layout
-    // (ctab) and 64-entry chunks, else the count runs the plain path
-    const size_t mixed_min_lds = al16(sizeof(uint2) * G.lut32.size()) + al16((size_t)G.nbc * 8) +
-                                 al16(sizeof(int) * G.ctab.size()) +
-                                 (size_t)W * al16(68 * (sizeof(float4) + sizeof(int) + sizeof(double)) +
-                                                  kQueueCap * sizeof(uint2));
-    const bool mixed = path == NC_PATH_MIXED && G.mixed_ok && mixed_min_lds <= ctx->lds_hw;
-    const int block = W * kWave;
-    const int lmode = G.rppi ? (G.survey ? kListRppiSurvey : kListRppiBox) : mixed ? kListMixed : kListPlain;
-    // --- rppi key stream: the reachable dense rp triples (sorted ones and the
-    // next triple, where a pi index equal to npi carries the flat index),
-    // their ranks, and per term a run of buckets of 2^shift compact bins
-    // rppi output: the key stream by default for boxes (6x the direct atomics
-    // on the 1M-point box); direct atomics for surveys, whose pair arithmetic
-    // bounds both paths (2.0 s direct against 2.4 s streamed on 500k data +
-    // 2.5M randoms), unless NC_RPPI_STREAM forces one
-    bool stream = G.rppi && (ctx->rppi_stream > 0 || (ctx->rppi_stream < 0 && !G.survey));
-    std::vector<uint16_t> rs_rank;
-    std::vector<int> rs_inv;
-    std::vector<int4> rs_term(nterms);
-    int rs_nbk = 0;
-    bool rs_anyw = false;
-    int rs_maxshift = 0;
-    if (stream) {
-        const int nb = G.nb, dmax = G.npi == 1 ? 2 : 1;
-        const long long n3 = (long long)nb * nb * nb;
-        std::vector<char> reach((size_t)n3, 0);
-        for (int k1 = 0; k1 < nb; ++k1)
-            for (int k2 = k1; k2 < nb; ++k2)
-                for (int k3 = k2; k3 < nb; ++k3) {
-                    const long long T = k3 + (long long)k2 * nb + (long long)k1 * nb * nb;
-                    for (int d = 0; d <= dmax; ++d)
-                        if (T + d < n3) reach[(size_t)(T + d)] = 1;
-                }
-        rs_rank.assign((size_t)n3, 0xffff);
-        for (long long T = 0; T < n3; ++T)
-            if (reach[(size_t)T]) {
-                rs_rank[(size_t)T] = (uint16_t)rs_inv.size();
-                rs_inv.push_back((int)T);
-            }
-        const long long cbins = (long long)rs_inv.size() * G.npi * G.npi;
-        int sh_u = 0, sh_w = 0;
-        while (sh_u < 15 && ((size_t)4 << (sh_u + 1)) <= ctx->lds_hw) ++sh_u;
-        while (sh_w < 14 && ((size_t)8 << (sh_w + 1)) <= ctx->lds_hw) ++sh_w;
-        if (rs_inv.size() >= 65535 || cbins >= (1LL << 31)) stream = false;
-        for (int i = 0; i < nterms && stream; ++i) {
-            const Term &t = terms[i];
-            const bool w = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
-            const int sh = w ? sh_w : sh_u;
-            const long long nbt = (cbins + (1LL << sh) - 1) >> sh;
-            if (nbt > kStreamBuckets) stream = false;
-            rs_term[i] = make_int4(rs_nbk, (int)nbt, sh, 0);
-            rs_nbk += (int)nbt;
-            rs_anyw = rs_anyw || w;
-            rs_maxshift = std::max(rs_maxshift, sh + (w ? 3 : 2));  // log2 of the binning LDS bytes
-        }
-    }
-    size_t rank_bytes = stream ? al16(sizeof(uint16_t) * rs_rank.size()) : 0;
-    int rs_nbm = 8;  // per-wave bucket slots: the largest bucket count of a term, rounded up to 8
-    for (int i = 0; i < nterms && stream; ++i) rs_nbm = std::max(rs_nbm, (rs_term[i].y + 7) & ~7);
-    size_t stage_bytes = stream ? al16(stream_wave_bytes(rs_anyw, rs_nbm)) : 0;
-    // per-wave LDS: staged list chunk (mixed: float4 + bin, + weight unless
-    // the layout is the unit-weight kHistDense) and the recheck queue (4- or
-    // 8-byte entries, as QueueEntry)
-    auto wave_lds = [&](int c, int hm) {
-        const size_t cp = (size_t)c + 4;
-        if (G.rppi) return al16(cp * sizeof(RElem)) + stage_bytes;
-        return mixed ? al16(cp * (sizeof(float4) + sizeof(int) + (hm == kHistDense ? 0 : sizeof(double))) +
-                            kQueueCap * (hm == kHistDense ? sizeof(uint32_t) : sizeof(uint2)))
-                     : al16(cp * (sizeof(Elem) + sizeof(double)));
-    };
-    // --- tables.  Global copies of all; the kernel copies into LDS what its
-    // variant reads in the pair loop: plain sq + lut, mixed lut32, and the
-    // slot table t3 (kHistT3) or ctab (kHistCtab).  The LDS histogram holds the
-    // C(nb+2,3) sorted bins, or nb^3 dense u32 counts (kHistDense).
-    const size_t g_sq = 0;
-    const size_t g_lut = al16(g_sq + sizeof(double) * G.sq.size());
-    const size_t g_ctab = al16(g_lut + sizeof(uint16_t) * G.lut.size());
-    const size_t g_lut32 = al16(g_ctab + sizeof(int) * G.ctab.size());
-    const size_t g_t3 = al16(g_lut32 + sizeof(uint2) * G.lut32.size());
-    const size_t tab_bytes = al16(g_t3 + sizeof(uint16_t) * G.t3.size());
-    std::vector<char> tabs(tab_bytes);
-    std::memcpy(tabs.data() + g_sq, G.sq.data(), sizeof(double) * G.sq.size());
-    std::memcpy(tabs.data() + g_lut, G.lut.data(), sizeof(uint16_t) * G.lut.size());
-    std::memcpy(tabs.data() + g_ctab, G.ctab.data(), sizeof(int) * G.ctab.size());
-    if (!G.lut32.empty()) std::memcpy(tabs.data() + g_lut32, G.lut32.data(), sizeof(uint2) * G.lut32.size());
-    if (!G.t3.empty()) std::memcpy(tabs.data() + g_t3, G.t3.data(), sizeof(uint16_t) * G.t3.size());
-    const size_t nb3 = (size_t)G.nb * G.nb * G.nb;
-    auto hist_size = [&](int hm) {
-        return hm == kHistGlobal    ? (size_t)0
-               : hm == kHistDense   ? al16(nb3 * 4)
-               : hm == kHistExact   ? al16((size_t)G.nbc * 16)
-               : is_f32(hm)         ? al16((size_t)G.nbc * 12)
-                                    : al16((size_t)G.nbc * 8);
-    };
-    auto lds_tables = [&](int hm, size_t &o_sq, size_t &o_lut, size_t &o_ctab, size_t &o_lut32, size_t &o_t3) {
-        size_t o = 0;
-        o_sq = o_lut = o_ctab = o_lut32 = o_t3 = 0;
-        if (!mixed) {
-            o_sq = o;
-            o = al16(o + sizeof(double) * G.sq.size());
-            o_lut = o;
-            o = al16(o + sizeof(uint16_t) * G.lut.size());
-        } else {
-            o_lut32 = o;
-            o = al16(o + sizeof(uint2) * G.lut32.size());
-        }
-        if (t3_like(hm)) {
-            o_t3 = o;
-            o = al16(o + sizeof(uint16_t) * G.t3.size());
-        } else if (ctab_like(hm)) {
-            o_ctab = o;
-            o = al16(o + sizeof(int) * G.ctab.size());
-        }
-        return o + (hm == kHistGlobal ? rank_bytes : 0);  // the rank table ends the tables
-    };
-    size_t off_sq, off_lut, off_ctab, off_lut32, off_t3;
-    NC_MEM(ctx, m, bufs.tables, tab_bytes);
-    NC_HIP(ctx, hipMemcpyAsync(bufs.tables.p, tabs.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
-    NC_MEM(ctx, m, bufs.errw, 16);
-    // 16 queue counters (zeroed before every launch), then the queue's
-    // kMaxSeg + 1 segment bounds
-    NC_MEM(ctx, m, bufs.counter, sizeof(unsigned) * (16 + kMaxSeg + 1));
-    NC_MEM(ctx, m, bufs.bound, 16);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.bound.p, 0u, 4LL);
-    for (const Term &t : terms)
-        for (nc_catalog *c : {t.c1, t.c2, t.c3})
-            if (c->n > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
-
-    // --- cell lists, one per distinct catalogue
-    NC_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    std::vector<CatGrid> grids;
-    for (const Term &t : terms) {
-        nc_catalog *cs[3] = {t.c1, t.c2, t.c3};
-        for (int r = 0; r < 3; ++r) {
-            bool found = false;
-            for (CatGrid &g : grids)
-                if (g.cat == cs[r]) {
-                    found = true;
-                    g.is_p1 = g.is_p1 || r == 0;
-                }
-            if (!found) grids.push_back({cs[r], r == 0, false});
-        }
-    }
-    NC_MEM(ctx, m, bufs.cflags, sizeof(unsigned) * grids.size());
-    auto enqueue_builds = [&]() -> int {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.errw.p, 0u, 4LL);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, (uint32_t *)bufs.cflags.p, 0u,
-                           (long long)grids.size());
-        for (size_t gi = 0; gi < grids.size(); ++gi) {
-            CatGrid &g = grids[gi];
-            const int r = build_grid(ctx, m, bufs, G, g.cat, G.survey && g.is_p1, (unsigned *)bufs.cflags.p + gi,
-                                     g.expect_exact);
-            if (r) return r;
-        }
-        return NC_OK;
-    };
-    rc = enqueue_builds();
-    if (rc) return rc;
-    NC_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    GridGeo gg;
-    for (int a = 0; a < 3; ++a) {
-        gg.cs[a] = G.cs[a];
-        gg.nc[a] = G.nc[a];
-    }
-    gg.survey = G.survey;
-    const int cb = (int)((G.ncells + 255) / 256);
-    for (const Term &t : terms) {
-        hipLaunchKernelGGL(k_nbr_bound, dim3(cb), dim3(256), 0, ctx->stream, (const uint32_t *)t.c1->start.p,
-                           (const uint32_t *)t.c2->start.p, gg, (unsigned *)bufs.bound.p);
-        if (t.c3 != t.c2)
-            hipLaunchKernelGGL(k_nbr_bound, dim3(cb), dim3(256), 0, ctx->stream, (const uint32_t *)t.c1->start.p,
-                               (const uint32_t *)t.c3->start.p, gg, (unsigned *)bufs.bound.p);
-    }
-    NC_HIP(ctx, hipGetLastError());
-    unsigned host_flags[2] = {0, 0};
-    std::vector<unsigned> cflags(grids.size());
-    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], bufs.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    NC_HIP(ctx, hipMemcpyAsync(&host_flags[1], bufs.bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    NC_HIP(ctx, hipMemcpyAsync(cflags.data(), bufs.cflags.p, sizeof(unsigned) * cflags.size(), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
-    const int cap = (int)((std::max(1u, host_flags[1]) + 15u) & ~15u);
-    for (size_t gi = 0; gi < grids.size(); ++gi) grids[gi].cat->exact = cflags[gi] == 0u;
-    // Terms over one catalogue (c1 == c2 == c3) count every unordered triple
-    // once when its gridding is exact (threeD only); survey terms with c1 ==
-    // c2 != c3 every unordered (p1, p2) pair.  A launch with weighted terms
-    // runs its unit-weight terms on the plain two-fold path (its kernel
-    // variant carries these paths for weighted terms only).
-    std::vector<char> tri(nterms, kTriNone);
-    int ntri = 0, nswap = 0;
-    for (int i = 0; i < nterms; ++i) {
-        const Term &t = terms[i];
-        const bool own = all_unit || !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
-        const bool one = ctx->tri && !G.rppi && own && t.c1 == t.c2 && t.c1->exact && t.c1->n > 0;
-        tri[i] = !one ? kTriNone : t.c2 == t.c3 ? kTriSix : G.survey ? kTriSwap : kTriNone;
-        ntri += tri[i] == kTriSix;
-        nswap += tri[i] == kTriSwap;
-    }
-    // Catalogues whose lists skip slabs beyond rmax (prune2 / prune3) and the
-    // six-fold primaries rely on exact gridding: a replayed plan checks it.
-    auto prunes = [&](const nc_catalog *c) { return !G.rppi && c->exact; };
-    for (CatGrid &g : grids) {
-        g.expect_exact = false;
-        for (int i = 0; i < nterms; ++i)
-            g.expect_exact = g.expect_exact || (tri[i] && terms[i].c1 == g.cat) ||
-                             (prunes(g.cat) && (terms[i].c2 == g.cat || terms[i].c3 == g.cat));
-    }
-    // slab pruning radius: rmax plus a margin far above the rounding of the
-    // cell and slab bounds (and of a point's slab) at these coordinates
-    double bmax = 0.0;
-    for (int a = 0; a < 3; ++a) bmax = std::max(bmax, std::fabs(G.box[a]));
-    const double prune_r = G.edges[G.nb] * (1.0 + 1e-9) + 1e-9 * bmax;
-    const double prune_r2 = prune_r * prune_r;
-
-    // --- histogram layout: dense u32 counts when every term has unit weights,
-    // they fit and no count can wrap between merges (one chunk adds at most
-    // chunk * cap^2 pairs); else the slot table; else ctab.  The plain path
-    // needs the single-step bin lookup for either of the first two.
-    // The budget is first half the CU's LDS (two workgroups per CU); a layout
-    // that only fits the whole of it runs one workgroup per CU.
-    auto fits = [&](int hm, size_t budget) {
-        return hist_size(hm) + lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3) + W * wave_lds(64, hm) <=
-               budget;
-    };
-    // Longest list the launch may meet: the 27-cell candidate bound, or, when
-    // only the exact lengths can admit the dense layout, the longest list
-    // k_list_bound finds (one extra pass over the primaries).
-    // A primary adds at most n(n-1) to the counts (n: its list length), three
-    // times that on the six-fold path, twice on the pair-swap path.
-    const double kWrap = 2147483648.0;
-    const double mult = ntri > 0 ? 3.0 : nswap > 0 ? 2.0 : 1.0;
-    const bool dense_cand = !G.rppi && (mixed || G.lut_single_step) && all_unit;
-    int nmax = cap;
-    // (the mixed path takes the dense layout over a whole CU, the plain path
-    // only within half of one)
-    const bool need_bound = dense_cand && (double)ctx->chunk * mult * (double)cap * (double)cap >= kWrap &&
-                            fits(kHistDense, mixed ? ctx->lds_hw : ctx->lds_max);
-    if (need_bound && ctx->autotuning && ctx->tune_list_bound.count(uids)) {
-        nmax = ctx->tune_list_bound[uids];
-    } else if (need_bound) {
-        KArgs B;
-        std::memset(&B, 0, sizeof(B));
-        for (int a = 0; a < 3; ++a) {
-            B.box[a] = G.box[a];
-            B.cs[a] = G.cs[a];
-            B.nc[a] = G.nc[a];
-        }
-        B.survey = G.survey;
-        B.r2min = G.sq[0];
-        B.r2max = G.sq[G.nb];
-        B.prune_r2 = prune_r2;
-        B.cap = 0;  // count only
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.bound.p, 0u, 1LL);
-        for (const Term &t : terms) {
-            TermDesc d = term_desc(t);
-            d.prune2 = prunes(t.c2);
-            d.prune3 = prunes(t.c3);
-            if (d.n1 == 0 || t.c2->n == 0 || t.c3->n == 0) continue;
-            // only the primaries of the slice counted (the same split as the work queue)
-            const int p0 = (int)((long long)d.n1 * ctx->slice / ctx->nslices);
-            const int p1 = (int)((long long)d.n1 * (ctx->slice + 1) / ctx->nslices);
-            if (p1 <= p0) continue;
-            const int grid = (int)std::min<long long>((p1 - p0 + 3) / 4, (long long)ctx->ncu * 32);
-            hipLaunchKernelGGL(k_list_bound, dim3(grid), dim3(256), 0, ctx->stream, B, d, p0, p1,
-                               (unsigned *)bufs.bound.p);
-        }
-        NC_HIP(ctx, hipGetLastError());
-        unsigned lb = 0;
-        NC_HIP(ctx, hipMemcpyAsync(&lb, bufs.bound.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-        NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        nmax = std::min(cap, (int)std::max(1u, lb));
-        if (ctx->autotuning) ctx->tune_list_bound[uids] = nmax;
-    }
-    // primaries per work item: the tuned chunk, cut where the dense layout
-    // needs it so that one work item cannot add 2^31 (merges follow work items)
-    int qchunk = ctx->chunk;
-    const double per_primary = mult * (double)nmax * (double)nmax;
-    const bool dense_ok = dense_cand && per_primary < kWrap;
-    if (dense_ok && (double)qchunk * per_primary >= kWrap) qchunk = std::max(1, (int)((kWrap - 1.0) / per_primary));
-    auto choose = [&](size_t budget) {
-        if (G.rppi) return kHistGlobal;
-        if (dense_ok && fits(kHistDense, budget)) return kHistDense;
-        if ((mixed || G.lut_single_step) && !G.t3.empty() && fits(kHistT3, budget)) return kHistT3;
-        return kHistCtab;
-    };
-    size_t budget = ctx->lds_max;
-    int hm = choose(budget);
-    if (exact) {
-        if (G.t3.empty() || !(mixed || G.lut_single_step))
-            return fail(ctx, NC_ERR_LIMIT, "exact accumulation supports at most 35 bins per side");
-        hm = kHistExact;
-        if (!fits(hm, budget)) budget = ctx->lds_hw;
-    } else if (f32acc) {
-        if (G.t3.empty() || !(mixed || G.lut_single_step))
-            return fail(ctx, NC_ERR_LIMIT, "FP32 accumulation supports at most 35 bins per side");
-        // the t3 table where it fits beside the FP32 sums, else ctab
-        hm = fits(kHistF32, budget) || (!fits(kHistF32C, budget) && fits(kHistF32, ctx->lds_hw)) ? kHistF32
-                                                                                                 : kHistF32C;
-        if (!fits(hm, budget)) budget = ctx->lds_hw;
-    } else if (mixed && dense_ok && hm != kHistDense && fits(kHistDense, ctx->lds_hw)) {
-        // mixed path: dense counts over a whole CU beat the sorted-slot
-        // layouts at half of it (25 bins, survey randoms: 4405 against 5428 ms)
-        budget = ctx->lds_hw;
-        hm = kHistDense;
-    } else if (!fits(hm, budget) && ctx->lds_hw > budget) {
-        budget = ctx->lds_hw;
-        hm = choose(budget);
-    }
-    if (stream && !fits(hm, budget)) {  // no room for the key stream: direct atomics
-        // (the autotuner skips such shapes rather than time the other path)
-        if (ctx->autotuning) return fail(ctx, NC_ERR_LIMIT, "no LDS room for the rppi key stream");
-        stream = false;
-        rank_bytes = stage_bytes = 0;
-        budget = fits(hm, ctx->lds_max) ? ctx->lds_max : ctx->lds_hw;
-    }
-    const size_t hist_bytes = hist_size(hm);
-    const size_t lds_tab_bytes = lds_tables(hm, off_sq, off_lut, off_ctab, off_lut32, off_t3);
-
-    // --- launch shape
-    int C = ctx->chunk_elems;
-    const size_t lds_tab = al16(hist_bytes + lds_tab_bytes);
-    // chunks are whole 64-entry blocks (the circulant diagonal blocks need it)
-    C = std::max(64, C - C % 64);
-    while (C > 64 && lds_tab + W * wave_lds(C, hm) > budget) C -= 64;
-    const size_t wave_bytes = wave_lds(C, hm);
-    const size_t lds_bytes = lds_tab + W * wave_bytes;
-    if (lds_bytes > budget) return fail(ctx, NC_ERR_LIMIT, "histogram does not fit in LDS");
-    // [weighted terms present][mixed][layout]
-    const void *kfns[2][2][3] = {
-        {{(const void *)k_triplets<false, kHistCtab, false>, (const void *)k_triplets<false, kHistT3, false>,
-          (const void *)k_triplets<false, kHistDense, false>},
-         {(const void *)k_triplets<true, kHistCtab, false>, (const void *)k_triplets<true, kHistT3, false>,
-          (const void *)k_triplets<true, kHistDense, false>}},
-        {{(const void *)k_triplets<false, kHistCtab, true>, (const void *)k_triplets<false, kHistT3, true>, nullptr},
-         {(const void *)k_triplets<true, kHistCtab, true>, (const void *)k_triplets<true, kHistT3, true>, nullptr}}};
-    const void *kfn = G.rppi ? (G.survey ? (stream ? (const void *)k_rppi<true, true> : (const void *)k_rppi<true, false>)
-                                         : (stream ? (const void *)k_rppi<false, true> : (const void *)k_rppi<false, false>))
-                      : hm == kHistExact ? (mixed ? (const void *)k_triplets<true, kHistExact, true>
-                                                  : (const void *)k_triplets<false, kHistExact, true>)
-                      : hm == kHistF32   ? (mixed ? (const void *)k_triplets<true, kHistF32, true>
-                                                  : (const void *)k_triplets<false, kHistF32, true>)
-                      : hm == kHistF32C  ? (mixed ? (const void *)k_triplets<true, kHistF32C, true>
-                                                  : (const void *)k_triplets<false, kHistF32C, true>)
-                                         : kfns[all_unit ? 0 : 1][mixed ? 1 : 0][hm];
-    if (kfn == nullptr) return fail(ctx, NC_ERR_LIMIT, "no kernel variant for the planned layout");
-    {
-        // some variants are compiled for smaller workgroups (more VGPRs per lane)
-        hipFuncAttributes fa;
-        NC_HIP(ctx, hipFuncGetAttributes(&fa, kfn));
-        if (block > fa.maxThreadsPerBlock)
-            return fail(ctx, NC_ERR_LIMIT, "waves_per_group " + std::to_string(W) + " exceeds the " +
-                                               std::to_string(fa.maxThreadsPerBlock / kWave) +
-                                               " waves this kernel variant is built for");
-    }
-    // --- terms and work queue
-    std::vector<TermDesc> td(nterms);
-    std::vector<Chunk> chunks;
-    std::vector<int> weighted(nterms);
-    long long nitems = 0;
-    for (int i = 0; i < nterms; ++i) {
-        const Term &t = terms[i];
-        TermDesc &d = td[i];
-        d = term_desc(t);
-        weighted[i] = d.weighted;
-        if (t.c2->n == 0 || t.c3->n == 0) continue;
-        d.tri = tri[i];
-        d.prune2 = prunes(t.c2);
-        d.prune3 = prunes(t.c3);
-        // the primary slice: positions [p0, p1) of the cell-sorted catalogue
-        const int p0 = (int)((long long)d.n1 * ctx->slice / ctx->nslices);
-        const int p1 = (int)((long long)d.n1 * (ctx->slice + 1) / ctx->nslices);
-        for (int b = p0; b < p1; b += qchunk, ++nitems)
-            if ((nitems / kSampleRun) % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(p1, b + qchunk), 0});
-    }
-    NC_MEM(ctx, m, bufs.terms, sizeof(TermDesc) * td.size());
-    NC_MEM(ctx, m, bufs.chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size()));
-    const long long ostride = exact ? 2 * G.out_words : G.out_words;  // u64 words of out per term
-    NC_MEM(ctx, m, bufs.out, sizeof(unsigned long long) * (size_t)nterms * ostride);
-    if (!td.empty())
-        NC_HIP(ctx, hipMemcpyAsync(bufs.terms.p, td.data(), sizeof(TermDesc) * td.size(), hipMemcpyHostToDevice,
-                                   ctx->stream));
-    if (!chunks.empty())
-        NC_HIP(ctx, hipMemcpyAsync(bufs.chunks.p, chunks.data(), sizeof(Chunk) * chunks.size(), hipMemcpyHostToDevice,
-                                   ctx->stream));
-
-    // --- persistent grid and the per-wave list scratch (on an arena: as many
-    // workgroups as the rest of it holds lists for)
-    int per_cu = ctx->wgs_per_cu;
-    if (per_cu <= 0) {
-        int occ = 0;
-        NC_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, block, lds_bytes));
-        per_cu = std::max(1, occ);
-    }
-    int grid = ctx->ncu * per_cu;
-    const long long slot_bytes = (long long)((2 * (size_t)cap * list_entry_bytes(lmode) + 255) & ~(size_t)255);
-    if (m.arena()) {
-        const long long fit = (long long)(m.left() / (size_t)slot_bytes) / W;
-        if (fit < 1)
-            return fail(ctx, NC_ERR_LIMIT, "the neighbour lists of one workgroup do not fit in the no-alloc scratch (" +
-                                               std::to_string(m.size >> 20) + " MiB)");
-        grid = (int)std::min<long long>(grid, fit);
-    }
-    const long long nwaves = (long long)grid * W;
-    if ((double)nwaves * (double)slot_bytes > 64.0 * (1ULL << 30))
-        return fail(ctx, NC_ERR_LIMIT, "neighbour lists need more than 64 GiB of scratch");
-    NC_MEM(ctx, m, bufs.scratch, (size_t)(nwaves * slot_bytes));
-
-    KArgs A;
-    std::memset(&A, 0, sizeof(A));
-    A.terms = (const TermDesc *)bufs.terms.p;
-    A.chunks = (const Chunk *)bufs.chunks.p;
-    A.nchunks = (int)chunks.size();
-    A.counter = (unsigned *)bufs.counter.p;
-    // Queue segments, one per XCD (workgroup b runs on XCD b % 8): each XCD
-    // works through its own run of cell-ordered chunks, so the neighbourhoods
-    // its primaries scan stay in its L2.  The chunk list is cut at equal
-    // estimated work (primaries x |c2| x |c3|); drained segments are helped
-    // with by the other XCDs.
-    std::vector<int> seg(kMaxSeg + 1, 0);  // host copy, kept (like chunks) until the copy has run
-    {
-        const int nseg = std::max(1, std::min(kMaxSeg, grid));
-        std::vector<double> cw(chunks.size());
-        double wtot = 0.0;
-        for (size_t c = 0; c < chunks.size(); ++c) {
-            const Term &t = terms[chunks[c].term];
-            cw[c] = (double)(chunks[c].end - chunks[c].begin) * (double)t.c2->n * (double)t.c3->n;
-            wtot += cw[c];
-        }
-        int sgi = 1;
-        double acc = 0.0;
-        for (size_t c = 0; c < chunks.size() && sgi < nseg; ++c) {
-            acc += cw[c];
-            while (sgi < nseg && acc >= wtot * sgi / nseg) seg[sgi++] = (int)c + 1;
-        }
-        while (sgi <= nseg) seg[sgi++] = (int)chunks.size();
-        A.nseg = nseg;
-        A.seg = (const int *)(A.counter + 16);
-        NC_HIP(ctx, hipMemcpyAsync((void *)A.seg, seg.data(), sizeof(int) * (size_t)(nseg + 1), hipMemcpyHostToDevice,
-                                   ctx->stream));
-    }
-    A.err = (unsigned *)bufs.errw.p;
-    A.out = (unsigned long long *)bufs.out.p;
-    A.scratch = (char *)bufs.scratch.p;
-    A.slot_bytes = slot_bytes;
-    A.cap = cap;
-    A.nmax = hm == kHistDense ? nmax : cap;
-    A.chunk_elems = C;
-    for (int a = 0; a < 3; ++a) {
-        A.box[a] = G.box[a];
-        A.cs[a] = G.cs[a];
-        A.nc[a] = G.nc[a];
-    }
-    A.survey = G.survey;
-    A.r2min = G.sq[0];
-    A.r2max = G.sq[G.nb];
-    A.prune_r2 = prune_r2;
-    A.nb = G.nb;
-    A.nbc = (int)G.out_words;
-    A.npi = G.npi;
-    A.pimax2 = (double)G.npi * (double)G.npi;
-    A.smax2 = G.sq[G.nb] + A.pimax2;  // rpMax*rpMax + piMax*piMax, cudaNcorr.cu:186
-    A.ori_x = G.ori_x;
-    A.lut_scale = G.lut_scale;
-    A.lut_m = G.lut_m;
-    A.sq = (const double *)((char *)bufs.tables.p + g_sq);
-    A.lut = (const uint16_t *)((char *)bufs.tables.p + g_lut);
-    A.ctab = (const int *)((char *)bufs.tables.p + g_ctab);
-    A.lut32 = (const uint2 *)((char *)bufs.tables.p + g_lut32);
-    A.t3 = (const uint16_t *)((char *)bufs.tables.p + g_t3);
-    A.sigma32 = G.sigma32;
-    A.lut32_lo = G.lut32_lo;
-    A.lut32_shift = G.lut32_shift;
-    A.lut32_c0 = G.lut32_c0;
-    A.lut32_n = (int)G.lut32.size();
-    A.g32 = G.g32;
-    A.tri_t = G.tri_t;
-    A.nslots = hm == kHistDense    ? (int)((nb3 + 1) / 2)
-               : hm == kHistGlobal ? 0
-               : hm == kHistExact  ? 2 * G.nbc
-               : is_f32(hm)        ? (int)(hist_size(hm) / 8)
-                                   : G.nbc;
-    A.out_stride = (int)ostride;
-    A.xscale = exact ? std::ldexp(1.0, xk) : 0.0;
-    // FP32 accumulation: merged at the first work-item boundary after 2^22
-    // pairs, which bounds the FP64 rounding of the compensations (see the
-    // header) far below the FP32 products' own 2^-46
-    A.flush_pairs = is_f32(hm) ? std::min<unsigned long long>(ctx->dense_flush, 1ull << 22) : ctx->dense_flush;
-    A.off_sq = (int)hist_bytes;
-    A.off_lut = (int)(hist_bytes + off_lut);
-    A.off_ctab = (int)(hist_bytes + off_ctab);
-    A.off_lut32 = (int)(hist_bytes + off_lut32);
-    A.off_t3 = (int)(hist_bytes + off_t3);
-    A.off_wave = (int)lds_tab;
-    A.wave_bytes = (int)wave_bytes;
-    A.lds_bytes = (int)lds_bytes;
-    NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-
-    // --- rppi key stream: tables, rounds, bucket regions, binning work items.
-    // A dry run over every S-th work item counts the keys per bucket and per
-    // work item (kept for the same catalogues and params).  The work queue is
-    // cut into rounds of consecutive work items whose keys fit 60% of the
-    // free device memory; a round's regions get each bucket's keys in
-    // proportion to its term's share of the round, plus the runs' slack.
-    // Keys beyond a region's capacity still count, through direct atomics.
-    struct Round {
-        int first, n;          // work items [first, first + n)
-        int item0, nitems;     // binning work items
-    };
-    std::vector<Round> rounds;
-    std::vector<unsigned long long> caps, rcaps;  // wanted keys per bucket; allocated per (round, bucket)
-    std::vector<double> cest;                     // estimated keys per work item
-    std::vector<double> rs_ckey;
-    if (stream && A.nchunks > 0) {
-        A.off_rank = (int)(hist_bytes + lds_tab_bytes - rank_bytes);
-        A.off_stage = (int)(wave_bytes - stage_bytes);
-        A.rs_lds_off = (int)stream_lds_offset(rs_anyw);
-        A.rs_nbm = rs_nbm;
-        const size_t tb_rank = al16(sizeof(int4) * (size_t)nterms);
-        const size_t tb_inv = al16(tb_rank + sizeof(uint16_t) * rs_rank.size());
-        const size_t tb_bytes = al16(tb_inv + sizeof(int) * rs_inv.size());
-        std::vector<char> tb(tb_bytes);
-        std::memcpy(tb.data(), rs_term.data(), sizeof(int4) * (size_t)nterms);
-        std::memcpy(tb.data() + tb_rank, rs_rank.data(), sizeof(uint16_t) * rs_rank.size());
-        std::memcpy(tb.data() + tb_inv, rs_inv.data(), sizeof(int) * rs_inv.size());
-        NC_HIP(ctx, dgrow(ctx->rs_tab, tb_bytes));
-        NC_HIP(ctx, hipMemcpyAsync(ctx->rs_tab.p, tb.data(), tb_bytes, hipMemcpyHostToDevice, ctx->stream));
-        A.rs_term = (const int4 *)ctx->rs_tab.p;
-        A.rs_rank = (const uint16_t *)((char *)ctx->rs_tab.p + tb_rank);
-        A.rs_inv = (const int *)((char *)ctx->rs_tab.p + tb_inv);
-        const size_t nch = chunks.size();
-        rs_ckey = pkey;
-        for (uint64_t u : uids) rs_ckey.push_back((double)u);
-        for (int u : wkey) rs_ckey.push_back((double)u);
-        rs_ckey.push_back((double)ctx->chunk);  // the work items the estimates refer to
-        if (ctx->sample_stride == 1 && ctx->rs_key == rs_ckey && ctx->rs_caps.size() == (size_t)rs_nbk &&
-            ctx->rs_cest.size() == nch) {
-            caps = ctx->rs_caps;
-            cest = ctx->rs_cest;
-        } else {
-            const size_t S = std::max<size_t>(1, std::min<size_t>(32, nch / 256));
-            std::vector<Chunk> sample;
-            for (size_t i = 0; i < nch; i += S) sample.push_back(chunks[i]);
-            NC_HIP(ctx, dgrow(ctx->rs_sample, sizeof(Chunk) * sample.size()));
-            NC_HIP(ctx, dgrow(ctx->rs_cursor, sizeof(unsigned long long) * (size_t)rs_nbk));
-            NC_HIP(ctx, dgrow(ctx->rs_chunk_keys, sizeof(unsigned long long) * sample.size()));
-            NC_HIP(ctx, hipMemcpyAsync(ctx->rs_sample.p, sample.data(), sizeof(Chunk) * sample.size(),
-                                       hipMemcpyHostToDevice, ctx->stream));
-            KArgs As = A;
-            As.chunks = (const Chunk *)ctx->rs_sample.p;
-            As.nchunks = (int)sample.size();
-            As.rs_dry = 1;
-            As.rs_cursor = (unsigned long long *)ctx->rs_cursor.p;
-            As.rs_chunk_keys = (unsigned long long *)ctx->rs_chunk_keys.p;
-            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 16LL);
-            hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, As.rs_cursor, 0ull, (long long)rs_nbk);
-            hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, As.rs_chunk_keys, 0ull,
-                               (long long)sample.size());
-            void *args[] = {&As};
-            NC_HIP(ctx, hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream));
-            std::vector<unsigned long long> n(rs_nbk), ck(sample.size());
-            NC_HIP(ctx, hipMemcpyAsync(n.data(), As.rs_cursor, sizeof(unsigned long long) * n.size(),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-            NC_HIP(ctx, hipMemcpyAsync(ck.data(), As.rs_chunk_keys, sizeof(unsigned long long) * ck.size(),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-            NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            caps.resize(rs_nbk);
-            for (int b = 0; b < rs_nbk; ++b)
-                caps[b] = S == 1 ? n[b] + 64 : (unsigned long long)((double)n[b] * (double)S * 1.3) + 4096;
-            cest.resize(nch);
-            for (size_t i = 0; i < nch; ++i) cest[i] = (double)ck[i / S] * (S == 1 ? 1.0 : 1.3);
-        }
-        // memory: 60% of the free device memory, counting the context's own
-        // region buffers; the runs' slack (every bucket, every wave) first
-        size_t freeb = 0, totalb = 0;
-        NC_HIP(ctx, hipMemGetInfo(&freeb, &totalb));
-        const double per = rs_anyw ? 12.0 : 4.0;
-        const double room = 0.6 * ((double)freeb + (double)ctx->rs_keys.bytes + (double)ctx->rs_vals.bytes) / per;
-        double need = 0.0;
-        for (unsigned long long c : caps) need += (double)c;
-        int blk = 64;  // a wave reserves blk slots of a region at a time
-        while (blk < 4096 && 2.0 * blk <= need / std::max(1, rs_nbk) / (4.0 * (double)nwaves)) blk *= 2;
-        while (blk > 64 && (double)rs_nbk * (double)nwaves * blk > 0.25 * room) blk /= 2;
-        A.rs_blk = blk;
-        const unsigned long long slack = (unsigned long long)nwaves * (unsigned long long)blk;
-        double K = std::max(0.0, room - (double)rs_nbk * (double)slack);  // keys per round
-        double total_est = 0.0;
-        for (double e : cest) total_est += e;
-        K = std::max({K, 1e7, total_est / 256.0});  // at most ~256 rounds (past the room: allocation fails)
-        if (ctx->rs_round_keys > 0.0) K = std::max(ctx->rs_round_keys, total_est / 256.0);
-        // per term: estimated keys, and work items (for terms the sample saw no key of)
-        std::vector<double> test(nterms, 0.0), tcnt(nterms, 0.0);
-        for (size_t i = 0; i < nch; ++i) {
-            test[chunks[i].term] += cest[i];
-            tcnt[chunks[i].term] += 1.0;
-        }
-        for (size_t i = 0; i < nch;) {
-            double acc = 0.0;
-            size_t j = i;
-            while (j < nch && (j == i || acc + cest[j] <= K)) acc += cest[j++];
-            rounds.push_back({(int)i, (int)(j - i), 0, 0});
-            i = j;
-        }
-        const int R = (int)rounds.size();
-        rcaps.assign((size_t)R * rs_nbk, 0ull);
-        std::vector<unsigned long long> base((size_t)R * (rs_nbk + 1), 0ull);
-        unsigned long long kmax = 0;
-        std::vector<RBItem> items;
-        for (int r = 0; r < R; ++r) {
-            std::vector<double> fe(nterms, 0.0), fc(nterms, 0.0);
-            for (int i = rounds[r].first; i < rounds[r].first + rounds[r].n; ++i) {
-                fe[chunks[i].term] += cest[i];
-                fc[chunks[i].term] += 1.0;
-            }
-            double want = 0.0;
-            std::vector<double> w(rs_nbk, 0.0);
-            std::vector<char> live(rs_nbk, 0);  // buckets of the terms with work in this round
-            for (int t = 0; t < nterms; ++t) {
-                if (fc[t] == 0.0) continue;
-                const double f = test[t] > 0.0 ? fe[t] / test[t] : fc[t] / tcnt[t];
-                for (int bb = 0; bb < rs_term[t].y; ++bb) {
-                    w[rs_term[t].x + bb] = (double)caps[rs_term[t].x + bb] * f * ctx->rs_scale;
-                    live[rs_term[t].x + bb] = 1;
-                    want += w[rs_term[t].x + bb];
-                }
-            }
-            const double sc = want > K ? K / want : 1.0;  // one work item past the room: overflow
-            unsigned long long *bs = base.data() + (size_t)r * (rs_nbk + 1);
-            for (int b = 0; b < rs_nbk; ++b) {
-                const unsigned long long c =
-                    live[b] ? (((unsigned long long)(w[b] * sc) + slack + 63) & ~63ull) : 0ull;
-                rcaps[(size_t)r * rs_nbk + b] = c;
-                bs[b + 1] = bs[b] + c;
-            }
-            kmax = std::max(kmax, bs[rs_nbk]);
-            // binning work items: slices in proportion to the region sizes
-            // (about four per CU in all), each under 2^30 keys
-            rounds[r].item0 = (int)items.size();
-            const double total = std::max(1.0, (double)bs[rs_nbk]);
-            for (int t = 0; t < nterms; ++t) {
-                if (fc[t] == 0.0) continue;
-                const int4 tr = rs_term[t];
-                for (int bb = 0; bb < tr.y; ++bb) {
-                    const int b = tr.x + bb;
-                    const unsigned long long cb = bs[b + 1] - bs[b];
-                    if (cb == 0) continue;
-                    long long sl = std::llround(4.0 * ctx->ncu * (double)cb / total);
-                    sl = std::max<long long>({1LL, sl, (long long)((cb + (1ull << 30) - 1) >> 30)});
-                    for (long long j = 0; j < sl; ++j)
-                        items.push_back({b, t, tr.z, weighted[t] ? 1 : 0, bb << tr.z, (int)j, (int)sl, 0});
-                }
-            }
-            rounds[r].nitems = (int)items.size() - rounds[r].item0;
-        }
-        NC_HIP(ctx, dgrow(ctx->rs_base, sizeof(unsigned long long) * base.size()));
-        NC_HIP(ctx, dgrow(ctx->rs_cursor, sizeof(unsigned long long) * (size_t)R * rs_nbk));
-        NC_HIP(ctx, hipMemcpyAsync(ctx->rs_base.p, base.data(), sizeof(unsigned long long) * base.size(),
-                                   hipMemcpyHostToDevice, ctx->stream));
-        const size_t kbytes = sizeof(unsigned) * (size_t)kmax;
-        const size_t vbytes = rs_anyw ? sizeof(double) * (size_t)kmax : 0;
-        if (kbytes > ctx->rs_keys.bytes || vbytes > ctx->rs_vals.bytes || vbytes == 0) {
-            // release both before growing either: the room above counted them
-            NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            if (kbytes > ctx->rs_keys.bytes) dfree(ctx->rs_keys);
-            if (vbytes > ctx->rs_vals.bytes || vbytes == 0) dfree(ctx->rs_vals);
-        }
-        NC_HIP(ctx, dgrow(ctx->rs_keys, kbytes));
-        if (rs_anyw) NC_HIP(ctx, dgrow(ctx->rs_vals, vbytes));
-        A.rs_keys = (unsigned *)ctx->rs_keys.p;
-        A.rs_vals = rs_anyw ? (double *)ctx->rs_vals.p : nullptr;
-        NC_HIP(ctx, dgrow(ctx->rs_items, sizeof(RBItem) * std::max<size_t>(1, items.size())));
-        if (!items.empty())
-            NC_HIP(ctx, hipMemcpyAsync(ctx->rs_items.p, items.data(), sizeof(RBItem) * items.size(),
-                                       hipMemcpyHostToDevice, ctx->stream));
-    }
-    // the launch arguments of round r
-    auto round_args = [&](int r) {
-        KArgs Ar = A;
-        const Round &rd = rounds[r];
-        Ar.chunks = (const Chunk *)bufs.chunks.p + rd.first;
-        Ar.nchunks = rd.n;
-        Ar.rs_base = (const unsigned long long *)ctx->rs_base.p + (size_t)r * (rs_nbk + 1);
-        Ar.rs_cursor = (unsigned long long *)ctx->rs_cursor.p + (size_t)r * rs_nbk;
-        Ar.rs_items = (const RBItem *)ctx->rs_items.p + rd.item0;
-        Ar.rs_nitems = rd.nitems;
-        return Ar;
-    };
-    const size_t bin_lds = (size_t)1 << rs_maxshift;
-
-    auto enqueue_count = [&]() -> int {
-        hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, ctx->stream, (unsigned long long *)bufs.out.p, 0ull,
-                           (long long)nterms * ostride);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u, 16LL);
-        if (A.nchunks > 0 && !stream) {
-            void *args[] = {&A};
-            if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
-                return NC_ERR_HIP;
-        }
-        if (A.nchunks > 0 && stream) {
-            hipLaunchKernelGGL(k_fill_u64, dim3(8), dim3(256), 0, ctx->stream, (unsigned long long *)ctx->rs_cursor.p,
-                               0ull, (long long)rounds.size() * rs_nbk);
-            for (int r = 0; r < (int)rounds.size(); ++r) {
-                KArgs Ar = round_args(r);
-                void *args[] = {&Ar};
-                if (r > 0)
-                    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, (uint32_t *)bufs.counter.p, 0u,
-                                       16LL);
-                if (hipLaunchKernel(kfn, dim3(grid), dim3(block), args, lds_bytes, ctx->stream) != hipSuccess)
-                    return NC_ERR_HIP;
-                if (Ar.rs_nitems > 0 && hipLaunchKernel((const void *)k_rppi_bin, dim3(Ar.rs_nitems), dim3(1024),
-                                                        args, bin_lds, ctx->stream) != hipSuccess)
-                    return NC_ERR_HIP;
-            }
-        }
-        return hipGetLastError() == hipSuccess ? NC_OK : NC_ERR_HIP;
-    };
-    if (enqueue_count() != NC_OK) return fail(ctx, NC_ERR_HIP, "triplet kernel launch failed");
-    NC_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-    std::vector<unsigned long long> res((size_t)nterms * ostride);
-    NC_HIP(ctx, hipMemcpyAsync(res.data(), bufs.out.p, sizeof(unsigned long long) * res.size(), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    NC_HIP(ctx, hipMemcpyAsync(&host_flags[0], bufs.errw.p, 4, hipMemcpyDeviceToHost, ctx->stream));
-    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    float ms_b = 0.f, ms_c = 0.f;
-    NC_HIP(ctx, hipEventElapsedTime(&ms_b, ctx->ev[0], ctx->ev[1]));
-    NC_HIP(ctx, hipEventElapsedTime(&ms_c, ctx->ev[2], ctx->ev[3]));
-    ctx->timing.ms_build = ms_b;
-    ctx->timing.ms_count = ms_c;
-    ctx->timing.ms_total = ms_b + ms_c;
-    const int nrounds = stream && A.nchunks > 0 ? (int)rounds.size() : 0;
-    ctx->timing.n_kernel = A.nchunks > 0 ? (stream ? 2 * nrounds : 1) : 0;
-    ctx->timing.hist_layout = stream ? kHistStream : hm;
-    ctx->timing.lds_bytes = (int32_t)lds_bytes;
-    ctx->timing.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
-    ctx->timing.tri_terms = ntri;
-    ctx->timing.swap_terms = nswap;
-    // key stream: keep the region sizes; after an overflow (counted, but
-    // through direct atomics) scale the buckets that overflowed by what this
-    // run reserved, and capture no graph so that the next call uses that
-    bool rs_overflow = false;
-    if (nrounds > 0) {
-        std::vector<unsigned long long> n((size_t)nrounds * rs_nbk);
-        NC_HIP(ctx, hipMemcpy(n.data(), ctx->rs_cursor.p, sizeof(unsigned long long) * n.size(),
-                              hipMemcpyDeviceToHost));
-        for (int b = 0; b < rs_nbk; ++b) {
-            double grow = 1.0;
-            for (int r = 0; r < nrounds; ++r) {  // n: slots reserved, at least the keys
-                const unsigned long long got = n[(size_t)r * rs_nbk + b], had = rcaps[(size_t)r * rs_nbk + b];
-                if (got > had) grow = std::max(grow, (double)got / (double)std::max(had, 1ull));
-            }
-            if (grow > 1.0) {
-                rs_overflow = true;
-                caps[b] = (unsigned long long)((double)caps[b] * grow * 1.125) + 64;
-            }
-        }
-        if (ctx->sample_stride == 1) {  // not from an autotuning sample
-            ctx->rs_key = rs_ckey;
-            ctx->rs_caps = caps;
-            ctx->rs_cest = cest;
-        }
-    }
-    if (host_flags[0] & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
-    if (host_flags[0] & kErrOverflow) return fail(ctx, NC_ERR_HIP, "internal error: neighbour list overflow");
-    if (host_flags[0] & kErrMode) return fail(ctx, NC_ERR_HIP, "internal error: a term does not match the weighting of its kernel variant");
-
-    convert_out(G.nb, G.out_words, G.rppi, nterms, weighted, res.data(), hist, counts, xk);
-
-    // --- capture the launch sequence for replay (builds, then count); any
-    // failure leaves the eager result valid and no plan cached
-    if (use_graph && !rs_overflow) {
-        BatchPlan p;
-        p.uids = uids;
-        p.wkey = wkey;
-        p.pkey = pkey;
-        p.tkey = tkey;
-        p.weighted = weighted;
-        p.nb = G.nb;
-        p.nbc = G.out_words;
-        p.xk = xk;
-        p.rppi = G.rppi;
-        p.nkernel = A.nchunks > 0 ? (stream ? 2 * nrounds : 1) : 0;
-        p.hist_layout = stream ? kHistStream : hm;
-        p.lds_bytes = (int)lds_bytes;
-        p.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
-        p.tri_terms = ntri;
-        p.swap_terms = nswap;
-        bool ok = hipHostMalloc((void **)&p.h_out, sizeof(unsigned long long) * std::max<size_t>(1, res.size())) ==
-                      hipSuccess &&
-                  hipHostMalloc((void **)&p.h_err, 16) == hipSuccess;
-        for (int part = 0; part < 2 && ok; ++part) {
-            hipGraph_t g = nullptr;
-            ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-            if (!ok) break;
-            const int erc = part == 0 ? enqueue_builds() : enqueue_count();
-            ok = hipStreamEndCapture(ctx->stream, &g) == hipSuccess && erc == NC_OK && g != nullptr;
-            if (ok) ok = hipGraphInstantiate(part == 0 ? &p.build_exec : &p.count_exec, g, nullptr, nullptr, 0) ==
-                         hipSuccess;
-            if (g) (void)hipGraphDestroy(g);
-        }
-        (void)hipGetLastError();
-        p.valid = ok;
-        ctx->plan = p;
-        if (!ok) plan_drop(ctx);
-    }
-    return NC_OK;
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// C-ABI
-
-// Device address of noalloc_scratch on `device` (current): looked up once
-// per device with hipGetSymbolAddress and kept; each GPU holds its own copy.
-static std::mutex g_noalloc_mu;
-static std::vector<char *> g_noalloc_ptr;
-static hipError_t noalloc_base(int device, char **out) {
-    std::lock_guard<std::mutex> lock(g_noalloc_mu);
-    if ((size_t)device >= g_noalloc_ptr.size()) g_noalloc_ptr.resize((size_t)device + 1, nullptr);
-    if (!g_noalloc_ptr[(size_t)device]) {
-        void *p = nullptr;
-        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
-        if (e != hipSuccess) return e;
-        g_noalloc_ptr[(size_t)device] = static_cast<char *>(p);
-    }
-    *out = g_noalloc_ptr[(size_t)device];
-    return hipSuccess;
-}
-
-// The points of a catalogue that wraps caller-owned device memory
-// (nc_catalog_wrap_device) may be overwritten between counts, weights
-// included.  Whether all weights are 1 decides the histogram layout, the
-// batch split and whether integer counts are returned, so it is derived again
-// from the current contents before every count over such catalogues (one
-// read of their weights and one synchronisation).  A captured plan keys on
-// these flags, so a changed weight status is never replayed.
-int refresh_weights(nc_context *ctx, Mem &m, const std::vector<Term> &terms) {
-    std::vector<nc_catalog *> w;
-    for (const Term &t : terms)
-        for (nc_catalog *c : {t.c1, t.c2, t.c3})
-            if (c && !c->owns && c->n > 0 && std::find(w.begin(), w.end(), c) == w.end()) w.push_back(c);
-    if (w.empty()) return NC_OK;
-    NC_HIP(ctx, hipSetDevice(ctx->device));
-    DBuf arena_flags;
-    DBuf &fl = m.arena() ? arena_flags : ctx->buf.wflags;
-    NC_MEM(ctx, m, fl, 4 * sizeof(unsigned) * w.size());
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, (uint32_t *)fl.p, 0u,
-                       4LL * (long long)w.size());
-    for (size_t i = 0; i < w.size(); ++i) {
-        const int grid = (int)std::min<long long>((w[i]->n + 255) / 256, 1024);
-        hipLaunchKernelGGL(k_weight_stats, dim3(grid), dim3(256), 0, ctx->stream, (const double4 *)w[i]->d_raw,
-                           (long long)w[i]->n, (unsigned *)fl.p + 4 * i);
-    }
-    NC_HIP(ctx, hipGetLastError());
-    std::vector<unsigned> f(4 * w.size());
-    NC_HIP(ctx, hipMemcpyAsync(f.data(), fl.p, sizeof(unsigned) * f.size(), hipMemcpyDeviceToHost,
-                               ctx->stream));
-    NC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (size_t i = 0; i < w.size(); ++i) {
-        w[i]->unit_w = f[4 * i] == 0u;
-        w[i]->wfinite = (f[4 * i + 1] & 1u) == 0u;
-        w[i]->wf32 = (f[4 * i + 1] & 2u) == 0u;
-        const unsigned long long b = (unsigned long long)f[4 * i + 2] | ((unsigned long long)f[4 * i + 3] << 32);
-        std::memcpy(&w[i]->wmax, &b, 8);
-    }
-    return NC_OK;
-}
-
-// A count over terms, as nc_count_batched runs it.  Unit-weight terms beside
-// weighted ones (the survey's RRR beside DDD, DDR, DRR) may take the dense u32
-// layout, which the weighted ones cannot, and the six-fold path, which a
-// launch with weighted terms keeps for those: they run as a launch of their
-// own whenever a dense histogram could fit in a CU's LDS, on either path (so
-// both paths give every primary the same share of the triplets, and the
-// primary slices of the two paths agree).  Those two launches are not
-// captured in a graph.
-int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
-                uint64_t *counts, bool use_graph) {
-    const int nterms = (int)t.size();
-    Mem m;  // the context's own buffers
-    m.fail = &ctx->fail_alloc;
-    if (const int rc = refresh_weights(ctx, m, t)) return rc;
-    if (ctx->batch_split && params && params->bin_type == NC_BIN_THREED &&
-        params->nedges >= 2 && params->nedges <= 41) {
-        std::vector<int> iu, iw;
-        for (int i = 0; i < nterms; ++i) {
-            if (!t[i].c1 || !t[i].c2 || !t[i].c3) return fail(ctx, NC_ERR_ARG, "null catalogue");
-            (t[i].c1->unit_w && t[i].c2->unit_w && t[i].c3->unit_w ? iu : iw).push_back(i);
-        }
-        const size_t nb = (size_t)params->nedges - 1;
-        if (!iu.empty() && !iw.empty() && nb * nb * nb * 4 + 48 * 1024 <= ctx->lds_hw) {
-            if (!hist) return fail(ctx, NC_ERR_ARG, "null params or output");
-            const size_t len = (size_t)nc_hist_len(params);
-            std::vector<double> h[2];
-            std::vector<uint64_t> c[2];
-            nc_timing tm[2];
-            for (int g = 0; g < 2; ++g) {
-                const std::vector<int> &idx = g ? iw : iu;
-                std::vector<Term> sub;
-                for (int i : idx) sub.push_back(t[i]);
-                h[g].resize(idx.size() * len);
-                if (counts) c[g].resize(idx.size() * len);
-                const int rc =
-                    run_count(ctx, params, sub, path, h[g].data(), counts ? c[g].data() : nullptr, false, m);
-                if (rc) return rc;
-                tm[g] = ctx->timing;
-                for (size_t k = 0; k < idx.size(); ++k) {
-                    std::memcpy(hist + (size_t)idx[k] * len, h[g].data() + k * len, sizeof(double) * len);
-                    if (counts)
-                        std::memcpy(counts + (size_t)idx[k] * len, c[g].data() + k * len, sizeof(uint64_t) * len);
-                }
-            }
-            // the sum of both launches; layout and LDS of the unit-weight one
-            ctx->timing.ms_build = tm[0].ms_build + tm[1].ms_build;
-            ctx->timing.ms_count = tm[0].ms_count + tm[1].ms_count;
-            ctx->timing.ms_total = tm[0].ms_total + tm[1].ms_total;
-            ctx->timing.n_kernel = tm[0].n_kernel + tm[1].n_kernel;
-            ctx->timing.hist_layout = tm[0].hist_layout;
-            ctx->timing.lds_bytes = tm[0].lds_bytes;
-            ctx->timing.path = tm[0].path;
-            ctx->timing.tri_terms = tm[0].tri_terms + tm[1].tri_terms;
-            ctx->timing.swap_terms = tm[0].swap_terms + tm[1].swap_terms;
-            return NC_OK;
-        }
-    }
-    return run_count(ctx, params, t, path, hist, counts, use_graph, m);
-}
-
-extern "C" {
-
-int nc_create(nc_context **out, int device) {
-    if (!out) return NC_ERR_ARG;
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return NC_ERR_HIP;
-    nc_context *ctx = new (std::nothrow) nc_context();
-    if (!ctx) return NC_ERR_NOMEM;
-    ctx->device = device;
-    hipDeviceProp_t prop;
-    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
-        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete ctx;
-        return NC_ERR_HIP;
-    }
-    {
-        const hipError_t e = noalloc_base(device, &ctx->noalloc);
-        if (e != hipSuccess) ctx->noalloc_err = std::string("hipGetSymbolAddress(noalloc_scratch): ") + hipGetErrorString(e);
-    }
-    ctx->ncu = prop.multiProcessorCount;
-    ctx->dev_name = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
-    ctx->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 81920) - 256;
-    ctx->lds_hw = prop.sharedMemPerBlock - 256;
-    if (const char *e = std::getenv("NC_RPPI_STREAM")) ctx->rppi_stream = std::strcmp(e, "0") != 0 ? 1 : 0;
-    if (const char *e = std::getenv("NC_RPPI_ROUND_KEYS")) {
-        const double v = std::strtod(e, nullptr);
-        if (v >= 1.0) ctx->rs_round_keys = v;
-    }
-    if (const char *e = std::getenv("NC_RPPI_REGION_SCALE")) {
-        const double v = std::strtod(e, nullptr);
-        if (v > 0.0 && v <= 1.0) ctx->rs_scale = v;
-    }
-    if (const char *e = std::getenv("NC_BATCH_SPLIT")) ctx->batch_split = std::strcmp(e, "0") != 0;
-    if (const char *e = std::getenv("NC_TRI")) ctx->tri = std::strcmp(e, "0") != 0;
-    if (const char *e = std::getenv("NC_FAIL_DEVICE_ALLOC")) ctx->fail_alloc = std::strtoll(e, nullptr, 10);
-    if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
-        const unsigned long long v = std::strtoull(e, nullptr, 10);
-        if (v > 0 && v <= (1ull << 31)) ctx->dense_flush = v;
-    }
-    for (auto &e : ctx->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            nc_destroy(ctx);
-            return NC_ERR_HIP;
-        }
-    *out = ctx;
-    return NC_OK;
-}
-
-int nc_destroy(nc_context *ctx) {
-    if (!ctx) return NC_ERR_ARG;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    plan_drop(ctx);
-    for (DBuf *b : {&ctx->buf.tables, &ctx->buf.terms, &ctx->buf.chunks, &ctx->buf.out, &ctx->buf.counter,
-                    &ctx->buf.errw, &ctx->buf.scratch, &ctx->buf.bound, &ctx->buf.wflags, &ctx->buf.cflags, &ctx->rs_tab, &ctx->rs_cursor, &ctx->rs_base, &ctx->rs_items, &ctx->rs_sample,
-                    &ctx->rs_chunk_keys, &ctx->rs_keys, &ctx->rs_vals})
-        dfree(*b);
-    for (auto &e : ctx->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    for (nc_catalog *c : ctx->cats) c->ctx = nullptr;
-    delete ctx;
-    return NC_OK;
-}
-
-const char *nc_last_error(const nc_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
-
-int64_t nc_hist_size(int nedges) { return nedges < 2 ? 0 : (int64_t)(nedges - 1) * (nedges - 1) * (nedges - 1); }
-
-int64_t nc_compact_size(int nedges) { return nedges < 2 ? 0 : choose_compact(nedges - 1); }
-
-int64_t nc_hist_len(const nc_params *p) {
-    if (!p || p->nedges < 2) return 0;
-    const int64_t nb = p->nedges - 1;
-    if (p->bin_type == NC_BIN_RPPI) return p->num_pi_bins < 1 ? 0 : nb * nb * nb * p->num_pi_bins * p->num_pi_bins;
-    return p->bin_type == NC_BIN_THREED ? nb * nb * nb : 0;
-}
-
-int nc_catalog_create(nc_context *ctx, const double *xyzw, int64_t n, nc_catalog **cat) {
-    if (!ctx || !cat || n < 0 || (n > 0 && !xyzw)) return ctx ? fail(ctx, NC_ERR_ARG, "bad catalogue") : NC_ERR_ARG;
-    if (n > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
-    *cat = nullptr;
-    NC_HIP(ctx, hipSetDevice(ctx->device));
-    nc_catalog *c = new (std::nothrow) nc_catalog();
-    if (!c) return fail(ctx, NC_ERR_NOMEM, "host allocation");
-    c->ctx = ctx;
-    c->uid = ctx->next_uid++;
-    c->n = n;
-    c->owns = true;
-    for (int64_t i = 0; i < n; ++i) {
-        const double w = xyzw[4 * i + 3];
-        c->unit_w = c->unit_w && w == 1.0;
-        c->wf32 = c->wf32 && fp32_weight(w);
-        if (std::isfinite(w))
-            c->wmax = std::max(c->wmax, std::fabs(w));
-        else
-            c->wfinite = false;
-    }
-    hipError_t e = dgrow_checked(&ctx->fail_alloc, c->raw, sizeof(double) * 4 * (size_t)n);
-    if (e == hipSuccess && n > 0)
-        e = hipMemcpyAsync(c->raw.p, xyzw, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-        dfree(c->raw);
-        delete c;
-        return fail(ctx, e == hipErrorOutOfMemory ? NC_ERR_NOMEM : NC_ERR_HIP, hipGetErrorString(e));
-    }
-    c->d_raw = (const double *)c->raw.p;
-    c->device = ctx->device;
-    ctx->cats.push_back(c);
-    *cat = c;
-    return NC_OK;
-}
-
-int nc_catalog_wrap_device(nc_context *ctx, const double *d_xyzw, int64_t n, nc_catalog **cat) {
-    if (!ctx || !cat || n < 0 || (n > 0 && !d_xyzw)) return ctx ? fail(ctx, NC_ERR_ARG, "bad catalogue") : NC_ERR_ARG;
-    if (n > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
-    *cat = nullptr;
-    NC_HIP(ctx, hipSetDevice(ctx->device));
-    nc_catalog *c = new (std::nothrow) nc_catalog();
-    if (!c) return fail(ctx, NC_ERR_NOMEM, "host allocation");
-    c->ctx = ctx;
-    c->uid = ctx->next_uid++;
-    c->n = n;
-    c->owns = false;
-    c->d_raw = d_xyzw;
-    // weight statistics: derived from the current contents before every count
-    c->device = ctx->device;
-    ctx->cats.push_back(c);
-    *cat = c;
-    return NC_OK;
-}
-
-int nc_catalog_destroy(nc_catalog *c) {
-    if (!c) return NC_ERR_ARG;
-    if (c->ctx) {
-        (void)hipSetDevice(c->ctx->device);
-        (void)hipStreamSynchronize(c->ctx->stream);
-        const std::vector<uint64_t> &u = c->ctx->plan.uids;
-        if (std::find(u.begin(), u.end(), c->uid) != u.end()) plan_drop(c->ctx);
-        auto &v = c->ctx->cats;
-        v.erase(std::remove(v.begin(), v.end(), c), v.end());
-    } else {
-        (void)hipSetDevice(c->device);
-    }
-    for (DBuf *b : {&c->raw, &c->key, &c->rank, &c->perm, &c->cnt, &c->start, &c->scan, &c->x, &c->y, &c->z, &c->w})
-        dfree(*b);
-    delete c;
-    return NC_OK;
-}
-
-int64_t nc_catalog_size(const nc_catalog *c) { return c ? c->n : -1; }
-
-int nc_count(nc_context *ctx, const nc_params *params, nc_catalog *c1, nc_catalog *c2, nc_catalog *c3, int path,
-             double *hist, uint64_t *counts) {
-    if (!ctx) return NC_ERR_ARG;
-    std::vector<Term> t{{c1, c2, c3}};
-    Mem m;
-    m.fail = &ctx->fail_alloc;
-    if (const int rc = refresh_weights(ctx, m, t)) return rc;
-    return run_count(ctx, params, t, path, hist, counts, false, m);
-}
-
-int nc_count_host(nc_context *ctx, const nc_params *params, const double *p1, int64_t n1, const double *p2, int64_t n2,
-                  const double *p3, int64_t n3, int path, double *hist, uint64_t *counts) {
-    if (!ctx) return NC_ERR_ARG;
-    nc_catalog *c[3] = {nullptr, nullptr, nullptr};
-    // identical host buffers share one device catalogue (and with it the
-    // (p2,p3) exchange symmetry)
-    int rc = nc_catalog_create(ctx, p1, n1, &c[0]);
-    if (!rc) {
-        if (p2 == p1 && n2 == n1)
-            c[1] = c[0];
-        else
-            rc = nc_catalog_create(ctx, p2, n2, &c[1]);
-    }
-    if (!rc) {
-        if (p3 == p2 && n3 == n2)
-            c[2] = c[1];
-        else if (p3 == p1 && n3 == n1)
-            c[2] = c[0];
-        else
-            rc = nc_catalog_create(ctx, p3, n3, &c[2]);
-    }
-    if (!rc) rc = nc_count(ctx, params, c[0], c[1], c[2], path, hist, counts);
-    if (c[2] && c[2] != c[1] && c[2] != c[0]) nc_catalog_destroy(c[2]);
-    if (c[1] && c[1] != c[0]) nc_catalog_destroy(c[1]);
-    if (c[0]) nc_catalog_destroy(c[0]);
-    return rc;
-}
-
-int nc_count_noalloc(nc_context *ctx, const nc_params *params, const double *d_p1, int64_t n1, const double *d_p2,
-                     int64_t n2, const double *d_p3, int64_t n3, int path, double *hist, uint64_t *counts) {
-    if (!ctx) return NC_ERR_ARG;
-    if (!ctx->noalloc) return fail(ctx, NC_ERR_HIP, "no-alloc scratch unavailable: " + ctx->noalloc_err);
-    if (!params || params->bin_type != NC_BIN_THREED)
-        return fail(ctx, NC_ERR_ARG, "nc_count_noalloc counts threeD bins only");
-    const double *d[3] = {d_p1, d_p2, d_p3};
-    const int64_t n[3] = {n1, n2, n3};
-    for (int i = 0; i < 3; ++i) {
-        if (n[i] < 0 || (n[i] > 0 && !d[i])) return fail(ctx, NC_ERR_ARG, "bad catalogue");
-        if (n[i] > 2147483647LL) return fail(ctx, NC_ERR_LIMIT, "catalogues are limited to 2^31-1 points");
-    }
-    // catalogues wrapping the caller's points, on the stack (never registered
-    // with the context, never freed: their buffers live in the arena); the
-    // same pointer and size passed twice is one catalogue, as in nc_count_host
-    nc_catalog cat[3];
-    nc_catalog *c[3];
-    for (int i = 0; i < 3; ++i) {
-        c[i] = &cat[i];
-        for (int j = 0; j < i; ++j)
-            if (d[j] == d[i] && n[j] == n[i]) {
-                c[i] = c[j];
-                break;
-            }
-        if (c[i] != &cat[i]) continue;
-        cat[i].ctx = ctx;
-        cat[i].device = ctx->device;
-        cat[i].n = n[i];
-        cat[i].d_raw = d[i];
-        cat[i].owns = false;
-    }
-    Mem m;
-    m.base = ctx->noalloc;
-    m.size = kNoAllocBytes;
-    NC_HIP(ctx, hipSetDevice(ctx->device));
-    std::vector<Term> t{{c[0], c[1], c[2]}};
-    if (const int rc = refresh_weights(ctx, m, t)) return rc;
-    return run_count(ctx, params, t, path, hist, counts, false, m);
-}
-
-int64_t nc_noalloc_capacity(void) { return (int64_t)kNoAllocBytes; }
-
-int nc_count_batched(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1,
-                     nc_catalog *const *c2, nc_catalog *const *c3, int path, double *hist, uint64_t *counts) {
-    if (!ctx) return NC_ERR_ARG;
-    if (nterms < 1 || !c1 || !c2 || !c3) return fail(ctx, NC_ERR_ARG, "bad term list");
-    std::vector<Term> t(nterms);
-    for (int i = 0; i < nterms; ++i) t[i] = {c1[i], c2[i], c3[i]};
-    return count_terms(ctx, params, t, path, hist, counts, true);
-}
-
-int nc_last_timing(const nc_context *ctx, nc_timing *out) {
-    if (!ctx || !out) return NC_ERR_ARG;
-    *out = ctx->timing;
-    return NC_OK;
-}
-
-// The lookup of MixedPol::group on the host: same clamp, cell, entry and
-// comparisons, in FP32 on scaled values (the scaling is exact).
-int nc_mixed_classify(const double *edges, int nedges, const float *d, int64_t n, int32_t *out, int32_t *entries) {
-    if (!edges || nedges < 2 || n < 0 || (n > 0 && (!d || !out))) return NC_ERR_ARG;
-    if (nedges > 41) return NC_ERR_LIMIT;
-    Geometry G;
-    G.edges.assign(edges, edges + nedges);
-    if (!(G.edges[0] >= 0.0)) return NC_ERR_BINS;
-    for (int i = 1; i < nedges; ++i)
-        if (!(G.edges[i] > G.edges[i - 1])) return NC_ERR_BINS;
-    G.nb = nedges - 1;
-    G.sq.resize(nedges);
-    for (int i = 0; i < nedges; ++i) G.sq[i] = G.edges[i] * G.edges[i];
-    make_mixed_tables(G);
-    if (!G.mixed_ok) return NC_ERR_LIMIT;
-    if (entries) *entries = (int32_t)G.lut32.size();
-    const float s = (float)(G.sigma32 * G.sigma32);
-    for (int64_t i = 0; i < n; ++i) {
-        const float x = d[i] * s;
-        if (std::isnan(x)) {  // no separation: outside every bin
-            out[i] = -1;
-            continue;
-        }
-        const float c = std::min(std::max(x, G.lut32_lo), 4.0f);
-        uint32_t bits;
-        std::memcpy(&bits, &c, 4);
-        const uint2 e = G.lut32[(bits >> G.lut32_shift) - (uint32_t)G.lut32_c0];
-        float E;
-        std::memcpy(&E, &e.y, 4);
-        const float dE = x - E;
-        const int k = (int)e.x - (std::signbit(dE) ? 1 : 0);
-        out[i] = std::fabs(dE) <= G.g32 ? -2 : (unsigned)k < (unsigned)G.nb ? k : -1;
-    }
-    return NC_OK;
-}
-
-int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_cu, int list_chunk) {
-    if (!ctx) return NC_ERR_ARG;
-    if (list_chunk > 0) {
-        if (list_chunk % 64 != 0 || list_chunk > 4096)
-            return fail(ctx, NC_ERR_ARG, "list_chunk must be a multiple of 64, at most 4096");
-        ctx->chunk_elems = list_chunk;
-    }
-    if (waves_per_group > 0) {
-        if (waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4 && waves_per_group != 8 &&
-            waves_per_group != 16)
-            return fail(ctx, NC_ERR_ARG, "waves_per_group must be 1, 2, 4, 8 or 16");
-        ctx->waves_per_group = waves_per_group;
-    }
-    if (chunk > 0) ctx->chunk = chunk;
-    if (wgs_per_cu >= 0) ctx->wgs_per_cu = wgs_per_cu;
-    return NC_OK;
-}
-
-int nc_set_accumulation(nc_context *ctx, int mode) {
-    if (!ctx) return NC_ERR_ARG;
-    if (mode != NC_ACCUM_FP64 && mode != NC_ACCUM_EXACT && mode != NC_ACCUM_FP32)
-        return fail(ctx, NC_ERR_ARG, "unknown accumulation mode");
-    ctx->accum = mode;
-    return NC_OK;
-}
-
-int nc_set_primary_slice(nc_context *ctx, int slice, int nslices) {
-    if (!ctx) return NC_ERR_ARG;
-    if (nslices < 1 || slice < 0 || slice >= nslices) return fail(ctx, NC_ERR_ARG, "need 0 <= slice < nslices");
-    ctx->slice = slice;
-    ctx->nslices = nslices;
-    return NC_OK;
-}
-
-int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *wgs_per_cu, int *list_chunk) {
-    if (!ctx) return NC_ERR_ARG;
-    if (list_chunk) *list_chunk = ctx->chunk_elems;
-    if (waves_per_group) *waves_per_group = ctx->waves_per_group;
-    if (chunk) *chunk = ctx->chunk;
-    if (wgs_per_cu) *wgs_per_cu = ctx->wgs_per_cu;
-    return NC_OK;
-}
-
-int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1, nc_catalog *const *c2,
-                nc_catalog *const *c3, int path, nc_tuning *best) {
-    if (!ctx) return NC_ERR_ARG;
-    if (nterms < 1 || !c1 || !c2 || !c3 || !params || !best) return fail(ctx, NC_ERR_ARG, "bad autotune arguments");
-    std::vector<Term> t(nterms);
-    int64_t npts = 0, nprim = 0;
-    for (int i = 0; i < nterms; ++i) {
-        t[i] = {c1[i], c2[i], c3[i]};
-        if (!c1[i] || !c2[i] || !c3[i]) return fail(ctx, NC_ERR_ARG, "null catalogue");
-        npts += c1[i]->n + c2[i]->n + c3[i]->n;
-        nprim += c1[i]->n;
-    }
-    // cache key: device, path, survey, bins, sizes
-    char key[256];
-    std::snprintf(key, sizeof(key), "%s path=%d survey=%d nb=%d terms=%d points=%lld", ctx->dev_name.c_str(), path,
-                  params->survey ? 1 : 0, params->nedges - 1, nterms, (long long)npts);
-    const char *cache = std::getenv("NC_TUNE_CACHE");
-    if (cache && *cache) {
-        if (FILE *f = std::fopen(cache, "r")) {
-            char line[512];
-            while (std::fgets(line, sizeof(line), f)) {
-                const size_t kl = std::strlen(key);
-                int w, c, pc, lc;
-                float ms;
-                if (std::strncmp(line, key, kl) == 0 && line[kl] == '|' &&
-                    std::sscanf(line + kl + 1, "%d %d %d %d %f", &w, &c, &pc, &lc, &ms) == 5) {
-                    std::fclose(f);
-                    ctx->waves_per_group = w;
-                    ctx->chunk = c;
-                    ctx->wgs_per_cu = pc;
-                    ctx->chunk_elems = lc;
-                    *best = {w, c, pc, lc, ms, 1};
-                    return NC_OK;
-                }
-            }
-            std::fclose(f);
-        }
-    }
-    const int saved[4] = {ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
-    std::vector<double> hist((size_t)nterms * nc_hist_len(params));
-    // sample: about 64 work items per workgroup slot of the chip at the largest chunk
-    const long long items = std::max<long long>(1, nprim / 16);
-    ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
-    ctx->autotuning = true;
-    ctx->tune_list_bound.clear();
-    const int saved_slice[2] = {ctx->slice, ctx->nslices};  // the sample is drawn from all primaries
-    ctx->slice = 0;
-    ctx->nslices = 1;
-    nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
-    int rc = NC_OK;
-    // 16-wave workgroups only where one workgroup per CU is forced anyway
-    // (more than half the CU's LDS); elsewhere two 8-wave ones do better
-    bool big_lds = false;
-    {
-        // a first run (8 waves per workgroup, the current chunks) sizes the
-        // sample to about 80 ms, enough to separate shapes a few percent apart
-        ctx->waves_per_group = 8;
-        ctx->wgs_per_cu = 0;
-        rc = count_terms(ctx, params, t, path, hist.data(), nullptr, false);
-        big_lds = rc == NC_OK && (size_t)ctx->timing.lds_bytes > ctx->lds_max;
-        const double ms = ctx->timing.ms_count;
-        if (rc == NC_OK && ms > 0.0)
-            ctx->sample_stride = (int)std::max(
-                1.0, std::min<double>((double)items, std::floor(ctx->sample_stride * (ms / 80.0))));
-        if (rc != NC_OK && rc != NC_ERR_LIMIT) goto done;
-        rc = NC_OK;
-    }
-    for (int w : {4, 8, 16})
-        for (int c : {16, 64})
-            for (int lc : {64, 128, 256}) {
-                if (w == 16 && !big_lds) continue;
-                ctx->waves_per_group = w;
-                ctx->chunk = c;
-                ctx->wgs_per_cu = 0;
-                ctx->chunk_elems = lc;
-                float ms = 1e30f;
-                int r = NC_OK;
-                for (int rep = 0; rep < 2 && r == NC_OK; ++rep) {  // the better of two runs
-                    r = count_terms(ctx, params, t, path, hist.data(), nullptr, false);
-                    if (r == NC_OK) ms = std::min(ms, ctx->timing.ms_count);
-                }
-                if (r == NC_ERR_LIMIT) {  // this shape does not fit: try the next
-                    if (rc == NC_OK) rc = r;
-                    continue;
-                }
-                if (r != NC_OK) {
-                    rc = r;
-                    goto done;
-                }
-                if (ms < b.ms) b = {w, c, 0, lc, ms, 0};
-            }
-    if (b.ms < 1e30f) rc = NC_OK;  // at least one shape ran
-done:
-    ctx->sample_stride = 1;
-    ctx->autotuning = false;
-    ctx->tune_list_bound.clear();
-    ctx->slice = saved_slice[0];
-    ctx->nslices = saved_slice[1];
-    plan_drop(ctx);
-    if (rc) {
-        ctx->waves_per_group = saved[0];
-        ctx->chunk = saved[1];
-        ctx->wgs_per_cu = saved[2];
-        ctx->chunk_elems = saved[3];
-        return rc;
-    }
-    ctx->waves_per_group = b.waves_per_group;
-    ctx->chunk = b.chunk;
-    ctx->wgs_per_cu = b.wgs_per_cu;
-    ctx->chunk_elems = b.list_chunk;
-    *best = b;
-    if (cache && *cache)
-        if (FILE *f = std::fopen(cache, "a")) {
-            std::fprintf(f, "%s|%d %d %d %d %.4f\n", key, b.waves_per_group, b.chunk, b.wgs_per_cu, b.list_chunk,
-                         b.ms);
-            std::fclose(f);
-        }
-    return NC_OK;
-}
-
-}  // extern "C"
+// Device address of noalloc_scratch on the current device (the host-side
+// name is a stand-in, registered with this translation unit's code object).
+hipError_t noalloc_symbol_address(void **p) { return hipGetSymbolAddress(p, HIP_SYMBOL(noalloc_scratch)); }
 

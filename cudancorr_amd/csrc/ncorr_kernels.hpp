@@ -164,4 +164,103 @@ struct KArgs {
     unsigned long long *rs_chunk_keys;     // dry: keys of each work item (region sizing per round)
 };
 
+
+// ---------------------------------------------------------------------------
+// Definitions shared by the kernels and the host-side planner.
+
+struct GridGeo {
+    double cs[3];
+    int nc[3];
+    int survey;
+};
+
+constexpr int kQueueCap = 320;  // deferred exact rechecks per wave (mixed path), >= 4*64 + 64
+
+// Neighbour-list flavours: threeD plain (Elem + weight), threeD mixed (MList),
+// rppi box and rppi survey (RElem).
+constexpr int kListPlain = 0, kListMixed = 1, kListRppiBox = 2, kListRppiSurvey = 3;
+
+// One rppi list entry: image position, to the primary rp^2 (box) or the
+// squared 3D separation (survey) r, then q, and the weight.  Box: q holds the
+// side's FP32 root sqrtf(rp^2), rp bin and pi bin floor(sqrtf(pi^2)) packed
+// (rp_side_pack), so a pair only evaluates its third side; survey: x - ori.x.
+struct RElem {
+    double x, y, z, r, q, w;
+};
+
+// Bytes of one list entry in a wave's scratch slot.
+__host__ __device__ constexpr size_t list_entry_bytes(int lm) {
+    return lm == kListMixed ? sizeof(float4) + sizeof(int2) + sizeof(int)
+                            : lm == kListPlain ? sizeof(Elem) + sizeof(double) : sizeof(RElem);
+}
+
+// LDS histogram layouts of the triplet kernel (HM):
+//   kHistCtab   compact C(nb+2,3) sorted slots (u64 counts / f64 sums), slot
+//               index through ctab; the plain path's FP64 bin lookup may take
+//               several steps;
+//   kHistT3     compact sorted slots, slot byte offset from the t3 table;
+//   kHistDense  dense nb^3 u32 counts indexed (ka, kb, k) unsorted, folded
+//               into the sorted slots when flushed (unit-weight terms only).
+// kHistT3 and kHistDense also take the single-step FP64 bin lookup.
+constexpr int kHistCtab = 0, kHistT3 = 1, kHistDense = 2;
+constexpr int kHistGlobal = 3;  // rppi: the dense histogram in global memory, no LDS copy
+constexpr int kHistStream = 4;  // rppi: key stream binned per bucket in LDS (reported layout)
+// kHistExact: the sorted slots of kHistT3 as 128-bit fixed-point integers (a
+// low-word array, then a high-word array): weight products are truncated to
+// multiples of 2^-K and summed exactly, so the sums do not depend on the order
+// of the additions (nc_set_accumulation NC_ACCUM_EXACT).
+constexpr int kHistExact = 5;
+// kHistF32: the sorted slots of kHistT3 as FP32 running sums with FP64
+// compensation (nc_set_accumulation NC_ACCUM_FP32): an array of nbc FP64
+// compensations at the slot offsets of kHistT3, then (at byte xhi = 8 nbc) an
+// array of nbc FP32 sums.  Each accepted pair's weight product is formed in
+// FP32 as a float-float pair (hi, lo) and hi is added to the FP32 sum with a
+// returning LDS atomic; the rounding error of that addition (TwoSum of the old
+// sum and hi, exact) and lo go to the compensation.  Sum + compensation is the
+// exact sum of the float-float products up to the FP64 rounding of the
+// compensation's own additions; the two are folded into the FP64 totals at
+// every merge.
+constexpr int kHistF32 = 6;
+// kHistF32C: the layout of kHistF32 with the slot offsets of kHistCtab (the
+// reference's index arithmetic), for bin counts whose t3 table and FP32 sums
+// do not fit in LDS together.
+constexpr int kHistF32C = 7;
+__host__ __device__ constexpr bool is_f32(int hm) { return hm == kHistF32 || hm == kHistF32C; }
+// layouts addressed through the t3 slot table, or through ctab
+__host__ __device__ constexpr bool t3_like(int hm) { return hm == kHistT3 || hm == kHistExact || hm == kHistF32; }
+__host__ __device__ constexpr bool ctab_like(int hm) { return hm == kHistCtab || hm == kHistF32C; }
+// layouts whose merges are also forced by a count of pairs added (flush_pairs)
+__host__ __device__ constexpr bool counts_pairs(int hm) { return hm == kHistDense || is_f32(hm); }
+// The mixed path's FP32 lookup (lut32, copied into LDS) holds in its entries'
+// .x the bin count in the layout's units: bytes of a dense count (4), of a t3
+// entry (2), or bins (ctab); the shift of .x.
+__host__ __device__ constexpr int lut32_units(int hm) { return hm == kHistDense ? 2 : t3_like(hm) ? 1 : 0; }
+
+// Per-wave staging of the rppi key stream: up to kStage keys (and values);
+// a row adds at most 2*64, so the wave flushes once it holds more than
+// kStage - 128.  kStreamBuckets bounds the buckets of one term.  Per bucket
+// of its current term a wave holds a reserved run [cur, end) of its region
+// (taken rs_blk slots at a time, so the bucket cursors see one atomic per
+// run, not one per flush); slots it leaves unused get kRsPad.
+constexpr int kStage = 512;
+constexpr int kStreamBuckets = 128;
+constexpr unsigned kRsPad = 0xffffffffu;
+
+__host__ __device__ constexpr size_t stream_lds_offset(bool weighted) {
+    return ((size_t)kStage * 4 + (weighted ? (size_t)kStage * 8 : 0) + (size_t)kStage * 2 + 15) & ~(size_t)15;
+}
+__host__ __device__ constexpr size_t stream_wave_bytes(bool weighted, int nbm) {
+    return stream_lds_offset(weighted) + (size_t)kStage * 2 + (size_t)nbm * 60;
+}
+
+// An FP32 weight for the FP32 accumulation (NC_ACCUM_FP32): a value FP32
+// holds exactly, 0 or of magnitude in [2^-40, 2^40], so that products of
+// three such weights (and six times them) stay normal FP32 numbers.
+__host__ __device__ inline bool fp32_weight(double w) {
+    const double a = w < 0.0 ? -w : w;
+    return (double)(float)w == w && (w == 0.0 || (a >= 0x1p-40 && a <= 0x1p40));
+}
+
+constexpr size_t kNoAllocBytes = (size_t)256 << 20;
+
 }  // namespace nck

@@ -1,6 +1,6 @@
 """CPU emulation of the FP32 accumulation arithmetic (NC_ACCUM_FP32): the
 float-float weight product and the TwoSum compensation the kernel applies per
-accepted pair (hist_add_f32 in cudancorr_amd/csrc/ncorr_hip.hip), replayed in
+accepted pair (hist_add_f32 in cudancorr_amd/csrc/ncorr_device.hip), replayed in
 numpy float32 / float64 in the same order of operations, against exact sums
 (math.fsum of the exact FP64 products of FP32 weights).  Also checks the
 header's accumulation enum against the Python binding."""

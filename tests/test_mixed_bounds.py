@@ -1,7 +1,7 @@
 """CPU check of the error bounds behind the mixed (FP32) path -- no GPU.
 
-The mixed kernel (cudancorr_amd/csrc/ncorr_hip.hip, make_mixed_tables and
-pair_loop_mixed) decides a pair without FP64 arithmetic when
+The mixed kernel (cudancorr_amd/csrc/ncorr_device.hip, MixedPol; its bounds are
+set by make_mixed_tables in cudancorr_amd/csrc/ncorr_geometry.hip) decides a pair without FP64 arithmetic when
   * its FP32 r23^2 is farther than g32 = 80uR^2 from every squared edge, and
   * s_a*s_b - dot^2 > tri_t = 80uR^2 * 2R^2 (1 + 1e-5) + 64uR^4, the largest value of
     80uR^2 (s_a + s_b) + 64uR^4 over list entries (triangle far from degenerate),
