@@ -1379,7 +1379,11 @@ struct MixedPol {
             // bitwise, so the predicates stay lane masks in scalar registers
             amb[u] = ok[u] & (amb_b | (valid & amb_t));
             acc[u] = ok[u] & valid & !amb_b & !amb_t;
-            slot[u] = pair_slot_x<HM>(A, t, ka[u], kb, kx);
+            // the row bin (an LDS read) in a register here, ahead of the
+            // branches below, so that no read waits inside them
+            int kr = ka[u];
+            asm volatile("" : "+v"(kr));
+            slot[u] = pair_slot_x<HM>(A, t, kr, kb, kx);
         }
         if constexpr (WEIGHTED && is_f32(HM)) hist_add_f32<U>(t, slot, wa, wbf, acc);
 #pragma unroll
