@@ -416,19 +416,28 @@ __device__ __forceinline__ void rp_side_unpack(double q, float &f, int &k, int &
 }
 
 
-// Arrays of a mixed-path list inside a wave's scratch slot.
+// Arrays of a mixed-path list inside a wave's scratch slot (24 B per entry;
+// the exact rechecks rebuild the entry's FP64 side, and with it its FP32
+// root, from the catalogue).
 struct MList {
     float4 *R;  // x, y, z relative to p1 (FP32), s = |r|^2 (FP32)
-    int2 *KF;   // exact bin of |p1-q|^2 | neighbour entry << 8, sqrtf(float(|p1-q|^2)) bits
+    int *K;     // exact bin of |p1-q|^2 | neighbour entry << 8
     int *I;     // index of the point in the cell-sorted catalogue
 };
 __device__ __forceinline__ MList mlist(const char *base, int cap) {
     MList m;
     m.R = reinterpret_cast<float4 *>(const_cast<char *>(base));
-    m.KF = reinterpret_cast<int2 *>(const_cast<char *>(base) + (size_t)cap * sizeof(float4));
-    m.I = reinterpret_cast<int *>(const_cast<char *>(base) + (size_t)cap * (sizeof(float4) + sizeof(int2)));
+    m.K = reinterpret_cast<int *>(const_cast<char *>(base) + (size_t)cap * sizeof(float4));
+    m.I = reinterpret_cast<int *>(const_cast<char *>(base) + (size_t)cap * (sizeof(float4) + sizeof(int)));
     return m;
 }
+
+// The primary of a pair loop: its cell (for the image shifts of the exact
+// rechecks) and its position.
+struct PrimRef {
+    int cx, cy, cz;
+    double px, py, pz;
+};
 
 // Catalogue arrays behind a mixed list (for weights and exact rechecks).
 struct CatRef {
@@ -526,7 +535,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             const double sg = A.sigma32;  // power of 2: the scaling is exact
             const float rx = (float)((qx - px) * sg), ry = (float)((qy - py) * sg), rz = (float)((qz - pz) * sg);
             ml.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
-            ml.KF[pos] = make_int2(k | (s << 8), __float_as_int(f));
+            ml.K[pos] = k | (s << 8);
             ml.I[pos] = j;
         } else {
             Elem e;
@@ -1221,21 +1230,25 @@ struct QueueEntry<kHistDense> {
 // shift of its neighbour entry, rebuilt as in build_list), one pair per lane.
 template <int ORD, bool WEIGHTED, int HM>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
-                            const CatRef &c2, const CatRef &c3, int cx, int cy, int cz,
+                            const CatRef &c2, const CatRef &c3, const PrimRef &pr,
                             const typename QueueEntry<HM>::T *q, int qn, int lane) {
     const MList m2 = mlist(l2, A.cap), m3 = mlist(l3, A.cap);
     wave_sync();
     for (int i = lane; i < qn; i += kWave) {
         const uint2 e = QueueEntry<HM>::unpack(q[i]);
-        const int2 ka = m2.KF[e.x], kb = m3.KF[e.y];
+        const int ka = m2.K[e.x], kb = m3.K[e.y];
         const int ja = m2.I[e.x], jb = m3.I[e.y];
         double sax, say, saz, sbx, sby, sbz;
-        neighbour(A, cx, cy, cz, ka.x >> 8, sax, say, saz);
-        neighbour(A, cx, cy, cz, kb.x >> 8, sbx, sby, sbz);
+        neighbour(A, pr.cx, pr.cy, pr.cz, ka >> 8, sax, say, saz);
+        neighbour(A, pr.cx, pr.cy, pr.cz, kb >> 8, sbx, sby, sbz);
+        // the image positions and FP32 roots of the two sides, as the list build made them
+        const double ax = c2.x[ja] + sax, ay = c2.y[ja] + say, az = c2.z[ja] + saz;
+        const double bx = c3.x[jb] + sbx, by = c3.y[jb] + sby, bz = c3.z[jb] + sbz;
+        const float fa = __builtin_sqrtf((float)sqrsep(pr.px - ax, pr.py - ay, pr.pz - az));
+        const float fb = __builtin_sqrtf((float)sqrsep(pr.px - bx, pr.py - by, pr.pz - bz));
         const double wa = WEIGHTED ? c2.w[ja] : 0.0, wb = WEIGHTED ? c3.w[jb] : 0.0;
-        exact_pair<ORD, WEIGHTED, HM>(A, t, c2.x[ja] + sax, c2.y[ja] + say, c2.z[ja] + saz, ka.x & 0xff,
-                                        __int_as_float(ka.y), w1, w1 * wa, wa, c3.x[jb] + sbx, c3.y[jb] + sby,
-                                        c3.z[jb] + sbz, kb.x & 0xff, __int_as_float(kb.y), w1 * wb, wb, true);
+        exact_pair<ORD, WEIGHTED, HM>(A, t, ax, ay, az, ka & 0xff, fa, w1, w1 * wa, wa, bx, by, bz, kb & 0xff, fb,
+                                        w1 * wb, wb, true);
     }
     wave_sync();
 }
@@ -1262,7 +1275,7 @@ struct MixedPol {
     MList m2, m3;
     CatRef c2, c3;
     const char *l2, *l3;
-    int cx, cy, cz;
+    PrimRef pr;
     double w1;
     int lane;
     float4 *la;
@@ -1282,8 +1295,8 @@ struct MixedPol {
     bool vb;
 
     __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
-                        int lane_, const CatRef &cat2, const CatRef &cat3, int cx_, int cy_, int cz_)
-        : A(A_), t(t_), c2(cat2), c3(cat3), l2(l2_), l3(l3_), cx(cx_), cy(cy_), cz(cz_), w1(w1_), lane(lane_) {
+                        int lane_, const CatRef &cat2, const CatRef &cat3, const PrimRef &pr_)
+        : A(A_), t(t_), c2(cat2), c3(cat3), l2(l2_), l3(l3_), pr(pr_), w1(w1_), lane(lane_) {
         C = A.chunk_elems;
         const int Cp = C + 4;
         m2 = mlist(l2, A.cap);
@@ -1316,7 +1329,7 @@ struct MixedPol {
             double w = 0.0;
             if (i < last - first) {
                 r = m.R[first + i];
-                k = row_bin<HM>(A, m.KF[first + i].x & 0xff);
+                k = row_bin<HM>(A, m.K[first + i] & 0xff);
                 if (WEIGHTED) w = pre_weighted<WEIGHTED, HM>() ? wrow * c.w[m.I[first + i]] : c.w[m.I[first + i]];
                 if (is_f32(HM)) w = f32_pair(w);
             }
@@ -1337,7 +1350,7 @@ struct MixedPol {
         wb = 0.0;
         if (vb) {
             rb = m.R[bidx];
-            kb = lane_bin<HM>(A, m.KF[bidx].x & 0xff);
+            kb = lane_bin<HM>(A, m.K[bidx] & 0xff);
             if (WEIGHTED) wb = (from3 ? c3 : c2).w[m.I[bidx]];
         }
         if constexpr (!pre_weighted<WEIGHTED, HM>()) w1b = w1 * wb;
@@ -1345,7 +1358,7 @@ struct MixedPol {
     }
 
     __device__ void drain() {
-        drain_queue<ORD, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, cx, cy, cz, q, qn, lane);
+        drain_queue<ORD, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, pr, q, qn, lane);
         qn = 0;
     }
 
@@ -1526,9 +1539,9 @@ __device__ __forceinline__ void flush_hist(const KArgs &A, int term, int weighte
 template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? kOrdSym : kOrdOne>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
-                                          const CatRef &c3, int cx, int cy, int cz) {
+                                          const CatRef &c3, const PrimRef &pr) {
     if constexpr (MIXED) {
-        MixedPol<SYM, ORD, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, cx, cy, cz);
+        MixedPol<SYM, ORD, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, pr);
         traverse(P, n2, n3);
         if (P.qn > 0) P.drain();
     } else {
@@ -1571,7 +1584,8 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
     if (nh + nbu > A.nmax) return false;
     wave_sync();
     const CatRef c = {T.x2, T.y2, T.z2, T.w2};
-    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, kOrdSix>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, cx, cy, cz);
+    const PrimRef pr = {cx, cy, cz, px, py, pz};
+    if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, kOrdSix>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, pr);
     int nrow = nh;
     if (nbu > 0) {
         nrow = uniform(build_list<LM, kSelLow, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
@@ -1579,8 +1593,8 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
         if (nrow + nbu > A.nmax) return false;
         wave_sync();
         if (nrow > 0)
-            pair_loop<MIXED, HM, false, WEIGHTED, kOrdSym>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, cx, cy, cz);
-        if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, cx, cy, cz);
+            pair_loop<MIXED, HM, false, WEIGHTED, kOrdSym>(A, t, pw, l2, nrow, l3, nbu, wlds, lane, c, c, pr);
+        if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, pr);
     }
     if constexpr (counts_pairs(HM))
         wpairs += 3ull * (unsigned long long)nh * (unsigned long long)max(nh - 1, 0) +
@@ -1751,6 +1765,7 @@ k_triplets(KArgs A) {
             const char *lb = T.sym ? l2 : l3;
             const CatRef c2 = {T.x2, T.y2, T.z2, T.w2};
             const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3};
+            const PrimRef pr = {cx, cy, cz, px, py, pz};
             if constexpr (!WT) {
                 if (T.weighted) {  // never planned so
                     if (lane == 0) atomicOr(A.err, kErrMode);
@@ -1764,23 +1779,22 @@ k_triplets(KArgs A) {
             if (T.sym) {
                 if (T.weighted) {
                     if constexpr (WT)
-                        pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                        pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 } else {
-                    pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 }
             } else if (T.tri == kTriSwap) {
                 // (the planner takes this only for terms of the variant's weighting)
                 if (T.weighted == (int)WT)
-                    pair_loop<MIXED, HM, false, WT, kOrdSwap>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy,
-                                                              cz);
+                    pair_loop<MIXED, HM, false, WT, kOrdSwap>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 else if (lane == 0)
                     atomicOr(A.err, kErrMode);
             } else {
                 if (T.weighted) {
                     if constexpr (WT)
-                        pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                        pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 } else {
-                    pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, cx, cy, cz);
+                    pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 }
             }
             wave_sync();
