@@ -103,6 +103,7 @@ TermDesc term_desc(const Term &t) {
     d.n1 = (int)t.c1->n;
     d.sym = t.c2 == t.c3;
     d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
+    d.wprim = 0;  // set by run_count for FP64 accumulation
     d.tri = 0;  // set by run_count once the builds have shown exact gridding
     d.prune2 = d.prune3 = 0;  // likewise
     return d;
@@ -1089,6 +1090,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         const Term &t = terms[i];
         TermDesc &d = td[i];
         d = term_desc(t);
+        d.wprim = d.weighted && t.c2->unit_w && t.c3->unit_w && !acc.exact && !acc.f32;
         weighted[i] = d.weighted;
         if (t.c2->n == 0 || t.c3->n == 0) continue;
         d.tri = sym.tri[i];

@@ -1228,7 +1228,8 @@ struct QueueEntry<kHistDense> {
 // Mixed path, deferred exact rechecks: every queued (S2 entry, S3 entry) pair
 // is evaluated with the FP64 image positions (catalogue point + periodic
 // shift of its neighbour entry, rebuilt as in build_list), one pair per lane.
-template <int ORD, bool WEIGHTED, int HM>
+// PRIMW: the term's secondaries carry unit weights (wa = wb = 1).
+template <int ORD, bool WEIGHTED, int HM, bool PRIMW = false>
 __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const char *l2, const char *l3,
                             const CatRef &c2, const CatRef &c3, const PrimRef &pr,
                             const typename QueueEntry<HM>::T *q, int qn, int lane) {
@@ -1246,9 +1247,9 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
         const double bx = c3.x[jb] + sbx, by = c3.y[jb] + sby, bz = c3.z[jb] + sbz;
         const float fa = __builtin_sqrtf((float)sqrsep(pr.px - ax, pr.py - ay, pr.pz - az));
         const float fb = __builtin_sqrtf((float)sqrsep(pr.px - bx, pr.py - by, pr.pz - bz));
-        const double wa = WEIGHTED ? c2.w[ja] : 0.0, wb = WEIGHTED ? c3.w[jb] : 0.0;
-        exact_pair<ORD, WEIGHTED, HM>(A, t, ax, ay, az, ka & 0xff, fa, w1, w1 * wa, wa, bx, by, bz, kb & 0xff, fb,
-                                        w1 * wb, wb, true);
+        const double wa = PRIMW ? 1.0 : WEIGHTED ? c2.w[ja] : 0.0, wb = PRIMW ? 1.0 : WEIGHTED ? c3.w[jb] : 0.0;
+        exact_pair<ORD, WEIGHTED || PRIMW, HM>(A, t, ax, ay, az, ka & 0xff, fa, w1, w1 * wa, wa, bx, by, bz,
+                                                 kb & 0xff, fb, w1 * wb, wb, true);
     }
     wave_sync();
 }
@@ -1264,7 +1265,10 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
 // the error bound (make_mixed_tables), is queued for the exact FP64
 // evaluation.
 // Rows: float4 {r, s} + bin + weight in LDS; lanes in registers.
-template <bool SYM, int ORD, bool WEIGHTED, int HM>
+// PRIMW (with WEIGHTED false): a weighted term whose secondaries carry unit
+// weights; every ordered triplet adds (w1 * 1) * 1 = w1, so an accepted pair
+// adds the wave-uniform ord * w1 and no weights are staged or loaded.
+template <bool SYM, int ORD, bool WEIGHTED, int HM, bool PRIMW = false>
 struct MixedPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 4;
@@ -1290,7 +1294,7 @@ struct MixedPol {
     float4 rb;
     int kb, bidx;
     double wb, w1b;
-    double wrow;  // the staged rows' weight factor (pre_weighted)
+    double wrow;  // the staged rows' weight factor (pre_weighted; PRIMW: the value of an accepted pair)
     float wbf;    // the lane's weight in FP32 (kHistF32: FP32 weights)
     bool vb;
 
@@ -1358,7 +1362,7 @@ struct MixedPol {
     }
 
     __device__ void drain() {
-        drain_queue<ORD, WEIGHTED, HM>(A, t, w1, l2, l3, c2, c3, pr, q, qn, lane);
+        drain_queue<ORD, WEIGHTED, HM, PRIMW>(A, t, w1, l2, l3, c2, c3, pr, q, qn, lane);
         qn = 0;
     }
 
@@ -1398,11 +1402,14 @@ struct MixedPol {
             asm volatile("" : "+v"(kr));
             slot[u] = pair_slot_x<HM>(A, t, kr, kb, kx);
         }
+        static_assert(!PRIMW || (!WEIGHTED && (HM == kHistT3 || HM == kHistCtab)), "FP64 slot sums");
         if constexpr (WEIGHTED && is_f32(HM)) hist_add_f32<U>(t, slot, wa, wbf, acc);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (acc[u] && !(WEIGHTED && is_f32(HM))) {
-                if constexpr (pre_weighted<WEIGHTED, HM>())
+                if constexpr (PRIMW)
+                    hist_add_value(t, slot[u], wrow);
+                else if constexpr (pre_weighted<WEIGHTED, HM>())
                     hist_add_value(t, slot[u], wa[u] * wb);
                 else if (SWAP)
                     hist_add<ORD, WEIGHTED, HM>(t, slot[u], w1, w1b, wb, w1 * wa[u], wa[u]);
@@ -1536,12 +1543,14 @@ __device__ __forceinline__ void flush_hist(const KArgs &A, int term, int weighte
 
 // ORD: ordered triplets per evaluated pair (hist_add); by default kOrdSym for
 // a symmetric traversal (p2 and p3 from one list) and kOrdOne otherwise.
-template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? kOrdSym : kOrdOne>
+// PRIMW: a weighted term with unit-weight secondaries (MixedPol; the plain
+// path takes it as any weighted term).
+template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? kOrdSym : kOrdOne, bool PRIMW = false>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
                                           const CatRef &c3, const PrimRef &pr) {
     if constexpr (MIXED) {
-        MixedPol<SYM, ORD, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane, c2, c3, pr);
+        MixedPol<SYM, ORD, WEIGHTED && !PRIMW, HM, PRIMW> P(A, t, w1, l2, l3, wlds, lane, c2, c3, pr);
         traverse(P, n2, n3);
         if (P.qn > 0) P.drain();
     } else {
@@ -1778,8 +1787,15 @@ k_triplets(KArgs A) {
             }
             if (T.sym) {
                 if (T.weighted) {
-                    if constexpr (WT)
+                    if constexpr (WT && primw_ok(HM)) {
+                        if (T.wprim)
+                            pair_loop<MIXED, HM, true, true, kOrdSym, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2,
+                                                                            c3, pr);
+                        else
+                            pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
+                    } else if constexpr (WT) {
                         pair_loop<MIXED, HM, true, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
+                    }
                 } else {
                     pair_loop<MIXED, HM, true, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 }
@@ -1791,8 +1807,15 @@ k_triplets(KArgs A) {
                     atomicOr(A.err, kErrMode);
             } else {
                 if (T.weighted) {
-                    if constexpr (WT)
+                    if constexpr (WT && primw_ok(HM)) {
+                        if (T.wprim)
+                            pair_loop<MIXED, HM, false, true, kOrdOne, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane,
+                                                                             c2, c3, pr);
+                        else
+                            pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
+                    } else if constexpr (WT) {
                         pair_loop<MIXED, HM, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
+                    }
                 } else {
                     pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 }

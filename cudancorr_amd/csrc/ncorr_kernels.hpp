@@ -53,6 +53,9 @@ struct TermDesc {
     int n1;
     int sym;       // c2 == c3: count unordered (p2,p3) pairs once, doubled
     int weighted;  // accumulate (w1*w2)*w3 in FP64, else integer counts
+    // weighted, with unit weights on c2 and c3 (FP64 accumulation): every
+    // ordered triplet adds w1, so the pair loops stage no weights
+    int wprim;
     // kTriSix: c1 == c2 == c3, exactly gridded: every unordered triple of
     // points outside the last cells is evaluated once, from its lowest-index
     // point, and counted six times (see k_triplets).
@@ -227,6 +230,8 @@ constexpr int kHistF32 = 6;
 constexpr int kHistF32C = 7;
 __host__ __device__ constexpr bool is_f32(int hm) { return hm == kHistF32 || hm == kHistF32C; }
 // layouts addressed through the t3 slot table, or through ctab
+// layouts whose weighted terms may take the unit-secondary (wprim) pair loops
+__host__ __device__ constexpr bool primw_ok(int hm) { return hm == kHistT3 || hm == kHistCtab; }
 __host__ __device__ constexpr bool t3_like(int hm) { return hm == kHistT3 || hm == kHistExact || hm == kHistF32; }
 __host__ __device__ constexpr bool ctab_like(int hm) { return hm == kHistCtab || hm == kHistF32C; }
 // layouts whose merges are also forced by a count of pairs added (flush_pairs)
