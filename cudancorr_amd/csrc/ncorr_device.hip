@@ -94,6 +94,16 @@ __device__ __forceinline__ double uniform(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// A catalogue array as a global-memory pointer.  The catalogue pointers reach
+// the kernels inside TermDesc, so the compiler cannot tell their address
+// space and would load through flat instructions, which count against the
+// LDS wait counter too: every LDS wait after them (the list build's lane
+// searches, the pair loops' table reads) would also wait for those loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gptr(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -659,7 +669,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
                     }
                 }
             }
-            int b = (int)st[cell * kSlabs + s0], end = (int)st[cell * kSlabs + s1];
+            int b = (int)gptr(st)[cell * kSlabs + s0], end = (int)gptr(st)[cell * kSlabs + s1];
             if constexpr (SEL == kSelSplit)
                 if (!last) b = max(b, pidx + 1);  // last cells whole (into base_b), others above pidx
             if constexpr (SEL == kSelLow) end = last ? b : min(end, pidx);  // not last, below pidx
@@ -718,10 +728,10 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
                 // image position: point + shift (device.cu:282-284); weights
                 // only for the lists that carry them (the mixed list reads them
                 // from the catalogue where they are needed)
-                qx[u] = X[jj[u]];
-                qy[u] = Y[jj[u]];
-                qz[u] = Z[jj[u]];
-                if (!MIXED) qw[u] = Wt[jj[u]];
+                qx[u] = gptr(X)[jj[u]];
+                qy[u] = gptr(Y)[jj[u]];
+                qz[u] = gptr(Z)[jj[u]];
+                if (!MIXED) qw[u] = gptr(Wt)[jj[u]];
             }
         }
 #pragma unroll
@@ -918,6 +928,15 @@ __device__ __forceinline__ void hist_add_value(const LdsTabs &t, int off, double
 __device__ __forceinline__ double f32_pair(double x) {
     const float hi = (float)x;
     return __builtin_bit_cast(double, make_float2(hi, (float)(x - (double)hi)));
+}
+
+// The staged weight of a row: c w1 w_row (pre-weighted layouts, wrow = c w1),
+// the catalogue weight otherwise; FP32 accumulation keeps it as a pair.
+template <bool WEIGHTED, int HM>
+__device__ __forceinline__ double staged_weight(double wrow, double w) {
+    if (pre_weighted<WEIGHTED, HM>()) w = wrow * w;
+    if (is_f32(HM)) w = f32_pair(w);
+    return w;
 }
 // The U pairs of a pair-loop group together: every returning atomic is issued
 // before the first sum it returns is used, so their latencies overlap.
@@ -1334,8 +1353,7 @@ struct MixedPol {
             if (i < last - first) {
                 r = m.R[first + i];
                 k = row_bin<HM>(A, m.K[first + i] & 0xff);
-                if (WEIGHTED) w = pre_weighted<WEIGHTED, HM>() ? wrow * c.w[m.I[first + i]] : c.w[m.I[first + i]];
-                if (is_f32(HM)) w = f32_pair(w);
+                if (WEIGHTED) w = staged_weight<WEIGHTED, HM>(wrow, gptr(c.w)[m.I[first + i]]);
             }
             la[i] = r;
             lk[i] = k;
@@ -1355,7 +1373,7 @@ struct MixedPol {
         if (vb) {
             rb = m.R[bidx];
             kb = lane_bin<HM>(A, m.K[bidx] & 0xff);
-            if (WEIGHTED) wb = (from3 ? c3 : c2).w[m.I[bidx]];
+            if (WEIGHTED) wb = gptr((from3 ? c3 : c2).w)[m.I[bidx]];
         }
         if constexpr (!pre_weighted<WEIGHTED, HM>()) w1b = w1 * wb;
         wbf = (float)wb;
