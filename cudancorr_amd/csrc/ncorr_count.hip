@@ -104,6 +104,8 @@ TermDesc term_desc(const Term &t) {
     d.sym = t.c2 == t.c3;
     d.weighted = !(t.c1->unit_w && t.c2->unit_w && t.c3->unit_w);
     d.wprim = 0;  // set by run_count for FP64 accumulation
+    d.wunit2 = t.c2->unit_w;
+    d.wunit3 = t.c3->unit_w;
     d.tri = 0;  // set by run_count once the builds have shown exact gridding
     d.prune2 = d.prune3 = 0;  // likewise
     return d;

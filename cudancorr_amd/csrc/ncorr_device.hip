@@ -452,6 +452,7 @@ struct PrimRef {
 // Catalogue arrays behind a mixed list (for weights and exact rechecks).
 struct CatRef {
     const double *x, *y, *z, *w;
+    int unit;  // every weight is 1: the pair loops read none
 };
 
 // Compact the neighbours of primary (px,py,pz) in catalogue (X,Y,Z,Wt,st)
@@ -1353,7 +1354,7 @@ struct MixedPol {
             if (i < last - first) {
                 r = m.R[first + i];
                 k = row_bin<HM>(A, m.K[first + i] & 0xff);
-                if (WEIGHTED) w = staged_weight<WEIGHTED, HM>(wrow, gptr(c.w)[m.I[first + i]]);
+                if (WEIGHTED) w = staged_weight<WEIGHTED, HM>(wrow, c.unit ? 1.0 : gptr(c.w)[m.I[first + i]]);
             }
             la[i] = r;
             lk[i] = k;
@@ -1373,7 +1374,8 @@ struct MixedPol {
         if (vb) {
             rb = m.R[bidx];
             kb = lane_bin<HM>(A, m.K[bidx] & 0xff);
-            if (WEIGHTED) wb = gptr((from3 ? c3 : c2).w)[m.I[bidx]];
+            const CatRef &c = from3 ? c3 : c2;
+            if (WEIGHTED) wb = c.unit ? 1.0 : gptr(c.w)[m.I[bidx]];
         }
         if constexpr (!pre_weighted<WEIGHTED, HM>()) w1b = w1 * wb;
         wbf = (float)wb;
@@ -1610,7 +1612,7 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
     const int nbu = uniform(nb);
     if (nh + nbu > A.nmax) return false;
     wave_sync();
-    const CatRef c = {T.x2, T.y2, T.z2, T.w2};
+    const CatRef c = {T.x2, T.y2, T.z2, T.w2, T.wunit2};
     const PrimRef pr = {cx, cy, cz, px, py, pz};
     if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, kOrdSix>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, pr);
     int nrow = nh;
@@ -1790,8 +1792,8 @@ k_triplets(KArgs A) {
             if (n2u == 0 || n3u == 0) continue;
             wave_sync();
             const char *lb = T.sym ? l2 : l3;
-            const CatRef c2 = {T.x2, T.y2, T.z2, T.w2};
-            const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3};
+            const CatRef c2 = {T.x2, T.y2, T.z2, T.w2, T.wunit2};
+            const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3, T.wunit3};
             const PrimRef pr = {cx, cy, cz, px, py, pz};
             if constexpr (!WT) {
                 if (T.weighted) {  // never planned so

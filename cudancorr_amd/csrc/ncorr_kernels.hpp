@@ -56,6 +56,7 @@ struct TermDesc {
     // weighted, with unit weights on c2 and c3 (FP64 accumulation): every
     // ordered triplet adds w1, so the pair loops stage no weights
     int wprim;
+    int wunit2, wunit3;  // c2 / c3 has unit weights (its weights are not read)
     // kTriSix: c1 == c2 == c3, exactly gridded: every unordered triple of
     // points outside the last cells is evaluated once, from its lowest-index
     // point, and counted six times (see k_triplets).
