@@ -45,7 +45,7 @@ class NcTiming(ctypes.Structure):
                 ("n_kernel", ctypes.c_int64),
                 ("hist_layout", ctypes.c_int32),
                 ("lds_bytes", ctypes.c_int32), ("path", ctypes.c_int32), ("tri_terms", ctypes.c_int32),
-                ("swap_terms", ctypes.c_int32)]
+                ("swap_terms", ctypes.c_int32), ("partner_terms", ctypes.c_int32)]
 
 
 class NcTuning(ctypes.Structure):

@@ -42,6 +42,7 @@ int nc_create(nc_context **out, int device) {
         if (v > 0.0 && v <= 1.0) ctx->rs_scale = v;
     }
     if (const char *e = std::getenv("NC_BATCH_SPLIT")) ctx->batch_split = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("NC_FUSE_TERMS")) ctx->fuse_terms = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_TRI")) ctx->tri = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_FAIL_DEVICE_ALLOC")) ctx->fail_alloc = std::strtoll(e, nullptr, 10);
     if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {

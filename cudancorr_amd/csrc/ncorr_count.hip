@@ -106,6 +106,7 @@ TermDesc term_desc(const Term &t) {
     d.wprim = 0;  // set by run_count for FP64 accumulation
     d.wunit2 = t.c2->unit_w;
     d.wunit3 = t.c3->unit_w;
+    d.partner = -1;  // set by run_count
     d.tri = 0;  // set by run_count once the builds have shown exact gridding
     d.prune2 = d.prune3 = 0;  // likewise
     return d;
@@ -204,6 +205,7 @@ int plan_replay(nc_context *ctx, double *hist, uint64_t *counts, bool &retry) {
     ctx->timing.path = p.path;
     ctx->timing.tri_terms = p.tri_terms;
     ctx->timing.swap_terms = p.swap_terms;
+    ctx->timing.partner_terms = p.partner_terms;
     if (*p.h_err & kErrDomain) return fail(ctx, NC_ERR_DOMAIN, "a point lies outside the cell grid of the box");
     if (*p.h_err & (kErrOverflow | kErrReplan)) {  // catalogue contents changed under a wrapped pointer: rebuild eagerly
         plan_drop(ctx);
@@ -360,6 +362,7 @@ struct LdsLayout {
     const Geometry &G;
     bool mixed;
     size_t stage_bytes = 0, rank_bytes = 0;
+    int nhist = 1;  // 2: a second histogram for partner terms
 
     size_t wave(int c, int hm) const {
         const size_t cp = (size_t)c + 4;
@@ -399,7 +402,7 @@ struct LdsLayout {
     }
     bool fits(int hm, int W, size_t budget) const {
         LdsOffsets o;
-        return hist(hm) + tables(hm, o) + W * wave(64, hm) <= budget;
+        return nhist * hist(hm) + tables(hm, o) + W * wave(64, hm) <= budget;
     }
 };
 
@@ -574,16 +577,18 @@ const void *launch_kernel(const Geometry &G, bool stream, int hm, bool mixed, bo
 // Queue segments, one per XCD (workgroup b runs on XCD b % 8): each XCD works
 // through its own run of cell-ordered work items, so the neighbourhoods its
 // primaries scan stay in its L2.  The item list is cut at equal estimated work
-// (primaries x |c2| x |c3|); drained segments are helped with by the other
-// XCDs.  seg: nseg + 1 bounds.
-int queue_segments(const std::vector<Chunk> &chunks, const std::vector<Term> &terms, int grid, std::vector<int> &seg) {
+// (primaries x |c2| x |c3|, plus primaries x |c3|^2 / 2 for a partner term);
+// drained segments are helped with by the other XCDs.  seg: nseg + 1 bounds.
+int queue_segments(const std::vector<Chunk> &chunks, const std::vector<Term> &terms, const std::vector<int> &partner,
+                   int grid, std::vector<int> &seg) {
     const int nseg = std::max(1, std::min(kMaxSeg, grid));
     seg.assign(kMaxSeg + 1, 0);
     std::vector<double> cw(chunks.size());
     double wtot = 0.0;
     for (size_t c = 0; c < chunks.size(); ++c) {
         const Term &t = terms[chunks[c].term];
-        cw[c] = (double)(chunks[c].end - chunks[c].begin) * (double)t.c2->n * (double)t.c3->n;
+        const double n3 = (double)t.c3->n;
+        cw[c] = (double)(chunks[c].end - chunks[c].begin) * ((double)t.c2->n * n3 + (partner[chunks[c].term] >= 0 ? 0.5 * n3 * n3 : 0.0));
         wtot += cw[c];
     }
     int sgi = 1;
@@ -1051,8 +1056,40 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         lay.rank_bytes = lay.stage_bytes = 0;
         budget = fits(hm, ctx->lds_max) ? ctx->lds_max : ctx->lds_hw;
     }
+    // Partner terms: (c1, c3, c3) beside (c1, c2, c3), c2 != c3 (DRR beside
+    // DDR): the primaries and the list of c3 are the same, so the work items
+    // of the first count both, into two histograms; the partner gets no work
+    // items.  Sorted-slot layouts only, where the second histogram fits.
+    std::vector<int> partner(nterms, -1), absorbed(nterms, 0);
+    if (ctx->fuse_terms && !G.rppi && primw_ok(hm)) {
+        bool any = false;
+        for (int i = 0; i < nterms; ++i) {
+            const Term &a = terms[i];
+            if (absorbed[i] || a.c2 == a.c3 || sym.tri[i] == kTriSix || a.c2->n == 0 || a.c3->n == 0) continue;
+            for (int j = 0; j < nterms; ++j) {
+                const Term &b = terms[j];
+                if (j == i || absorbed[j] || partner[j] >= 0 || sym.tri[j] != kTriNone) continue;
+                if (b.c1 == a.c1 && b.c2 == a.c3 && b.c3 == a.c3) {
+                    partner[i] = j;
+                    absorbed[j] = 1;
+                    any = true;
+                    break;
+                }
+            }
+        }
+        if (any) {
+            lay.nhist = 2;
+            if (!fits(hm, budget)) {
+                lay.nhist = 1;
+                partner.assign(nterms, -1);
+                absorbed.assign(nterms, 0);
+            }
+        }
+    }
+    int nabsorbed = 0;
+    for (int a : absorbed) nabsorbed += a;
     LdsOffsets off;
-    const size_t hist_bytes = lay.hist(hm);
+    const size_t hist_bytes = lay.nhist * lay.hist(hm);
     const size_t lds_tab_bytes = lay.tables(hm, off);
 
     // --- launch shape
@@ -1093,8 +1130,9 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         TermDesc &d = td[i];
         d = term_desc(t);
         d.wprim = d.weighted && t.c2->unit_w && t.c3->unit_w && !acc.exact && !acc.f32;
+        d.partner = partner[i];
         weighted[i] = d.weighted;
-        if (t.c2->n == 0 || t.c3->n == 0) continue;
+        if (t.c2->n == 0 || t.c3->n == 0 || absorbed[i]) continue;
         d.tri = sym.tri[i];
         d.prune2 = prunes(G, t.c2);
         d.prune3 = prunes(G, t.c3);
@@ -1143,7 +1181,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.nchunks = (int)chunks.size();
     A.counter = (unsigned *)bufs.counter.p;
     std::vector<int> seg;  // host copy, kept (like chunks) until the copy has run
-    A.nseg = queue_segments(chunks, terms, L.grid, seg);
+    A.nseg = queue_segments(chunks, terms, partner, L.grid, seg);
     A.seg = (const int *)(A.counter + 16);
     NC_HIP(ctx, hipMemcpyAsync((void *)A.seg, seg.data(), sizeof(int) * (size_t)(A.nseg + 1), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -1201,6 +1239,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.off_lut32 = (int)(hist_bytes + off.lut32);
     A.off_t3 = (int)(hist_bytes + off.t3);
     A.off_wave = (int)lds_tab;
+    A.off_hist2 = lay.nhist == 2 ? (int)lay.hist(hm) : 0;
     A.wave_bytes = (int)wave_bytes;
     A.lds_bytes = (int)L.lds_bytes;
     NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
@@ -1238,6 +1277,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     ctx->timing.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
     ctx->timing.tri_terms = sym.ntri;
     ctx->timing.swap_terms = sym.nswap;
+    ctx->timing.partner_terms = nabsorbed;
     // key stream: keep the region sizes, and capture no graph after an
     // overflow so that the next call uses the grown ones
     bool rs_overflow = false;
@@ -1269,6 +1309,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         p.path = mixed ? NC_PATH_MIXED : NC_PATH_PLAIN;
         p.tri_terms = sym.ntri;
         p.swap_terms = sym.nswap;
+        p.partner_terms = nabsorbed;
         capture_plan(ctx, m, bufs, G, grids, L, p);
     }
     return NC_OK;
@@ -1384,6 +1425,7 @@ int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term
             ctx->timing.path = tm[0].path;
             ctx->timing.tri_terms = tm[0].tri_terms + tm[1].tri_terms;
             ctx->timing.swap_terms = tm[0].swap_terms + tm[1].swap_terms;
+            ctx->timing.partner_terms = tm[0].partner_terms + tm[1].partner_terms;
             return NC_OK;
         }
     }

@@ -1674,7 +1674,11 @@ k_triplets(KArgs A) {
     t.xscale = A.xscale;
     t.xhi = A.nbc * 8;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    // the partner terms' histogram (TermDesc.partner) follows the first
+    unsigned long long *hist2 = reinterpret_cast<unsigned long long *>(lds + A.off_hist2);
     for (int i = tid; i < A.nslots; i += nthr) t.hist[i] = 0ull;
+    if (A.off_hist2 > 0)
+        for (int i = tid; i < A.nslots; i += nthr) hist2[i] = 0ull;
     if (tid == 0) {
         s_pairs = 0ull;
         s_seg = 0;
@@ -1703,7 +1707,7 @@ k_triplets(KArgs A) {
     char *slot = A.scratch + (long long)(blockIdx.x * nw + wave) * A.slot_bytes;
     char *l2 = slot;
     char *l3 = slot + (size_t)A.cap * list_entry_bytes(MIXED ? kListMixed : kListPlain);
-    int cur = -1, cur_weighted = 0;
+    int cur = -1, cur_weighted = 0, cur_partner = -1, partner_weighted = 0;
     unsigned long long wpairs = 0ull;  // pairs this wave added in the current chunk
     __syncthreads();
     const int home = (int)(blockIdx.x % (unsigned)A.nseg);
@@ -1735,8 +1739,11 @@ k_triplets(KArgs A) {
         const bool full = counts_pairs(HM) && uniform((int)(s_pairs >= A.flush_pairs)) != 0;
         if (ch.term != cur || full) {
             if (cur >= 0) flush_hist<HM>(A, cur, cur_weighted, t.hist, &s_pairs);
+            if (cur_partner >= 0) flush_hist<HM>(A, cur_partner, partner_weighted, hist2, &s_pairs);
             cur = ch.term;
             cur_weighted = T.weighted;
+            cur_partner = T.partner;
+            partner_weighted = T.partner >= 0 ? A.terms[T.partner].weighted : 0;
         }
         for (;;) {
             // every lane takes part (lane 0 adds 1, the others 0): a branch on
@@ -1789,7 +1796,7 @@ k_triplets(KArgs A) {
                 if (lane == 0) atomicOr(A.err, kErrOverflow);
                 continue;
             }
-            if (n2u == 0 || n3u == 0) continue;
+            if (n3u == 0 || (n2u == 0 && T.partner < 0)) continue;
             wave_sync();
             const char *lb = T.sym ? l2 : l3;
             const CatRef c2 = {T.x2, T.y2, T.z2, T.w2, T.wunit2};
@@ -1805,7 +1812,9 @@ k_triplets(KArgs A) {
                 const unsigned long long a2 = (unsigned long long)n2u, a3 = (unsigned long long)n3u;
                 wpairs += T.tri == kTriSwap ? 2ull * a2 * a3 : T.sym ? a2 * (a2 - 1ull) : a2 * a3;
             }
-            if (T.sym) {
+            if (n2u == 0) {
+                // only the partner term has pairs (below)
+            } else if (T.sym) {
                 if (T.weighted) {
                     if constexpr (WT && primw_ok(HM)) {
                         if (T.wprim)
@@ -1840,6 +1849,27 @@ k_triplets(KArgs A) {
                     pair_loop<MIXED, HM, false, false>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2, c3, pr);
                 }
             }
+            if constexpr (primw_ok(HM)) {
+                // the partner term (c1, c3, c3) over the list of c3 just built,
+                // into the second histogram
+                if (T.partner >= 0 && n3u > 1) {
+                    wave_sync();
+                    const TermDesc &TB = A.terms[T.partner];
+                    LdsTabs t2 = t;
+                    t2.hist = hist2;
+                    if (uniform(TB.weighted)) {
+                        if constexpr (WT) {
+                            if (uniform(TB.wprim))
+                                pair_loop<MIXED, HM, true, true, kOrdSym, true>(A, t2, pw, l3, n3u, l3, n3u, wlds,
+                                                                                lane, c3, c3, pr);
+                            else
+                                pair_loop<MIXED, HM, true, true>(A, t2, pw, l3, n3u, l3, n3u, wlds, lane, c3, c3, pr);
+                        }
+                    } else {
+                        pair_loop<MIXED, HM, true, false>(A, t2, pw, l3, n3u, l3, n3u, wlds, lane, c3, c3, pr);
+                    }
+                }
+            }
             wave_sync();
         }
         if constexpr (counts_pairs(HM)) {
@@ -1849,6 +1879,7 @@ k_triplets(KArgs A) {
         __syncthreads();
     }
     if (cur >= 0) flush_hist<HM>(A, cur, cur_weighted, t.hist, &s_pairs);
+    if (cur_partner >= 0) flush_hist<HM>(A, cur_partner, partner_weighted, hist2, &s_pairs);
 }
 
 // ---------------------------------------------------------------------------

@@ -133,7 +133,8 @@ struct BatchPlan {
     unsigned long long *h_out = nullptr;
     unsigned *h_err = nullptr;
     std::vector<int> weighted;
-    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0, tri_terms = 0, swap_terms = 0;
+    int nb = 0, nkernel = 0, hist_layout = 0, rppi = 0, lds_bytes = 0, path = 0, tri_terms = 0, swap_terms = 0,
+        partner_terms = 0;
     int xk = kNoExact;  // exact layout: fixed-point scale exponent
     long long nbc = 0;  // output words per term (slots; twice as many u64 words in the exact layout)
 };
@@ -171,6 +172,7 @@ struct nc_context {
     // a (catalogues, params) key; NC_RPPI_STREAM=0 turns the stream off
     int rppi_stream = -1;  // rppi key stream: 1 on, 0 off (direct atomics), -1 box on, survey off
     bool batch_split = true;  // unit-weight terms of a mixed-weight batch as their own launch
+    bool fuse_terms = true;   // (c1, c3, c3) counted beside (c1, c2, c3) from the same lists (TermDesc.partner)
     bool tri = true;          // NC_TRI=0: terms over one catalogue never use the six-fold symmetry
     int accum = NC_ACCUM_FP64;  // nc_set_accumulation
     long long fail_alloc = -1;  // NC_FAIL_DEVICE_ALLOC (test knob; see dgrow_checked)

@@ -57,6 +57,10 @@ struct TermDesc {
     // ordered triplet adds w1, so the pair loops stage no weights
     int wprim;
     int wunit2, wunit3;  // c2 / c3 has unit weights (its weights are not read)
+    // partner: a term (c1, c3, c3) counted with this one (c1, c2, c3 != c2)
+    // from the same primaries and the same list of c3, into the workgroup's
+    // second histogram (-1: none); a partner term has no work items of its own
+    int partner;
     // kTriSix: c1 == c2 == c3, exactly gridded: every unordered triple of
     // points outside the last cells is evaluated once, from its lowest-index
     // point, and counted six times (see k_triplets).
@@ -148,6 +152,7 @@ struct KArgs {
     const uint16_t *t3;           // nb*nb*(nb+2): (ka*nb+kb)*(nb+2)+k+1 -> byte offset of the compact slot
     // dynamic LDS layout (bytes)
     int off_sq, off_lut, off_ctab, off_lut32, off_t3, off_wave, wave_bytes, lds_bytes;
+    int off_hist2;  // byte offset of the partner terms' histogram (0: none)
     // rppi key stream (k_rppi with STREAM): binned triplets leave as compact
     // keys c = rank(T)*npi^2 + rem, grouped by bucket c >> shift into
     // per-bucket regions, then k_rppi_bin adds each bucket in LDS.

@@ -98,6 +98,9 @@ typedef struct {
     int32_t swap_terms;  /* survey terms whose first two catalogues are one (c1 == c2 != c3, DDR): the
                             triplets (p1, p2, p3) and (p2, p1, p3) were evaluated once, from the lower
                             index, and counted twice (the same conditions as tri_terms) */
+    int32_t partner_terms; /* terms (c1, c3, c3) counted beside a term (c1, c2, c3), c2 != c3, of the same
+                            call from its primaries and its lists of c3 (DRR beside DDR), into a second
+                            workgroup histogram; sorted-slot layouts, NC_FUSE_TERMS=0 turns it off */
 } nc_timing;
 
 /* Create a context on HIP device `device` (replaces the implicit CUDA context
