@@ -44,6 +44,7 @@ int nc_create(nc_context **out, int device) {
     if (const char *e = std::getenv("NC_BATCH_SPLIT")) ctx->batch_split = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_FUSE_TERMS")) ctx->fuse_terms = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_TRI")) ctx->tri = std::strcmp(e, "0") != 0;
+    if (const char *e = std::getenv("NC_TRI_LAST")) ctx->tri_last = std::strcmp(e, "0") != 0;
     if (const char *e = std::getenv("NC_FAIL_DEVICE_ALLOC")) ctx->fail_alloc = std::strtoll(e, nullptr, 10);
     if (const char *e = std::getenv("NC_DENSE_FLUSH_PAIRS")) {
         const unsigned long long v = std::strtoull(e, nullptr, 10);

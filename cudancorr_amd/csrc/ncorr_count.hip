@@ -1240,6 +1240,7 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     A.off_t3 = (int)(hist_bytes + off.t3);
     A.off_wave = (int)lds_tab;
     A.off_hist2 = lay.nhist == 2 ? (int)lay.hist(hm) : 0;
+    A.tri_last6 = ctx->tri_last && !G.survey && G.nc[0] >= 3 && G.nc[1] >= 3 && G.nc[2] >= 3;
     A.wave_bytes = (int)wave_bytes;
     A.lds_bytes = (int)L.lds_bytes;
     NC_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));

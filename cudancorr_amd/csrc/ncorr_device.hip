@@ -447,7 +447,21 @@ __device__ __forceinline__ MList mlist(const char *base, int cap) {
 struct PrimRef {
     int cx, cy, cz;
     double px, py, pz;
+    unsigned lastmask;  // bit e: neighbour entry e of the primary is a last cell (box)
 };
+
+// Neighbour entries e = (dx+1)*9 + (dy+1)*3 + (dz+1) of a primary's cell
+// whose cells are adjacent to the cell of entry s (every axis offset within
+// one): as a 27-bit mask.  With at least three cells per axis, two entries
+// whose offsets differ by two on an axis are images on either side of the
+// primary's cell.
+__device__ __forceinline__ unsigned entry_adjacent(int s) {
+    const int ix = s / 9, iy = (s / 3) % 3, iz = s % 3;
+    auto axis = [](int i) { return i == 0 ? 3u : i == 1 ? 7u : 6u; };
+    const unsigned mz = axis(iz), my = axis(iy), mx = axis(ix);
+    const unsigned yz = ((my & 1u) ? mz : 0u) | ((my & 2u) ? mz << 3 : 0u) | ((my & 4u) ? mz << 6 : 0u);
+    return ((mx & 1u) ? yz : 0u) | ((mx & 2u) ? yz << 9 : 0u) | ((mx & 4u) ? yz << 18 : 0u);
+}
 
 // Catalogue arrays behind a mixed list (for weights and exact rechecks).
 struct CatRef {
@@ -509,7 +523,8 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
                                           double px, double py, double pz, int cx, int cy, int cz,
                                           char *__restrict__ base,
                                           int lane, const double *sq, const uint16_t *lut, int pidx = 0,
-                                          char *__restrict__ base_b = nullptr, int *nb_out = nullptr, int first = 0) {
+                                          char *__restrict__ base_b = nullptr, int *nb_out = nullptr, int first = 0,
+                                          unsigned *lastmask_out = nullptr) {
     constexpr bool MIXED = LM == kListMixed;
     int n = SEL == kSelLow ? first : 0;
     int nbl = 0;
@@ -553,7 +568,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             e.x = qx;
             e.y = qy;
             e.z = qz;
-            e.k = k;
+            e.k = k | (s << 8);  // exact bin | neighbour entry
             e.f = f;
             reinterpret_cast<Elem *>(b)[pos] = e;
             reinterpret_cast<double *>(b + (size_t)A.cap * sizeof(Elem))[pos] = qw;
@@ -694,6 +709,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
     // q + 0.0 would be q in every later use)
     const bool any_shift = __ballot((cpk & 63) != 0) != 0ull;
     const bool any_flag = __ballot(cpk != 0) != 0ull;
+    if (lastmask_out) *lastmask_out = (unsigned)__ballot((cpk & 64) != 0);
     // Candidates v of [0, total), two 64-lane blocks per step with all their
     // loads in flight; a lane finds its entry by a binary search over the
     // offsets of lanes 0..31.
@@ -1063,7 +1079,13 @@ __device__ __forceinline__ void traverse(Pol &P, int n2, int n3) {
 // FP32 triangle test with the hardware sqrt (<= 1 ulp) -- redone with the
 // correctly rounded sqrtf when the slack is within 8 ulp of the largest side
 // -- and the histogram updates.
-template <bool SYM, int ORD, bool WEIGHTED, int HM>
+// LASTP: the two-fold loop of a last-cell primary when one-catalogue terms
+// count their last-cell triples six-fold (KArgs.tri_last6): a pair of list
+// entries whose cells are adjacent counts only if both cells are last cells
+// (the other triples it closes are counted six-fold by a primary outside the
+// last cells).  Rows carry their neighbour entry in the top bits of the
+// staged bin; lanes hold the mask of row entries they skip.
+template <bool SYM, int ORD, bool WEIGHTED, int HM, bool LASTP = false>
 struct PlainPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 2;
@@ -1079,10 +1101,11 @@ struct PlainPol {
     double wb, w1b;
     float wbf;  // the lane's weight in FP32 (kHistF32: FP32 weights)
     bool vb;
+    unsigned lastmask, skipb;  // LASTP
 
     __device__ PlainPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2, const char *l3, char *wlds,
-                        int lane_)
-        : A(A_), t(t_), w1(w1_), lane(lane_) {
+                        int lane_, unsigned lastmask_)
+        : A(A_), t(t_), w1(w1_), lane(lane_), lastmask(lastmask_) {
         wrow = w1 * (double)ord_count(ORD);
         C = A.chunk_elems;
         L2 = reinterpret_cast<const Elem *>(l2);
@@ -1103,7 +1126,7 @@ struct PlainPol {
             double w = 0.0;
             if (i < last - first) {
                 e = L[first + i];
-                e.k = row_bin<HM>(A, e.k);
+                e.k = row_bin<HM>(A, e.k & 0xff) | (LASTP ? (e.k >> 8) << 24 : 0);
                 if (WEIGHTED) w = pre_weighted<WEIGHTED, HM>() ? wrow * W[first + i] : W[first + i];
                 if (is_f32(HM)) w = f32_pair(w);
             } else {  // padding: far away, never in range
@@ -1124,7 +1147,11 @@ struct PlainPol {
         wb = 0.0;
         if (vb) {
             eb = (from3 ? L3 : L2)[b];
-            eb.k = lane_bin<HM>(A, eb.k);
+            if constexpr (LASTP) {
+                const int sb = eb.k >> 8;
+                skipb = entry_adjacent(sb) & ~(((lastmask >> sb) & 1u) ? lastmask : 0u);
+            }
+            eb.k = lane_bin<HM>(A, eb.k & 0xff);
             if (WEIGHTED) wb = (from3 ? W3 : W2)[b];
         } else {
             eb.x = eb.y = eb.z = -1e150;
@@ -1143,11 +1170,17 @@ struct PlainPol {
         const float near_rel = 8.0f * 1.1920929e-7f;
         double d[U];
         bool inr[U];
-        int k[U];
+        int k[U], kra[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
+            bool live = ok[u];
+            kra[u] = ea[u].k;
+            if constexpr (LASTP) {
+                live = live & (((skipb >> ((unsigned)kra[u] >> 24)) & 1u) == 0u);
+                kra[u] &= 0xffffff;
+            }
             d[u] = sqrsep(ea[u].x - eb.x, ea[u].y - eb.y, ea[u].z - eb.z);
-            inr[u] = ok[u] & (d[u] > r2min) & (d[u] < r2max);
+            inr[u] = live & (d[u] > r2min) & (d[u] < r2max);
             const double dj = inr[u] ? d[u] : 0.0;
             k[u] = t.lut[(int)(dj * scale)];
         }
@@ -1175,14 +1208,14 @@ struct PlainPol {
         if constexpr (WEIGHTED && is_f32(HM)) {
             int slot[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) slot[u] = acc[u] ? pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]) : 0;
+            for (int u = 0; u < U; ++u) slot[u] = acc[u] ? pair_slot<HM>(A, t, kra[u], eb.k, k[u]) : 0;
             hist_add_f32<U>(t, slot, wa, wbf, acc);
             return;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (acc[u]) {
-                const int slot = pair_slot<HM>(A, t, ea[u].k, eb.k, k[u]);
+                const int slot = pair_slot<HM>(A, t, kra[u], eb.k, k[u]);
                 if constexpr (pre_weighted<WEIGHTED, HM>())
                     hist_add_value(t, slot, wa[u] * wb);
                 else if (SWAP)
@@ -1288,7 +1321,8 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
 // PRIMW (with WEIGHTED false): a weighted term whose secondaries carry unit
 // weights; every ordered triplet adds (w1 * 1) * 1 = w1, so an accepted pair
 // adds the wave-uniform ord * w1 and no weights are staged or loaded.
-template <bool SYM, int ORD, bool WEIGHTED, int HM, bool PRIMW = false>
+// LASTP: as PlainPol's.
+template <bool SYM, int ORD, bool WEIGHTED, int HM, bool PRIMW = false, bool LASTP = false>
 struct MixedPol {
     static constexpr bool kSym = SYM;
     static constexpr int U = 4;
@@ -1317,6 +1351,7 @@ struct MixedPol {
     double wrow;  // the staged rows' weight factor (pre_weighted; PRIMW: the value of an accepted pair)
     float wbf;    // the lane's weight in FP32 (kHistF32: FP32 weights)
     bool vb;
+    unsigned skipb;  // LASTP: row entries this lane skips
 
     __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
                         int lane_, const CatRef &cat2, const CatRef &cat3, const PrimRef &pr_)
@@ -1353,7 +1388,8 @@ struct MixedPol {
             double w = 0.0;
             if (i < last - first) {
                 r = m.R[first + i];
-                k = row_bin<HM>(A, m.K[first + i] & 0xff);
+                const int kk = m.K[first + i];
+                k = row_bin<HM>(A, kk & 0xff) | (LASTP ? (kk >> 8) << 24 : 0);
                 if (WEIGHTED) w = staged_weight<WEIGHTED, HM>(wrow, c.unit ? 1.0 : gptr(c.w)[m.I[first + i]]);
             }
             la[i] = r;
@@ -1373,7 +1409,12 @@ struct MixedPol {
         wb = 0.0;
         if (vb) {
             rb = m.R[bidx];
-            kb = lane_bin<HM>(A, m.K[bidx] & 0xff);
+            const int kk = m.K[bidx];
+            kb = lane_bin<HM>(A, kk & 0xff);
+            if constexpr (LASTP) {
+                const int sb = kk >> 8;
+                skipb = entry_adjacent(sb) & ~(((pr.lastmask >> sb) & 1u) ? pr.lastmask : 0u);
+            }
             const CatRef &c = from3 ? c3 : c2;
             if (WEIGHTED) wb = c.unit ? 1.0 : gptr(c.w)[m.I[bidx]];
         }
@@ -1420,6 +1461,12 @@ struct MixedPol {
             // branches below, so that no read waits inside them
             int kr = ka[u];
             asm volatile("" : "+v"(kr));
+            if constexpr (LASTP) {
+                const bool live = ((skipb >> ((unsigned)kr >> 24)) & 1u) == 0u;
+                amb[u] = amb[u] & live;
+                acc[u] = acc[u] & live;
+                kr &= 0xffffff;
+            }
             slot[u] = pair_slot_x<HM>(A, t, kr, kb, kx);
         }
         static_assert(!PRIMW || (!WEIGHTED && (HM == kHistT3 || HM == kHistCtab)), "FP64 slot sums");
@@ -1565,16 +1612,17 @@ __device__ __forceinline__ void flush_hist(const KArgs &A, int term, int weighte
 // a symmetric traversal (p2 and p3 from one list) and kOrdOne otherwise.
 // PRIMW: a weighted term with unit-weight secondaries (MixedPol; the plain
 // path takes it as any weighted term).
-template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? kOrdSym : kOrdOne, bool PRIMW = false>
+template <bool MIXED, int HM, bool SYM, bool WEIGHTED, int ORD = SYM ? kOrdSym : kOrdOne, bool PRIMW = false,
+          bool LASTP = false>
 __device__ __forceinline__ void pair_loop(const KArgs &A, const LdsTabs &t, double w1, const char *l2, int n2,
                                           const char *l3, int n3, char *wlds, int lane, const CatRef &c2,
                                           const CatRef &c3, const PrimRef &pr) {
     if constexpr (MIXED) {
-        MixedPol<SYM, ORD, WEIGHTED && !PRIMW, HM, PRIMW> P(A, t, w1, l2, l3, wlds, lane, c2, c3, pr);
+        MixedPol<SYM, ORD, WEIGHTED && !PRIMW, HM, PRIMW, LASTP> P(A, t, w1, l2, l3, wlds, lane, c2, c3, pr);
         traverse(P, n2, n3);
         if (P.qn > 0) P.drain();
     } else {
-        PlainPol<SYM, ORD, WEIGHTED, HM> P(A, t, w1, l2, l3, wlds, lane);
+        PlainPol<SYM, ORD, WEIGHTED, HM, LASTP> P(A, t, w1, l2, l3, wlds, lane, pr.lastmask);
         traverse(P, n2, n3);
     }
 }
@@ -1615,6 +1663,26 @@ __device__ __forceinline__ bool tri_primary(const KArgs &A, const LdsTabs &t, co
     const CatRef c = {T.x2, T.y2, T.z2, T.w2, T.wunit2};
     const PrimRef pr = {cx, cy, cz, px, py, pz};
     if (nh > 1) pair_loop<MIXED, HM, true, WEIGHTED, kOrdSix>(A, t, pw, l2, nh, l2, nh, wlds, lane, c, c, pr);
+    if (A.tri_last6) {
+        // Triples with last-cell points, six-fold too: every triple of this
+        // primary's lists that has its sides in range is closed under the
+        // reference's neighbour rule (A's cell is a full cell on every axis,
+        // so entries on either side of it are at least rmax apart), and it is
+        // counted here when this primary is its lowest-index point outside
+        // the last cells: pairs of the higher entries of list 2 with the
+        // last-cell entries, and pairs of last-cell entries.  Last-cell
+        // primaries skip such triples (LASTP).
+        if (nbu > 0) {
+            if (nh > 0)
+                pair_loop<MIXED, HM, false, WEIGHTED, kOrdSix>(A, t, pw, l2, nh, l3, nbu, wlds, lane, c, c, pr);
+            if (nbu > 1) pair_loop<MIXED, HM, true, WEIGHTED, kOrdSix>(A, t, pw, l3, nbu, l3, nbu, wlds, lane, c, c, pr);
+        }
+        if constexpr (counts_pairs(HM)) {
+            const unsigned long long h = (unsigned long long)nh, b = (unsigned long long)nbu;
+            wpairs += 3ull * h * (h > 0 ? h - 1 : 0) + 6ull * h * b + 3ull * b * (b > 0 ? b - 1 : 0);
+        }
+        return true;
+    }
     int nrow = nh;
     if (nbu > 0) {
         nrow = uniform(build_list<LM, kSelLow, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
@@ -1778,13 +1846,15 @@ k_triplets(KArgs A) {
             // index above the primary's
             constexpr int LM = MIXED ? kListMixed : kListPlain;
             int n2;
+            unsigned lastmask = 0u;
             if (T.tri == kTriSwap) {
                 int nb_none = 0;
                 n2 = build_list<LM, kSelSplit, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx, cy, cz, l2, lane, t.sq,
                                                t.lut, p, l3, &nb_none);
             } else {
                 n2 = build_list<LM, kSelAll, lut32_units(HM)>(A, T.x2, T.y2, T.z2, T.w2, T.st2, T.prune2, px, py, pz, cx,
-                                                              cy, cz, l2, lane, t.sq, t.lut);
+                                                              cy, cz, l2, lane, t.sq, t.lut, 0, nullptr, nullptr, 0,
+                                                              &lastmask);
             }
             const int n3 =
                 T.sym ? n2
@@ -1801,7 +1871,9 @@ k_triplets(KArgs A) {
             const char *lb = T.sym ? l2 : l3;
             const CatRef c2 = {T.x2, T.y2, T.z2, T.w2, T.wunit2};
             const CatRef c3 = T.sym ? c2 : CatRef{T.x3, T.y3, T.z3, T.w3, T.wunit3};
-            const PrimRef pr = {cx, cy, cz, px, py, pz};
+            const PrimRef pr = {cx, cy, cz, px, py, pz, (unsigned)uniform((int)lastmask)};
+            // a last-cell primary of a six-fold term (its other triples are counted six-fold)
+            const bool lastp = T.tri == kTriSix && A.tri_last6;
             if constexpr (!WT) {
                 if (T.weighted) {  // never planned so
                     if (lane == 0) atomicOr(A.err, kErrMode);
@@ -1814,6 +1886,15 @@ k_triplets(KArgs A) {
             }
             if (n2u == 0) {
                 // only the partner term has pairs (below)
+            } else if (lastp) {
+                if (T.weighted) {
+                    if constexpr (WT)
+                        pair_loop<MIXED, HM, true, true, kOrdSym, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane,
+                                                                               c2, c3, pr);
+                } else {
+                    pair_loop<MIXED, HM, true, false, kOrdSym, false, true>(A, t, pw, l2, n2u, lb, n3u, wlds, lane, c2,
+                                                                            c3, pr);
+                }
             } else if (T.sym) {
                 if (T.weighted) {
                     if constexpr (WT && primw_ok(HM)) {

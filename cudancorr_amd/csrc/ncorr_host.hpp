@@ -174,6 +174,7 @@ struct nc_context {
     bool batch_split = true;  // unit-weight terms of a mixed-weight batch as their own launch
     bool fuse_terms = true;   // (c1, c3, c3) counted beside (c1, c2, c3) from the same lists (TermDesc.partner)
     bool tri = true;          // NC_TRI=0: terms over one catalogue never use the six-fold symmetry
+    bool tri_last = true;     // NC_TRI_LAST=0: their last-cell triples stay two-fold
     int accum = NC_ACCUM_FP64;  // nc_set_accumulation
     long long fail_alloc = -1;  // NC_FAIL_DEVICE_ALLOC (test knob; see dgrow_checked)
     double rs_scale = 1.0;       // NC_RPPI_REGION_SCALE: region capacities x this (tests of the overflow path)

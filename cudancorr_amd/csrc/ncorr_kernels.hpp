@@ -153,6 +153,9 @@ struct KArgs {
     // dynamic LDS layout (bytes)
     int off_sq, off_lut, off_ctab, off_lut32, off_t3, off_wave, wave_bytes, lds_bytes;
     int off_hist2;  // byte offset of the partner terms' histogram (0: none)
+    // one-catalogue terms count their last-cell triples six-fold too (box,
+    // at least three cells per axis); last-cell primaries skip those triples
+    int tri_last6;
     // rppi key stream (k_rppi with STREAM): binned triplets leave as compact
     // keys c = rank(T)*npi^2 + rem, grouped by bucket c >> shift into
     // per-bucket regions, then k_rppi_bin adds each bucket in LDS.
