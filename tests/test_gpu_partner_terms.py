@@ -4,7 +4,8 @@ primaries and its lists of c3, into a second workgroup histogram.  Checked
 against the oracle on both paths (counts bit for bit, weighted sums within
 1e-9), against a context with the pairing turned off (NC_FUSE_TERMS=0), with
 primaries whose list of c2 is empty (their partner pairs must still count),
-over three catalogues, and under graph replay."""
+over three catalogues, in a periodic box with partial last cells, and under
+graph replay."""
 import os
 
 import numpy as np
@@ -143,3 +144,29 @@ def test_paired_equals_unpaired_and_replays(ctx, cats):
         for x in (D, R, D2, R2):
             x.close()
         off.close()
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_partner_in_a_box_with_partial_last_cells(ctx, path):
+    """(D, D, R) with (D, R, R) in a periodic box of 4.7 cells per axis: the
+    partner's two-fold loop over the first term's lists of R, last-cell
+    primaries included."""
+    _log(f"partner box path={path}")
+    from cudancorr_amd.gen import neyman_scott_box
+
+    L = 47.0
+    d = neyman_scott_box(40, 25, 2.0, L, seed=95)
+    d[:, 3] = np.random.default_rng(96).lognormal(0.0, 0.3, d.shape[0]).astype(np.float32)
+    r = np.ones((3000, 4))
+    r[:, :3] = np.random.default_rng(98).uniform(0.0, L, size=(3000, 3)).astype(np.float32)
+    edges = edges_linear(0.0, 10.0, 8)
+    box = np.array([L, L, L])
+    D, R = ctx.catalog(d), ctx.catalog(r)
+    try:
+        h, _ = ctx.count_batched([(D, D, R), (D, R, R)], edges, box, path=path)
+        assert ctx.timing().partner_terms == 1
+        _wcheck(h[0], oracle.count(d, d, r, edges, box)[1])
+        _wcheck(h[1], oracle.count(d, r, r, edges, box)[1])
+    finally:
+        D.close()
+        R.close()
