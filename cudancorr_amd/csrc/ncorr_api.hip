@@ -406,7 +406,7 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
         rc = NC_OK;
     }
     for (int w : {4, 8, 16})
-        for (int c : {16, 64, 256})
+        for (int c : {16, 64})
             for (int lc : {64, 128, 256}) {
                 if (w == 16 && !big_lds) continue;
                 ctx->waves_per_group = w;

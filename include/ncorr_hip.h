@@ -256,7 +256,7 @@ int nc_set_primary_slice(nc_context *ctx, int slice, int nslices);
 /* Autotuner (no counterpart in the reference): times the triplet kernel (the
  * better of two runs) on a strided sample of the work items of these terms,
  * sized by a first run to about 80 ms, for every candidate of
- * waves_per_group {4, 8} (and 16 when 8 waves need more than half the CU's LDS) x chunk {16, 64, 256} x list_chunk {64, 128, 256} (wgs_per_cu = as many as fit;
+ * waves_per_group {4, 8} (and 16 when 8 waves need more than half the CU's LDS) x chunk {16, 64} x list_chunk {64, 128, 256} (wgs_per_cu = as many as fit;
  * shapes whose LDS does not fit are skipped), keeps the fastest in ctx and returns it in *best (ms =
  * device time of the sample).  With the environment variable NC_TUNE_CACHE
  * naming a file, results are looked up in and appended to that file, keyed by
