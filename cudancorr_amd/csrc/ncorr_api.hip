@@ -310,6 +310,7 @@ int nc_set_tuning(nc_context *ctx, int waves_per_group, int chunk, int wgs_per_c
     }
     if (chunk > 0) ctx->chunk = chunk;
     if (wgs_per_cu >= 0) ctx->wgs_per_cu = wgs_per_cu;
+    ctx->tune_unit.on = false;  // one tuning for every launch again
     return NC_OK;
 }
 
@@ -338,17 +339,15 @@ int nc_get_tuning(const nc_context *ctx, int *waves_per_group, int *chunk, int *
     return NC_OK;
 }
 
-int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1, nc_catalog *const *c2,
-                nc_catalog *const *c3, int path, nc_tuning *best) {
-    if (!ctx) return NC_ERR_ARG;
-    if (nterms < 1 || !c1 || !c2 || !c3 || !params || !best) return fail(ctx, NC_ERR_ARG, "bad autotune arguments");
-    std::vector<Term> t(nterms);
+// The autotuner over one set of terms (nc_autotune), leaving the best shape
+// in ctx.
+static int tune_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path,
+                      nc_tuning *best) {
+    const int nterms = (int)t.size();
     int64_t npts = 0, nprim = 0;
-    for (int i = 0; i < nterms; ++i) {
-        t[i] = {c1[i], c2[i], c3[i]};
-        if (!c1[i] || !c2[i] || !c3[i]) return fail(ctx, NC_ERR_ARG, "null catalogue");
-        npts += c1[i]->n + c2[i]->n + c3[i]->n;
-        nprim += c1[i]->n;
+    for (const Term &x : t) {
+        npts += x.c1->n + x.c2->n + x.c3->n;
+        nprim += x.c1->n;
     }
     // cache key: device, path, survey, bins, sizes
     char key[256];
@@ -378,15 +377,17 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
     }
     const int saved[4] = {ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
     std::vector<double> hist((size_t)nterms * nc_hist_len(params));
-    // sample: about 64 work items per workgroup slot of the chip at the largest chunk
-    const long long items = std::max<long long>(1, nprim / 16);
-    ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2 * 64));
+    // sample: to start with, about two runs of kSampleRun primaries per CU
+    const long long items = std::max<long long>(1, nprim / kSampleRun);  // runs of kSampleRun primaries
+    ctx->sample_stride = (int)std::max<long long>(1, items / ((long long)ctx->ncu * 2));
     ctx->autotuning = true;
     ctx->tune_list_bound.clear();
     const int saved_slice[2] = {ctx->slice, ctx->nslices};  // the sample is drawn from all primaries
     ctx->slice = 0;
     ctx->nslices = 1;
     nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
+    // the best shape for each work-item size, for the second stage
+    nc_tuning best_c[2] = {{0, 0, 0, 0, 1e30f, 0}, {0, 0, 0, 0, 1e30f, 0}};
     int rc = NC_OK;
     // 16-wave workgroups only where one workgroup per CU is forced anyway
     // (more than half the CU's LDS); elsewhere two 8-wave ones do better
@@ -428,8 +429,39 @@ int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog
                     goto done;
                 }
                 if (ms < b.ms) b = {w, c, 0, lc, ms, 0};
+                nc_tuning &bc = best_c[c == 16 ? 0 : 1];
+                if (ms < bc.ms) bc = {w, c, 0, lc, ms, 0};
             }
     if (b.ms < 1e30f) rc = NC_OK;  // at least one shape ran
+    // Second stage: where the sample is a small part of the count, runs of
+    // primaries scattered over the catalogues need not rank the work-item
+    // sizes as a count over consecutive cells does, so the best shape of each
+    // item size is timed again on one contiguous primary slice of eight times
+    // the sample's size (the whole count where that is less).
+    if (rc == NC_OK && ctx->sample_stride > 1 && best_c[0].ms < 1e30f && best_c[1].ms < 1e30f) {
+        const int ns = std::max(1, ctx->sample_stride / 8);
+        ctx->sample_stride = 1;
+        ctx->slice = ns / 2;
+        ctx->nslices = ns;
+        nc_tuning b2 = {0, 0, 0, 0, 1e30f, 0};
+        for (const nc_tuning &cand : best_c) {
+            ctx->waves_per_group = cand.waves_per_group;
+            ctx->chunk = cand.chunk;
+            ctx->wgs_per_cu = 0;
+            ctx->chunk_elems = cand.list_chunk;
+            float ms = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                const int r = count_terms(ctx, params, t, path, hist.data(), nullptr, false);
+                if (r != NC_OK) {
+                    rc = r;
+                    goto done;
+                }
+                ms = std::min(ms, ctx->timing.ms_count);
+            }
+            if (ms < b2.ms) b2 = {cand.waves_per_group, cand.chunk, 0, cand.list_chunk, ms, 0};
+        }
+        b = b2;
+    }
 done:
     ctx->sample_stride = 1;
     ctx->autotuning = false;
@@ -455,6 +487,39 @@ done:
                          b.ms);
             std::fclose(f);
         }
+    return NC_OK;
+}
+
+int nc_autotune(nc_context *ctx, const nc_params *params, int nterms, nc_catalog *const *c1, nc_catalog *const *c2,
+                nc_catalog *const *c3, int path, nc_tuning *best) {
+    if (!ctx) return NC_ERR_ARG;
+    if (nterms < 1 || !c1 || !c2 || !c3 || !params || !best) return fail(ctx, NC_ERR_ARG, "bad autotune arguments");
+    std::vector<Term> t(nterms);
+    for (int i = 0; i < nterms; ++i) {
+        t[i] = {c1[i], c2[i], c3[i]};
+        if (!c1[i] || !c2[i] || !c3[i]) return fail(ctx, NC_ERR_ARG, "null catalogue");
+    }
+    Mem m;
+    m.fail = &ctx->fail_alloc;
+    if (const int rc = refresh_weights(ctx, m, t)) return rc;
+    ctx->tune_unit.on = false;
+    std::vector<int> iu, iw;
+    if (!split_batch(ctx, params, t, iu, iw)) return tune_terms(ctx, params, t, path, best);
+    // the unit-weight and the weighted launch of a split batch, each tuned on its own
+    std::vector<Term> tu, tw;
+    for (int i : iu) tu.push_back(t[i]);
+    for (int i : iw) tw.push_back(t[i]);
+    nc_tuning bu, bw;
+    if (const int rc = tune_terms(ctx, params, tu, path, &bu)) return rc;
+    if (const int rc = tune_terms(ctx, params, tw, path, &bw)) return rc;
+    ctx->tune_unit.on = true;
+    ctx->tune_unit.waves_per_group = bu.waves_per_group;
+    ctx->tune_unit.chunk = bu.chunk;
+    ctx->tune_unit.wgs_per_cu = bu.wgs_per_cu;
+    ctx->tune_unit.list_chunk = bu.list_chunk;
+    *best = bw;
+    best->ms = bu.ms + bw.ms;
+    best->from_cache = bu.from_cache && bw.from_cache;
     return NC_OK;
 }
 

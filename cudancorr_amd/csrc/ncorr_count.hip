@@ -1124,7 +1124,6 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
     std::vector<TermDesc> td(nterms);
     std::vector<Chunk> chunks;
     std::vector<int> weighted(nterms);
-    long long nitems = 0;
     for (int i = 0; i < nterms; ++i) {
         const Term &t = terms[i];
         TermDesc &d = td[i];
@@ -1138,8 +1137,8 @@ int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &term
         d.prune3 = prunes(G, t.c3);
         int p0, p1;  // the primary slice: positions [p0, p1) of the cell-sorted catalogue
         slice_range(ctx, d.n1, p0, p1);
-        for (int b = p0; b < p1; b += qchunk, ++nitems)
-            if ((nitems / kSampleRun) % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(p1, b + qchunk), 0});
+        for (int b = p0; b < p1; b += qchunk)
+            if (((b - p0) / kSampleRun) % ctx->sample_stride == 0) chunks.push_back({i, b, std::min(p1, b + qchunk), 0});
     }
     NC_MEM(ctx, m, bufs.terms, sizeof(TermDesc) * td.size());
     NC_MEM(ctx, m, bufs.chunks, sizeof(Chunk) * std::max<size_t>(1, chunks.size()));
@@ -1380,21 +1379,32 @@ int refresh_weights(nc_context *ctx, Mem &m, const std::vector<Term> &terms) {
 // both paths give every primary the same share of the triplets, and the
 // primary slices of the two paths agree).  Those two launches are not
 // captured in a graph.
+bool split_batch(const nc_context *ctx, const nc_params *params, const std::vector<Term> &t, std::vector<int> &iu,
+                 std::vector<int> &iw) {
+    iu.clear();
+    iw.clear();
+    if (!ctx->batch_split || !params || params->bin_type != NC_BIN_THREED || params->nedges < 2 ||
+        params->nedges > 41)
+        return false;
+    for (size_t i = 0; i < t.size(); ++i) {
+        if (!t[i].c1 || !t[i].c2 || !t[i].c3) return false;
+        (t[i].c1->unit_w && t[i].c2->unit_w && t[i].c3->unit_w ? iu : iw).push_back((int)i);
+    }
+    const size_t nb = (size_t)params->nedges - 1;
+    return !iu.empty() && !iw.empty() && nb * nb * nb * 4 + 48 * 1024 <= ctx->lds_hw;
+}
+
 int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
                 uint64_t *counts, bool use_graph) {
     const int nterms = (int)t.size();
     Mem m;  // the context's own buffers
     m.fail = &ctx->fail_alloc;
     if (const int rc = refresh_weights(ctx, m, t)) return rc;
-    if (ctx->batch_split && params && params->bin_type == NC_BIN_THREED &&
-        params->nedges >= 2 && params->nedges <= 41) {
+    for (const Term &x : t)
+        if (!x.c1 || !x.c2 || !x.c3) return fail(ctx, NC_ERR_ARG, "null catalogue");
+    {
         std::vector<int> iu, iw;
-        for (int i = 0; i < nterms; ++i) {
-            if (!t[i].c1 || !t[i].c2 || !t[i].c3) return fail(ctx, NC_ERR_ARG, "null catalogue");
-            (t[i].c1->unit_w && t[i].c2->unit_w && t[i].c3->unit_w ? iu : iw).push_back(i);
-        }
-        const size_t nb = (size_t)params->nedges - 1;
-        if (!iu.empty() && !iw.empty() && nb * nb * nb * 4 + 48 * 1024 <= ctx->lds_hw) {
+        if (split_batch(ctx, params, t, iu, iw)) {
             if (!hist) return fail(ctx, NC_ERR_ARG, "null params or output");
             const size_t len = (size_t)nc_hist_len(params);
             std::vector<double> h[2];
@@ -1406,8 +1416,23 @@ int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term
                 for (int i : idx) sub.push_back(t[i]);
                 h[g].resize(idx.size() * len);
                 if (counts) c[g].resize(idx.size() * len);
+                // the unit-weight launch with its own tuning, where nc_autotune found one
+                const int saved[4] = {ctx->waves_per_group, ctx->chunk, ctx->wgs_per_cu, ctx->chunk_elems};
+                const bool own = g == 0 && ctx->tune_unit.on;
+                if (own) {
+                    ctx->waves_per_group = ctx->tune_unit.waves_per_group;
+                    ctx->chunk = ctx->tune_unit.chunk;
+                    ctx->wgs_per_cu = ctx->tune_unit.wgs_per_cu;
+                    ctx->chunk_elems = ctx->tune_unit.list_chunk;
+                }
                 const int rc =
                     run_count(ctx, params, sub, path, h[g].data(), counts ? c[g].data() : nullptr, false, m);
+                if (own) {
+                    ctx->waves_per_group = saved[0];
+                    ctx->chunk = saved[1];
+                    ctx->wgs_per_cu = saved[2];
+                    ctx->chunk_elems = saved[3];
+                }
                 if (rc) return rc;
                 tm[g] = ctx->timing;
                 for (size_t k = 0; k < idx.size(); ++k) {

@@ -102,9 +102,11 @@ using nch::CountBufs;
 using nch::DBuf;
 using nch::kNoExact;
 
-// The autotuner times contiguous runs of work items, so that the workgroups
-// running at one time share neighbourhoods (in L2) as in a full count.
-constexpr long long kSampleRun = 64;
+// The autotuner times contiguous runs of kSampleRun primaries (whole work
+// items from the run's first primary on), so that the workgroups running at
+// one time share neighbourhoods (in L2) as in a full count, and so that every
+// shape, whatever its work-item size, times the same primaries.
+constexpr long long kSampleRun = 1024;
 
 struct nc_catalog {
     nc_context *ctx = nullptr;  // null once the context is destroyed
@@ -161,9 +163,15 @@ struct nc_context {
     // dense counts are merged once this many pairs were added (< 2^31 keeps
     // u32 counts from wrapping; NC_DENSE_FLUSH_PAIRS lowers it for tests)
     unsigned long long dense_flush = 1ull << 31;
-    int sample_stride = 1;    // autotuner: queue only every n-th run of kSampleRun work items
+    int sample_stride = 1;    // autotuner: queue only the work items of every n-th run of kSampleRun primaries
     int slice = 0, nslices = 1;  // nc_set_primary_slice: primaries of slice `slice` of `nslices` only
     bool autotuning = false;  // in nc_autotune's timed runs
+    // the tuning of the unit-weight launch of a split batch (count_terms), when
+    // nc_autotune timed the two launches apart; nc_set_tuning clears it
+    struct {
+        bool on = false;
+        int waves_per_group = 0, chunk = 0, wgs_per_cu = 0, list_chunk = 0;
+    } tune_unit;
     // longest list of each launch's terms (keyed by their catalogues), found
     // once per nc_autotune call: its runs share the inputs
     std::map<std::vector<uint64_t>, int> tune_list_bound;
@@ -228,6 +236,10 @@ void plan_drop(nc_context *ctx);
 int run_count(nc_context *ctx, const nc_params *P, const std::vector<Term> &terms, int path, double *hist,
               uint64_t *counts, bool use_graph, Mem &m);
 int refresh_weights(nc_context *ctx, Mem &m, const std::vector<Term> &terms);
+// Whether count_terms runs the unit-weight terms of t (iu) and the weighted
+// ones (iw) as two launches.
+bool split_batch(const nc_context *ctx, const nc_params *params, const std::vector<Term> &t, std::vector<int> &iu,
+                 std::vector<int> &iw);
 int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
                 uint64_t *counts, bool use_graph);
 hipError_t noalloc_base(int device, char **out);

@@ -260,7 +260,13 @@ int nc_set_primary_slice(nc_context *ctx, int slice, int nslices);
  * shapes whose LDS does not fit are skipped), keeps the fastest in ctx and returns it in *best (ms =
  * device time of the sample).  With the environment variable NC_TUNE_CACHE
  * naming a file, results are looked up in and appended to that file, keyed by
- * device, path, survey flag, bin count and catalogue sizes. */
+ * device, path, survey flag, bin count and catalogue sizes.  Where the sample
+ * is a small part of the count, the best shape of each chunk is timed again
+ * on one contiguous primary slice eight times the sample's size.  A batch that
+ * nc_count_batched runs as two launches (unit-weight terms beside weighted
+ * ones) is tuned launch by launch: the context keeps the unit-weight launch's
+ * shape for that launch, *best is the weighted launch's (ms: the sum of the
+ * two), and nc_set_tuning returns to one shape for every launch. */
 typedef struct {
     int waves_per_group, chunk, wgs_per_cu, list_chunk;
     float ms;
