@@ -1396,7 +1396,6 @@ bool split_batch(const nc_context *ctx, const nc_params *params, const std::vect
 
 int count_terms(nc_context *ctx, const nc_params *params, const std::vector<Term> &t, int path, double *hist,
                 uint64_t *counts, bool use_graph) {
-    const int nterms = (int)t.size();
     Mem m;  // the context's own buffers
     m.fail = &ctx->fail_alloc;
     if (const int rc = refresh_weights(ctx, m, t)) return rc;

@@ -393,10 +393,10 @@ __global__ void k_weight_stats(const double4 *__restrict__ raw, long long n, uns
 // S2 x S3 that tests r23 only.
 //
 // Plain list entry (FP64, 40 B):   Elem {x,y,z image position, k, f} + weight
-// Mixed list entry (28 B):         float4 {x,y,z relative to p1, s = |r|^2}
-//                                  + int2 {k | neighbour entry << 8, f} + catalogue index
-// (the FP64 image position of an entry is rebuilt from the catalogue index and
-// its neighbour entry's shift, exactly as the list build computed it)
+// Mixed list entry (20 B):         float4 {x,y,z relative to p1, k | neighbour entry << 8}
+//                                  + catalogue index
+// (the FP64 image position of an entry, and f, are rebuilt from the catalogue
+// index and its neighbour entry's shift, exactly as the list build computed them)
 // k = exact bin of |p1-q|^2, f = sqrtf(float(|p1-q|^2)) as in device.cu:173.
 
 
@@ -426,20 +426,29 @@ __device__ __forceinline__ void rp_side_unpack(double q, float &f, int &k, int &
 }
 
 
-// Arrays of a mixed-path list inside a wave's scratch slot (24 B per entry;
+// Arrays of a mixed-path list inside a wave's scratch slot (20 B per entry;
 // the exact rechecks rebuild the entry's FP64 side, and with it its FP32
-// root, from the catalogue).
+// root, from the catalogue).  The fourth word of R holds the entry's key,
+// not s = |r|^2: the pair loops recompute s from r when they stage rows or
+// load lanes (the same FP32 operations as the list build, so the same
+// value), which keeps one 16 B load per entry and pass.
 struct MList {
-    float4 *R;  // x, y, z relative to p1 (FP32), s = |r|^2 (FP32)
-    int *K;     // exact bin of |p1-q|^2 | neighbour entry << 8
+    float4 *R;  // x, y, z relative to p1 (FP32); .w: the key's bits
     int *I;     // index of the point in the cell-sorted catalogue
 };
 __device__ __forceinline__ MList mlist(const char *base, int cap) {
     MList m;
     m.R = reinterpret_cast<float4 *>(const_cast<char *>(base));
-    m.K = reinterpret_cast<int *>(const_cast<char *>(base) + (size_t)cap * sizeof(float4));
-    m.I = reinterpret_cast<int *>(const_cast<char *>(base) + (size_t)cap * (sizeof(float4) + sizeof(int)));
+    m.I = reinterpret_cast<int *>(const_cast<char *>(base) + (size_t)cap * sizeof(float4));
     return m;
+}
+// An entry's key: exact bin of |p1-q|^2 | neighbour entry << 8.
+__device__ __forceinline__ int mkey(const MList &m, int i) {
+    return reinterpret_cast<const int *>(m.R + i)[3];
+}
+// s = |r|^2 of an entry's relative position (FP32, as the list build makes it)
+__device__ __forceinline__ float rsq(float4 r) {
+    return __builtin_fmaf(r.z, r.z, __builtin_fmaf(r.y, r.y, r.x * r.x));
 }
 
 // The primary of a pair loop: its cell (for the image shifts of the exact
@@ -560,8 +569,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             const MList ml = mlist(b, A.cap);
             const double sg = A.sigma32;  // power of 2: the scaling is exact
             const float rx = (float)((qx - px) * sg), ry = (float)((qy - py) * sg), rz = (float)((qz - pz) * sg);
-            ml.R[pos] = make_float4(rx, ry, rz, __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx)));
-            ml.K[pos] = k | (s << 8);
+            ml.R[pos] = make_float4(rx, ry, rz, __int_as_float(k | (s << 8)));
             ml.I[pos] = j;
         } else {
             Elem e;
@@ -1290,7 +1298,7 @@ __device__ void drain_queue(const KArgs &A, const LdsTabs &t, double w1, const c
     wave_sync();
     for (int i = lane; i < qn; i += kWave) {
         const uint2 e = QueueEntry<HM>::unpack(q[i]);
-        const int ka = m2.K[e.x], kb = m3.K[e.y];
+        const int ka = mkey(m2, e.x), kb = mkey(m3, e.y);
         const int ja = m2.I[e.x], jb = m3.I[e.y];
         double sax, say, saz, sbx, sby, sbz;
         neighbour(A, pr.cx, pr.cy, pr.cz, ka >> 8, sax, say, saz);
@@ -1388,7 +1396,8 @@ struct MixedPol {
             double w = 0.0;
             if (i < last - first) {
                 r = m.R[first + i];
-                const int kk = m.K[first + i];
+                const int kk = __float_as_int(r.w);
+                r.w = rsq(r);
                 k = row_bin<HM>(A, kk & 0xff) | (LASTP ? (kk >> 8) << 24 : 0);
                 if (WEIGHTED) w = staged_weight<WEIGHTED, HM>(wrow, c.unit ? 1.0 : gptr(c.w)[m.I[first + i]]);
             }
@@ -1409,7 +1418,8 @@ struct MixedPol {
         wb = 0.0;
         if (vb) {
             rb = m.R[bidx];
-            const int kk = m.K[bidx];
+            const int kk = __float_as_int(rb.w);
+            rb.w = rsq(rb);
             kb = lane_bin<HM>(A, kk & 0xff);
             if constexpr (LASTP) {
                 const int sb = kk >> 8;

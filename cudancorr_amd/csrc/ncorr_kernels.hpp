@@ -202,7 +202,7 @@ struct RElem {
 
 // Bytes of one list entry in a wave's scratch slot.
 __host__ __device__ constexpr size_t list_entry_bytes(int lm) {
-    return lm == kListMixed ? sizeof(float4) + sizeof(int) + sizeof(int)
+    return lm == kListMixed ? sizeof(float4) + sizeof(int)
                             : lm == kListPlain ? sizeof(Elem) + sizeof(double) : sizeof(RElem);
 }
 
