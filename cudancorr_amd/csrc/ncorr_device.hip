@@ -716,11 +716,21 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
     // then the per-candidate shift and flag are skipped (wave-uniform tests;
     // q + 0.0 would be q in every later use)
     const bool any_shift = __ballot((cpk & 63) != 0) != 0ull;
-    const bool any_flag = __ballot(cpk != 0) != 0ull;
     if (lastmask_out) *lastmask_out = (unsigned)__ballot((cpk & 64) != 0);
     // Candidates v of [0, total), two 64-lane blocks per step with all their
-    // loads in flight; a lane finds its entry by a binary search over the
-    // offsets of lanes 0..31.
+    // loads in flight.  A candidate belongs to the last non-empty entry whose
+    // offset is <= v (offsets of non-empty entries increase strictly).  The
+    // non-empty entries are walked once per primary under scalar control, in
+    // order: a block [w0, w0 + 64) takes over the entry current at w0 and
+    // then each entry starting before w0 + 64, lanes at or past its offset
+    // switching to it.  Lane e < 27 holds its entry's catalogue index offset
+    // cbeg - cexcl (candidate v of the entry is point v + that) and the word
+    // cpk | e << 8 the list entries record.
+    const int cdel = cbeg - cexcl, cword = cpk | (lane << 8);
+    unsigned pend = (unsigned)__ballot((lane < 27) & (clen > 0));  // entries not yet reached
+    int e_nx = pend ? __builtin_ctz(pend) : 0;
+    int s_nx = pend ? __builtin_amdgcn_readlane(cexcl, e_nx) : INT_MAX;  // offset of the next entry
+    int cur_del = 0, cur_word = 0;  // the last entry reached
     constexpr int kB = 2;
     for (int v0 = 0; v0 < total; v0 += kB * kWave) {
         int jj[kB], pk[kB], ent[kB];
@@ -728,23 +738,24 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
         double qx[kB], qy[kB], qz[kB], qw[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            const int v = v0 + u * kWave + lane;
+            const int w0 = v0 + u * kWave, v = w0 + lane;
             ok[u] = v < total;
-            int r = 0, er = 0;
-#pragma unroll
-            for (int st2 = 16; st2 >= 1; st2 >>= 1) {
-                const int x = __shfl(cexcl, r + st2);
-                if (x <= v) {
-                    r += st2;
-                    er = x;
+            int dl = cur_del, wd = cur_word;
+            while (s_nx < w0 + kWave) {  // wave-uniform
+                cur_del = __builtin_amdgcn_readlane(cdel, e_nx);
+                cur_word = __builtin_amdgcn_readlane(cword, e_nx);
+                if (v >= s_nx) {
+                    dl = cur_del;
+                    wd = cur_word;
                 }
+                pend &= pend - 1u;
+                e_nx = pend ? __builtin_ctz(pend) : 0;
+                s_nx = pend ? __builtin_amdgcn_readlane(cexcl, e_nx) : INT_MAX;
             }
-            // every lane shuffles (a shuffle reads 0 from a lane outside the
-            // EXEC mask, so none may sit under the lane's own validity test);
-            // the index of an invalid lane is never used
-            ent[u] = r;
-            jj[u] = __shfl(cbeg, r) + (v - er);
-            pk[u] = any_flag ? __shfl(cpk, r) : 0;
+            // the values of an invalid lane are never used
+            ent[u] = wd >> 8;
+            jj[u] = dl + v;
+            pk[u] = wd & 127;
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
