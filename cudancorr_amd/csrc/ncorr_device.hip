@@ -94,6 +94,18 @@ __device__ __forceinline__ double uniform(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// A wave-uniform 32-bit value held in a VGPR (the compiler would keep it in a
+// scalar register).  Inside the pair loops the scalar registers run out, and
+// a constant spilled to a VGPR lane costs a v_readlane_b32 (a VALU issue slot)
+// at every use; a VGPR operand costs nothing.
+template <typename T>
+__device__ __forceinline__ T in_vgpr(T x) {
+    static_assert(sizeof(T) == 4, "32-bit values");
+    unsigned r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(__builtin_bit_cast(unsigned, x)));
+    return __builtin_bit_cast(T, r);
+}
+
 // A catalogue array as a global-memory pointer.  The catalogue pointers reach
 // the kernels inside TermDesc, so the compiler cannot tell their address
 // space and would load through flat instructions, which count against the
@@ -517,11 +529,11 @@ constexpr int kSelAll = 0, kSelSplit = 1, kSelLow = 2;
 // the shift of the table's .x.
 template <int KS>
 __device__ __forceinline__ int mixed_entry_bin(const KArgs &A, const uint2 *lb32, double d, const double *sq,
-                                               const uint16_t *lut) {
+                                               const uint16_t *lut, float lo32, int sh32, float g32) {
     const float ds = (float)(d * (A.sigma32 * A.sigma32));
-    const uint2 e = lb32[__float_as_uint(__builtin_amdgcn_fmed3f(ds, A.lut32_lo, 4.0f)) >> A.lut32_shift];
+    const uint2 e = lb32[__float_as_uint(__builtin_amdgcn_fmed3f(ds, lo32, 4.0f)) >> sh32];
     const float dE = ds - __uint_as_float(e.y);
-    if (fabsf(dE) <= A.g32) return bin_of(d, A.lut_scale, lut, sq, A.nb);
+    if (fabsf(dE) <= g32) return bin_of(d, A.lut_scale, lut, sq, A.nb);
     return (int)(e.x >> KS) - (dE < 0.0f ? 1 : 0);
 }
 
@@ -561,9 +573,15 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
     // the mixed path's FP32 lookup in LDS (entry c - c0 at lb32[c])
     extern __shared__ __attribute__((aligned(16))) char lds_tables[];
     const uint2 *lb32 = MIXED ? reinterpret_cast<const uint2 *>(lds_tables + A.off_lut32) - A.lut32_c0 : nullptr;
+    // constants of the per-candidate code as VGPR operands (the scalar
+    // registers run out here, and a spilled one costs a v_readlane per use)
+    const int capv = in_vgpr(A.cap);
+    const float lo32v = MIXED ? in_vgpr(A.lut32_lo) : 0.0f, g32v = MIXED ? in_vgpr(A.g32) : 0.0f;
+    const int sh32v = MIXED ? in_vgpr(A.lut32_shift) : 0;
     // one threeD list entry (exact bin k and sqrtf of its side) at position pos of the list at b
     auto store = [&](char *b, int pos, int j, double d, double qx, double qy, double qz, double qw, int s) {
-        const int k = MIXED ? mixed_entry_bin<KS>(A, lb32, d, sq, lut) : bin_of(d, A.lut_scale, lut, sq, A.nb);
+        const int k = MIXED ? mixed_entry_bin<KS>(A, lb32, d, sq, lut, lo32v, sh32v, g32v)
+                            : bin_of(d, A.lut_scale, lut, sq, A.nb);
         const float f = __builtin_sqrtf((float)d);
         if constexpr (MIXED) {
             const MList ml = mlist(b, A.cap);
@@ -606,7 +624,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             }
             const unsigned long long m = __ballot(pass);
             const int pos = n + lane_rank(m);
-            if (pass & (pos < A.cap)) {
+            if (pass & (pos < capv)) {
                 if constexpr (LM == kListRppiBox)  // the side's root, rp bin and pi bin (device.cu:214-250)
                     q = rp_side_pack(__builtin_sqrtf((float)r), bin_of(r, A.lut_scale, lut, sq, A.nb),
                                      (int)floorf(__builtin_sqrtf((float)q)));
@@ -620,14 +638,14 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
         if constexpr (SEL == kSelAll) {
             const unsigned long long m = __ballot(in);
             const int pos = n + lane_rank(m);
-            if (in & (pos < A.cap)) store(base, pos, j, d, qx, qy, qz, qw, s);
+            if (in & (pos < capv)) store(base, pos, j, d, qx, qy, qz, qw, s);
             n += __popcll(m);
         } else if constexpr (SEL == kSelSplit) {
             const bool ph = in & !last & (j > pidx), pb = in & last;
             const unsigned long long mh = __ballot(ph), mb = __ballot(pb);
             const int posh = n + lane_rank(mh), posb = nbl + lane_rank(mb);
             // the two lists take disjoint entries: one store path
-            const bool sh = ph & (posh < A.cap), sb = pb & (posb < A.cap);
+            const bool sh = ph & (posh < capv), sb = pb & (posb < capv);
             if (sh | sb) store(sh ? base : base_b, sh ? posh : posb, j, d, qx, qy, qz, qw, s);
             n += __popcll(mh);
             nbl += __popcll(mb);
@@ -635,7 +653,7 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
             const bool pl = in & !last & (j < pidx);
             const unsigned long long m = __ballot(pl);
             const int pos = n + lane_rank(m);
-            if (pl & (pos < A.cap)) store(base, pos, j, d, qx, qy, qz, qw, s);
+            if (pl & (pos < capv)) store(base, pos, j, d, qx, qy, qz, qw, s);
             n += __popcll(m);
         }
     };
@@ -731,6 +749,9 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
     int e_nx = pend ? __builtin_ctz(pend) : 0;
     int s_nx = pend ? __builtin_amdgcn_readlane(cexcl, e_nx) : INT_MAX;  // offset of the next entry
     int cur_del = 0, cur_word = 0;  // the last entry reached
+    // a point of the first non-empty range: the loads of lanes past the end
+    // read it (always in bounds, never kept), so they need no branch
+    const int jsafe = pend ? __builtin_amdgcn_readlane(cbeg, e_nx) : 0;
     constexpr int kB = 2;
     for (int v0 = 0; v0 < total; v0 += kB * kWave) {
         int jj[kB], pk[kB], ent[kB];
@@ -759,16 +780,15 @@ __device__ __forceinline__ int build_list(const KArgs &A, const double *__restri
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            qx[u] = qy[u] = qz[u] = qw[u] = 0.0;
-            if (ok[u]) {
-                // image position: point + shift (device.cu:282-284); weights
-                // only for the lists that carry them (the mixed list reads them
-                // from the catalogue where they are needed)
-                qx[u] = gptr(X)[jj[u]];
-                qy[u] = gptr(Y)[jj[u]];
-                qz[u] = gptr(Z)[jj[u]];
-                if (!MIXED) qw[u] = gptr(Wt)[jj[u]];
-            }
+            if (v0 + u * kWave >= total) break;  // wave-uniform, as below
+            // image position: point + shift (device.cu:282-284); weights
+            // only for the lists that carry them (the mixed list reads them
+            // from the catalogue where they are needed)
+            const int jl = ok[u] ? jj[u] : jsafe;
+            qx[u] = gptr(X)[jl];
+            qy[u] = gptr(Y)[jl];
+            qz[u] = gptr(Z)[jl];
+            qw[u] = MIXED ? 0.0 : gptr(Wt)[jl];
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
@@ -858,17 +878,6 @@ __device__ __forceinline__ int shl_add(int a, int b) {
     }
 }
 
-// A wave-uniform 32-bit value held in a VGPR (the compiler would keep it in a
-// scalar register).  Inside the pair loops the scalar registers run out, and
-// a constant spilled to a VGPR lane costs a v_readlane_b32 (a VALU issue slot)
-// at every use; a VGPR operand costs nothing.
-template <typename T>
-__device__ __forceinline__ T in_vgpr(T x) {
-    static_assert(sizeof(T) == 4, "32-bit values");
-    unsigned r;
-    asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(__builtin_bit_cast(unsigned, x)));
-    return __builtin_bit_cast(T, r);
-}
 
 // The same with the third bin in layout units: 4k (kHistDense), 2k (kHistT3)
 // or k (kHistCtab).
