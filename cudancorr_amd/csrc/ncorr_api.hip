@@ -388,6 +388,7 @@ static int tune_terms(nc_context *ctx, const nc_params *params, const std::vecto
     nc_tuning b = {saved[0], saved[1], saved[2], saved[3], 1e30f, 0};
     // the best shape for each work-item size, for the second stage
     nc_tuning best_c[2] = {{0, 0, 0, 0, 1e30f, 0}, {0, 0, 0, 0, 1e30f, 0}};
+    std::vector<nc_tuning> runs;  // every shape that ran, with its time
     int rc = NC_OK;
     // 16-wave workgroups only where one workgroup per CU is forced anyway
     // (more than half the CU's LDS); elsewhere two 8-wave ones do better
@@ -428,10 +429,22 @@ static int tune_terms(nc_context *ctx, const nc_params *params, const std::vecto
                     rc = r;
                     goto done;
                 }
-                if (ms < b.ms) b = {w, c, 0, lc, ms, 0};
-                nc_tuning &bc = best_c[c == 16 ? 0 : 1];
-                if (ms < bc.ms) bc = {w, c, 0, lc, ms, 0};
+                runs.push_back({w, c, 0, lc, ms, 0});
             }
+    // The fastest shape, except that a larger list chunk within kTuneTie of
+    // it is preferred: it re-reads the HBM lists fewer times (list chunks 64
+    // and 128 often time within a percent of each other).
+    for (int g = -1; g < 2; ++g) {  // all shapes, then each work-item size
+        nc_tuning &dst = g < 0 ? b : best_c[g];
+        float fastest = 1e30f;
+        for (const nc_tuning &x : runs)
+            if (g < 0 || (x.chunk == 16) == (g == 0)) fastest = std::min(fastest, x.ms);
+        for (const nc_tuning &x : runs)
+            if ((g < 0 || (x.chunk == 16) == (g == 0)) && x.ms <= fastest * (1.0f + kTuneTie) &&
+                (dst.ms >= 1e30f || x.list_chunk > dst.list_chunk ||
+                 (x.list_chunk == dst.list_chunk && x.ms < dst.ms)))
+                dst = x;
+    }
     if (b.ms < 1e30f) rc = NC_OK;  // at least one shape ran
     // Second stage: where the sample is a small part of the count, runs of
     // primaries scattered over the catalogues need not rank the work-item

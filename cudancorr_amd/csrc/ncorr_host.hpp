@@ -107,6 +107,9 @@ using nch::kNoExact;
 // one time share neighbourhoods (in L2) as in a full count, and so that every
 // shape, whatever its work-item size, times the same primaries.
 constexpr long long kSampleRun = 1024;
+// Shapes whose sample time is within this fraction of the fastest count as
+// tied; among them the autotuner takes the largest list chunk.
+constexpr float kTuneTie = 0.005f;
 
 struct nc_catalog {
     nc_context *ctx = nullptr;  // null once the context is destroyed
