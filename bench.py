@@ -19,7 +19,7 @@ timed region; the timed region is bracketed by hipDeviceSynchronize.
 Survey workloads (--workload survey, survey-small): the four terms DDD, DDR,
 DRR, RRR of a cap/shell survey (weighted data, unit-weight randoms) as in
 source/main.cu:72-95.  Each term's primaries are split into --slices slices of
-their cell-sorted order (nc_set_primary_slice, the same catalogue handles);
+their cell-sorted order (16 for survey, 1 for survey-small) (nc_set_primary_slice, the same catalogue handles);
 each slice is one nc_count_batched call over the four terms (mixed) or four
 nc_count calls (plain), and one step is all slices.  Binned triplets
 of the weighted terms come from an untimed unit-weight run of the same terms;
@@ -208,7 +208,7 @@ def run_survey(args):
     log(f"generated {d.shape[0]} data + {r.shape[0]} randoms in {time.perf_counter() - t0:.1f}s")
     hip = _hip()
     ctx = Context(0)
-    K = max(1, args.slices)
+    K = max(1, args.slices if args.slices is not None else (1 if args.workload == "survey-small" else 16))
     du = d.copy()
     du[:, 3] = 1.0
     D, R, DU = ctx.catalog(d), ctx.catalog(r), ctx.catalog(du)
@@ -653,7 +653,9 @@ def main():
                     help="boxes: 64 clustered periodic boxes (default); survey: 4M data + 20M randoms, "
                          "four terms; survey-small: 500k data + 2.5M randoms; rppi-box: rppi binning of "
                          "the 1M-point box; rppi-survey: rppi binning of the survey-small catalogues")
-    ap.add_argument("--slices", type=int, default=16, help="survey: primaries split into this many launches")
+    ap.add_argument("--slices", type=int, default=None,
+                    help="survey: primaries split into this many launches (default 16 for survey, about 6 s "
+                         "each; 1 for survey-small, whose whole count takes about 60 ms)")
     ap.add_argument("--boxes", type=int, default=64)
     ap.add_argument("--no-plain", action="store_true", help="skip the 64-call plain leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
