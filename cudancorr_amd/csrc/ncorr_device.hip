@@ -1549,8 +1549,20 @@ struct MixedPol {
         }
     }
 
+    // The diagonal block [b0, b0 + m) against itself: lane l meets the row
+    // (l + st) mod m for st = 1 .. m/2 (the last step of an even m on the
+    // lower half of the lanes only), each unordered pair once.  For l < m,
+    // l + st < 2m, so (l + st) mod m is the smaller of l + st and l + st - m
+    // as unsigned values.  A lane past m (m < 64) gets l + st - m <= 63 + m/2
+    // - m < 64 from the same formula: its read stays inside the staged chunk
+    // (b0 - row_first + 63 < C), and vb masks its pairs.
     __device__ void circ(int b0, int m) {
         const int smax = m / 2;
+        const bool lower = lane < smax;  // lanes of the last step when m is even
+        const float4 *lab = la + (b0 - row_first);
+        const int *lkb = lk + (b0 - row_first);
+        const double *lwb = lw + (b0 - row_first);
+        const unsigned mu = (unsigned)m;
         for (int s = 1; s <= smax; s += U) {
             float4 ra[U];
             int ka[U], ai[U];
@@ -1559,14 +1571,13 @@ struct MixedPol {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int st = s + u;
-                int off = lane + st;
-                off -= off >= m ? m : 0;
-                ok[u] = vb && st <= smax && (2 * st != m || lane < st);
-                const int li = b0 - row_first + (ok[u] ? off : 0);
-                ra[u] = la[li];
-                ka[u] = lk[li];
-                wa[u] = WEIGHTED ? lw[li] : 0.0;
-                ai[u] = row_first + li;
+                const unsigned x = (unsigned)(lane + st), xm = x - mu;
+                const int off = (int)(xm < x ? xm : x);
+                ok[u] = vb & (st <= smax) & ((2 * st != m) | lower);
+                ra[u] = lab[off];
+                ka[u] = lkb[off];
+                wa[u] = WEIGHTED ? lwb[off] : 0.0;
+                ai[u] = b0 + off;
             }
             group<false>(ra, ka, wa, ai, ok);
         }
