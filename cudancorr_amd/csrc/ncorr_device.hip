@@ -1380,6 +1380,7 @@ struct MixedPol {
     float wbf;    // the lane's weight in FP32 (kHistF32: FP32 weights)
     bool vb;
     unsigned skipb;  // LASTP: row entries this lane skips
+    int rot_b0;      // circ_full: the diagonal block's first list index
 
     __device__ MixedPol(const KArgs &A_, const LdsTabs &t_, double w1_, const char *l2_, const char *l3_, char *wlds,
                         int lane_, const CatRef &cat2, const CatRef &cat3, const PrimRef &pr_)
@@ -1459,7 +1460,9 @@ struct MixedPol {
 
     // U pairs: rows (ra, ka, wa, list index ai) with the lane entry.  SWAP:
     // the rows are S3 entries and the lanes S2 entries.
-    template <bool SWAP>
+    // ROT: the rows are the lanes' own block rotated (circ_full): ai holds
+    // the step st, and the row's list index is rot_b0 + ((lane + st) & 63).
+    template <bool SWAP, bool ROT = false>
     __device__ __forceinline__ void group(const float4 (&ra)[U], const int (&ka)[U], const double (&wa)[U],
                                           const int (&ai)[U], const bool (&ok)[U]) {
         float d[U], dot[U], ss[U];
@@ -1520,9 +1523,11 @@ struct MixedPol {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const unsigned long long m = __ballot(amb[u]);
-                if (amb[u])
-                    q[qn + lane_rank(m)] = SWAP ? QueueEntry<HM>::pack((unsigned)bidx, (unsigned)ai[u])
-                                                : QueueEntry<HM>::pack((unsigned)ai[u], (unsigned)bidx);
+                if (amb[u]) {
+                    const unsigned ar = ROT ? (unsigned)(rot_b0 + ((lane + ai[u]) & (kWave - 1))) : (unsigned)ai[u];
+                    q[qn + lane_rank(m)] = SWAP ? QueueEntry<HM>::pack((unsigned)bidx, ar)
+                                                : QueueEntry<HM>::pack(ar, (unsigned)bidx);
+                }
                 qn += __popcll(m);
             }
             // one group adds at most U*64 entries: drain once the queue could overflow
@@ -1557,6 +1562,10 @@ struct MixedPol {
     // - m < 64 from the same formula: its read stays inside the staged chunk
     // (b0 - row_first + 63 < C), and vb masks its pairs.
     __device__ void circ(int b0, int m) {
+        if (m == kWave) {
+            circ_full(b0);
+            return;
+        }
         const int smax = m / 2;
         const bool lower = lane < smax;  // lanes of the last step when m is even
         const float4 *lab = la + (b0 - row_first);
@@ -1580,6 +1589,49 @@ struct MixedPol {
                 ai[u] = b0 + off;
             }
             group<false>(ra, ka, wa, ai, ok);
+        }
+    }
+
+    // A full diagonal block (m = 64) without row reads: each lane loads the
+    // staged row form of its own entry once, and the copies move one lane
+    // down per step (DPP wave_rol:1, lane l taking lane l + 1 mod 64), so at
+    // step st lane l holds entry (l + st) mod 64 -- the circulant pairing of
+    // circ, with no LDS gather per pair.
+    static __device__ __forceinline__ int rol1(int x) { return __builtin_amdgcn_mov_dpp(x, 0x134, 0xF, 0xF, false); }
+    static __device__ __forceinline__ float rol1(float x) { return __int_as_float(rol1(__float_as_int(x))); }
+    static __device__ __forceinline__ double rol1(double x) {
+        const long long b = __double_as_longlong(x);
+        const unsigned lo = (unsigned)rol1((int)b), hi = (unsigned)rol1((int)(b >> 32));
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+    __device__ void circ_full(int b0) {
+        const int i0 = b0 - row_first + lane;
+        float4 r = la[i0];
+        int k = lk[i0];
+        double w = WEIGHTED ? lw[i0] : 0.0;
+        const bool lower = lane < kWave / 2;  // lanes of the last step
+        rot_b0 = b0;
+        static_assert((kWave / 2) % U == 0, "whole groups of steps");
+        for (int s = 1; s <= kWave / 2; s += U) {
+            float4 ra[U];
+            int ka[U], ai[U];
+            double wa[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                r.x = rol1(r.x);
+                r.y = rol1(r.y);
+                r.z = rol1(r.z);
+                r.w = rol1(r.w);
+                k = rol1(k);
+                if (WEIGHTED) w = rol1(w);
+                ra[u] = r;
+                ka[u] = k;
+                wa[u] = w;
+                ai[u] = s + u;
+                ok[u] = vb & ((s + u != kWave / 2) | lower);
+            }
+            group<false, true>(ra, ka, wa, ai, ok);
         }
     }
 };
